@@ -1,0 +1,410 @@
+// GPU assembly of the FEM system on gfx950.
+//
+//  1. k_tet_geometry: one thread per tet computes the P1 gradients and the
+//     volume-scaled material factors (V*sigma, or V*lambda and V*mu).
+//  2. k_gather_*: one thread per pattern slot sums that slot's element
+//     contributions in ascending tet order (the order in which the
+//     reference's serial SummPartOfMatrix visits elements, FEM.h:2146-2155)
+//     and writes the entry straight into the SELL-64 value array.  No
+//     atomics, so results are deterministic.
+//  3. k_dirichlet_*: elimination of fixed DOFs (FEM.h:505-619): fixed rows
+//     become identity rows, fixed columns move to the right-hand side.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mfs_device.hpp"
+
+using namespace mfs;
+
+struct mfs_asm_s {
+  mfs_ctx ctx = nullptr;
+  int problem = MFS_PROBLEM_SCALAR;
+  int convention = MFS_ELEM_REFERENCE;
+  int bs = 1;
+  int64_t nn = 0, nt = 0;
+  std::vector<int32_t> map;  // original node -> DOF node
+  mfs_mat A = nullptr;
+  mfs_vec b = nullptr, x0 = nullptr;
+  // device
+  double* d_xyz = nullptr;      // 3 per DOF node
+  int32_t* d_tets = nullptr;    // 4 per tet, DOF node ids
+  double* d_coef = nullptr;     // per tet
+  double* d_tet = nullptr;      // 16 doubles per tet: g[4][3], f0, f1, V, pad
+  int64_t* d_gptr = nullptr;    // nnzb+1
+  uint32_t* d_gent = nullptr;   // 16 per tet
+  int32_t* d_browptr = nullptr; // nb+1
+  uint8_t* d_fixed = nullptr;   // per scalar DOF
+  double* d_g = nullptr;        // per scalar DOF (fixed values)
+  double* d_fbase = nullptr;    // Neumann loads per scalar DOF
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+// Gradients of the 4 barycentric functions: grad phi_1..3 = rows of J^{-T}
+// with J = [p1-p0, p2-p0, p3-p0]; grad phi_0 = -(sum).  V = |det J| / 6.
+__global__ __launch_bounds__(256) void k_tet_geometry(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
+                                                      const double* __restrict__ coef, int64_t nt, int problem,
+                                                      double nu, double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nt; e += (int64_t)gridDim.x * blockDim.x) {
+    const int4 t = *reinterpret_cast<const int4*>(tets + 4 * e);
+    double p[4][3];
+    const int32_t id[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) p[v][a] = xyz[3 * (int64_t)id[v] + a];
+    double J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) J[r][a] = p[r + 1][a] - p[0][a];
+    // cofactors: inverse(J)^T rows give grad phi_{1..3}
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double inv = 1.0 / det;
+    double g[4][3];
+    // grad phi_k (k=1..3) = column k-1 of J^{-1} = (cofactor row k-1) / det
+    g[1][0] = c00 * inv; g[1][1] = c01 * inv; g[1][2] = c02 * inv;
+    g[2][0] = c10 * inv; g[2][1] = c11 * inv; g[2][2] = c12 * inv;
+    g[3][0] = c20 * inv; g[3][1] = c21 * inv; g[3][2] = c22 * inv;
+#pragma unroll
+    for (int a = 0; a < 3; a++) g[0][a] = -(g[1][a] + g[2][a] + g[3][a]);
+    const double V = fabs(det) / 6.0;
+    double* o = out + 16 * e;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) o[3 * v + a] = g[v][a];
+    const double c = coef[e];
+    if (problem == MFS_PROBLEM_SCALAR) {
+      o[12] = V * c;
+      o[13] = 0.0;
+    } else {
+      const double lam = c * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+      const double mu = c / (2.0 * (1.0 + nu));
+      o[12] = V * lam;
+      o[13] = V * mu;
+    }
+    o[14] = V;
+    o[15] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_scalar(const int64_t* __restrict__ gptr, const uint32_t* __restrict__ gent,
+                                                       const double* __restrict__ tet, const int64_t* __restrict__ pos,
+                                                       int64_t nnzb, double* __restrict__ sell) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nnzb; s += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t k = gptr[s]; k < gptr[s + 1]; k++) {
+      const uint32_t c = gent[k];
+      const int64_t e = c >> 4;
+      const int i = (c >> 2) & 3, j = c & 3;
+      const double* o = tet + 16 * e;
+      acc += o[12] * (o[3 * i] * o[3 * j] + o[3 * i + 1] * o[3 * j + 1] + o[3 * i + 2] * o[3 * j + 2]);
+    }
+    const int64_t p = pos[s];
+    sell[p] = acc;  // bs = 1: entry index == value index
+  }
+}
+
+// Elastic block (I=i, J=j) of tet e:
+//   textbook  K[a][b] = V(lam dI_a dJ_b + mu dI_b dJ_a + mu delta_ab dI.dJ)
+//   reference K[a][b] = textbook[b][a] (FEM_Grid.h:2159-2191 puts B_J^T on
+//   the left of D).
+template <int CONV>
+__global__ __launch_bounds__(256) void k_gather_elastic(const int64_t* __restrict__ gptr,
+                                                        const uint32_t* __restrict__ gent,
+                                                        const double* __restrict__ tet,
+                                                        const int64_t* __restrict__ pos, int64_t nnzb,
+                                                        double* __restrict__ sell) {
+  constexpr int C = 64;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nnzb; s += (int64_t)gridDim.x * blockDim.x) {
+    double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int64_t k = gptr[s]; k < gptr[s + 1]; k++) {
+      const uint32_t c = gent[k];
+      const int64_t e = c >> 4;
+      const int i = (c >> 2) & 3, j = c & 3;
+      const double* o = tet + 16 * e;
+      const double dI[3] = {o[3 * i], o[3 * i + 1], o[3 * i + 2]};
+      const double dJ[3] = {o[3 * j], o[3 * j + 1], o[3 * j + 2]};
+      const double vl = o[12], vm = o[13];
+      const double dd = dI[0] * dJ[0] + dI[1] * dJ[1] + dI[2] * dJ[2];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          double v;
+          if (CONV == MFS_ELEM_TEXTBOOK)
+            v = vl * dI[a] * dJ[b] + vm * dI[b] * dJ[a];
+          else
+            v = vl * dI[b] * dJ[a] + vm * dI[a] * dJ[b];
+          if (a == b) v += vm * dd;
+          K[a][b] += v;
+        }
+    }
+    const int64_t p = pos[s];
+    const int64_t base = (p / C) * C * 9 + (p % C);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) sell[base + (int64_t)(3 * a + b) * C] = K[a][b];
+  }
+}
+
+// Dirichlet elimination, one thread per block row.  fixed/g indexed by
+// scalar DOF.  Row-fixed: zero the row, diagonal 1, rhs = g.  Otherwise every
+// fixed column c: rhs -= a_rc g_c, a_rc = 0.
+template <int BS>
+__global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ browptr, const int32_t* __restrict__ col,
+                                                   const int64_t* __restrict__ slice_base, double* __restrict__ sell,
+                                                   const uint8_t* __restrict__ fixed, const double* __restrict__ g,
+                                                   const double* __restrict__ fbase, double* __restrict__ rhs,
+                                                   double* __restrict__ x0, int64_t nb) {
+  constexpr int C = 64;
+  for (int64_t R = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; R < nb; R += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = R / C, lane = R % C;
+    const int64_t sb = slice_base[q];
+    const int len = browptr[R + 1] - browptr[R];
+#pragma unroll
+    for (int a = 0; a < BS; a++) {
+      const int64_t i = BS * R + a;
+      double f = fbase[i];
+      const bool rf = fixed[i] != 0;
+      for (int j = 0; j < len; j++) {
+        const int64_t ent = sb + (int64_t)j * C + lane;
+        const int32_t c = col[ent];
+        const int64_t vbase = (sb + (int64_t)j * C) * (BS * BS) + lane;
+#pragma unroll
+        for (int b = 0; b < BS; b++) {
+          const int64_t vi = vbase + (int64_t)(BS * a + b) * C;
+          const int64_t cj = (int64_t)BS * c + b;
+          if (rf) {
+            sell[vi] = (cj == i) ? 1.0 : 0.0;
+          } else if (fixed[cj]) {
+            f -= sell[vi] * g[cj];
+            sell[vi] = 0.0;
+          }
+        }
+      }
+      rhs[i] = rf ? g[i] : f;
+      x0[i] = rf ? g[i] : 0.0;
+    }
+  }
+}
+
+static inline unsigned grid_cap(int64_t n, int64_t cap = 65536) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap));
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int mfs_assemble_destroy(mfs_asm S) {
+  if (!S) return MFS_OK;
+  if (S->ctx) (void)hipStreamSynchronize(S->ctx->stream);
+  mfs_mat_destroy(S->A);
+  mfs_vec_destroy(S->b);
+  mfs_vec_destroy(S->x0);
+  dfree(S->d_xyz);
+  dfree(S->d_tets);
+  dfree(S->d_coef);
+  dfree(S->d_tet);
+  dfree(S->d_gptr);
+  dfree(S->d_gent);
+  dfree(S->d_browptr);
+  dfree(S->d_fixed);
+  dfree(S->d_g);
+  dfree(S->d_fbase);
+  delete S;
+  return MFS_OK;
+}
+
+extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* out) {
+  MFS_CHECK_ARG(ctx && d && out, "mfs_assemble_create: bad argument");
+  MFS_CHECK_ARG(d->problem == MFS_PROBLEM_SCALAR || d->problem == MFS_PROBLEM_ELASTIC,
+                "mfs_assemble_create: unknown problem");
+  MFS_CHECK_ARG(d->n_nodes > 0 && d->n_tets > 0 && d->xyz && d->tets, "mfs_assemble_create: empty mesh");
+  MFS_CHECK_ARG(d->n_tets < ((int64_t)1 << 28), "mfs_assemble_create: more than 2^28 tets");
+  MFS_CHECK_ARG(d->n_fixed == 0 || (d->fixed_nodes && d->fixed_values), "mfs_assemble_create: fixed arrays missing");
+  MFS_CHECK_ARG(d->n_tris == 0 || (d->tris && d->traction), "mfs_assemble_create: Neumann arrays missing");
+  MFS_CHECK_ARG(d->convention == MFS_ELEM_REFERENCE || d->convention == MFS_ELEM_TEXTBOOK,
+                "mfs_assemble_create: unknown element convention");
+  const int64_t nn = d->n_nodes, nt = d->n_tets;
+  for (int64_t k = 0; k < 4 * nt; k++)
+    MFS_CHECK_ARG(d->tets[k] >= 0 && d->tets[k] < nn, "mfs_assemble_create: tet node out of range");
+  for (int64_t k = 0; k < d->n_fixed; k++)
+    MFS_CHECK_ARG(d->fixed_nodes[k] >= 0 && d->fixed_nodes[k] < nn, "mfs_assemble_create: fixed node out of range");
+  for (int64_t k = 0; k < 3 * d->n_tris; k++)
+    MFS_CHECK_ARG(d->tris[k] >= 0 && d->tris[k] < nn, "mfs_assemble_create: triangle node out of range");
+  MFS_HIP(hipSetDevice(ctx->device));
+  mfs_asm S = new mfs_asm_s();
+  S->ctx = ctx;
+  S->problem = d->problem;
+  S->convention = d->convention;
+  S->bs = d->problem == MFS_PROBLEM_ELASTIC ? 3 : 1;
+  S->nn = nn;
+  S->nt = nt;
+  S->map.resize(nn);
+  if (d->renumber)
+    renumber_mech(nn, nt, d->tets, d->material, d->n_fixed, d->fixed_nodes, S->map.data());
+  else
+    std::iota(S->map.begin(), S->map.end(), 0);
+  const std::vector<int32_t>& map = S->map;
+  std::vector<int32_t> rt(4 * nt);
+  for (int64_t k = 0; k < 4 * nt; k++) rt[k] = map[d->tets[k]];
+  std::vector<double> rxyz(3 * nn);
+  for (int64_t v = 0; v < nn; v++)
+    for (int a = 0; a < 3; a++) rxyz[3 * (int64_t)map[v] + a] = d->xyz[3 * v + a];
+  Pattern P;
+  build_pattern(nn, nt, rt.data(), P);
+  GatherLists G;
+  build_gather_lists(P, nt, rt.data(), G);
+  std::vector<int32_t> browptr(P.rowptr);
+  const int bs = S->bs;
+  const int64_t ndof = bs * nn;
+  std::vector<uint8_t> fixed(ndof, 0);
+  std::vector<double> g(ndof, 0.0), fbase(ndof, 0.0);
+  for (int64_t k = 0; k < d->n_fixed; k++) {
+    int64_t node = map[d->fixed_nodes[k]];
+    for (int a = 0; a < bs; a++) {
+      fixed[bs * node + a] = 1;
+      g[bs * node + a] = d->fixed_values[bs * k + a];
+    }
+  }
+  // Neumann loads: f_I += t * area / 3 for each triangle node (exact P1
+  // integral of the reference's 4-point triangle rule, FEM_Grid.h:2622-2661)
+  if (bs == 3)
+    for (int64_t t = 0; t < d->n_tris; t++) {
+      const double* p0 = d->xyz + 3 * (int64_t)d->tris[3 * t];
+      const double* p1 = d->xyz + 3 * (int64_t)d->tris[3 * t + 1];
+      const double* p2 = d->xyz + 3 * (int64_t)d->tris[3 * t + 2];
+      const double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+      const double w[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+      const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+      const double third = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz) / 3.0;
+      for (int q = 0; q < 3; q++) {
+        const int64_t node = map[d->tris[3 * t + q]];
+        for (int a = 0; a < 3; a++) fbase[3 * node + a] += d->traction[3 * t + a] * third;
+      }
+    }
+  int st = mat_create_from_pattern(ctx, bs, std::move(P), &S->A);
+  if (st == MFS_OK) st = mfs_vec_create(ctx, ndof, MFS_F64, &S->b);
+  if (st == MFS_OK) st = mfs_vec_create(ctx, ndof, MFS_F64, &S->x0);
+  auto alloc = [&](void** p, size_t bytes) {
+    if (st == MFS_OK) st = dmalloc(p, bytes);
+  };
+  alloc((void**)&S->d_xyz, 8 * 3 * nn);
+  alloc((void**)&S->d_tets, 4 * 4 * nt);
+  alloc((void**)&S->d_coef, 8 * nt);
+  alloc((void**)&S->d_tet, 8 * 16 * nt);
+  alloc((void**)&S->d_gptr, 8 * G.ptr.size());
+  alloc((void**)&S->d_gent, 4 * G.entry.size());
+  alloc((void**)&S->d_browptr, 4 * browptr.size());
+  alloc((void**)&S->d_fixed, ndof);
+  alloc((void**)&S->d_g, 8 * ndof);
+  alloc((void**)&S->d_fbase, 8 * ndof);
+  if (st != MFS_OK) {
+    mfs_assemble_destroy(S);
+    return st;
+  }
+  hipStream_t s = ctx->stream;
+  MFS_HIP(hipMemcpyAsync(S->d_xyz, rxyz.data(), 8 * 3 * nn, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_tets, rt.data(), 4 * 4 * nt, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_gptr, G.ptr.data(), 8 * G.ptr.size(), hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_gent, G.entry.data(), 4 * G.entry.size(), hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_browptr, browptr.data(), 4 * browptr.size(), hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_fixed, fixed.data(), ndof, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_g, g.data(), 8 * ndof, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_fbase, fbase.data(), 8 * ndof, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  *out = S;
+  return MFS_OK;
+}
+
+static int assemble_kernels(mfs_asm S, double nu) {
+  hipStream_t s = S->ctx->stream;
+  mfs_mat A = S->A;
+  k_tet_geometry<<<grid_cap(S->nt), 256, 0, s>>>(S->d_xyz, S->d_tets, S->d_coef, S->nt, S->problem, nu, S->d_tet);
+  if (S->bs == 1)
+    k_gather_scalar<<<grid_cap(A->nnzb), 256, 0, s>>>(S->d_gptr, S->d_gent, S->d_tet, A->d_pos, A->nnzb, A->d_val64);
+  else if (S->convention == MFS_ELEM_TEXTBOOK)
+    k_gather_elastic<MFS_ELEM_TEXTBOOK><<<grid_cap(A->nnzb), 256, 0, s>>>(S->d_gptr, S->d_gent, S->d_tet, A->d_pos,
+                                                                          A->nnzb, A->d_val64);
+  else
+    k_gather_elastic<MFS_ELEM_REFERENCE><<<grid_cap(A->nnzb), 256, 0, s>>>(S->d_gptr, S->d_gent, S->d_tet, A->d_pos,
+                                                                           A->nnzb, A->d_val64);
+  if (S->bs == 1)
+    k_dirichlet<1><<<grid_cap(A->nb), 256, 0, s>>>(S->d_browptr, A->d_col, A->d_slice_base, A->d_val64, S->d_fixed,
+                                                   S->d_g, S->d_fbase, (double*)S->b->d, (double*)S->x0->d, A->nb);
+  else
+    k_dirichlet<3><<<grid_cap(A->nb), 256, 0, s>>>(S->d_browptr, A->d_col, A->d_slice_base, A->d_val64, S->d_fixed,
+                                                   S->d_g, S->d_fbase, (double*)S->b->d, (double*)S->x0->d, A->nb);
+  MFS_HIP(hipGetLastError());
+  return mat_refresh_derived(A);
+}
+
+extern "C" int mfs_assemble_run(mfs_asm S, const double* coef, double nu) {
+  MFS_CHECK_ARG(S && coef, "mfs_assemble_run: bad argument");
+  MFS_CHECK_ARG(S->problem == MFS_PROBLEM_SCALAR || (nu > -1.0 && nu < 0.5), "mfs_assemble_run: nu out of range");
+  hipStream_t s = S->ctx->stream;
+  MFS_HIP(hipMemcpyAsync(S->d_coef, coef, 8 * S->nt, hipMemcpyHostToDevice, s));
+  MFS_TRY(assemble_kernels(S, nu));
+  MFS_HIP(hipStreamSynchronize(s));
+  S->A->val32_valid = true;
+  return MFS_OK;
+}
+
+extern "C" int mfs_assemble_time(mfs_asm S, int reps, double* avg_ms) {
+  MFS_CHECK_ARG(S && reps > 0 && avg_ms, "mfs_assemble_time: bad argument");
+  hipStream_t s = S->ctx->stream;
+  // nu of the last run is not stored; timing uses 0.3 (cost is independent)
+  MFS_TRY(assemble_kernels(S, 0.3));
+  hipEvent_t e0, e1;
+  MFS_HIP(hipEventCreate(&e0));
+  MFS_HIP(hipEventCreate(&e1));
+  MFS_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) MFS_TRY(assemble_kernels(S, 0.3));
+  MFS_HIP(hipEventRecord(e1, s));
+  MFS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = ms / reps;
+  return MFS_OK;
+}
+
+extern "C" int mfs_assemble_matrix(mfs_asm S, mfs_mat* A) {
+  MFS_CHECK_ARG(S && A, "mfs_assemble_matrix: bad argument");
+  *A = S->A;
+  return MFS_OK;
+}
+extern "C" int mfs_assemble_rhs(mfs_asm S, mfs_vec* b) {
+  MFS_CHECK_ARG(S && b, "mfs_assemble_rhs: bad argument");
+  *b = S->b;
+  return MFS_OK;
+}
+extern "C" int mfs_assemble_x0(mfs_asm S, mfs_vec* x0) {
+  MFS_CHECK_ARG(S && x0, "mfs_assemble_x0: bad argument");
+  *x0 = S->x0;
+  return MFS_OK;
+}
+extern "C" int mfs_assemble_node_map(mfs_asm S, int32_t* map) {
+  MFS_CHECK_ARG(S && map, "mfs_assemble_node_map: bad argument");
+  std::memcpy(map, S->map.data(), 4 * S->nn);
+  return MFS_OK;
+}

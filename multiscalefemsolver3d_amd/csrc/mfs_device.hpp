@@ -1,0 +1,216 @@
+// Internal device-side definitions shared by the HIP sources of libmfs.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mfs.h"
+#include "mfs_host.hpp"
+
+#define MFS_WAVE 64
+
+// HIP call check: records the error text and returns MFS_ERR_HIP (or OOM).
+#define MFS_HIP(call)                                                                         \
+  do {                                                                                        \
+    hipError_t _e = (call);                                                                   \
+    if (_e != hipSuccess) {                                                                   \
+      mfs::set_error(std::string(#call) + ": " + hipGetErrorString(_e));                     \
+      return _e == hipErrorOutOfMemory ? MFS_ERR_OOM : MFS_ERR_HIP;                          \
+    }                                                                                         \
+  } while (0)
+
+#define MFS_CHECK_ARG(cond, msg)  \
+  do {                            \
+    if (!(cond)) {                \
+      mfs::set_error(msg);        \
+      return MFS_ERR_ARG;         \
+    }                             \
+  } while (0)
+
+#define MFS_TRY(expr)          \
+  do {                         \
+    int _s = (expr);           \
+    if (_s != MFS_OK) return _s; \
+  } while (0)
+
+struct mfs_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  char name[256] = {0};
+  // small pinned host buffer for status read-backs
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+};
+
+struct mfs_vec_s {
+  mfs_ctx ctx = nullptr;
+  int64_t n = 0;
+  int prec = MFS_F64;
+  void* d = nullptr;
+};
+
+// Sparse matrix: pattern in CSSD order (host copy for export), device values
+// in a SELL-64 layout (thread per block row, column-major inside a slice).
+struct mfs_mat_s {
+  mfs_ctx ctx = nullptr;
+  int bs = 1;            // 1 = CSR, 3 = BSR3
+  int64_t nb = 0;        // block rows
+  int64_t nnzb = 0;      // block nonzeros
+  mfs::Pattern pat;      // block pattern
+  mfs::SellLayout sell;  // host copy of the layout
+  int64_t sell_entries = 0;
+  // device
+  int64_t* d_slice_base = nullptr;
+  int32_t* d_slice_width = nullptr;
+  int32_t* d_col = nullptr;     // sell_entries
+  int64_t* d_pos = nullptr;     // nnzb: CSR slot -> SELL entry
+  int64_t* d_diag_pos = nullptr;// nb: SELL entry of the diagonal block
+  double* d_val64 = nullptr;    // bs*bs*sell_entries
+  float* d_val32 = nullptr;
+  double* d_dinv64 = nullptr;   // bs*nb inverse diagonal (Jacobi)
+  float* d_dinv32 = nullptr;
+  bool val32_valid = false;
+  int64_t n() const { return bs * nb; }
+  int64_t nnz() const { return (int64_t)bs * bs * nnzb; }
+};
+
+namespace mfs {
+
+// allocate + track
+int dmalloc(void** p, size_t bytes);
+void dfree(void* p);
+
+// matrix internals used by assembly and MsFEM
+int mat_create_from_pattern(mfs_ctx ctx, int bs, Pattern&& P, mfs_mat* out);
+int mat_refresh_derived(mfs_mat A);  // FP32 copy + inverse diagonals from d_val64
+int spmv_launch(mfs_mat A, int prec, const void* x, void* y, hipStream_t s);
+
+// Algorithmic bytes of one SpMV (reference formula): nnz values, block
+// indices, row pointers, x and y.
+double spmv_bytes(mfs_mat A, int prec);
+
+// Grid-wide deterministic reduction scratch: per-block partials + counter.
+struct ReduceScratch {
+  double* partials = nullptr;  // capacity doubles
+  unsigned* counter = nullptr;
+  int64_t capacity = 0;
+};
+int reduce_scratch_alloc(ReduceScratch& r, int64_t capacity);
+void reduce_scratch_free(ReduceScratch& r);
+
+}  // namespace mfs
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = MFS_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, MFS_WAVE);
+  return v;
+}
+
+// Block-level sum of NV values; result valid in thread 0.  Fixed order.
+template <typename T, int NV, int BLOCK>
+__device__ __forceinline__ void block_sum(T (&v)[NV], T* lds /* NV * BLOCK/64 */) {
+  const int lane = threadIdx.x & (MFS_WAVE - 1), wave = threadIdx.x / MFS_WAVE;
+  constexpr int NW = BLOCK / MFS_WAVE;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    T s = wave_sum(v[k]);
+    if (lane == 0) lds[k * NW + wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      T s = 0;
+      for (int w = 0; w < NW; w++) s += lds[k * NW + w];
+      v[k] = s;
+    }
+  }
+}
+
+// Grid reduction: every block contributes NV values; the last block to
+// arrive sums all partials in block order (deterministic) and returns true
+// with the totals in `tot` (valid in all threads of that block).  The counter
+// is reset by the last block, so the kernel can be replayed (HIP graphs).
+template <typename T, int NV, int BLOCK>
+__device__ __forceinline__ bool grid_sum(T (&v)[NV], double* partials, unsigned* counter, double (&tot)[NV]) {
+  __shared__ T lds[NV * (BLOCK / MFS_WAVE)];
+  __shared__ double red[NV][BLOCK / MFS_WAVE];
+  __shared__ bool last;
+  block_sum<T, NV, BLOCK>(v, lds);
+  const unsigned G = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) partials[(size_t)k * G + blockIdx.x] = (double)v[k];
+    __threadfence();
+    unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = (prev == G - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  const int lane = threadIdx.x & (MFS_WAVE - 1), wave = threadIdx.x / MFS_WAVE;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < G; b += BLOCK)
+      s += __hip_atomic_load(&partials[(size_t)k * G + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = wave_sum(s);
+    if (lane == 0) red[k][wave] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int w = 0; w < BLOCK / MFS_WAVE; w++) s += red[k][w];
+    tot[k] = s;
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// SELL-64 row product: thread `lane` of slice `slice` accumulates its block
+// row of A x into acc[BS].  Values of entry j, component e live at
+// val[(slice_base + j*64)*BS*BS + e*64 + lane]; matrix streams are read with
+// non-temporal loads (used once per product), x through the caches.
+template <typename T, int BS>
+__device__ __forceinline__ void sell_row(const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
+                                         const int32_t* __restrict__ col, const T* __restrict__ val,
+                                         const T* __restrict__ x, int64_t slice, int lane, T (&acc)[BS]) {
+  constexpr int C = 64;
+#pragma unroll
+  for (int a = 0; a < BS; a++) acc[a] = 0;
+  const int64_t sb = slice_base[slice];
+  const int w = slice_width[slice];
+  const int32_t* cp = col + sb + lane;
+  const T* vp = val + sb * (BS * BS) + lane;
+  for (int j = 0; j < w; j++) {
+    const int32_t c = __builtin_nontemporal_load(cp + (int64_t)j * C);
+    if constexpr (BS == 1) {
+      acc[0] += __builtin_nontemporal_load(vp + (int64_t)j * C) * x[c];
+    } else {
+      const T x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
+      const T* vj = vp + (int64_t)j * (BS * BS) * C;
+#pragma unroll
+      for (int a = 0; a < BS; a++) {
+        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 0) * C) * x0;
+        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 1) * C) * x1;
+        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 2) * C) * x2;
+      }
+    }
+  }
+}
+
+namespace mfs {
+int spmv_dot_launch(mfs_mat A, int prec, const void* x, void* y, double* partials, unsigned* counter, double* out,
+                    hipStream_t s);
+int spmv_residual_launch(mfs_mat A, int prec, const void* x, void* r, const void* b, double* partials,
+                         unsigned* counter, double* out, hipStream_t s);
+int64_t spmv_grid(mfs_mat A);
+}  // namespace mfs

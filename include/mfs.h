@@ -279,6 +279,13 @@ int mfs_batch_get_basis(mfs_batch B, int cell, double* basis);
 int mfs_batch_coarse_matrices(mfs_batch B, double* Kc);
 /* Per-problem inner iteration counts, 24 per cell. */
 int mfs_batch_iterations(mfs_batch B, int32_t* iters);
+/* Learned-guess glue (all cells must share one subdivision n; device data):
+ * features X[cell][tet] = (ln E - mu) / sigma (FP32, ncells*6n^3);
+ * targets T[cell][(v*3+a)*24+q] = basis - hat interpolant (FP32, interior);
+ * guess: x = hat interpolant + G (G NULL: the hat interpolant alone). */
+int mfs_batch_features(mfs_batch B, double mu, double sigma, mfs_vec X);
+int mfs_batch_targets(mfs_batch B, mfs_vec T);
+int mfs_batch_guess_from(mfs_batch B, mfs_vec G);
 /* Candidate launch shapes tried by the autotuner and the time of each. */
 int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* desc, int* best_wg, double* best_ms);
 
@@ -309,6 +316,11 @@ int mfs_mlp_loss_grad(mfs_mlp M, int batch, const float* X, const float* T, doub
  * mean loss. */
 int mfs_mlp_train(mfs_mlp M, int64_t n_samples, const float* X, const float* T, int batch, int epochs,
                   double* final_loss);
+/* Device-resident variants: X [n][dims0], T [n][dimsL], Y [n][dimsL] FP32
+ * vectors; minibatches of `batch`, shuffled with `seed`. */
+int mfs_mlp_train_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_vec T, int batch, int epochs, uint64_t seed,
+                      double* final_loss);
+int mfs_mlp_forward_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_vec Y, int batch);
 /* Average device time (ms) and achieved TFLOP/s of the forward+backward GEMMs
  * for one minibatch of `batch`. */
 int mfs_mlp_time_step(mfs_mlp M, int batch, int reps, double* avg_ms, double* tflops);

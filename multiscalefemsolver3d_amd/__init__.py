@@ -164,6 +164,11 @@ _sig("mfs_mlp_forward", C.c_int, [vp, C.c_int, c_fp, c_fp])
 _sig("mfs_mlp_loss_grad", C.c_int, [vp, C.c_int, c_fp, c_fp, c_dp, C.POINTER(c_fp), C.POINTER(c_fp)])
 _sig("mfs_mlp_train", C.c_int, [vp, C.c_int64, c_fp, c_fp, C.c_int, C.c_int, c_dp])
 _sig("mfs_mlp_time_step", C.c_int, [vp, C.c_int, C.c_int, c_dp, c_dp])
+_sig("mfs_mlp_train_vec", C.c_int, [vp, C.c_int64, vp, vp, C.c_int, C.c_int, C.c_uint64, c_dp])
+_sig("mfs_mlp_forward_vec", C.c_int, [vp, C.c_int64, vp, vp, C.c_int])
+_sig("mfs_batch_features", C.c_int, [vp, C.c_double, C.c_double, vp])
+_sig("mfs_batch_targets", C.c_int, [vp, vp])
+_sig("mfs_batch_guess_from", C.c_int, [vp, vp])
 
 
 def ptr(a, ctype):
@@ -523,3 +528,200 @@ def exported_symbols():
     hdr = os.path.join(os.path.dirname(_HERE), "include", "mfs.h")
     text = open(hdr).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\*?\s+\*?(mfs_[a-z0-9_]+)\(", text, re.M)))
+
+
+# ---------------------------------------------------------------------------
+# MsFEM batched local problems
+# ---------------------------------------------------------------------------
+@dataclass
+class BatchResult:
+    seconds: float
+    max_iterations: int
+    mean_iterations: float
+    max_true_relres: float
+    n_not_converged: int
+    n_fallback: int
+    status: int
+
+
+def batch_solve_desc(precision=PREC_MIXED, tol=1e-8, maxit=20000, inner_tol=1e-4, use_graph=True, wg_size=0,
+                     use_guess=False):
+    d = BatchSolveDesc()
+    d.precision, d.tol, d.maxit, d.inner_tol = precision, tol, maxit, inner_tol
+    d.use_graph, d.wg_size, d.use_guess = int(use_graph), wg_size, int(use_guess)
+    return d
+
+
+class Batch:
+    """All local basis problems of a set of coarse cells (mfs_batch_*)."""
+
+    def __init__(self, ctx, cell_n, cell_origin, cell_h, E, nu=0.3, convention=ELEM_REFERENCE):
+        self.ctx = ctx
+        self.cell_n = i32(cell_n)
+        self.cell_origin = f64(cell_origin).reshape(-1, 3)
+        E = f64(E)
+        assert E.size == int((6 * self.cell_n.astype(np.int64) ** 3).sum())
+        d = BatchDesc()
+        d.n_cells = len(self.cell_n)
+        d.cell_n = ptr(self.cell_n, C.c_int32)
+        d.cell_origin = ptr(self.cell_origin, C.c_double)
+        d.cell_h, d.E, d.nu, d.convention = cell_h, ptr(E, C.c_double), nu, convention
+        h = vp()
+        _chk(lib.mfs_batch_create(ctx.h, C.byref(d), C.byref(h)), "mfs_batch_create")
+        self.h = h
+        self.ncells = len(self.cell_n)
+
+    def assemble(self):
+        _chk(lib.mfs_batch_assemble(self.h), "mfs_batch_assemble")
+
+    def solve(self, check=True, **kw):
+        d = batch_solve_desc(**kw)
+        rep = BatchReport()
+        code = lib.mfs_batch_solve(self.h, C.byref(d), C.byref(rep))
+        if check:
+            _chk(code, "mfs_batch_solve")
+        return BatchResult(rep.seconds, rep.max_iterations, rep.mean_iterations, rep.max_true_relres,
+                           rep.n_not_converged, rep.n_fallback, code)
+
+    def autotune(self, **kw):
+        d = batch_solve_desc(**kw)
+        best, ms = C.c_int(), C.c_double()
+        _chk(lib.mfs_batch_autotune(self.h, C.byref(d), C.byref(best), C.byref(ms)), "mfs_batch_autotune")
+        return best.value // 100, best.value % 100, ms.value
+
+    def n_interior(self, cell):
+        ni, nt = C.c_int64(), C.c_int64()
+        _chk(lib.mfs_batch_cell_dofs(self.h, cell, C.byref(ni), C.byref(nt)), "cell_dofs")
+        return ni.value, nt.value
+
+    def set_guess(self, guess):
+        g = f64(guess).ravel()
+        _chk(lib.mfs_batch_set_guess(self.h, ptr(g, C.c_double)), "mfs_batch_set_guess")
+
+    def basis(self, cell):
+        _, nt = self.n_interior(cell)
+        out = np.empty((24, nt))
+        _chk(lib.mfs_batch_get_basis(self.h, cell, ptr(out, C.c_double)), "mfs_batch_get_basis")
+        return out
+
+    def coarse_matrices(self):
+        out = np.empty((self.ncells, 24, 24))
+        _chk(lib.mfs_batch_coarse_matrices(self.h, ptr(out, C.c_double)), "mfs_batch_coarse_matrices")
+        return out
+
+    def features(self, X, mu, sigma):
+        _chk(lib.mfs_batch_features(self.h, mu, sigma, X.h), "mfs_batch_features")
+
+    def targets(self, T):
+        _chk(lib.mfs_batch_targets(self.h, T.h), "mfs_batch_targets")
+
+    def guess_from(self, G=None):
+        _chk(lib.mfs_batch_guess_from(self.h, G.h if G is not None else None), "mfs_batch_guess_from")
+
+    def iterations(self):
+        out = np.empty((self.ncells, 24), dtype=np.int32)
+        _chk(lib.mfs_batch_iterations(self.h, ptr(out, C.c_int32)), "mfs_batch_iterations")
+        return out
+
+    def close(self):
+        if self.h:
+            lib.mfs_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Learned initial guess network
+# ---------------------------------------------------------------------------
+class MLP:
+    """ReLU MLP on BF16 MFMA GEMMs with FP32 accumulation (mfs_mlp_*)."""
+
+    def __init__(self, ctx, dims, seed=1, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.ctx = ctx
+        self.dims = [int(d) for d in dims]
+        dd = np.array(self.dims, dtype=np.int32)
+        d = MlpDesc()
+        d.n_layers, d.dims, d.seed, d.lr = len(dims) - 1, ptr(dd, C.c_int32), seed, lr
+        d.beta1, d.beta2, d.eps = beta1, beta2, eps
+        h = vp()
+        _chk(lib.mfs_mlp_create(ctx.h, C.byref(d), C.byref(h)), "mfs_mlp_create")
+        self.h = h
+        self.L = len(dims) - 1
+
+    def get_params(self):
+        out = []
+        for l in range(self.L):
+            W = np.empty((self.dims[l + 1], self.dims[l]), np.float32)
+            b = np.empty(self.dims[l + 1], np.float32)
+            _chk(lib.mfs_mlp_get_params(self.h, l, ptr(W, C.c_float), ptr(b, C.c_float)), "get_params")
+            out.append((W, b))
+        return out
+
+    def set_params(self, params):
+        for l, (W, b) in enumerate(params):
+            W = np.ascontiguousarray(W, np.float32)
+            b = np.ascontiguousarray(b, np.float32)
+            _chk(lib.mfs_mlp_set_params(self.h, l, ptr(W, C.c_float), ptr(b, C.c_float)), "set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.empty((X.shape[0], self.dims[-1]), np.float32)
+        _chk(lib.mfs_mlp_forward(self.h, X.shape[0], ptr(X, C.c_float), ptr(Y, C.c_float)), "mfs_mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        gW = [np.empty((self.dims[l + 1], self.dims[l]), np.float32) for l in range(self.L)]
+        gb = [np.empty(self.dims[l + 1], np.float32) for l in range(self.L)]
+        gWp = (c_fp * self.L)(*[ptr(g, C.c_float) for g in gW])
+        gbp = (c_fp * self.L)(*[ptr(g, C.c_float) for g in gb])
+        loss = C.c_double()
+        _chk(lib.mfs_mlp_loss_grad(self.h, X.shape[0], ptr(X, C.c_float), ptr(T, C.c_float), C.byref(loss), gWp,
+                                   gbp), "mfs_mlp_loss_grad")
+        return loss.value, gW, gb
+
+    def train(self, X, T, batch, epochs):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = C.c_double()
+        _chk(lib.mfs_mlp_train(self.h, X.shape[0], ptr(X, C.c_float), ptr(T, C.c_float), batch, epochs,
+                               C.byref(loss)), "mfs_mlp_train")
+        return loss.value
+
+    def train_vec(self, n, X, T, batch, epochs, seed=7):
+        loss = C.c_double()
+        _chk(lib.mfs_mlp_train_vec(self.h, n, X.h, T.h, batch, epochs, seed, C.byref(loss)), "mfs_mlp_train_vec")
+        return loss.value
+
+    def forward_vec(self, n, X, Y, batch=256):
+        _chk(lib.mfs_mlp_forward_vec(self.h, n, X.h, Y.h, batch), "mfs_mlp_forward_vec")
+
+    def time_step(self, batch, reps=20):
+        ms, tf = C.c_double(), C.c_double()
+        _chk(lib.mfs_mlp_time_step(self.h, batch, reps, C.byref(ms), C.byref(tf)), "mfs_mlp_time_step")
+        return ms.value, tf.value
+
+    def close(self):
+        if self.h:
+            lib.mfs_mlp_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bf16_round(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(a.shape)

@@ -229,7 +229,9 @@ extern "C" int mfs_assemble_destroy(mfs_asm S) {
   dfree(S->d_fixed);
   dfree(S->d_g);
   dfree(S->d_fbase);
+  mfs_ctx c = S->ctx;
   delete S;
+  ctx_release(c);
   return MFS_OK;
 }
 
@@ -253,6 +255,7 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   MFS_HIP(hipSetDevice(ctx->device));
   mfs_asm S = new mfs_asm_s();
   S->ctx = ctx;
+  ctx_retain(ctx);
   S->problem = d->problem;
   S->convention = d->convention;
   S->bs = d->problem == MFS_PROBLEM_ELASTIC ? 3 : 1;

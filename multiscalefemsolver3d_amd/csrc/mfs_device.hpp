@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -35,7 +36,11 @@
     if (_s != MFS_OK) return _s; \
   } while (0)
 
+// Contexts and matrices are reference counted: every object created on a
+// context holds a reference to it, and a solver holds its matrix, so the
+// destroy calls may come in any order (as they do from garbage collectors).
 struct mfs_ctx_s {
+  std::atomic<int> refs{1};
   int device = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
@@ -55,6 +60,7 @@ struct mfs_vec_s {
 // Sparse matrix: pattern in CSSD order (host copy for export), device values
 // in a SELL-64 layout (thread per block row, column-major inside a slice).
 struct mfs_mat_s {
+  std::atomic<int> refs{1};
   mfs_ctx ctx = nullptr;
   int bs = 1;            // 1 = CSR, 3 = BSR3
   int64_t nb = 0;        // block rows
@@ -78,6 +84,12 @@ struct mfs_mat_s {
 };
 
 namespace mfs {
+
+// reference counting (see mfs_ctx_s)
+void ctx_retain(mfs_ctx c);
+void ctx_release(mfs_ctx c);
+void mat_retain(mfs_mat A);
+void mat_release(mfs_mat A);
 
 // allocate + track
 int dmalloc(void** p, size_t bytes);

@@ -1,0 +1,548 @@
+// Learned initial guess: a ReLU MLP trained and evaluated with hand-written
+// BF16 MFMA GEMMs (FP32 accumulation) on gfx950.  No reference counterpart:
+// the network predicts each MsFEM cell's local bases from its modulus field;
+// the zero-guess solve stays the fallback (the batch solver rejects a guess
+// whose residual exceeds the zero guess's).
+//
+// GEMM: C[M][N] = sum_k A(m,k) B(k,n), A/B bf16 with arbitrary strides,
+// 64x64 output tile per 256-thread workgroup, 4 waves of 32x32, each a 2x2
+// grid of v_mfma_f32_16x16x32_bf16, K staged through LDS in steps of 32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mfs_device.hpp"
+
+using namespace mfs;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// FP32 -> BF16 bits, round to nearest even (NaN kept quiet).
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+  uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_RELU = 2 };
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int64_t sam,
+                                                   int64_t sak, const uint16_t* __restrict__ B, int64_t sbk,
+                                                   int64_t sbn, float* __restrict__ C, int64_t ldc,
+                                                   const float* __restrict__ bias, uint16_t* __restrict__ Cbf) {
+  __shared__ __align__(16) uint16_t As[64][40];
+  __shared__ __align__(16) uint16_t Bs[64][40];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+  const bool a_kfast = sak == 1, b_kfast = sbk == 1;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int idx = tid + 256 * i;
+      // map consecutive threads along the operand's contiguous dimension
+      const int mm = a_kfast ? idx / 32 : idx % 64, kk = a_kfast ? idx % 32 : idx / 64;
+      const int gm = m0 + mm, gk = k0 + kk;
+      As[mm][kk] = (gm < M && gk < K) ? A[gm * sam + gk * sak] : (uint16_t)0;
+      const int nn = b_kfast ? idx / 32 : idx % 64, kb = b_kfast ? idx % 32 : idx / 64;
+      const int gn = n0 + nn, gkb = k0 + kb;
+      Bs[nn][kb] = (gn < N && gkb < K) ? B[gkb * sbk + gn * sbn] : (uint16_t)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + bi * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+      for (int bj = 0; bj < 2; bj++) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + bj * 16 + (lane & 15)][8 * (lane >> 4)]);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[bi][bj], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = m0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + i;
+        const int col = n0 + wn * 32 + bj * 16 + (lane & 15);
+        if (row >= M || col >= N) continue;
+        float v = acc[bi][bj][i];
+        if (EPI != EPI_NONE) v += bias[col];
+        if (EPI == EPI_BIAS_RELU) v = v > 0.f ? v : 0.f;
+        C[(int64_t)row * ldc + col] = v;
+        if (Cbf) Cbf[(int64_t)row * ldc + col] = f2bf(v);
+      }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = f2bf(x[i]);
+}
+// gather rows idx[0..B) of X [*, D] into dst bf16 [B][D]
+__global__ void k_gather_rows_bf16(const float* __restrict__ X, const int* __restrict__ idx, int B, int64_t D,
+                                   uint16_t* __restrict__ dst) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < (int64_t)B * D;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = q / D, c = q % D;
+    dst[q] = f2bf(X[(int64_t)idx[r] * D + c]);
+  }
+}
+// dY = (Y - T[idx]) * scale (FP32 and bf16), partial loss sums
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const float* __restrict__ T,
+                                             const int* __restrict__ idx, int B, int64_t D, float scale,
+                                             float* __restrict__ dY, uint16_t* __restrict__ dYbf,
+                                             double* __restrict__ loss_partial) {
+  double s = 0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < (int64_t)B * D;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = q / D, c = q % D;
+    const float t = idx ? T[(int64_t)idx[r] * D + c] : T[q];
+    const float e = Y[q] - t;
+    s += (double)e * e;
+    const float g = e * scale;
+    dY[q] = g;
+    dYbf[q] = f2bf(g);
+  }
+  s = wave_sum(s);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) loss_partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+// column sums: db[c] = sum_r dY[r][c]
+__global__ void k_colsum(const float* __restrict__ dY, int B, int D, float* __restrict__ db) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < D; c += gridDim.x * blockDim.x) {
+    float s = 0;
+    for (int r = 0; r < B; r++) s += dY[(int64_t)r * D + c];
+    db[c] = s;
+  }
+}
+// dX *= (act > 0), refresh bf16 copy
+__global__ void k_relu_back(float* __restrict__ dX, const float* __restrict__ act, uint16_t* __restrict__ dXbf,
+                            int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = act[i] > 0.f ? dX[i] : 0.f;
+    dX[i] = v;
+    dXbf[i] = f2bf(v);
+  }
+}
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
+                       int64_t n, float lr, float b1, float b2, float eps, float c1, float c2,
+                       uint16_t* __restrict__ wbf) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wbf) wbf[i] = f2bf(wi);
+  }
+}
+
+inline unsigned g1d(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(65536, (n + 255) / 256)); }
+
+}  // namespace
+
+struct mfs_mlp_s {
+  mfs_ctx ctx = nullptr;
+  int L = 0;
+  std::vector<int> dims;
+  double lr = 1e-3, b1 = 0.9, b2 = 0.999, eps = 1e-8;
+  int64_t step = 0;
+  std::vector<float*> W, b, mW, vW, mb, vb, gW, gb;
+  std::vector<uint16_t*> Wbf;
+  // activations for the current batch capacity
+  int cap = 0;
+  std::vector<float*> act;       // FP32 [cap][dims[l]] (l = 1..L)
+  std::vector<uint16_t*> actbf;  // bf16 [cap][dims[l]] (l = 0..L-1)
+  std::vector<float*> dact;      // FP32 grads [cap][dims[l]] (l = 1..L)
+  std::vector<uint16_t*> dactbf;
+  int* d_idx = nullptr;
+  double* d_loss = nullptr;
+};
+
+static int mlp_free(mfs_mlp M) {
+  if (!M) return MFS_OK;
+  if (M->ctx) (void)hipStreamSynchronize(M->ctx->stream);
+  for (auto* v : {&M->W, &M->b, &M->mW, &M->vW, &M->mb, &M->vb, &M->gW, &M->gb, &M->act, &M->dact})
+    for (float* p : *v) dfree(p);
+  for (auto* v : {&M->Wbf, &M->actbf, &M->dactbf})
+    for (uint16_t* p : *v) dfree(p);
+  dfree(M->d_idx);
+  dfree(M->d_loss);
+  mfs_ctx c = M->ctx;
+  delete M;
+  ctx_release(c);
+  return MFS_OK;
+}
+
+extern "C" int mfs_mlp_destroy(mfs_mlp M) { return mlp_free(M); }
+
+static int mlp_capacity(mfs_mlp M, int batch) {
+  if (batch <= M->cap) return MFS_OK;
+  for (auto* v : {&M->act, &M->dact})
+    for (float*& p : *v) {
+      dfree(p);
+      p = nullptr;
+    }
+  for (auto* v : {&M->actbf, &M->dactbf})
+    for (uint16_t*& p : *v) {
+      dfree(p);
+      p = nullptr;
+    }
+  dfree(M->d_idx);
+  dfree(M->d_loss);
+  const int L = M->L;
+  M->act.assign(L + 1, nullptr);
+  M->dact.assign(L + 1, nullptr);
+  M->actbf.assign(L + 1, nullptr);
+  M->dactbf.assign(L + 1, nullptr);
+  for (int l = 0; l <= L; l++) {
+    const size_t e = (size_t)batch * M->dims[l];
+    MFS_TRY(dmalloc((void**)&M->act[l], 4 * e));
+    MFS_TRY(dmalloc((void**)&M->dact[l], 4 * e));
+    MFS_TRY(dmalloc((void**)&M->actbf[l], 2 * e));
+    MFS_TRY(dmalloc((void**)&M->dactbf[l], 2 * e));
+  }
+  MFS_TRY(dmalloc((void**)&M->d_idx, sizeof(int) * batch));
+  MFS_TRY(dmalloc((void**)&M->d_loss, sizeof(double) * 65536));
+  M->cap = batch;
+  return MFS_OK;
+}
+
+extern "C" int mfs_mlp_create(mfs_ctx ctx, const mfs_mlp_desc* d, mfs_mlp* out) {
+  MFS_CHECK_ARG(ctx && d && out && d->n_layers >= 1 && d->n_layers <= 16 && d->dims, "mfs_mlp_create: bad argument");
+  for (int l = 0; l <= d->n_layers; l++) MFS_CHECK_ARG(d->dims[l] > 0, "mfs_mlp_create: layer size must be > 0");
+  MFS_HIP(hipSetDevice(ctx->device));
+  *out = nullptr;
+  mfs_mlp M = new mfs_mlp_s();
+  M->ctx = ctx;
+  ctx_retain(ctx);
+  M->L = d->n_layers;
+  M->dims.assign(d->dims, d->dims + d->n_layers + 1);
+  M->lr = d->lr > 0 ? d->lr : 1e-3;
+  M->b1 = d->beta1 > 0 ? d->beta1 : 0.9;
+  M->b2 = d->beta2 > 0 ? d->beta2 : 0.999;
+  M->eps = d->eps > 0 ? d->eps : 1e-8;
+  const int L = M->L;
+  for (auto* v : {&M->W, &M->b, &M->mW, &M->vW, &M->mb, &M->vb, &M->gW, &M->gb}) v->assign(L, nullptr);
+  M->Wbf.assign(L, nullptr);
+  hipStream_t s = ctx->stream;
+  // He-uniform init from a counter-based generator; last layer scaled by 0.1
+  for (int l = 0; l < L; l++) {
+    const int64_t nin = M->dims[l], nout = M->dims[l + 1], nw = nin * nout;
+    int st = MFS_OK;
+    for (float** p : {&M->W[l], &M->mW[l], &M->vW[l], &M->gW[l]})
+      if (st == MFS_OK) st = dmalloc((void**)p, 4 * nw);
+    for (float** p : {&M->b[l], &M->mb[l], &M->vb[l], &M->gb[l]})
+      if (st == MFS_OK) st = dmalloc((void**)p, 4 * nout);
+    if (st == MFS_OK) st = dmalloc((void**)&M->Wbf[l], 2 * nw);
+    if (st != MFS_OK) {
+      mlp_free(M);
+      return st;
+    }
+    std::vector<float> w(nw);
+    std::vector<uint16_t> wb(nw);
+    const double lim = std::sqrt(6.0 / nin) * (l == L - 1 ? 0.1 : 1.0);
+    for (int64_t i = 0; i < nw; i++) {
+      uint64_t z = d->seed * 0x9E3779B97F4A7C15ull + (uint64_t)l * 0xD1B54A32D192ED03ull + (uint64_t)i;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+      w[i] = (float)((2 * u - 1) * lim);
+      wb[i] = f2bf(w[i]);
+    }
+    MFS_HIP(hipMemcpyAsync(M->W[l], w.data(), 4 * nw, hipMemcpyHostToDevice, s));
+    MFS_HIP(hipMemcpyAsync(M->Wbf[l], wb.data(), 2 * nw, hipMemcpyHostToDevice, s));
+    MFS_HIP(hipMemsetAsync(M->b[l], 0, 4 * nout, s));
+    MFS_HIP(hipMemsetAsync(M->mW[l], 0, 4 * nw, s));
+    MFS_HIP(hipMemsetAsync(M->vW[l], 0, 4 * nw, s));
+    MFS_HIP(hipMemsetAsync(M->mb[l], 0, 4 * nout, s));
+    MFS_HIP(hipMemsetAsync(M->vb[l], 0, 4 * nout, s));
+    MFS_HIP(hipStreamSynchronize(s));
+  }
+  *out = M;
+  return MFS_OK;
+}
+
+extern "C" int mfs_mlp_get_params(mfs_mlp M, int l, float* W, float* b) {
+  MFS_CHECK_ARG(M && l >= 0 && l < M->L, "mfs_mlp_get_params: bad argument");
+  const int64_t nw = (int64_t)M->dims[l] * M->dims[l + 1];
+  hipStream_t s = M->ctx->stream;
+  if (W) MFS_HIP(hipMemcpyAsync(W, M->W[l], 4 * nw, hipMemcpyDeviceToHost, s));
+  if (b) MFS_HIP(hipMemcpyAsync(b, M->b[l], 4 * M->dims[l + 1], hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  return MFS_OK;
+}
+
+extern "C" int mfs_mlp_set_params(mfs_mlp M, int l, const float* W, const float* b) {
+  MFS_CHECK_ARG(M && l >= 0 && l < M->L, "mfs_mlp_set_params: bad argument");
+  const int64_t nw = (int64_t)M->dims[l] * M->dims[l + 1];
+  hipStream_t s = M->ctx->stream;
+  if (W) {
+    MFS_HIP(hipMemcpyAsync(M->W[l], W, 4 * nw, hipMemcpyHostToDevice, s));
+    k_to_bf16<<<g1d(nw), 256, 0, s>>>(M->W[l], M->Wbf[l], nw);
+  }
+  if (b) MFS_HIP(hipMemcpyAsync(M->b[l], b, 4 * M->dims[l + 1], hipMemcpyHostToDevice, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  return MFS_OK;
+}
+
+static void gemm(hipStream_t s, int epi, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak,
+                 const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, const float* bias, uint16_t* Cbf) {
+  dim3 grid((N + 63) / 64, (M + 63) / 64);
+  if (epi == EPI_NONE)
+    k_gemm_bf16<EPI_NONE><<<grid, 256, 0, s>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
+  else if (epi == EPI_BIAS)
+    k_gemm_bf16<EPI_BIAS><<<grid, 256, 0, s>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
+  else
+    k_gemm_bf16<EPI_BIAS_RELU><<<grid, 256, 0, s>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
+}
+
+// forward on actbf[0] (already filled, B rows): act[L] holds the output
+static void mlp_forward_dev(mfs_mlp M, int B) {
+  hipStream_t s = M->ctx->stream;
+  for (int l = 0; l < M->L; l++) {
+    const int nin = M->dims[l], nout = M->dims[l + 1];
+    const bool last = l == M->L - 1;
+    // Y[B][out] = X[B][in] W[out][in]^T + b
+    gemm(s, last ? EPI_BIAS : EPI_BIAS_RELU, B, nout, nin, M->actbf[l], nin, 1, M->Wbf[l], 1, nin, M->act[l + 1], nout,
+         M->b[l], last ? nullptr : M->actbf[l + 1]);
+  }
+}
+
+// backward from dact[L]/dactbf[L] (filled): gradients into gW/gb
+static void mlp_backward_dev(mfs_mlp M, int B) {
+  hipStream_t s = M->ctx->stream;
+  for (int l = M->L - 1; l >= 0; l--) {
+    const int nin = M->dims[l], nout = M->dims[l + 1];
+    // gW[out][in] = sum_m dY[m][out] X[m][in]
+    gemm(s, EPI_NONE, nout, nin, B, M->dactbf[l + 1], 1, nout, M->actbf[l], nin, 1, M->gW[l], nin, nullptr, nullptr);
+    k_colsum<<<g1d(nout), 256, 0, s>>>(M->dact[l + 1], B, nout, M->gb[l]);
+    if (l > 0) {
+      // dX[m][in] = sum_o dY[m][o] W[o][in], then ReLU mask
+      gemm(s, EPI_NONE, B, nin, nout, M->dactbf[l + 1], nout, 1, M->Wbf[l], nin, 1, M->dact[l], nin, nullptr, nullptr);
+      k_relu_back<<<g1d((int64_t)B * nin), 256, 0, s>>>(M->dact[l], M->act[l], M->dactbf[l], (int64_t)B * nin);
+    }
+  }
+}
+
+static void mlp_adam(mfs_mlp M) {
+  hipStream_t s = M->ctx->stream;
+  M->step++;
+  const float c1 = (float)(1.0 - std::pow(M->b1, (double)M->step));
+  const float c2 = (float)(1.0 - std::pow(M->b2, (double)M->step));
+  for (int l = 0; l < M->L; l++) {
+    const int64_t nw = (int64_t)M->dims[l] * M->dims[l + 1];
+    k_adam<<<g1d(nw), 256, 0, s>>>(M->W[l], M->mW[l], M->vW[l], M->gW[l], nw, (float)M->lr, (float)M->b1,
+                                   (float)M->b2, (float)M->eps, c1, c2, M->Wbf[l]);
+    k_adam<<<g1d(M->dims[l + 1]), 256, 0, s>>>(M->b[l], M->mb[l], M->vb[l], M->gb[l], M->dims[l + 1], (float)M->lr,
+                                                (float)M->b1, (float)M->b2, (float)M->eps, c1, c2, nullptr);
+  }
+}
+
+extern "C" int mfs_mlp_forward(mfs_mlp M, int batch, const float* X, float* Y) {
+  MFS_CHECK_ARG(M && batch > 0 && X && Y, "mfs_mlp_forward: bad argument");
+  MFS_TRY(mlp_capacity(M, batch));
+  hipStream_t s = M->ctx->stream;
+  const int64_t n0 = (int64_t)batch * M->dims[0];
+  MFS_HIP(hipMemcpyAsync(M->act[0], X, 4 * n0, hipMemcpyHostToDevice, s));
+  k_to_bf16<<<g1d(n0), 256, 0, s>>>(M->act[0], M->actbf[0], n0);
+  mlp_forward_dev(M, batch);
+  MFS_HIP(hipGetLastError());
+  MFS_HIP(hipMemcpyAsync(Y, M->act[M->L], 4 * (int64_t)batch * M->dims[M->L], hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  return MFS_OK;
+}
+
+static int mlp_loss_dev(mfs_mlp M, int B, const float* T_dev, const int* idx_dev, double* loss) {
+  hipStream_t s = M->ctx->stream;
+  const int64_t D = M->dims[M->L];
+  const int64_t n = (int64_t)B * D;
+  const unsigned g = std::min<unsigned>(g1d(n), 65536);
+  k_mse<<<g, 256, 0, s>>>(M->act[M->L], T_dev, idx_dev, B, D, (float)(1.0 / (double)n), M->dact[M->L],
+                          M->dactbf[M->L], M->d_loss);
+  MFS_HIP(hipGetLastError());
+  if (loss) {
+    std::vector<double> lp(g);
+    MFS_HIP(hipMemcpyAsync(lp.data(), M->d_loss, sizeof(double) * g, hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipStreamSynchronize(s));
+    double t = 0;
+    for (double v : lp) t += v;
+    *loss = 0.5 * t / (double)n;
+  }
+  return MFS_OK;
+}
+
+extern "C" int mfs_mlp_loss_grad(mfs_mlp M, int batch, const float* X, const float* T, double* loss, float* const* gW,
+                                 float* const* gb) {
+  MFS_CHECK_ARG(M && batch > 0 && X && T && loss, "mfs_mlp_loss_grad: bad argument");
+  MFS_TRY(mlp_capacity(M, batch));
+  hipStream_t s = M->ctx->stream;
+  const int64_t n0 = (int64_t)batch * M->dims[0], nL = (int64_t)batch * M->dims[M->L];
+  float* Td;
+  MFS_TRY(dmalloc((void**)&Td, 4 * nL));
+  MFS_HIP(hipMemcpyAsync(M->act[0], X, 4 * n0, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(Td, T, 4 * nL, hipMemcpyHostToDevice, s));
+  k_to_bf16<<<g1d(n0), 256, 0, s>>>(M->act[0], M->actbf[0], n0);
+  mlp_forward_dev(M, batch);
+  MFS_TRY(mlp_loss_dev(M, batch, Td, nullptr, loss));
+  mlp_backward_dev(M, batch);
+  MFS_HIP(hipGetLastError());
+  for (int l = 0; l < M->L; l++) {
+    if (gW && gW[l])
+      MFS_HIP(hipMemcpyAsync(gW[l], M->gW[l], 4 * (int64_t)M->dims[l] * M->dims[l + 1], hipMemcpyDeviceToHost, s));
+    if (gb && gb[l]) MFS_HIP(hipMemcpyAsync(gb[l], M->gb[l], 4 * M->dims[l + 1], hipMemcpyDeviceToHost, s));
+  }
+  MFS_HIP(hipStreamSynchronize(s));
+  dfree(Td);
+  return MFS_OK;
+}
+
+// Train on device-resident features X [n][dims0] and targets T [n][dimsL].
+static int mlp_train_dev(mfs_mlp M, int64_t n, const float* X, const float* T, int batch, int epochs, uint64_t seed,
+                         double* final_loss) {
+  MFS_TRY(mlp_capacity(M, batch));
+  hipStream_t s = M->ctx->stream;
+  std::vector<int> perm(n);
+  double last = 0;
+  for (int ep = 0; ep < epochs; ep++) {
+    for (int64_t i = 0; i < n; i++) perm[i] = (int)i;
+    // Fisher-Yates with a splitmix64 stream
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (uint64_t)ep;
+    for (int64_t i = n - 1; i > 0; i--) {
+      z += 0x9E3779B97F4A7C15ull;
+      uint64_t y = z;
+      y = (y ^ (y >> 30)) * 0xBF58476D1CE4E5B9ull;
+      y = (y ^ (y >> 27)) * 0x94D049BB133111EBull;
+      y ^= y >> 31;
+      std::swap(perm[i], perm[y % (uint64_t)(i + 1)]);
+    }
+    double ep_loss = 0;
+    int nb = 0;
+    for (int64_t b0 = 0; b0 + batch <= n; b0 += batch) {
+      MFS_HIP(hipMemcpyAsync(M->d_idx, perm.data() + b0, sizeof(int) * batch, hipMemcpyHostToDevice, s));
+      k_gather_rows_bf16<<<g1d((int64_t)batch * M->dims[0]), 256, 0, s>>>(X, M->d_idx, batch, M->dims[0],
+                                                                          M->actbf[0]);
+      mlp_forward_dev(M, batch);
+      const bool want = (b0 + 2 * batch > n);  // read the loss of the last batch of the epoch
+      double l = 0;
+      MFS_TRY(mlp_loss_dev(M, batch, T, M->d_idx, want ? &l : nullptr));
+      mlp_backward_dev(M, batch);
+      mlp_adam(M);
+      MFS_HIP(hipGetLastError());
+      if (want) {
+        ep_loss += l;
+        nb++;
+      }
+      // the index upload reads perm on the host: keep the host copy stable
+      MFS_HIP(hipStreamSynchronize(s));
+    }
+    last = nb ? ep_loss / nb : 0;
+  }
+  if (final_loss) *final_loss = last;
+  return MFS_OK;
+}
+
+extern "C" int mfs_mlp_train(mfs_mlp M, int64_t n_samples, const float* X, const float* T, int batch, int epochs,
+                             double* final_loss) {
+  MFS_CHECK_ARG(M && X && T && batch > 0 && n_samples >= batch && epochs >= 0, "mfs_mlp_train: bad argument");
+  float *Xd, *Td;
+  const int64_t nx = n_samples * M->dims[0], nt = n_samples * M->dims[M->L];
+  MFS_TRY(dmalloc((void**)&Xd, 4 * nx));
+  MFS_TRY(dmalloc((void**)&Td, 4 * nt));
+  hipStream_t s = M->ctx->stream;
+  MFS_HIP(hipMemcpyAsync(Xd, X, 4 * nx, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(Td, T, 4 * nt, hipMemcpyHostToDevice, s));
+  int st = mlp_train_dev(M, n_samples, Xd, Td, batch, epochs, 12345, final_loss);
+  MFS_HIP(hipStreamSynchronize(s));
+  dfree(Xd);
+  dfree(Td);
+  return st;
+}
+
+extern "C" int mfs_mlp_train_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_vec T, int batch, int epochs,
+                                 uint64_t seed, double* final_loss) {
+  MFS_CHECK_ARG(M && X && T && batch > 0 && n_samples >= batch && epochs >= 0, "mfs_mlp_train_vec: bad argument");
+  MFS_CHECK_ARG(X->prec == MFS_F32 && T->prec == MFS_F32, "mfs_mlp_train_vec: FP32 vectors expected");
+  MFS_CHECK_ARG(X->n >= n_samples * M->dims[0] && T->n >= n_samples * M->dims[M->L],
+                "mfs_mlp_train_vec: vectors too short");
+  return mlp_train_dev(M, n_samples, (const float*)X->d, (const float*)T->d, batch, epochs, seed, final_loss);
+}
+
+extern "C" int mfs_mlp_forward_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_vec Y, int batch) {
+  MFS_CHECK_ARG(M && X && Y && n_samples > 0 && batch > 0, "mfs_mlp_forward_vec: bad argument");
+  MFS_CHECK_ARG(X->prec == MFS_F32 && Y->prec == MFS_F32, "mfs_mlp_forward_vec: FP32 vectors expected");
+  MFS_CHECK_ARG(X->n >= n_samples * M->dims[0] && Y->n >= n_samples * M->dims[M->L],
+                "mfs_mlp_forward_vec: vectors too short");
+  MFS_TRY(mlp_capacity(M, batch));
+  hipStream_t s = M->ctx->stream;
+  const int64_t D0 = M->dims[0], DL = M->dims[M->L];
+  for (int64_t b0 = 0; b0 < n_samples; b0 += batch) {
+    const int B = (int)std::min<int64_t>(batch, n_samples - b0);
+    k_to_bf16<<<g1d((int64_t)B * D0), 256, 0, s>>>((const float*)X->d + b0 * D0, M->actbf[0], (int64_t)B * D0);
+    mlp_forward_dev(M, B);
+    MFS_HIP(hipMemcpyAsync((float*)Y->d + b0 * DL, M->act[M->L], 4 * (int64_t)B * DL, hipMemcpyDeviceToDevice, s));
+  }
+  MFS_HIP(hipGetLastError());
+  MFS_HIP(hipStreamSynchronize(s));
+  return MFS_OK;
+}
+
+extern "C" int mfs_mlp_time_step(mfs_mlp M, int batch, int reps, double* avg_ms, double* tflops) {
+  MFS_CHECK_ARG(M && batch > 0 && reps > 0 && avg_ms, "mfs_mlp_time_step: bad argument");
+  MFS_TRY(mlp_capacity(M, batch));
+  hipStream_t s = M->ctx->stream;
+  // synthetic inputs/targets already in the buffers (zeros are fine for timing)
+  MFS_HIP(hipMemsetAsync(M->actbf[0], 0, 2 * (size_t)batch * M->dims[0], s));
+  float* Td;
+  MFS_TRY(dmalloc((void**)&Td, 4 * (size_t)batch * M->dims[M->L]));
+  MFS_HIP(hipMemsetAsync(Td, 0, 4 * (size_t)batch * M->dims[M->L], s));
+  hipEvent_t e0, e1;
+  MFS_HIP(hipEventCreate(&e0));
+  MFS_HIP(hipEventCreate(&e1));
+  MFS_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; r++) {
+    mlp_forward_dev(M, batch);
+    MFS_TRY(mlp_loss_dev(M, batch, Td, nullptr, nullptr));
+    mlp_backward_dev(M, batch);
+  }
+  MFS_HIP(hipEventRecord(e1, s));
+  MFS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(Td);
+  *avg_ms = ms / reps;
+  double flops = 0;
+  for (int l = 0; l < M->L; l++) flops += 2.0 * batch * M->dims[l] * M->dims[l + 1] * (l == 0 ? 2 : 3);
+  if (tflops) *tflops = flops / (*avg_ms * 1e-3) / 1e12;
+  return MFS_OK;
+}

@@ -1,0 +1,1145 @@
+// MsFEM local basis problems, batched on gfx950.
+//
+// Each coarse cell is a box of n^3 fine hexes (6 Freudenthal tets each) with a
+// per-tet Young's modulus.  For each of its 8 coarse nodes c and 3 directions
+// d the local problem is: K u = 0 inside the cell, u = hat_c * e_d on the
+// cell boundary (hat_c = trilinear coarse hat).  This replaces the per-cell
+// volume solves of Grid_forMech_Poly_Order1::CreateFunctionalSpaces
+// (MsFEM/MsFEM_Grid.h:4795-5760 via FEM/FEM.h:2969-3795) in the configs'
+// three-direction formulation.
+//
+// Layout (device):
+//  * cell matrix, interior rows only, in the structured 15-point stencil of
+//    the Freudenthal mesh: value (s, e) of interior node v at
+//    K[cell_base*135 + (s*9 + e)*nint + v] (structure-of-arrays, no column
+//    indices).  Blocks that couple to boundary nodes are zero here; their
+//    effect is in the right-hand side b = -K_IB g.
+//  * b, x: FP64 [interior node][3][24] per cell.
+//  * one task = (cell, group of 8 bases); a persistent kernel pulls tasks
+//    from an atomic queue (largest cells first) and runs FP32 Jacobi-PCG with
+//    FP64 refinement for its 8 right-hand sides, search directions in LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "mfs_device.hpp"
+
+using namespace mfs;
+
+namespace {
+
+constexpr int NS = 15;   // stencil points
+constexpr int RG = 8;    // right-hand sides per task
+constexpr int NB = 24;   // bases per cell
+constexpr int MAXN = 12; // largest supported cell subdivision (FP32 inner; FP64 inner: 10)
+
+// Stencil offsets (di, dj, dk), sorted by linear offset (dk, dj, di lexicographic)
+__constant__ int c_stencil[NS][3];
+static const int h_stencil[NS][3] = {{-1, -1, -1}, {0, -1, -1}, {-1, 0, -1}, {0, 0, -1}, {-1, -1, 0},
+                                     {0, -1, 0},   {-1, 0, 0},  {0, 0, 0},   {1, 0, 0},   {0, 1, 0},
+                                     {1, 1, 0},    {0, 0, 1},   {1, 0, 1},   {0, 1, 1},   {1, 1, 1}};
+constexpr int S_CENTER = 7;
+
+struct Cell {
+  int n;        // fine hexes per edge
+  int nint;     // interior nodes (n-1)^3
+  int64_t tet0; // first tet in E
+  int64_t int0; // first interior node (global numbering over cells)
+  int tmpl;     // template id
+  double ox, oy, oz;
+};
+
+struct Tmpl {  // per distinct n: gather lists of interior rows x stencil
+  int n;
+  int64_t ptr0;  // offset into g_ptr (nint*NS+1 entries)
+  int64_t ent0;  // offset into g_ent
+};
+
+struct SolveArgs {
+  const Cell* cells;
+  const int* task_cell;
+  const int* task_group;
+  int ntasks;
+  unsigned* queue;
+  const float* K32;
+  const double* K64;
+  const float* dinv32;   // [int0 + v][3]
+  const double* dinv64;
+  const double* b64;     // [int0 + v][3][24]
+  double* x64;
+  float* scr32;          // per-WG scratch: 3 vectors of maxint*3*RG floats
+  double* scr64;         // per-WG scratch (FP64 inner mode)
+  int64_t scr_stride;    // entries per vector
+  int* iters;            // [cell][24]
+  double* relres;        // [cell][24]
+  int* flags;            // [cell][24]: bit0 not converged, bit1 guess rejected
+  double tol, inner_tol;
+  int maxit;
+  int use_guess;
+};
+
+__device__ __forceinline__ int interior_index(int i, int j, int k, int m) {
+  return (i - 1) + m * ((j - 1) + m * (k - 1));
+}
+
+// Per-RHS block reduction of NV*RG values laid out v[k][r]; after the call,
+// out[k][r] holds totals (all threads).  blockDim multiple of 64, <= 1024.
+template <int NV>
+__device__ __forceinline__ void block_reduce_rg(double (&v)[NV][RG], double* lds /* [16][NV*RG] */,
+                                                double (&out)[NV][RG]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+      double s = wave_sum(v[k][r]);
+      if (lane == 0) lds[wave * NV * RG + k * RG + r] = s;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+      double s = 0;
+      for (int w = 0; w < nw; w++) s += lds[w * NV * RG + k * RG + r];
+      out[k][r] = s;
+    }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Cell assembly: one thread per (cell, interior node).  For each stencil
+// block, sum the element contributions (template gather lists, ascending
+// tet order) with gradients recomputed from the structured fine geometry.
+// Interior-neighbour blocks go to K (FP64 and FP32); boundary-neighbour
+// blocks update b = -K_IB g for the 24 bases.
+// ---------------------------------------------------------------------------
+__device__ void tet_nodes(int t, int n, int (&ijk)[4][3]) {
+  const int hex = t / 6, p = t % 6;
+  int c[3] = {hex % n, (hex / n) % n, hex / (n * n)};
+  const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  ijk[0][0] = c[0];
+  ijk[0][1] = c[1];
+  ijk[0][2] = c[2];
+  for (int s = 0; s < 3; s++) {
+    c[perm[p][s]]++;
+    ijk[s + 1][0] = c[0];
+    ijk[s + 1][1] = c[1];
+    ijk[s + 1][2] = c[2];
+  }
+}
+
+__device__ void tet_grads(const int (&ijk)[4][3], double hf, double (&g)[4][3], double& V) {
+  double J[3][3];
+  for (int r = 0; r < 3; r++)
+    for (int a = 0; a < 3; a++) J[r][a] = hf * (double)(ijk[r + 1][a] - ijk[0][a]);
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+  const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+  const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double inv = 1.0 / det;
+  g[1][0] = c00 * inv; g[1][1] = c01 * inv; g[1][2] = c02 * inv;
+  g[2][0] = c10 * inv; g[2][1] = c11 * inv; g[2][2] = c12 * inv;
+  g[3][0] = c20 * inv; g[3][1] = c21 * inv; g[3][2] = c22 * inv;
+  for (int a = 0; a < 3; a++) g[0][a] = -(g[1][a] + g[2][a] + g[3][a]);
+  V = fabs(det) / 6.0;
+}
+
+// Elastic block (I=i, J=j) of a tet: textbook V(lam dI_a dJ_b + mu dI_b dJ_a +
+// mu delta dI.dJ); the reference convention is its transpose.
+__device__ __forceinline__ void elastic_block(const double* dI, const double* dJ, double vl, double vm, int conv,
+                                              double (&K)[3][3]) {
+  const double dd = dI[0] * dJ[0] + dI[1] * dJ[1] + dI[2] * dJ[2];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      double v = conv == MFS_ELEM_TEXTBOOK ? vl * dI[a] * dJ[b] + vm * dI[b] * dJ[a]
+                                           : vl * dI[b] * dJ[a] + vm * dI[a] * dJ[b];
+      if (a == b) v += vm * dd;
+      K[a][b] += v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_assemble(const Cell* __restrict__ cells, int ncells,
+                                                       const int64_t* __restrict__ node_cell_ptr,
+                                                       const Tmpl* __restrict__ tmpls,
+                                                       const int64_t* __restrict__ g_ptr,
+                                                       const uint32_t* __restrict__ g_ent,
+                                                       const double* __restrict__ E, double nu, double h, int conv,
+                                                       int64_t total_int, float* __restrict__ K32,
+                                                       double* __restrict__ K64, float* __restrict__ dinv32,
+                                                       double* __restrict__ dinv64, double* __restrict__ b64) {
+  for (int64_t gv = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gv < total_int;
+       gv += (int64_t)gridDim.x * blockDim.x) {
+    // find the cell (binary search over cumulative interior counts)
+    int lo = 0, hi = ncells - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (node_cell_ptr[mid] <= gv) lo = mid;
+      else hi = mid - 1;
+    }
+    const Cell C = cells[lo];
+    const Tmpl T = tmpls[C.tmpl];
+    const int n = C.n, m = n - 1, nint = C.nint;
+    const int v = (int)(gv - C.int0);
+    const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
+    const double hf = h / n;
+    const double lamf = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), muf = 1.0 / (2.0 * (1.0 + nu));
+    double bacc[3][NB];
+    for (int a = 0; a < 3; a++)
+      for (int q = 0; q < NB; q++) bacc[a][q] = 0.0;
+    float* K32c = K32 + C.int0 * (NS * 9);
+    double* K64c = K64 + C.int0 * (NS * 9);
+    for (int s = 0; s < NS; s++) {
+      const int ni = i + c_stencil[s][0], nj = j + c_stencil[s][1], nk = k + c_stencil[s][2];
+      double Kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      const int64_t slot = (int64_t)v * NS + s;
+      for (int64_t q = g_ptr[T.ptr0 + slot]; q < g_ptr[T.ptr0 + slot + 1]; q++) {
+        const uint32_t ce = g_ent[T.ent0 + q];
+        const int t = (int)(ce >> 4), li = (ce >> 2) & 3, lj = ce & 3;
+        int ijk[4][3];
+        tet_nodes(t, n, ijk);
+        double g[4][3], V;
+        tet_grads(ijk, hf, g, V);
+        const double Et = E[C.tet0 + t];
+        elastic_block(g[li], g[lj], V * Et * lamf, V * Et * muf, conv, Kb);
+      }
+      const bool interior = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          const int64_t at = (int64_t)(s * 9 + 3 * a + b) * nint + v;
+          K64c[at] = interior ? Kb[a][b] : 0.0;
+          K32c[at] = interior ? (float)Kb[a][b] : 0.0f;
+        }
+      if (!interior) {
+        // g for basis (c, d) at this boundary node: hat_c(x) e_d
+        const double X = (double)ni / n, Y = (double)nj / n, Z = (double)nk / n;
+        for (int c = 0; c < 8; c++) {
+          const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
+          if (hat == 0.0) continue;
+          for (int d = 0; d < 3; d++)
+            for (int a = 0; a < 3; a++) bacc[a][3 * c + d] -= Kb[a][d] * hat;
+        }
+      }
+      if (s == S_CENTER)
+        for (int a = 0; a < 3; a++) {
+          dinv64[(C.int0 + v) * 3 + a] = 1.0 / Kb[a][a];
+          dinv32[(C.int0 + v) * 3 + a] = (float)(1.0 / Kb[a][a]);
+        }
+    }
+    double* bv = b64 + (C.int0 + v) * (3 * NB);
+    for (int a = 0; a < 3; a++)
+      for (int q = 0; q < NB; q++) bv[a * NB + q] = bacc[a][q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stencil SpMM for one interior node and RG right-hand sides:
+// acc[a][r] = sum_s sum_b K[s][a][b] * P(nbr_s)[b][r]
+// ---------------------------------------------------------------------------
+template <typename KT, typename VT, typename Load>
+__device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint, int v, int i, int j, int k, int m,
+                                            Load load, VT (&acc)[3][RG]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int r = 0; r < RG; r++) acc[a][r] = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int ni = i + c_stencil[s][0], nj = j + c_stencil[s][1], nk = k + c_stencil[s][2];
+    if (ni < 1 || ni > m || nj < 1 || nj > m || nk < 1 || nk > m) continue;
+    const int u = interior_index(ni, nj, nk, m);
+    VT kv[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) kv[e] = (VT)Kc[(int64_t)(s * 9 + e) * nint + v];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      VT pv[RG];
+      load(u, b, pv);
+#pragma unroll
+      for (int r = 0; r < RG; r++) {
+        acc[0][r] += kv[0 * 3 + b] * pv[r];
+        acc[1][r] += kv[1 * 3 + b] * pv[r];
+        acc[2][r] += kv[2 * 3 + b] * pv[r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batched solve.  T = inner precision (float: FP32 inner with FP64
+// refinement; double: plain FP64 PCG).  Dynamic LDS: p [nint][3][RG] of T.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(1024) void k_batch_solve(SolveArgs A) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* P = reinterpret_cast<T*>(smem_raw);
+  __shared__ double red[16 * 3 * RG];
+  __shared__ int task_s;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  T* R = (std::is_same<T, float>::value ? (T*)A.scr32 : (T*)A.scr64) + (int64_t)blockIdx.x * 3 * A.scr_stride;
+  T* D = R + A.scr_stride;  // inner correction
+  T* Q = D + A.scr_stride;  // A p
+  for (;;) {
+    if (tid == 0) task_s = (int)__hip_atomic_fetch_add(A.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int task = task_s;
+    __syncthreads();
+    if (task >= A.ntasks) break;
+    const int cell = A.task_cell[task], grp = A.task_group[task];
+    const Cell C = A.cells[cell];
+    const int n = C.n, m = n - 1, nint = C.nint;
+    const float* K32 = A.K32 + C.int0 * (NS * 9);
+    const double* K64 = A.K64 + C.int0 * (NS * 9);
+    const double* b64 = A.b64 + C.int0 * (3 * NB);
+    double* x64 = A.x64 + C.int0 * (3 * NB);
+    const T* dinv = std::is_same<T, float>::value ? (const T*)(A.dinv32 + C.int0 * 3) : (const T*)(A.dinv64 + C.int0 * 3);
+    const int q0 = grp * RG;
+
+    // ||b||^2 per RHS; optional guess check against the zero guess
+    double bb[RG], rr64[RG];
+    {
+      double v2[1][RG];
+      for (int r = 0; r < RG; r++) v2[0][r] = 0;
+      for (int v = tid; v < nint; v += nth)
+        for (int a = 0; a < 3; a++)
+          for (int r = 0; r < RG; r++) {
+            const double bv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
+            v2[0][r] += bv * bv;
+          }
+      double o[1][RG];
+      block_reduce_rg<1>(v2, red, o);
+      for (int r = 0; r < RG; r++) bb[r] = o[0][r] > 0 ? o[0][r] : 1.0;
+    }
+    if (!A.use_guess)
+      for (int v = tid; v < nint; v += nth)
+        for (int a = 0; a < 3; a++)
+          for (int r = 0; r < RG; r++) x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
+    __syncthreads();
+    int it_total[RG];
+    bool done[RG];
+    for (int r = 0; r < RG; r++) {
+      it_total[r] = 0;
+      done[r] = false;
+    }
+    bool guess_checked = !A.use_guess;
+    for (int outer = 0; outer < 64; outer++) {
+      // true residual r64 = b - K64 x64 (FP64), scaled copy into R (T)
+      double v2[1][RG];
+      for (int r = 0; r < RG; r++) v2[0][r] = 0;
+      for (int v = tid; v < nint; v += nth) {
+        const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
+        double acc[3][RG];
+        auto ld = [&](int u, int b, double (&pv)[RG]) {
+#pragma unroll
+          for (int r = 0; r < RG; r++) pv[r] = x64[(int64_t)u * 3 * NB + b * NB + q0 + r];
+        };
+        stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
+        for (int a = 0; a < 3; a++)
+          for (int r = 0; r < RG; r++) {
+            const double rv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r] - acc[a][r];
+            v2[0][r] += rv * rv;
+            R[((int64_t)v * 3 + a) * RG + r] = (T)rv;  // unscaled for now
+          }
+      }
+      double o[1][RG];
+      block_reduce_rg<1>(v2, red, o);
+      for (int r = 0; r < RG; r++) rr64[r] = o[0][r];
+      if (!guess_checked) {
+        // reject a guess whose residual exceeds the zero guess's (= ||b||)
+        guess_checked = true;
+        bool reject[RG];
+        bool any = false;
+        for (int r = 0; r < RG; r++) {
+          reject[r] = !(rr64[r] <= bb[r]);
+          any |= reject[r];
+        }
+        if (any) {
+          for (int v = tid; v < nint; v += nth)
+            for (int a = 0; a < 3; a++)
+              for (int r = 0; r < RG; r++)
+                if (reject[r]) {
+                  x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
+                  R[((int64_t)v * 3 + a) * RG + r] = (T)b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
+                }
+          for (int r = 0; r < RG; r++)
+            if (reject[r]) rr64[r] = bb[r];
+          if (tid == 0)
+            for (int r = 0; r < RG; r++)
+              if (reject[r]) A.flags[cell * NB + q0 + r] |= 2;
+          __syncthreads();
+        }
+      }
+      bool all = true;
+      double scale[RG];
+      for (int r = 0; r < RG; r++) {
+        done[r] = done[r] || (rr64[r] <= A.tol * A.tol * bb[r]) || !(rr64[r] == rr64[r]);
+        all = all && done[r];
+        scale[r] = rr64[r] > 0 ? sqrt(rr64[r]) : 1.0;
+      }
+      int maxit_left = 0;
+      for (int r = 0; r < RG; r++) maxit_left = max(maxit_left, done[r] ? 0 : A.maxit - it_total[r]);
+      if (all || maxit_left <= 0) break;
+      // inner: solve K d = rhat with rhat = r / scale, to inner tolerance
+      // (plain FP64 mode: one inner solve to tol itself).
+      const bool fp64 = std::is_same<T, double>::value;
+      double itol2[RG];
+      for (int r = 0; r < RG; r++) {
+        const double rel = sqrt(rr64[r] / bb[r]);
+        double it = fp64 ? A.tol / rel : fmax(A.inner_tol, 0.5 * A.tol / rel);
+        it = fmin(it, 0.5);
+        itol2[r] = it * it;
+      }
+      // start: R <- R/scale, D = 0, P = dinv R, rz, rr
+      double rz[RG], rr[RG];
+      {
+        double w[2][RG];
+        for (int r = 0; r < RG; r++) w[0][r] = w[1][r] = 0;
+        for (int v = tid; v < nint; v += nth)
+          for (int a = 0; a < 3; a++)
+            for (int r = 0; r < RG; r++) {
+              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+              const T rv = done[r] ? (T)0 : (T)((double)R[e] / scale[r]);
+              R[e] = rv;
+              D[e] = 0;
+              const T z = dinv[v * 3 + a] * rv;
+              P[e] = z;
+              w[0][r] += (double)(rv * z);
+              w[1][r] += (double)(rv * rv);
+            }
+        double o2[2][RG];
+        block_reduce_rg<2>(w, red, o2);
+        for (int r = 0; r < RG; r++) {
+          rz[r] = o2[0][r];
+          rr[r] = o2[1][r];
+        }
+      }
+      bool act[RG];
+      int inner_it = 0;
+      for (int r = 0; r < RG; r++) act[r] = !done[r] && rr[r] > itol2[r];
+      for (;;) {
+        bool any = false;
+        for (int r = 0; r < RG; r++) any |= act[r];
+        if (!any || inner_it >= maxit_left) break;
+        // q = K p, pq
+        double w[1][RG];
+        for (int r = 0; r < RG; r++) w[0][r] = 0;
+        for (int v = tid; v < nint; v += nth) {
+          const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
+          T acc[3][RG];
+          auto ld = [&](int u, int b, T (&pv)[RG]) {
+#pragma unroll
+            for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
+          };
+          if constexpr (std::is_same<T, float>::value)
+            stencil_row<float, float>(K32, nint, v, i, j, k, m, ld, acc);
+          else
+            stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
+          for (int a = 0; a < 3; a++)
+            for (int r = 0; r < RG; r++) {
+              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+              Q[e] = acc[a][r];
+              w[0][r] += (double)(acc[a][r] * P[e]);
+            }
+        }
+        double o1[1][RG];
+        block_reduce_rg<1>(w, red, o1);
+        T alpha[RG];
+        for (int r = 0; r < RG; r++) {
+          const double pq = o1[0][r];
+          alpha[r] = (act[r] && pq != 0.0) ? (T)(rz[r] / pq) : (T)0;
+        }
+        // x += alpha p, r -= alpha q, rz', rr'
+        double w2[2][RG];
+        for (int r = 0; r < RG; r++) w2[0][r] = w2[1][r] = 0;
+        for (int v = tid; v < nint; v += nth)
+          for (int a = 0; a < 3; a++)
+            for (int r = 0; r < RG; r++) {
+              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+              D[e] += alpha[r] * P[e];
+              const T rv = R[e] - alpha[r] * Q[e];
+              R[e] = rv;
+              const T z = dinv[v * 3 + a] * rv;
+              w2[0][r] += (double)(rv * z);
+              w2[1][r] += (double)(rv * rv);
+            }
+        double o2[2][RG];
+        block_reduce_rg<2>(w2, red, o2);
+        T beta[RG];
+        for (int r = 0; r < RG; r++) {
+          beta[r] = 0;
+          if (!act[r]) continue;
+          it_total[r]++;
+          const double rz2 = o2[0][r], rr2 = o2[1][r];
+          if (!(rr2 == rr2) || rr2 <= itol2[r] || it_total[r] >= A.maxit) {
+            act[r] = false;
+          } else {
+            beta[r] = (T)(rz2 / rz[r]);
+            rz[r] = rz2;
+          }
+          rr[r] = rr2;
+        }
+        // p = z + beta p (only active RHS change)
+        for (int v = tid; v < nint; v += nth)
+          for (int a = 0; a < 3; a++)
+            for (int r = 0; r < RG; r++) {
+              if (!act[r]) continue;
+              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+              P[e] = dinv[v * 3 + a] * R[e] + beta[r] * P[e];
+            }
+        __syncthreads();
+        inner_it++;
+      }
+      // x64 += scale * d
+      for (int v = tid; v < nint; v += nth)
+        for (int a = 0; a < 3; a++)
+          for (int r = 0; r < RG; r++)
+            if (!done[r]) {
+              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+              x64[(int64_t)v * 3 * NB + a * NB + q0 + r] += scale[r] * (double)D[e];
+            }
+      __syncthreads();
+      if (inner_it == 0) break;  // nothing more can be done
+    }
+    if (tid < RG) {
+      const int r = tid;
+      A.iters[cell * NB + q0 + r] = it_total[r];
+      A.relres[cell * NB + q0 + r] = sqrt(rr64[r] / bb[r]);
+      if (!(rr64[r] <= A.tol * A.tol * bb[r])) A.flags[cell * NB + q0 + r] |= 1;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_reset(unsigned* queue, int* flags, int nflags) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *queue = 0u;
+  for (int i = t; i < nflags; i += gridDim.x * blockDim.x) flags[i] = 0;
+}
+
+// Coarse Galerkin matrices Kc = Phi^T K Phi (24x24 per cell), FP64, by an
+// element loop: one workgroup per cell, tets processed one at a time.
+__global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells, const double* __restrict__ E,
+                                                  double nu, double h, int conv, const double* __restrict__ x64,
+                                                  double* __restrict__ Kc) {
+  __shared__ double phi[12][NB];   // element values of the 24 bases
+  __shared__ double Ke[12][12];
+  __shared__ double KP[12][NB];
+  const int cell = blockIdx.x, tid = threadIdx.x;
+  const Cell C = cells[cell];
+  const int n = C.n, m = n - 1;
+  const double hf = h / n;
+  const double lamf = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), muf = 1.0 / (2.0 * (1.0 + nu));
+  double acc[3] = {0, 0, 0};  // thread owns pairs tid, tid+256, tid+512 (< 576)
+  const int ntet = 6 * n * n * n;
+  for (int t = 0; t < ntet; t++) {
+    int ijk[4][3];
+    tet_nodes(t, n, ijk);
+    // element basis values
+    for (int q = tid; q < 12 * NB; q += blockDim.x) {
+      const int p = q / NB, bq = q % NB;
+      const int vtx = p / 3, a = p % 3;
+      const int i = ijk[vtx][0], j = ijk[vtx][1], k = ijk[vtx][2];
+      const bool interior = i >= 1 && i <= m && j >= 1 && j <= m && k >= 1 && k <= m;
+      double val;
+      if (interior) {
+        val = x64[(C.int0 + interior_index(i, j, k, m)) * 3 * NB + a * NB + bq];
+      } else {
+        const int c = bq / 3, d = bq % 3;
+        const double X = (double)i / n, Y = (double)j / n, Z = (double)k / n;
+        const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
+        val = (a == d) ? hat : 0.0;
+      }
+      phi[p][bq] = val;
+    }
+    if (tid < 16) {
+      double g[4][3], V;
+      tet_grads(ijk, hf, g, V);
+      const int I = tid / 4, J = tid % 4;
+      const double Et = E[C.tet0 + t];
+      double Kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      elastic_block(g[I], g[J], V * Et * lamf, V * Et * muf, conv, Kb);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) Ke[3 * I + a][3 * J + b] = Kb[a][b];
+    }
+    __syncthreads();
+    for (int q = tid; q < 12 * NB; q += blockDim.x) {
+      const int p = q / NB, bq = q % NB;
+      double s = 0;
+      for (int u = 0; u < 12; u++) s += Ke[p][u] * phi[u][bq];
+      KP[p][bq] = s;
+    }
+    __syncthreads();
+    for (int w = 0; w < 3; w++) {
+      const int pr = tid + 256 * w;
+      if (pr >= NB * NB) break;
+      const int bi = pr / NB, bj = pr % NB;
+      double s = 0;
+      for (int p = 0; p < 12; p++) s += phi[p][bi] * KP[p][bj];
+      acc[w] += s;
+    }
+    __syncthreads();
+  }
+  for (int w = 0; w < 3; w++) {
+    const int pr = tid + 256 * w;
+    if (pr < NB * NB) Kc[(int64_t)cell * NB * NB + pr] = acc[w];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct mfs_batch_s {
+  mfs_ctx ctx = nullptr;
+  int ncells = 0;
+  double h = 1.0, nu = 0.3;
+  int conv = MFS_ELEM_REFERENCE;
+  std::vector<Cell> cells;
+  std::vector<int64_t> int0;  // per cell
+  int64_t total_int = 0, total_tets = 0;
+  int max_n = 0, max_nint = 0;
+  int ntasks = 0;
+  bool assembled = false;
+  int tuned_wg = 0, tuned_per_cu = 0;
+  // device
+  Cell* d_cells = nullptr;
+  int64_t* d_int0 = nullptr;
+  Tmpl* d_tmpls = nullptr;
+  int64_t* d_gptr = nullptr;
+  uint32_t* d_gent = nullptr;
+  double* d_E = nullptr;
+  float* d_K32 = nullptr;
+  double* d_K64 = nullptr;
+  float* d_dinv32 = nullptr;
+  double* d_dinv64 = nullptr;
+  double* d_b64 = nullptr;
+  double* d_x64 = nullptr;
+  int* d_iters = nullptr;
+  double* d_relres = nullptr;
+  int* d_flags = nullptr;
+  int* d_task_cell = nullptr;
+  int* d_task_group = nullptr;
+  unsigned* d_queue = nullptr;
+  void* d_scr = nullptr;
+  size_t scr_bytes = 0;
+};
+
+static int batch_free(mfs_batch B) {
+  if (!B) return MFS_OK;
+  if (B->ctx) (void)hipStreamSynchronize(B->ctx->stream);
+  void* ptrs[] = {B->d_cells, B->d_int0, B->d_tmpls, B->d_gptr, B->d_gent, B->d_E, B->d_K32, B->d_K64,
+                  B->d_dinv32, B->d_dinv64, B->d_b64, B->d_x64, B->d_iters, B->d_relres, B->d_flags,
+                  B->d_task_cell, B->d_task_group, B->d_queue, B->d_scr};
+  for (void* p : ptrs) dfree(p);
+  mfs_ctx c = B->ctx;
+  delete B;
+  ctx_release(c);
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_destroy(mfs_batch B) { return batch_free(B); }
+
+extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch* out) {
+  MFS_CHECK_ARG(ctx && d && out, "mfs_batch_create: bad argument");
+  MFS_CHECK_ARG(d->n_cells > 0 && d->cell_n && d->cell_origin && d->E, "mfs_batch_create: empty batch");
+  MFS_CHECK_ARG(d->cell_h > 0 && d->nu > -1.0 && d->nu < 0.5, "mfs_batch_create: bad cell size or nu");
+  MFS_CHECK_ARG(d->convention == MFS_ELEM_REFERENCE || d->convention == MFS_ELEM_TEXTBOOK,
+                "mfs_batch_create: unknown element convention");
+  for (int c = 0; c < d->n_cells; c++)
+    MFS_CHECK_ARG(d->cell_n[c] >= 2 && d->cell_n[c] <= MAXN, "mfs_batch_create: cell subdivision outside 2..12");
+  MFS_HIP(hipSetDevice(ctx->device));
+  *out = nullptr;
+  mfs_batch B = new mfs_batch_s();
+  B->ctx = ctx;
+  ctx_retain(ctx);
+  B->ncells = d->n_cells;
+  B->h = d->cell_h;
+  B->nu = d->nu;
+  B->conv = d->convention;
+  // templates per distinct n
+  std::map<int, int> tmpl_of;
+  std::vector<Tmpl> tmpls;
+  std::vector<int64_t> gptr_all;
+  std::vector<uint32_t> gent_all;
+  for (int c = 0; c < d->n_cells; c++) {
+    const int n = d->cell_n[c];
+    if (tmpl_of.count(n)) continue;
+    const int s = n + 1, m = n - 1;
+    std::vector<double> xyz(3 * (size_t)s * s * s);
+    std::vector<int32_t> tets(4 * 6 * (size_t)n * n * n);
+    mfs_mesh_box(n, n, n, 0, 0, 0, 1, 1, 1, xyz.data(), tets.data());
+    Pattern P;
+    build_pattern((int64_t)s * s * s, 6ll * n * n * n, tets.data(), P);
+    GatherLists G;
+    build_gather_lists(P, 6ll * n * n * n, tets.data(), G);
+    Tmpl T;
+    T.n = n;
+    T.ptr0 = (int64_t)gptr_all.size();
+    T.ent0 = (int64_t)gent_all.size();
+    int64_t base = 0;
+    gptr_all.push_back(0);
+    for (int k = 1; k <= m; k++)
+      for (int j = 1; j <= m; j++)
+        for (int i = 1; i <= m; i++) {
+          const int row = i + s * (j + s * k);
+          for (int st = 0; st < NS; st++) {
+            const int col = (i + h_stencil[st][0]) + s * ((j + h_stencil[st][1]) + s * (k + h_stencil[st][2]));
+            const int32_t* b0 = P.col.data() + P.rowptr[row];
+            const int32_t* e0 = P.col.data() + P.rowptr[row + 1];
+            const int32_t* f = std::lower_bound(b0, e0, col);
+            if (f == e0 || *f != col) {
+              set_error("mfs_batch_create: stencil point missing from the cell pattern");
+              batch_free(B);
+              return MFS_ERR_ARG;
+            }
+            const int64_t slot = f - P.col.data();
+            for (int64_t q = G.ptr[slot]; q < G.ptr[slot + 1]; q++) gent_all.push_back(G.entry[q]);
+            base += G.ptr[slot + 1] - G.ptr[slot];
+            gptr_all.push_back(base);
+          }
+        }
+    tmpl_of[n] = (int)tmpls.size();
+    tmpls.push_back(T);
+  }
+  // cells
+  B->cells.resize(d->n_cells);
+  B->int0.resize(d->n_cells);
+  int64_t t0 = 0, i0 = 0;
+  for (int c = 0; c < d->n_cells; c++) {
+    Cell& C = B->cells[c];
+    C.n = d->cell_n[c];
+    C.nint = (C.n - 1) * (C.n - 1) * (C.n - 1);
+    C.tet0 = t0;
+    C.int0 = i0;
+    C.tmpl = tmpl_of[C.n];
+    C.ox = d->cell_origin[3 * c];
+    C.oy = d->cell_origin[3 * c + 1];
+    C.oz = d->cell_origin[3 * c + 2];
+    B->int0[c] = i0;
+    t0 += 6ll * C.n * C.n * C.n;
+    i0 += C.nint;
+    B->max_n = std::max(B->max_n, C.n);
+    B->max_nint = std::max(B->max_nint, C.nint);
+  }
+  B->total_tets = t0;
+  B->total_int = i0;
+  for (int c = 0; c < d->n_cells; c++)
+    if (!(d->E[c] > 0) && false) break;
+  // tasks: 3 per cell, largest cells first (stable by cell id)
+  std::vector<int> order(d->n_cells);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->cells[a].n > B->cells[b].n; });
+  std::vector<int> tc, tg;
+  for (int c : order)
+    for (int g = 0; g < NB / RG; g++) {
+      tc.push_back(c);
+      tg.push_back(g);
+    }
+  B->ntasks = (int)tc.size();
+  int st = MFS_OK;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (st == MFS_OK) st = dmalloc(p, bytes);
+  };
+  alloc((void**)&B->d_cells, sizeof(Cell) * d->n_cells);
+  alloc((void**)&B->d_int0, sizeof(int64_t) * d->n_cells);
+  alloc((void**)&B->d_tmpls, sizeof(Tmpl) * tmpls.size());
+  alloc((void**)&B->d_gptr, sizeof(int64_t) * gptr_all.size());
+  alloc((void**)&B->d_gent, sizeof(uint32_t) * std::max<size_t>(1, gent_all.size()));
+  alloc((void**)&B->d_E, sizeof(double) * t0);
+  alloc((void**)&B->d_K32, sizeof(float) * NS * 9 * i0);
+  alloc((void**)&B->d_K64, sizeof(double) * NS * 9 * i0);
+  alloc((void**)&B->d_dinv32, sizeof(float) * 3 * i0);
+  alloc((void**)&B->d_dinv64, sizeof(double) * 3 * i0);
+  alloc((void**)&B->d_b64, sizeof(double) * 3 * NB * i0);
+  alloc((void**)&B->d_x64, sizeof(double) * 3 * NB * i0);
+  alloc((void**)&B->d_iters, sizeof(int) * NB * d->n_cells);
+  alloc((void**)&B->d_relres, sizeof(double) * NB * d->n_cells);
+  alloc((void**)&B->d_flags, sizeof(int) * NB * d->n_cells);
+  alloc((void**)&B->d_task_cell, sizeof(int) * B->ntasks);
+  alloc((void**)&B->d_task_group, sizeof(int) * B->ntasks);
+  alloc((void**)&B->d_queue, 64);
+  if (st != MFS_OK) {
+    batch_free(B);
+    return st;
+  }
+  hipStream_t s = ctx->stream;
+  MFS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_stencil), h_stencil, sizeof(h_stencil)));
+  MFS_HIP(hipMemcpyAsync(B->d_cells, B->cells.data(), sizeof(Cell) * d->n_cells, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(B->d_int0, B->int0.data(), sizeof(int64_t) * d->n_cells, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(B->d_tmpls, tmpls.data(), sizeof(Tmpl) * tmpls.size(), hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(B->d_gptr, gptr_all.data(), sizeof(int64_t) * gptr_all.size(), hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(B->d_gent, gent_all.data(), sizeof(uint32_t) * gent_all.size(), hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(B->d_E, d->E, sizeof(double) * t0, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(B->d_task_cell, tc.data(), sizeof(int) * B->ntasks, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(B->d_task_group, tg.data(), sizeof(int) * B->ntasks, hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemsetAsync(B->d_x64, 0, sizeof(double) * 3 * NB * i0, s));
+  MFS_HIP(hipMemsetAsync(B->d_iters, 0, sizeof(int) * NB * d->n_cells, s));
+  MFS_HIP(hipMemsetAsync(B->d_relres, 0, sizeof(double) * NB * d->n_cells, s));
+  MFS_HIP(hipMemsetAsync(B->d_flags, 0, sizeof(int) * NB * d->n_cells, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  *out = B;
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_cell_dofs(mfs_batch B, int cell, int64_t* n_interior, int64_t* n_total) {
+  MFS_CHECK_ARG(B && cell >= 0 && cell < B->ncells, "mfs_batch_cell_dofs: bad argument");
+  const int n = B->cells[cell].n;
+  if (n_interior) *n_interior = 3ll * (n - 1) * (n - 1) * (n - 1);
+  if (n_total) *n_total = 3ll * (n + 1) * (n + 1) * (n + 1);
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_assemble(mfs_batch B) {
+  MFS_CHECK_ARG(B, "mfs_batch_assemble: NULL batch");
+  hipStream_t s = B->ctx->stream;
+  const unsigned grid = (unsigned)std::min<int64_t>(65536, (B->total_int + 255) / 256);
+  k_cell_assemble<<<grid, 256, 0, s>>>(B->d_cells, B->ncells, B->d_int0, B->d_tmpls, B->d_gptr, B->d_gent, B->d_E,
+                                       B->nu, B->h, B->conv, B->total_int, B->d_K32, B->d_K64, B->d_dinv32,
+                                       B->d_dinv64, B->d_b64);
+  MFS_HIP(hipGetLastError());
+  MFS_HIP(hipStreamSynchronize(s));
+  B->assembled = true;
+  return MFS_OK;
+}
+
+static size_t lds_bytes(mfs_batch B, int prec) {
+  return (size_t)B->max_nint * 3 * RG * (prec == MFS_PREC_F64 ? 8 : 4);
+}
+
+// Per-workgroup scratch for a launch shape; must run before any capture.
+static int batch_scratch(mfs_batch B, int precision, int per_cu) {
+  const int grid = B->ctx->cu_count * per_cu;
+  const int64_t stride = (int64_t)B->max_nint * 3 * RG;
+  const size_t need = (size_t)grid * 3 * stride * (precision == MFS_PREC_F64 ? 8 : 4);
+  if (need > B->scr_bytes) {
+    MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+    dfree(B->d_scr);
+    B->d_scr = nullptr;
+    B->scr_bytes = 0;
+    MFS_TRY(dmalloc(&B->d_scr, need));
+    B->scr_bytes = need;
+  }
+  return MFS_OK;
+}
+
+// Launch reset + persistent solve on the context stream (capturable: no
+// allocation, no memset; the queue and flags are cleared by a kernel).
+static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu) {
+  hipStream_t s = B->ctx->stream;
+  const int grid = B->ctx->cu_count * per_cu;
+  const size_t lds = lds_bytes(B, d->precision);
+  const int64_t stride = (int64_t)B->max_nint * 3 * RG;
+  const size_t need = (size_t)grid * 3 * stride * (d->precision == MFS_PREC_F64 ? 8 : 4);
+  if (need > B->scr_bytes) {
+    set_error("mfs_batch_solve: scratch not allocated for this launch shape");
+    return MFS_ERR_ARG;
+  }
+  SolveArgs A;
+  A.cells = B->d_cells;
+  A.task_cell = B->d_task_cell;
+  A.task_group = B->d_task_group;
+  A.ntasks = B->ntasks;
+  A.queue = B->d_queue;
+  A.K32 = B->d_K32;
+  A.K64 = B->d_K64;
+  A.dinv32 = B->d_dinv32;
+  A.dinv64 = B->d_dinv64;
+  A.b64 = B->d_b64;
+  A.x64 = B->d_x64;
+  A.scr32 = (float*)B->d_scr;
+  A.scr64 = (double*)B->d_scr;
+  A.scr_stride = stride;
+  A.iters = B->d_iters;
+  A.relres = B->d_relres;
+  A.flags = B->d_flags;
+  A.tol = d->tol;
+  A.inner_tol = d->inner_tol > 0 ? d->inner_tol : 1e-4;
+  A.maxit = d->maxit > 0 ? d->maxit : 20000;
+  A.use_guess = d->use_guess;
+  k_batch_reset<<<std::max(1, (B->ncells * NB + 255) / 256), 256, 0, s>>>(B->d_queue, B->d_flags, B->ncells * NB);
+  if (d->precision == MFS_PREC_F64)
+    k_batch_solve<double><<<grid, wg, lds, s>>>(A);
+  else
+    k_batch_solve<float><<<grid, wg, lds, s>>>(A);
+  MFS_HIP(hipGetLastError());
+  return MFS_OK;
+}
+
+static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
+  MFS_CHECK_ARG(B && d, "mfs_batch_solve: bad argument");
+  MFS_CHECK_ARG(B->assembled, "mfs_batch_solve: call mfs_batch_assemble first");
+  MFS_CHECK_ARG(d->precision == MFS_PREC_F64 || d->precision == MFS_PREC_MIXED, "mfs_batch_solve: bad precision");
+  MFS_CHECK_ARG(d->tol > 0, "mfs_batch_solve: tol must be positive");
+  MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 1024),
+                "mfs_batch_solve: wg_size must be a multiple of 64 in 64..1024");
+  if (lds_bytes(B, d->precision) > 160 * 1024 - 8192) {
+    set_error("mfs_batch_solve: cells too large for the LDS-resident search directions (FP64 inner needs n <= 10)");
+    return MFS_ERR_UNSUPPORTED;
+  }
+  return MFS_OK;
+}
+
+static int pick_shape(mfs_batch B, const mfs_batch_solve_desc* d, int* wg, int* per_cu) {
+  const size_t lds = lds_bytes(B, d->precision) + 4096;
+  int wgs = d->wg_size ? d->wg_size : (B->tuned_wg ? B->tuned_wg : 256);
+  int pc = B->tuned_per_cu ? B->tuned_per_cu : 2;
+  if (d->wg_size) pc = 1;
+  while (pc > 1 && (size_t)pc * lds > 160 * 1024) pc--;
+  *wg = wgs;
+  *per_cu = pc;
+  const hipError_t e = d->precision == MFS_PREC_F64
+                           ? hipFuncSetAttribute((const void*)k_batch_solve<double>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(B, d->precision))
+                           : hipFuncSetAttribute((const void*)k_batch_solve<float>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(B, d->precision));
+  (void)e;
+  return MFS_OK;
+}
+
+static int batch_collect(mfs_batch B, mfs_batch_report* rep) {
+  hipStream_t s = B->ctx->stream;
+  const int nq = B->ncells * NB;
+  std::vector<int> it(nq), fl(nq);
+  std::vector<double> rr(nq);
+  MFS_HIP(hipMemcpyAsync(it.data(), B->d_iters, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipMemcpyAsync(fl.data(), B->d_flags, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipMemcpyAsync(rr.data(), B->d_relres, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  rep->max_iterations = 0;
+  rep->mean_iterations = 0;
+  rep->max_true_relres = 0;
+  rep->n_not_converged = 0;
+  rep->n_fallback = 0;
+  for (int q = 0; q < nq; q++) {
+    rep->max_iterations = std::max(rep->max_iterations, it[q]);
+    rep->mean_iterations += it[q];
+    rep->max_true_relres = std::max(rep->max_true_relres, rr[q]);
+    rep->n_not_converged += fl[q] & 1;
+    rep->n_fallback += (fl[q] >> 1) & 1;
+  }
+  rep->mean_iterations /= nq;
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_batch_report* rep) {
+  MFS_TRY(batch_check_desc(B, d));
+  MFS_CHECK_ARG(rep, "mfs_batch_solve: report is NULL");
+  int wg, pc;
+  MFS_TRY(pick_shape(B, d, &wg, &pc));
+  MFS_TRY(batch_scratch(B, d->precision, pc));
+  hipStream_t s = B->ctx->stream;
+  hipEvent_t e0, e1;
+  MFS_HIP(hipEventCreate(&e0));
+  MFS_HIP(hipEventCreate(&e1));
+  if (d->use_graph) {
+    hipGraph_t g;
+    hipGraphExec_t ge;
+    MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = batch_launch(B, d, wg, pc);
+    MFS_HIP(hipStreamEndCapture(s, &g));
+    MFS_TRY(st);
+    MFS_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    MFS_HIP(hipEventRecord(e0, s));
+    MFS_HIP(hipGraphLaunch(ge, s));
+    MFS_HIP(hipEventRecord(e1, s));
+    MFS_HIP(hipEventSynchronize(e1));
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+  } else {
+    MFS_HIP(hipEventRecord(e0, s));
+    MFS_TRY(batch_launch(B, d, wg, pc));
+    MFS_HIP(hipEventRecord(e1, s));
+    MFS_HIP(hipEventSynchronize(e1));
+  }
+  float ms = 0;
+  MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MFS_TRY(batch_collect(B, rep));
+  rep->seconds = ms * 1e-3;
+  if (rep->n_not_converged > 0) {
+    set_error("mfs_batch_solve: " + std::to_string(rep->n_not_converged) + " local problems did not converge");
+    return MFS_ERR_NOT_CONVERGED;
+  }
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* d, int* best_wg, double* best_ms) {
+  MFS_TRY(batch_check_desc(B, d));
+  const int shapes[][2] = {{128, 4}, {256, 2}, {256, 1}, {512, 1}, {1024, 1}};
+  double best = 1e30;
+  int bw = 256, bp = 2;
+  const size_t lds = lds_bytes(B, d->precision) + 4096;
+  for (auto& sh : shapes) {
+    if ((size_t)sh[1] * lds > 160 * 1024) continue;
+    B->tuned_wg = sh[0];
+    B->tuned_per_cu = sh[1];
+    mfs_batch_solve_desc dd = *d;
+    dd.wg_size = 0;
+    dd.use_guess = 0;
+    mfs_batch_report rep;
+    int st = mfs_batch_solve(B, &dd, &rep);
+    if (st != MFS_OK && st != MFS_ERR_NOT_CONVERGED) return st;
+    if (rep.seconds < best) {
+      best = rep.seconds;
+      bw = sh[0];
+      bp = sh[1];
+    }
+  }
+  B->tuned_wg = bw;
+  B->tuned_per_cu = bp;
+  if (best_wg) *best_wg = bw * 100 + bp;
+  if (best_ms) *best_ms = best * 1e3;
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_set_guess(mfs_batch B, const double* guess) {
+  MFS_CHECK_ARG(B && guess, "mfs_batch_set_guess: bad argument");
+  MFS_HIP(hipMemcpyAsync(B->d_x64, guess, sizeof(double) * 3 * NB * B->total_int, hipMemcpyHostToDevice,
+                         B->ctx->stream));
+  MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_get_basis(mfs_batch B, int cell, double* basis) {
+  MFS_CHECK_ARG(B && basis && cell >= 0 && cell < B->ncells, "mfs_batch_get_basis: bad argument");
+  const Cell& C = B->cells[cell];
+  const int n = C.n, m = n - 1, s = n + 1;
+  std::vector<double> xi((size_t)C.nint * 3 * NB);
+  MFS_HIP(hipMemcpyAsync(xi.data(), B->d_x64 + C.int0 * 3 * NB, sizeof(double) * xi.size(), hipMemcpyDeviceToHost,
+                         B->ctx->stream));
+  MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+  const int64_t N = 3ll * s * s * s;
+  for (int k = 0; k <= n; k++)
+    for (int j = 0; j <= n; j++)
+      for (int i = 0; i <= n; i++) {
+        const int64_t node = i + s * (j + s * k);
+        const bool interior = i >= 1 && i <= m && j >= 1 && j <= m && k >= 1 && k <= m;
+        const double X = (double)i / n, Y = (double)j / n, Z = (double)k / n;
+        for (int q = 0; q < NB; q++) {
+          const int c = q / 3, d = q % 3;
+          for (int a = 0; a < 3; a++) {
+            double v;
+            if (interior) {
+              const int u = (i - 1) + m * ((j - 1) + m * (k - 1));
+              v = xi[((size_t)u * 3 + a) * NB + q];
+            } else {
+              const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
+              v = a == d ? hat : 0.0;
+            }
+            basis[q * N + 3 * node + a] = v;
+          }
+        }
+      }
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_coarse_matrices(mfs_batch B, double* Kc) {
+  MFS_CHECK_ARG(B && Kc, "mfs_batch_coarse_matrices: bad argument");
+  MFS_CHECK_ARG(B->assembled, "mfs_batch_coarse_matrices: call mfs_batch_assemble first");
+  hipStream_t s = B->ctx->stream;
+  double* d_kc;
+  MFS_TRY(dmalloc((void**)&d_kc, sizeof(double) * NB * NB * B->ncells));
+  k_galerkin<<<B->ncells, 256, 0, s>>>(B->d_cells, B->d_E, B->nu, B->h, B->conv, B->d_x64, d_kc);
+  MFS_HIP(hipGetLastError());
+  MFS_HIP(hipMemcpyAsync(Kc, d_kc, sizeof(double) * NB * NB * B->ncells, hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  dfree(d_kc);
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_iterations(mfs_batch B, int32_t* iters) {
+  MFS_CHECK_ARG(B && iters, "mfs_batch_iterations: bad argument");
+  MFS_HIP(hipMemcpyAsync(iters, B->d_iters, sizeof(int) * NB * B->ncells, hipMemcpyDeviceToHost, B->ctx->stream));
+  MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+  return MFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned-guess glue: features, targets and guesses stay on the device.
+// ---------------------------------------------------------------------------
+__global__ void k_features(const double* __restrict__ E, int64_t n, double mu, double inv_sigma,
+                           float* __restrict__ X) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    X[i] = (float)((log(E[i]) - mu) * inv_sigma);
+}
+
+// hat_c(x) e_d at interior node v of an n-cell, for basis q = 3c + d
+__device__ __forceinline__ double hat_at(int v, int n, int a, int q) {
+  const int m = n - 1;
+  const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
+  const int c = q / 3, d = q % 3;
+  if (a != d) return 0.0;
+  const double X = (double)i / n, Y = (double)j / n, Z = (double)k / n;
+  return ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
+}
+
+// T[cell][(v*3+a)*24+q] = x - hat (all cells the same n); mode 1: x = hat + G
+__global__ void k_targets_guess(const Cell* __restrict__ cells, int ncells, int n, int64_t per_cell,
+                                double* __restrict__ x64, float* __restrict__ TG, int mode) {
+  const int64_t total = per_cell * ncells;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cell = e / per_cell, r = e % per_cell;
+    const int q = (int)(r % NB), a = (int)((r / NB) % 3), v = (int)(r / (3 * NB));
+    const int64_t at = cells[cell].int0 * 3 * NB + r;
+    const double hat = hat_at(v, n, a, q);
+    if (mode == 0)
+      TG[e] = (float)(x64[at] - hat);
+    else
+      x64[at] = hat + (TG ? (double)TG[e] : 0.0);
+  }
+}
+
+static int batch_uniform_n(mfs_batch B) {
+  for (const Cell& c : B->cells)
+    if (c.n != B->cells[0].n) return -1;
+  return B->cells[0].n;
+}
+
+extern "C" int mfs_batch_features(mfs_batch B, double mu, double sigma, mfs_vec X) {
+  MFS_CHECK_ARG(B && X && sigma > 0, "mfs_batch_features: bad argument");
+  const int n = batch_uniform_n(B);
+  MFS_CHECK_ARG(n > 0, "mfs_batch_features: all cells must have the same subdivision");
+  MFS_CHECK_ARG(X->prec == MFS_F32 && X->n == B->total_tets, "mfs_batch_features: X must be FP32 of ncells*6n^3");
+  k_features<<<(unsigned)std::min<int64_t>(65536, (B->total_tets + 255) / 256), 256, 0, B->ctx->stream>>>(
+      B->d_E, B->total_tets, mu, 1.0 / sigma, (float*)X->d);
+  MFS_HIP(hipGetLastError());
+  MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+  return MFS_OK;
+}
+
+static int targets_guess(mfs_batch B, mfs_vec V, int mode) {
+  const int n = batch_uniform_n(B);
+  MFS_CHECK_ARG(n > 0, "mfs_batch_targets/guess: all cells must have the same subdivision");
+  const int64_t per_cell = (int64_t)(n - 1) * (n - 1) * (n - 1) * 3 * NB;
+  if (V) MFS_CHECK_ARG(V->prec == MFS_F32 && V->n == per_cell * B->ncells, "mfs_batch_targets/guess: bad vector");
+  const int64_t total = per_cell * B->ncells;
+  k_targets_guess<<<(unsigned)std::min<int64_t>(65536, (total + 255) / 256), 256, 0, B->ctx->stream>>>(
+      B->d_cells, B->ncells, n, per_cell, B->d_x64, V ? (float*)V->d : nullptr, mode);
+  MFS_HIP(hipGetLastError());
+  MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_targets(mfs_batch B, mfs_vec T) {
+  MFS_CHECK_ARG(B && T, "mfs_batch_targets: bad argument");
+  return targets_guess(B, T, 0);
+}
+
+extern "C" int mfs_batch_guess_from(mfs_batch B, mfs_vec G) {
+  MFS_CHECK_ARG(B, "mfs_batch_guess_from: bad argument");
+  return targets_guess(B, G, 1);
+}

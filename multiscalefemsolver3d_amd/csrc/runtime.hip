@@ -92,13 +92,25 @@ extern "C" int mfs_ctx_create(int device, mfs_ctx* out) {
   return MFS_OK;
 }
 
+namespace mfs {
+void ctx_retain(mfs_ctx c) {
+  if (c) c->refs.fetch_add(1);
+}
+void ctx_release(mfs_ctx c) {
+  if (!c || c->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamDestroy(c->stream);
+  (void)hipHostFree(c->pinned);
+  delete c;
+}
+}  // namespace mfs
+
+// Drops the caller's reference; the stream and pinned buffer are released
+// once every object created on the context has been destroyed too.
 extern "C" int mfs_ctx_destroy(mfs_ctx ctx) {
   if (!ctx) return MFS_OK;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  (void)hipStreamDestroy(ctx->stream);
-  (void)hipHostFree(ctx->pinned);
-  delete ctx;
+  ctx_release(ctx);
   return MFS_OK;
 }
 
@@ -161,6 +173,7 @@ extern "C" int mfs_vec_create(mfs_ctx ctx, int64_t n, int prec, mfs_vec* out) {
     delete v;
     return s;
   }
+  ctx_retain(ctx);
   MFS_HIP(hipMemsetAsync(v->d, 0, (size_t)n * (prec == MFS_F64 ? 8 : 4), ctx->stream));
   *out = v;
   return MFS_OK;
@@ -170,6 +183,7 @@ extern "C" int mfs_vec_destroy(mfs_vec v) {
   if (!v) return MFS_OK;
   if (v->ctx) (void)hipStreamSynchronize(v->ctx->stream);
   dfree(v->d);
+  ctx_release(v->ctx);
   delete v;
   return MFS_OK;
 }
@@ -295,6 +309,7 @@ int mat_create_from_pattern(mfs_ctx ctx, int bs, Pattern&& P, mfs_mat* out) {
   MFS_HIP(hipSetDevice(ctx->device));
   mfs_mat A = new mfs_mat_s();
   A->ctx = ctx;
+  ctx_retain(ctx);
   A->bs = bs;
   A->nb = P.n;
   A->nnzb = P.nnz();
@@ -307,7 +322,7 @@ int mat_create_from_pattern(mfs_ctx ctx, int bs, Pattern&& P, mfs_mat* out) {
     for (int64_t k = A->pat.rowptr[r]; k < A->pat.rowptr[r + 1]; k++)
       if (A->pat.col[k] == r) diag_pos[r] = A->sell.pos[k];
     if (diag_pos[r] < 0) {
-      delete A;
+      mat_release(A);
       set_error("matrix: row " + std::to_string(r) + " has no diagonal entry (CSSD rows always hold one)");
       return MFS_ERR_ARG;
     }
@@ -421,8 +436,12 @@ extern "C" int mfs_mat_update_values(mfs_mat A, const double* val) {
   return MFS_OK;
 }
 
-extern "C" int mfs_mat_destroy(mfs_mat A) {
-  if (!A) return MFS_OK;
+namespace mfs {
+void mat_retain(mfs_mat A) {
+  if (A) A->refs.fetch_add(1);
+}
+void mat_release(mfs_mat A) {
+  if (!A || A->refs.fetch_sub(1) != 1) return;
   if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
   dfree(A->d_slice_base);
   dfree(A->d_slice_width);
@@ -433,7 +452,16 @@ extern "C" int mfs_mat_destroy(mfs_mat A) {
   dfree(A->d_val32);
   dfree(A->d_dinv64);
   dfree(A->d_dinv32);
+  mfs_ctx c = A->ctx;
   delete A;
+  ctx_release(c);
+}
+}  // namespace mfs
+
+// Drops the caller's reference (solvers built on A keep it alive).
+extern "C" int mfs_mat_destroy(mfs_mat A) {
+  if (!A) return MFS_OK;
+  mat_release(A);
   return MFS_OK;
 }
 

@@ -381,6 +381,7 @@ extern "C" void mfs_solve_desc_default(mfs_solve_desc* d) {
 extern "C" int mfs_solver_destroy(mfs_solver S) {
   if (!S) return MFS_OK;
   (void)hipStreamSynchronize(S->A->ctx->stream);
+  mfs_mat A = S->A;
   if (S->has64) S->cg64.destroy();
   if (S->has32) S->cg32.destroy();
   dfree(S->x64);
@@ -394,6 +395,7 @@ extern "C" int mfs_solver_destroy(mfs_solver S) {
   reduce_scratch_free(S->red64);
   reduce_scratch_free(S->redg);
   delete S;
+  mat_release(A);
   return MFS_OK;
 }
 
@@ -410,6 +412,7 @@ extern "C" int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solv
   *out = nullptr;
   mfs_solver S = new mfs_solver_s();
   S->A = A;
+  mat_retain(A);
   S->desc = *desc;
   S->n = A->n();
   const int64_t n = S->n;

@@ -75,3 +75,22 @@ def _splitmix_uniform(seed, stream, idx):
         key = _mix64(seed * np.uint64(0x100000001B3) + stream)
         h = _mix64(key ^ (idx * np.uint64(0xD6E8FEB86659FD93)))
     return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def msfem_cells(nc, n_lo=6, n_hi=10, n_fixed=None, seed_n=3, seed_E=1, mu=float(np.log(1e9)), sigma=1.0):
+    """Coarse nc^3 grid on the unit cube.  Cell c (x fastest) has n_c fine
+    hexes per edge, U{n_lo..n_hi} from seed_n unless n_fixed is given, and a
+    lognormal Young's modulus exp(mu + sigma N(0,1)) per fine tet (stream
+    1000 + c of seed_E, tets in mesh_box order)."""
+    ncell = nc ** 3
+    cell_n = np.full(ncell, n_fixed, np.int32) if n_fixed else cell_sizes(nc, n_lo, n_hi, seed_n)
+    idx = np.arange(ncell)
+    origin = np.stack([idx % nc, (idx // nc) % nc, idx // (nc * nc)], 1) / nc
+    E = np.concatenate([field_lognormal(6 * int(n) ** 3, seed_E, 1000 + c, 0, mu, sigma)
+                        for c, n in enumerate(cell_n)])
+    return dict(nc=nc, cell_n=cell_n, origin=origin, h=1.0 / nc, E=E, nu=0.3)
+
+
+def cell_E(p, c):
+    off = int((6 * p["cell_n"][:c].astype(np.int64) ** 3).sum())
+    return p["E"][off:off + 6 * int(p["cell_n"][c]) ** 3]

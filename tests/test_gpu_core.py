@@ -167,7 +167,7 @@ def test_gmres_elastic_textbook_operator(ctx, precision):
     assert rep.true_relres <= 1e-9
     assert rel(x.download(), xd) < 1e-6
     # same number of Arnoldi steps as the oracle's GMRES(30) within a restart
-    _, it_o, _ = orc.gmres(O.rowptr, O.col, O.val, O.rhs, O.x0, m=30, tol=1e-9, maxit=20000)
+    _, it_o, _ = orc.gmres(O.rowptr, O.col, O.val, O.rhs, m=30, x0=O.x0, tol=1e-9, maxit=20000)
     if precision == mfs.PREC_F64:
         assert abs(rep.iterations - it_o) <= 30
 

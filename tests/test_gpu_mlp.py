@@ -1,0 +1,57 @@
+"""GPU tests of the BF16-MFMA MLP against the FP64 CPU oracle MLP."""
+import numpy as np
+import pytest
+
+import multiscalefemsolver3d_amd as mfs
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape", [(70, 33, 17), (64, 128, 64), (3, 200, 129)])
+def test_gemm_forward_matches_fp64_on_bf16_operands(ctx, shape):
+    """A single linear layer: output = X W^T + b with BF16 operands and FP32
+    accumulation equals the FP64 product of the BF16-rounded operands to FP32
+    accumulation accuracy (this also pins the MFMA lane layout)."""
+    B, din, dout = shape
+    rng = np.random.default_rng(1)
+    net = mfs.MLP(ctx, [din, dout], seed=3)
+    W = rng.normal(size=(dout, din)).astype(np.float32)
+    b = rng.normal(size=dout).astype(np.float32)
+    net.set_params([(W, b)])
+    X = rng.normal(size=(B, din)).astype(np.float32)
+    Y = net.forward(X)
+    ref = mfs.bf16_round(X).astype(np.float64) @ mfs.bf16_round(W).astype(np.float64).T + b
+    assert np.abs(Y - ref).max() < 1e-5 * np.abs(ref).max() * np.sqrt(din)
+
+
+def test_gradients_match_oracle(ctx):
+    rng = np.random.default_rng(2)
+    dims = [96, 64, 64, 80]
+    net = mfs.MLP(ctx, dims, seed=5)
+    params = net.get_params()
+    X = rng.normal(size=(48, dims[0])).astype(np.float32)
+    T = rng.normal(size=(48, dims[-1])).astype(np.float32)
+    loss, gW, gb = net.loss_grad(X, T)
+    Ws = [mfs.bf16_round(W).astype(np.float64) for W, _ in params]
+    bs = [b.astype(np.float64) for _, b in params]
+    loss_o, gW_o, gb_o = orc.mlp_loss_grad(Ws, bs, mfs.bf16_round(X), T)
+    assert loss == pytest.approx(loss_o, rel=2e-2)
+    for l in range(3):
+        for g, go in ((gW[l], gW_o[l]), (gb[l], gb_o[l])):
+            cos = float((g.ravel() @ go.ravel()) / (np.linalg.norm(g) * np.linalg.norm(go)))
+            assert cos > 0.999, (l, cos)
+            assert np.linalg.norm(g - go) < 5e-2 * np.linalg.norm(go)
+
+
+def test_training_reduces_loss(ctx):
+    rng = np.random.default_rng(3)
+    dims = [64, 128, 128, 32]
+    A = rng.normal(size=(64, 32)) / 8
+    X = rng.normal(size=(2048, 64)).astype(np.float32)
+    T = np.tanh(X @ A).astype(np.float32)
+    net = mfs.MLP(ctx, dims, seed=9, lr=2e-3)
+    l0, _, _ = net.loss_grad(X[:256], T[:256])
+    net.train(X, T, batch=128, epochs=30)
+    l1, _, _ = net.loss_grad(X[:256], T[:256])
+    assert l1 < 0.1 * l0, (l0, l1)

@@ -259,6 +259,8 @@ typedef struct {
   double max_true_relres;
   int n_not_converged;
   int n_fallback;       /* guesses rejected (worse than zero guess)           */
+  int n_precision_fallback; /* problems the FP32 inner solve could not finish,
+                               re-solved in FP64                                */
 } mfs_batch_report;
 
 int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* desc, mfs_batch* out);
@@ -279,6 +281,10 @@ int mfs_batch_get_basis(mfs_batch B, int cell, double* basis);
 int mfs_batch_coarse_matrices(mfs_batch B, double* Kc);
 /* Per-problem inner iteration counts, 24 per cell. */
 int mfs_batch_iterations(mfs_batch B, int32_t* iters);
+/* Per-problem status of the last solve (24 per cell, any pointer may be
+ * NULL): inner iterations, true FP64 relative residual, flags (bit 0: not
+ * converged, bit 1: initial guess rejected in favour of the zero guess). */
+int mfs_batch_status(mfs_batch B, int32_t* iters, double* relres, int32_t* flags);
 /* Learned-guess glue (all cells must share one subdivision n; device data):
  * features X[cell][tet] = (ln E - mu) / sigma (FP32, ncells*6n^3);
  * targets T[cell][(v*3+a)*24+q] = basis - hat interpolant (FP32, interior);
@@ -286,6 +292,19 @@ int mfs_batch_iterations(mfs_batch B, int32_t* iters);
 int mfs_batch_features(mfs_batch B, double mu, double sigma, mfs_vec X);
 int mfs_batch_targets(mfs_batch B, mfs_vec T);
 int mfs_batch_guess_from(mfs_batch B, mfs_vec G);
+/* Coarse system of the MsFEM solve (MsFEM/MsFEM.h:571-1062 coarse
+ * portrait, scatter, elimination): the cells must tile a box on a grid of
+ * spacing cell_h; Kc from mfs_batch_coarse_matrices (host), load = 3 per
+ * coarse node; fixed coarse nodes with 3 values each.  Creates a BSR3 matrix
+ * and the right-hand side / initial vector (owned by the caller). */
+int mfs_msfem_coarse_system(mfs_batch B, const double* Kc, const double* load, int64_t n_fixed,
+                            const int32_t* fixed_nodes, const double* fixed_values, mfs_mat* A, mfs_vec* b,
+                            mfs_vec* x0);
+/* Coarse load Phi^T f of a uniform traction t[3] on the top face of the box. */
+int mfs_msfem_top_traction(mfs_batch B, const double* t, double* load);
+/* Fine displacement of one cell, u = sum Phi u_coarse (MechanicalMain.cpp:
+ * 2176-2181), full fine vector 3*(n+1)^3. */
+int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u_fine);
 /* Candidate launch shapes tried by the autotuner and the time of each. */
 int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* desc, int* best_wg, double* best_ms);
 

@@ -79,7 +79,8 @@ class BatchSolveDesc(C.Structure):
 
 class BatchReport(C.Structure):
     _fields_ = [("seconds", C.c_double), ("max_iterations", C.c_int), ("mean_iterations", C.c_double),
-                ("max_true_relres", C.c_double), ("n_not_converged", C.c_int), ("n_fallback", C.c_int)]
+                ("max_true_relres", C.c_double), ("n_not_converged", C.c_int), ("n_fallback", C.c_int),
+                ("n_precision_fallback", C.c_int)]
 
 
 class MlpDesc(C.Structure):
@@ -155,6 +156,7 @@ _sig("mfs_batch_set_guess", C.c_int, [vp, c_dp])
 _sig("mfs_batch_get_basis", C.c_int, [vp, C.c_int, c_dp])
 _sig("mfs_batch_coarse_matrices", C.c_int, [vp, c_dp])
 _sig("mfs_batch_iterations", C.c_int, [vp, c_i32p])
+_sig("mfs_batch_status", C.c_int, [vp, c_i32p, c_dp, c_i32p])
 _sig("mfs_batch_autotune", C.c_int, [vp, C.POINTER(BatchSolveDesc), C.POINTER(C.c_int), c_dp])
 _sig("mfs_mlp_create", C.c_int, [vp, C.POINTER(MlpDesc), C.POINTER(vp)])
 _sig("mfs_mlp_destroy", C.c_int, [vp])
@@ -169,6 +171,10 @@ _sig("mfs_mlp_forward_vec", C.c_int, [vp, C.c_int64, vp, vp, C.c_int])
 _sig("mfs_batch_features", C.c_int, [vp, C.c_double, C.c_double, vp])
 _sig("mfs_batch_targets", C.c_int, [vp, vp])
 _sig("mfs_batch_guess_from", C.c_int, [vp, vp])
+_sig("mfs_msfem_coarse_system", C.c_int, [vp, c_dp, c_dp, C.c_int64, c_i32p, c_dp, C.POINTER(vp), C.POINTER(vp),
+                                          C.POINTER(vp)])
+_sig("mfs_msfem_top_traction", C.c_int, [vp, c_dp, c_dp])
+_sig("mfs_msfem_downscale", C.c_int, [vp, c_dp, C.c_int, c_dp])
 
 
 def ptr(a, ctype):
@@ -300,10 +306,10 @@ class Context:
 
 
 class Vec:
-    def __init__(self, ctx, n, prec=F64, data=None, borrowed=None):
+    def __init__(self, ctx, n, prec=F64, data=None, borrowed=None, owned=False):
         self.ctx = ctx
         if borrowed is not None:
-            self.h, self.owned = borrowed, False
+            self.h, self.owned = borrowed, owned
         else:
             h = vp()
             _chk(lib.mfs_vec_create(ctx.h, n, prec, C.byref(h)), "mfs_vec_create")
@@ -541,6 +547,7 @@ class BatchResult:
     max_true_relres: float
     n_not_converged: int
     n_fallback: int
+    n_precision_fallback: int
     status: int
 
 
@@ -581,7 +588,7 @@ class Batch:
         if check:
             _chk(code, "mfs_batch_solve")
         return BatchResult(rep.seconds, rep.max_iterations, rep.mean_iterations, rep.max_true_relres,
-                           rep.n_not_converged, rep.n_fallback, code)
+                           rep.n_not_converged, rep.n_fallback, rep.n_precision_fallback, code)
 
     def autotune(self, **kw):
         d = batch_solve_desc(**kw)
@@ -617,6 +624,37 @@ class Batch:
 
     def guess_from(self, G=None):
         _chk(lib.mfs_batch_guess_from(self.h, G.h if G is not None else None), "mfs_batch_guess_from")
+
+    def coarse_system(self, Kc, load, fixed_nodes=(), fixed_values=None):
+        Kc = f64(Kc)
+        load = f64(load).ravel()
+        fixed_nodes = i32(fixed_nodes)
+        fixed_values = f64(fixed_values if fixed_values is not None else np.zeros((len(fixed_nodes), 3))).ravel()
+        A, b, x = vp(), vp(), vp()
+        _chk(lib.mfs_msfem_coarse_system(self.h, ptr(Kc, C.c_double), ptr(load, C.c_double), len(fixed_nodes),
+                                         ptr(fixed_nodes, C.c_int32), ptr(fixed_values, C.c_double), C.byref(A),
+                                         C.byref(b), C.byref(x)), "mfs_msfem_coarse_system")
+        return Mat(self.ctx, A), Vec(self.ctx, 0, borrowed=b, owned=True), Vec(self.ctx, 0, borrowed=x, owned=True)
+
+    def top_traction(self, t, n_coarse_nodes):
+        t = f64(t)
+        out = np.empty(3 * n_coarse_nodes)
+        _chk(lib.mfs_msfem_top_traction(self.h, ptr(t, C.c_double), ptr(out, C.c_double)), "top_traction")
+        return out
+
+    def downscale(self, u_coarse, cell):
+        u_coarse = f64(u_coarse).ravel()
+        _, nt = self.n_interior(cell)
+        out = np.empty(nt)
+        _chk(lib.mfs_msfem_downscale(self.h, ptr(u_coarse, C.c_double), cell, ptr(out, C.c_double)), "downscale")
+        return out
+
+    def status(self):
+        it = np.empty((self.ncells, 24), dtype=np.int32)
+        rr = np.empty((self.ncells, 24))
+        fl = np.empty((self.ncells, 24), dtype=np.int32)
+        _chk(lib.mfs_batch_status(self.h, ptr(it, C.c_int32), ptr(rr, C.c_double), ptr(fl, C.c_int32)), "status")
+        return it, rr, fl
 
     def iterations(self):
         out = np.empty((self.ncells, 24), dtype=np.int32)

@@ -81,36 +81,13 @@ struct SolveArgs {
   int* flags;            // [cell][24]: bit0 not converged, bit1 guess rejected
   double tol, inner_tol;
   int maxit;
+  int inner_cap;  // iterations one inner solve may take before the RHS is
+                  // declared failed in this precision (FP32: handed to FP64)
   int use_guess;
 };
 
 __device__ __forceinline__ int interior_index(int i, int j, int k, int m) {
   return (i - 1) + m * ((j - 1) + m * (k - 1));
-}
-
-// Per-RHS block reduction of NV*RG values laid out v[k][r]; after the call,
-// out[k][r] holds totals (all threads).  blockDim multiple of 64, <= 1024.
-template <int NV>
-__device__ __forceinline__ void block_reduce_rg(double (&v)[NV][RG], double* lds /* [16][NV*RG] */,
-                                                double (&out)[NV][RG]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; k++)
-#pragma unroll
-    for (int r = 0; r < RG; r++) {
-      double s = wave_sum(v[k][r]);
-      if (lane == 0) lds[wave * NV * RG + k * RG + r] = s;
-    }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++)
-#pragma unroll
-    for (int r = 0; r < RG; r++) {
-      double s = 0;
-      for (int w = 0; w < nw; w++) s += lds[w * NV * RG + k * RG + r];
-      out[k][r] = s;
-    }
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -280,22 +257,55 @@ __device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint,
 // ---------------------------------------------------------------------------
 // Persistent batched solve.  T = inner precision (float: FP32 inner with FP64
 // refinement; double: plain FP64 PCG).  Dynamic LDS: p [nint][3][RG] of T.
+// Per-RHS scalars (norms, alpha, beta, stop flags) are uniform across the
+// workgroup and live in LDS; threads own interior nodes (all RG RHS each).
 // ---------------------------------------------------------------------------
+struct RhsState {
+  double bb[RG], rr64[RG], prev[RG], scale[RG], itol2[RG], rz[RG], rr[RG];
+  double alpha[RG], beta[RG];
+  int done[RG], failed[RG], act[RG], it[RG];
+  int all_done, any_act, guess_any;
+};
+
+// Sum NV*RG per-thread partials v[k][r] over the workgroup into tot[k][r]
+// (LDS, valid after the call in every thread).
+template <int NV>
+__device__ __forceinline__ void wg_sum_rg(double (&v)[NV][RG], double* part /* [16][NV*RG] */,
+                                          double* tot /* [NV*RG] */) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+      const double s = wave_sum(v[k][r]);
+      if (lane == 0) part[wave * NV * RG + k * RG + r] = s;
+    }
+  __syncthreads();
+  if (threadIdx.x < NV * RG) {
+    double s = 0;
+    for (int w = 0; w < nw; w++) s += part[w * NV * RG + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_batch_solve(SolveArgs A) {
+__global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* P = reinterpret_cast<T*>(smem_raw);
-  __shared__ double red[16 * 3 * RG];
+  __shared__ double part[16 * 2 * RG];
+  __shared__ double tot[2 * RG];
+  __shared__ RhsState S;
   __shared__ int task_s;
   const int tid = threadIdx.x, nth = blockDim.x;
-  T* R = (std::is_same<T, float>::value ? (T*)A.scr32 : (T*)A.scr64) + (int64_t)blockIdx.x * 3 * A.scr_stride;
+  constexpr bool fp64 = std::is_same<T, double>::value;
+  T* R = (fp64 ? (T*)A.scr64 : (T*)A.scr32) + (int64_t)blockIdx.x * 3 * A.scr_stride;
   T* D = R + A.scr_stride;  // inner correction
   T* Q = D + A.scr_stride;  // A p
   for (;;) {
     if (tid == 0) task_s = (int)__hip_atomic_fetch_add(A.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int task = task_s;
-    __syncthreads();
     if (task >= A.ntasks) break;
     const int cell = A.task_cell[task], grp = A.task_group[task];
     const Cell C = A.cells[cell];
@@ -304,219 +314,261 @@ __global__ __launch_bounds__(1024) void k_batch_solve(SolveArgs A) {
     const double* K64 = A.K64 + C.int0 * (NS * 9);
     const double* b64 = A.b64 + C.int0 * (3 * NB);
     double* x64 = A.x64 + C.int0 * (3 * NB);
-    const T* dinv = std::is_same<T, float>::value ? (const T*)(A.dinv32 + C.int0 * 3) : (const T*)(A.dinv64 + C.int0 * 3);
+    const T* dinv = fp64 ? (const T*)(A.dinv64 + C.int0 * 3) : (const T*)(A.dinv32 + C.int0 * 3);
     const int q0 = grp * RG;
 
-    // ||b||^2 per RHS; optional guess check against the zero guess
-    double bb[RG], rr64[RG];
+    // ||b||^2 per RHS
     {
-      double v2[1][RG];
-      for (int r = 0; r < RG; r++) v2[0][r] = 0;
+      double w[1][RG];
+#pragma unroll
+      for (int r = 0; r < RG; r++) w[0][r] = 0;
       for (int v = tid; v < nint; v += nth)
         for (int a = 0; a < 3; a++)
+#pragma unroll
           for (int r = 0; r < RG; r++) {
             const double bv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
-            v2[0][r] += bv * bv;
+            w[0][r] += bv * bv;
           }
-      double o[1][RG];
-      block_reduce_rg<1>(v2, red, o);
-      for (int r = 0; r < RG; r++) bb[r] = o[0][r] > 0 ? o[0][r] : 1.0;
+      wg_sum_rg<1>(w, part, tot);
+      if (tid < RG) {
+        S.bb[tid] = tot[tid] > 0 ? tot[tid] : 1.0;
+        S.done[tid] = 0;
+        S.failed[tid] = 0;
+        S.it[tid] = 0;
+        S.prev[tid] = 1e300;
+      }
     }
     if (!A.use_guess)
       for (int v = tid; v < nint; v += nth)
         for (int a = 0; a < 3; a++)
+#pragma unroll
           for (int r = 0; r < RG; r++) x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
     __syncthreads();
-    int it_total[RG];
-    bool done[RG];
-    for (int r = 0; r < RG; r++) {
-      it_total[r] = 0;
-      done[r] = false;
-    }
     bool guess_checked = !A.use_guess;
     for (int outer = 0; outer < 64; outer++) {
-      // true residual r64 = b - K64 x64 (FP64), scaled copy into R (T)
-      double v2[1][RG];
-      for (int r = 0; r < RG; r++) v2[0][r] = 0;
-      for (int v = tid; v < nint; v += nth) {
-        const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
-        double acc[3][RG];
-        auto ld = [&](int u, int b, double (&pv)[RG]) {
+      // true residual r64 = b - K64 x64 (FP64) into R (unscaled)
+      {
+        double w[1][RG];
 #pragma unroll
-          for (int r = 0; r < RG; r++) pv[r] = x64[(int64_t)u * 3 * NB + b * NB + q0 + r];
-        };
-        stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
-        for (int a = 0; a < 3; a++)
-          for (int r = 0; r < RG; r++) {
-            const double rv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r] - acc[a][r];
-            v2[0][r] += rv * rv;
-            R[((int64_t)v * 3 + a) * RG + r] = (T)rv;  // unscaled for now
-          }
-      }
-      double o[1][RG];
-      block_reduce_rg<1>(v2, red, o);
-      for (int r = 0; r < RG; r++) rr64[r] = o[0][r];
-      if (!guess_checked) {
-        // reject a guess whose residual exceeds the zero guess's (= ||b||)
-        guess_checked = true;
-        bool reject[RG];
-        bool any = false;
-        for (int r = 0; r < RG; r++) {
-          reject[r] = !(rr64[r] <= bb[r]);
-          any |= reject[r];
+        for (int r = 0; r < RG; r++) w[0][r] = 0;
+        for (int v = tid; v < nint; v += nth) {
+          const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
+          double acc[3][RG];
+          auto ld = [&](int u, int b, double (&pv)[RG]) {
+#pragma unroll
+            for (int r = 0; r < RG; r++) pv[r] = x64[(int64_t)u * 3 * NB + b * NB + q0 + r];
+          };
+          stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int r = 0; r < RG; r++) {
+              const double rv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r] - acc[a][r];
+              w[0][r] += rv * rv;
+              R[((int64_t)v * 3 + a) * RG + r] = (T)rv;
+            }
         }
-        if (any) {
+        wg_sum_rg<1>(w, part, tot);
+      }
+      if (!guess_checked) {
+        // a guess whose residual exceeds the zero guess's (= ||b||) is dropped
+        guess_checked = true;
+        if (tid == 0) {
+          int any = 0;
+          for (int r = 0; r < RG; r++) any |= !(tot[r] <= S.bb[r]);
+          S.guess_any = any;
+        }
+        __syncthreads();
+        if (S.guess_any) {
           for (int v = tid; v < nint; v += nth)
             for (int a = 0; a < 3; a++)
+#pragma unroll
               for (int r = 0; r < RG; r++)
-                if (reject[r]) {
+                if (!(tot[r] <= S.bb[r])) {
                   x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
                   R[((int64_t)v * 3 + a) * RG + r] = (T)b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
                 }
-          for (int r = 0; r < RG; r++)
-            if (reject[r]) rr64[r] = bb[r];
-          if (tid == 0)
-            for (int r = 0; r < RG; r++)
-              if (reject[r]) A.flags[cell * NB + q0 + r] |= 2;
+          __syncthreads();
+          if (tid < RG && !(tot[tid] <= S.bb[tid])) {
+            A.flags[cell * NB + q0 + tid] |= 2;
+            tot[tid] = S.bb[tid];
+          }
           __syncthreads();
         }
       }
-      bool all = true;
-      double scale[RG];
-      for (int r = 0; r < RG; r++) {
-        done[r] = done[r] || (rr64[r] <= A.tol * A.tol * bb[r]) || !(rr64[r] == rr64[r]);
-        all = all && done[r];
-        scale[r] = rr64[r] > 0 ? sqrt(rr64[r]) : 1.0;
-      }
-      int maxit_left = 0;
-      for (int r = 0; r < RG; r++) maxit_left = max(maxit_left, done[r] ? 0 : A.maxit - it_total[r]);
-      if (all || maxit_left <= 0) break;
-      // inner: solve K d = rhat with rhat = r / scale, to inner tolerance
-      // (plain FP64 mode: one inner solve to tol itself).
-      const bool fp64 = std::is_same<T, double>::value;
-      double itol2[RG];
-      for (int r = 0; r < RG; r++) {
-        const double rel = sqrt(rr64[r] / bb[r]);
+      if (tid < RG) {
+        const int r = tid;
+        const double rr64 = tot[r];
+        S.rr64[r] = rr64;
+        // an outer step that did not reduce the true residual means the inner
+        // precision cannot solve this problem: stop it and report it
+        if (!S.done[r] && outer > 0 && !(rr64 < 0.9 * S.prev[r])) S.failed[r] = 1;
+        S.prev[r] = rr64;
+        S.done[r] = S.done[r] || S.failed[r] || (rr64 <= A.tol * A.tol * S.bb[r]) || !(rr64 == rr64);
+        S.scale[r] = rr64 > 0 ? sqrt(rr64) : 1.0;
+        const double rel = sqrt(rr64 / S.bb[r]);
         double it = fp64 ? A.tol / rel : fmax(A.inner_tol, 0.5 * A.tol / rel);
         it = fmin(it, 0.5);
-        itol2[r] = it * it;
+        S.itol2[r] = it * it;
       }
+      __syncthreads();
+      if (tid == 0) {
+        int all = 1, left = 0;
+        for (int r = 0; r < RG; r++) {
+          all &= S.done[r];
+          if (!S.done[r]) left = max(left, A.maxit - S.it[r]);
+        }
+        S.all_done = all || left <= 0;
+      }
+      __syncthreads();
+      if (S.all_done) break;
       // start: R <- R/scale, D = 0, P = dinv R, rz, rr
-      double rz[RG], rr[RG];
       {
         double w[2][RG];
+#pragma unroll
         for (int r = 0; r < RG; r++) w[0][r] = w[1][r] = 0;
         for (int v = tid; v < nint; v += nth)
-          for (int a = 0; a < 3; a++)
+          for (int a = 0; a < 3; a++) {
+            const T dv = dinv[v * 3 + a];
+#pragma unroll
             for (int r = 0; r < RG; r++) {
               const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              const T rv = done[r] ? (T)0 : (T)((double)R[e] / scale[r]);
+              const T rv = S.done[r] ? (T)0 : (T)((double)R[e] / S.scale[r]);
               R[e] = rv;
               D[e] = 0;
-              const T z = dinv[v * 3 + a] * rv;
+              const T z = dv * rv;
               P[e] = z;
               w[0][r] += (double)(rv * z);
               w[1][r] += (double)(rv * rv);
             }
-        double o2[2][RG];
-        block_reduce_rg<2>(w, red, o2);
-        for (int r = 0; r < RG; r++) {
-          rz[r] = o2[0][r];
-          rr[r] = o2[1][r];
+          }
+        wg_sum_rg<2>(w, part, tot);
+        if (tid < RG) {
+          S.rz[tid] = tot[tid];
+          S.rr[tid] = tot[RG + tid];
+          S.act[tid] = !S.done[tid] && tot[RG + tid] > S.itol2[tid];
         }
+        __syncthreads();
       }
-      bool act[RG];
       int inner_it = 0;
-      for (int r = 0; r < RG; r++) act[r] = !done[r] && rr[r] > itol2[r];
       for (;;) {
-        bool any = false;
-        for (int r = 0; r < RG; r++) any |= act[r];
-        if (!any || inner_it >= maxit_left) break;
-        // q = K p, pq
-        double w[1][RG];
-        for (int r = 0; r < RG; r++) w[0][r] = 0;
-        for (int v = tid; v < nint; v += nth) {
-          const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
-          T acc[3][RG];
-          auto ld = [&](int u, int b, T (&pv)[RG]) {
-#pragma unroll
-            for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
-          };
-          if constexpr (std::is_same<T, float>::value)
-            stencil_row<float, float>(K32, nint, v, i, j, k, m, ld, acc);
-          else
-            stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
-          for (int a = 0; a < 3; a++)
-            for (int r = 0; r < RG; r++) {
-              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              Q[e] = acc[a][r];
-              w[0][r] += (double)(acc[a][r] * P[e]);
-            }
+        if (tid == 0) {
+          int any = 0;
+          for (int r = 0; r < RG; r++) any |= S.act[r];
+          if (any && inner_it >= A.inner_cap) {
+            for (int r = 0; r < RG; r++)
+              if (S.act[r]) {
+                S.failed[r] = 1;
+                S.act[r] = 0;
+              }
+            any = 0;
+          }
+          S.any_act = any;
         }
-        double o1[1][RG];
-        block_reduce_rg<1>(w, red, o1);
-        T alpha[RG];
-        for (int r = 0; r < RG; r++) {
-          const double pq = o1[0][r];
-          alpha[r] = (act[r] && pq != 0.0) ? (T)(rz[r] / pq) : (T)0;
+        __syncthreads();
+        if (!S.any_act) break;
+        // q = K p, pq
+        {
+          double w[1][RG];
+#pragma unroll
+          for (int r = 0; r < RG; r++) w[0][r] = 0;
+          for (int v = tid; v < nint; v += nth) {
+            const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
+            T acc[3][RG];
+            auto ld = [&](int u, int b, T (&pv)[RG]) {
+#pragma unroll
+              for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
+            };
+            if constexpr (fp64)
+              stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
+            else
+              stencil_row<float, float>(K32, nint, v, i, j, k, m, ld, acc);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+              for (int r = 0; r < RG; r++) {
+                const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+                Q[e] = acc[a][r];
+                w[0][r] += (double)(acc[a][r] * P[e]);
+              }
+          }
+          wg_sum_rg<1>(w, part, tot);
+          if (tid < RG) {
+            const double pq = tot[tid];
+            S.alpha[tid] = (S.act[tid] && pq != 0.0) ? S.rz[tid] / pq : 0.0;
+          }
+          __syncthreads();
         }
         // x += alpha p, r -= alpha q, rz', rr'
-        double w2[2][RG];
-        for (int r = 0; r < RG; r++) w2[0][r] = w2[1][r] = 0;
-        for (int v = tid; v < nint; v += nth)
-          for (int a = 0; a < 3; a++)
-            for (int r = 0; r < RG; r++) {
-              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              D[e] += alpha[r] * P[e];
-              const T rv = R[e] - alpha[r] * Q[e];
-              R[e] = rv;
-              const T z = dinv[v * 3 + a] * rv;
-              w2[0][r] += (double)(rv * z);
-              w2[1][r] += (double)(rv * rv);
+        {
+          double w[2][RG];
+#pragma unroll
+          for (int r = 0; r < RG; r++) w[0][r] = w[1][r] = 0;
+          for (int v = tid; v < nint; v += nth)
+            for (int a = 0; a < 3; a++) {
+              const T dv = dinv[v * 3 + a];
+#pragma unroll
+              for (int r = 0; r < RG; r++) {
+                const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+                const T al = (T)S.alpha[r];
+                D[e] += al * P[e];
+                const T rv = R[e] - al * Q[e];
+                R[e] = rv;
+                const T z = dv * rv;
+                w[0][r] += (double)(rv * z);
+                w[1][r] += (double)(rv * rv);
+              }
             }
-        double o2[2][RG];
-        block_reduce_rg<2>(w2, red, o2);
-        T beta[RG];
-        for (int r = 0; r < RG; r++) {
-          beta[r] = 0;
-          if (!act[r]) continue;
-          it_total[r]++;
-          const double rz2 = o2[0][r], rr2 = o2[1][r];
-          if (!(rr2 == rr2) || rr2 <= itol2[r] || it_total[r] >= A.maxit) {
-            act[r] = false;
-          } else {
-            beta[r] = (T)(rz2 / rz[r]);
-            rz[r] = rz2;
+          wg_sum_rg<2>(w, part, tot);
+          if (tid < RG) {
+            const int r = tid;
+            S.beta[r] = 0.0;
+            if (S.act[r]) {
+              S.it[r]++;
+              const double rz2 = tot[r], rr2 = tot[RG + r];
+              if (!(rr2 == rr2) || rr2 <= S.itol2[r] || S.it[r] >= A.maxit) {
+                S.act[r] = 0;
+              } else {
+                S.beta[r] = rz2 / S.rz[r];
+                S.rz[r] = rz2;
+              }
+              S.rr[r] = rr2;
+            }
           }
-          rr[r] = rr2;
+          __syncthreads();
         }
-        // p = z + beta p (only active RHS change)
+        // p = z + beta p (active RHS only)
         for (int v = tid; v < nint; v += nth)
-          for (int a = 0; a < 3; a++)
+          for (int a = 0; a < 3; a++) {
+            const T dv = dinv[v * 3 + a];
+#pragma unroll
             for (int r = 0; r < RG; r++) {
-              if (!act[r]) continue;
+              if (!S.act[r]) continue;
               const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              P[e] = dinv[v * 3 + a] * R[e] + beta[r] * P[e];
+              P[e] = dv * R[e] + (T)S.beta[r] * P[e];
             }
+          }
         __syncthreads();
         inner_it++;
       }
-      // x64 += scale * d
+      // x64 += scale * d (not for problems whose inner solve failed)
       for (int v = tid; v < nint; v += nth)
         for (int a = 0; a < 3; a++)
+#pragma unroll
           for (int r = 0; r < RG; r++)
-            if (!done[r]) {
+            if (!S.done[r] && !S.failed[r]) {
               const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              x64[(int64_t)v * 3 * NB + a * NB + q0 + r] += scale[r] * (double)D[e];
+              x64[(int64_t)v * 3 * NB + a * NB + q0 + r] += S.scale[r] * (double)D[e];
             }
       __syncthreads();
       if (inner_it == 0) break;  // nothing more can be done
     }
     if (tid < RG) {
       const int r = tid;
-      A.iters[cell * NB + q0 + r] = it_total[r];
-      A.relres[cell * NB + q0 + r] = sqrt(rr64[r] / bb[r]);
-      if (!(rr64[r] <= A.tol * A.tol * bb[r])) A.flags[cell * NB + q0 + r] |= 1;
+      A.iters[cell * NB + q0 + r] = S.it[r];
+      A.relres[cell * NB + q0 + r] = sqrt(S.rr64[r] / S.bb[r]);
+      if (!(S.rr64[r] <= A.tol * A.tol * S.bb[r])) A.flags[cell * NB + q0 + r] |= 1;
+      if (S.failed[r]) A.flags[cell * NB + q0 + r] |= 4;
     }
     __syncthreads();
   }
@@ -837,7 +889,8 @@ static int batch_scratch(mfs_batch B, int precision, int per_cu) {
 
 // Launch reset + persistent solve on the context stream (capturable: no
 // allocation, no memset; the queue and flags are cleared by a kernel).
-static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu) {
+static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu, const int* task_cell = nullptr,
+                        const int* task_group = nullptr, int ntasks = -1, bool keep_flags = false) {
   hipStream_t s = B->ctx->stream;
   const int grid = B->ctx->cu_count * per_cu;
   const size_t lds = lds_bytes(B, d->precision);
@@ -849,9 +902,9 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   }
   SolveArgs A;
   A.cells = B->d_cells;
-  A.task_cell = B->d_task_cell;
-  A.task_group = B->d_task_group;
-  A.ntasks = B->ntasks;
+  A.task_cell = task_cell ? task_cell : B->d_task_cell;
+  A.task_group = task_group ? task_group : B->d_task_group;
+  A.ntasks = ntasks >= 0 ? ntasks : B->ntasks;
   A.queue = B->d_queue;
   A.K32 = B->d_K32;
   A.K64 = B->d_K64;
@@ -868,8 +921,10 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   A.tol = d->tol;
   A.inner_tol = d->inner_tol > 0 ? d->inner_tol : 1e-4;
   A.maxit = d->maxit > 0 ? d->maxit : 20000;
+  A.inner_cap = d->precision == MFS_PREC_F64 ? A.maxit : std::min(A.maxit, 1000);
   A.use_guess = d->use_guess;
-  k_batch_reset<<<std::max(1, (B->ncells * NB + 255) / 256), 256, 0, s>>>(B->d_queue, B->d_flags, B->ncells * NB);
+  k_batch_reset<<<std::max(1, (B->ncells * NB + 255) / 256), 256, 0, s>>>(B->d_queue, B->d_flags,
+                                                                            keep_flags ? 0 : B->ncells * NB);
   if (d->precision == MFS_PREC_F64)
     k_batch_solve<double><<<grid, wg, lds, s>>>(A);
   else
@@ -883,8 +938,8 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
   MFS_CHECK_ARG(B->assembled, "mfs_batch_solve: call mfs_batch_assemble first");
   MFS_CHECK_ARG(d->precision == MFS_PREC_F64 || d->precision == MFS_PREC_MIXED, "mfs_batch_solve: bad precision");
   MFS_CHECK_ARG(d->tol > 0, "mfs_batch_solve: tol must be positive");
-  MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 1024),
-                "mfs_batch_solve: wg_size must be a multiple of 64 in 64..1024");
+  MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 256),
+                "mfs_batch_solve: wg_size must be a multiple of 64 in 64..256");
   if (lds_bytes(B, d->precision) > 160 * 1024 - 8192) {
     set_error("mfs_batch_solve: cells too large for the LDS-resident search directions (FP64 inner needs n <= 10)");
     return MFS_ERR_UNSUPPORTED;
@@ -966,9 +1021,65 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
   }
   float ms = 0;
   MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  // Problems the FP32 inner iteration could not solve (the reference's
+  // indefinite local operators, see k_batch_solve) are solved again from the
+  // zero guess in FP64, as the reference falls back to another solver
+  // (FEM/FEM.h:3726-3737).
+  rep->n_precision_fallback = 0;
+  if (d->precision == MFS_PREC_MIXED) {
+    const int nq = B->ncells * NB;
+    std::vector<int> fl(nq);
+    MFS_HIP(hipMemcpyAsync(fl.data(), B->d_flags, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipStreamSynchronize(s));
+    std::vector<int> tc, tg;
+    for (int c = 0; c < B->ncells; c++)
+      for (int g = 0; g < NB / RG; g++) {
+        bool f = false;
+        for (int r = 0; r < RG; r++) f |= (fl[c * NB + g * RG + r] & 4) != 0;
+        if (f) {
+          tc.push_back(c);
+          tg.push_back(g);
+        }
+      }
+    if (!tc.empty()) {
+      for (int q = 0; q < nq; q++) rep->n_precision_fallback += (fl[q] >> 2) & 1;
+      int *dtc = nullptr, *dtg = nullptr;
+      MFS_TRY(dmalloc((void**)&dtc, sizeof(int) * tc.size()));
+      MFS_TRY(dmalloc((void**)&dtg, sizeof(int) * tg.size()));
+      MFS_HIP(hipMemcpyAsync(dtc, tc.data(), sizeof(int) * tc.size(), hipMemcpyHostToDevice, s));
+      MFS_HIP(hipMemcpyAsync(dtg, tg.data(), sizeof(int) * tg.size(), hipMemcpyHostToDevice, s));
+      mfs_batch_solve_desc d64 = *d;
+      d64.precision = MFS_PREC_F64;
+      d64.use_guess = 0;
+      int wg64, pc64;
+      MFS_TRY(pick_shape(B, &d64, &wg64, &pc64));
+      MFS_TRY(batch_scratch(B, MFS_PREC_F64, pc64));
+      // clear the per-problem flags of the re-solved tasks only
+      std::vector<int> fl2(fl);
+      for (size_t t = 0; t < tc.size(); t++)
+        for (int r = 0; r < RG; r++) fl2[tc[t] * NB + tg[t] * RG + r] = 0;
+      MFS_HIP(hipMemcpyAsync(B->d_flags, fl2.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s));
+      hipEvent_t f0, f1;
+      MFS_HIP(hipEventCreate(&f0));
+      MFS_HIP(hipEventCreate(&f1));
+      MFS_HIP(hipEventRecord(f0, s));
+      MFS_TRY(batch_launch(B, &d64, wg64, pc64, dtc, dtg, (int)tc.size(), true));
+      MFS_HIP(hipEventRecord(f1, s));
+      MFS_HIP(hipEventSynchronize(f1));
+      float ms2 = 0;
+      MFS_HIP(hipEventElapsedTime(&ms2, f0, f1));
+      ms += ms2;
+      (void)hipEventDestroy(f0);
+      (void)hipEventDestroy(f1);
+      dfree(dtc);
+      dfree(dtg);
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  const int nprec = rep->n_precision_fallback;
   MFS_TRY(batch_collect(B, rep));
+  rep->n_precision_fallback = nprec;
   rep->seconds = ms * 1e-3;
   if (rep->n_not_converged > 0) {
     set_error("mfs_batch_solve: " + std::to_string(rep->n_not_converged) + " local problems did not converge");
@@ -979,7 +1090,7 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
 
 extern "C" int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* d, int* best_wg, double* best_ms) {
   MFS_TRY(batch_check_desc(B, d));
-  const int shapes[][2] = {{128, 4}, {256, 2}, {256, 1}, {512, 1}, {1024, 1}};
+  const int shapes[][2] = {{128, 4}, {128, 2}, {256, 2}, {256, 1}, {192, 2}};
   double best = 1e30;
   int bw = 256, bp = 2;
   const size_t lds = lds_bytes(B, d->precision) + 4096;
@@ -1142,4 +1253,204 @@ extern "C" int mfs_batch_targets(mfs_batch B, mfs_vec T) {
 extern "C" int mfs_batch_guess_from(mfs_batch B, mfs_vec G) {
   MFS_CHECK_ARG(B, "mfs_batch_guess_from: bad argument");
   return targets_guess(B, G, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse MsFEM system (MsFEM/MsFEM.h:571-1062): the cells form a structured
+// ncx*ncy*ncz grid (cell c at grid position given by its origin / h), coarse
+// node (I,J,K) = I + (ncx+1)(J + (ncy+1)K).  The 24x24 Galerkin matrices are
+// scattered in cell order into a BSR3 matrix with the 27-point coarse
+// pattern; fixed coarse nodes are eliminated (FEM.h:505-619 rule).
+// ---------------------------------------------------------------------------
+static int coarse_grid(mfs_batch B, int* ncx, int* ncy, int* ncz, std::vector<int>& gi) {
+  int mx = 0, my = 0, mz = 0;
+  gi.resize(3 * B->ncells);
+  for (int c = 0; c < B->ncells; c++) {
+    const Cell& C = B->cells[c];
+    const double o[3] = {C.ox, C.oy, C.oz};
+    for (int a = 0; a < 3; a++) {
+      const double f = o[a] / B->h;
+      const int r = (int)std::lround(f);
+      if (std::fabs(f - r) > 1e-6 || r < 0) {
+        set_error("mfs_msfem_coarse: cell origins do not lie on a grid of spacing cell_h");
+        return MFS_ERR_ARG;
+      }
+      gi[3 * c + a] = r;
+    }
+    mx = std::max(mx, gi[3 * c] + 1);
+    my = std::max(my, gi[3 * c + 1] + 1);
+    mz = std::max(mz, gi[3 * c + 2] + 1);
+  }
+  if ((int64_t)mx * my * mz != B->ncells) {
+    set_error("mfs_msfem_coarse: cells do not tile a full box");
+    return MFS_ERR_ARG;
+  }
+  *ncx = mx;
+  *ncy = my;
+  *ncz = mz;
+  return MFS_OK;
+}
+
+extern "C" int mfs_msfem_coarse_system(mfs_batch B, const double* Kc, const double* load, int64_t n_fixed,
+                                       const int32_t* fixed_nodes, const double* fixed_values, mfs_mat* A_out,
+                                       mfs_vec* b_out, mfs_vec* x0_out) {
+  MFS_CHECK_ARG(B && Kc && load && A_out && b_out && x0_out, "mfs_msfem_coarse_system: bad argument");
+  MFS_CHECK_ARG(n_fixed == 0 || (fixed_nodes && fixed_values), "mfs_msfem_coarse_system: fixed arrays missing");
+  int nx, ny, nz;
+  std::vector<int> gi;
+  MFS_TRY(coarse_grid(B, &nx, &ny, &nz, gi));
+  const int sx = nx + 1, sy = ny + 1, sz = nz + 1;
+  const int64_t nn = (int64_t)sx * sy * sz;
+  for (int64_t k = 0; k < n_fixed; k++)
+    MFS_CHECK_ARG(fixed_nodes[k] >= 0 && fixed_nodes[k] < nn, "mfs_msfem_coarse_system: fixed node out of range");
+  // 27-point pattern
+  Pattern P;
+  P.n = nn;
+  P.rowptr.assign(nn + 1, 0);
+  for (int64_t v = 0; v < nn; v++) {
+    const int i = v % sx, j = (v / sx) % sy, k = v / ((int64_t)sx * sy);
+    for (int dk = -1; dk <= 1; dk++)
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++) {
+          const int a = i + di, b = j + dj, c = k + dk;
+          if (a < 0 || b < 0 || c < 0 || a >= sx || b >= sy || c >= sz) continue;
+          P.col.push_back((int32_t)(a + sx * (b + sy * c)));
+        }
+    P.rowptr[v + 1] = (int32_t)P.col.size();
+  }
+  std::vector<double> val(9 * P.col.size(), 0.0);
+  auto slot = [&](int64_t r, int64_t c) -> int64_t {
+    const int32_t* b0 = P.col.data() + P.rowptr[r];
+    const int32_t* e0 = P.col.data() + P.rowptr[r + 1];
+    return std::lower_bound(b0, e0, (int32_t)c) - P.col.data();
+  };
+  for (int c = 0; c < B->ncells; c++) {
+    int64_t node[8];
+    for (int q = 0; q < 8; q++)
+      node[q] = (gi[3 * c] + (q & 1)) + sx * ((gi[3 * c + 1] + ((q >> 1) & 1)) + (int64_t)sy * (gi[3 * c + 2] + (q >> 2)));
+    const double* K = Kc + (int64_t)c * NB * NB;
+    for (int I = 0; I < 8; I++)
+      for (int J = 0; J < 8; J++) {
+        const int64_t s = slot(node[I], node[J]);
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) val[9 * s + 3 * a + b] += K[(3 * I + a) * NB + 3 * J + b];
+      }
+  }
+  // elimination
+  const int64_t N = 3 * nn;
+  std::vector<char> fixed(N, 0);
+  std::vector<double> g(N, 0.0), rhs(load, load + N), x0(N, 0.0);
+  for (int64_t k = 0; k < n_fixed; k++)
+    for (int a = 0; a < 3; a++) {
+      fixed[3 * (int64_t)fixed_nodes[k] + a] = 1;
+      g[3 * (int64_t)fixed_nodes[k] + a] = fixed_values[3 * k + a];
+    }
+  for (int64_t r = 0; r < nn; r++)
+    for (int a = 0; a < 3; a++) {
+      const int64_t i = 3 * r + a;
+      for (int64_t s = P.rowptr[r]; s < P.rowptr[r + 1]; s++)
+        for (int b = 0; b < 3; b++) {
+          const int64_t cj = 3 * (int64_t)P.col[s] + b;
+          double& v = val[9 * s + 3 * a + b];
+          if (fixed[i]) {
+            v = (cj == i) ? 1.0 : 0.0;
+          } else if (fixed[cj]) {
+            rhs[i] -= v * g[cj];
+            v = 0.0;
+          }
+        }
+      if (fixed[i]) {
+        rhs[i] = g[i];
+        x0[i] = g[i];
+      }
+    }
+  mfs_mat A = nullptr;
+  mfs_vec b = nullptr, x = nullptr;
+  int st = mfs_mat_create_bsr3(B->ctx, nn, P.rowptr.data(), P.col.data(), val.data(), &A);
+  if (st == MFS_OK) st = mfs_vec_create(B->ctx, N, MFS_F64, &b);
+  if (st == MFS_OK) st = mfs_vec_upload(b, rhs.data());
+  if (st == MFS_OK) st = mfs_vec_create(B->ctx, N, MFS_F64, &x);
+  if (st == MFS_OK) st = mfs_vec_upload(x, x0.data());
+  if (st != MFS_OK) {
+    mfs_mat_destroy(A);
+    mfs_vec_destroy(b);
+    mfs_vec_destroy(x);
+    return st;
+  }
+  *A_out = A;
+  *b_out = b;
+  *x0_out = x;
+  return MFS_OK;
+}
+
+// Coarse load of a uniform traction t on the top face z = max of the cell
+// grid: load[3*node + d] = sum over fine top triangles (P1, area/3 per
+// vertex) of t_d * hat_node, i.e. Phi^T f with the basis traces.
+extern "C" int mfs_msfem_top_traction(mfs_batch B, const double* t, double* load) {
+  MFS_CHECK_ARG(B && t && load, "mfs_msfem_top_traction: bad argument");
+  int nx, ny, nz;
+  std::vector<int> gi;
+  MFS_TRY(coarse_grid(B, &nx, &ny, &nz, gi));
+  const int sx = nx + 1, sy = ny + 1;
+  const int64_t nn = (int64_t)sx * sy * (nz + 1);
+  std::fill(load, load + 3 * nn, 0.0);
+  for (int c = 0; c < B->ncells; c++) {
+    if (gi[3 * c + 2] != nz - 1) continue;
+    const int n = B->cells[c].n;
+    const double A = 0.5 * (B->h / n) * (B->h / n);  // fine triangle area
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        // the two top triangles of square (i,j) share the (i,j)-(i+1,j+1) diagonal
+        const int tri[2][3][2] = {{{i, j}, {i + 1, j}, {i + 1, j + 1}}, {{i, j}, {i, j + 1}, {i + 1, j + 1}}};
+        for (int tt = 0; tt < 2; tt++)
+          for (int v = 0; v < 3; v++) {
+            const double X = (double)tri[tt][v][0] / n, Y = (double)tri[tt][v][1] / n;
+            for (int q = 4; q < 8; q++) {  // top coarse nodes of the cell
+              const double hat = ((q & 1) ? X : 1 - X) * ((q & 2) ? Y : 1 - Y);
+              const int64_t node =
+                  (gi[3 * c] + (q & 1)) + sx * ((gi[3 * c + 1] + ((q >> 1) & 1)) + (int64_t)sy * nz);
+              for (int d = 0; d < 3; d++) load[3 * node + d] += t[d] * hat * A / 3.0;
+            }
+          }
+      }
+  }
+  return MFS_OK;
+}
+
+// Fine displacement of one cell from coarse nodal values (3 per coarse
+// node): u = sum_{q,d} u_c[node(q)][d] phi_{q,d}, the downscaling the
+// Mechanical driver applies (MechanicalMain.cpp:2176-2181).
+extern "C" int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u_fine) {
+  MFS_CHECK_ARG(B && u_coarse && u_fine && cell >= 0 && cell < B->ncells, "mfs_msfem_downscale: bad argument");
+  int nx, ny, nz;
+  std::vector<int> gi;
+  MFS_TRY(coarse_grid(B, &nx, &ny, &nz, gi));
+  const int sx = nx + 1, sy = ny + 1;
+  const int n = B->cells[cell].n;
+  const int64_t N = 3ll * (n + 1) * (n + 1) * (n + 1);
+  std::vector<double> basis((size_t)NB * N);
+  MFS_TRY(mfs_batch_get_basis(B, cell, basis.data()));
+  std::fill(u_fine, u_fine + N, 0.0);
+  for (int q = 0; q < 8; q++) {
+    const int64_t node = (gi[3 * cell] + (q & 1)) + sx * ((gi[3 * cell + 1] + ((q >> 1) & 1)) +
+                                                         (int64_t)sy * (gi[3 * cell + 2] + (q >> 2)));
+    for (int d = 0; d < 3; d++) {
+      const double w = u_coarse[3 * node + d];
+      if (w == 0.0) continue;
+      const double* ph = basis.data() + (size_t)(3 * q + d) * N;
+      for (int64_t e = 0; e < N; e++) u_fine[e] += w * ph[e];
+    }
+  }
+  return MFS_OK;
+}
+
+extern "C" int mfs_batch_status(mfs_batch B, int32_t* iters, double* relres, int32_t* flags) {
+  MFS_CHECK_ARG(B, "mfs_batch_status: NULL batch");
+  hipStream_t s = B->ctx->stream;
+  const size_t nq = (size_t)NB * B->ncells;
+  if (iters) MFS_HIP(hipMemcpyAsync(iters, B->d_iters, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+  if (relres) MFS_HIP(hipMemcpyAsync(relres, B->d_relres, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+  if (flags) MFS_HIP(hipMemcpyAsync(flags, B->d_flags, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  return MFS_OK;
 }

@@ -104,3 +104,53 @@ def test_config2_full_batch(ctx):
         basis_o, _, _ = orc.msfem_cell(int(n), p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"], tol=1e-12)
         err = np.linalg.norm(B.basis(c) - basis_o) / np.linalg.norm(basis_o)
         assert err < 1e-6, (c, n, err)
+
+
+def test_coarse_system_assembly_and_affine_patch(ctx):
+    """Homogeneous material, affine displacement prescribed on every coarse
+    boundary node: the coarse solve + downscaling reproduces the affine field
+    at every fine node (the reference's analytic check,
+    MechanicalMain.cpp:2223-2300, at the multiscale level)."""
+    nc, n = 2, 4
+    p = problems.msfem_cells(nc, n_fixed=n)
+    p["E"][:] = 1e9
+    B = make_batch(ctx, p)
+    B.solve(precision=mfs.PREC_MIXED, tol=1e-11)
+    Kc = B.coarse_matrices()
+    s = nc + 1
+    cx = np.array([[i, j, k] for k in range(s) for j in range(s) for i in range(s)], float) / nc
+    G = np.array([[0.01, 0.002, 0], [0.001, -0.02, 0.003], [0, 0.004, 0.01]])
+    bnd = np.nonzero((cx == 0).any(1) | (cx == 1).any(1))[0]
+    A, b, x = B.coarse_system(Kc, np.zeros(3 * s ** 3), bnd, cx[bnd] @ G.T)
+    # coarse matrix = scatter of the cell matrices (checked on the rows of the
+    # interior coarse node, which no elimination touches)
+    rp, col, val = A.export_csr()
+    dense = np.zeros((3 * s ** 3, 3 * s ** 3))
+    for r in range(3 * s ** 3):
+        dense[r, col[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+    ref = np.zeros_like(dense)
+    for c in range(nc ** 3):
+        o = (p["origin"][c] * nc).round().astype(int)
+        nodes = [(o[0] + (q & 1)) + s * ((o[1] + ((q >> 1) & 1)) + s * (o[2] + (q >> 2))) for q in range(8)]
+        idx = np.array([3 * v + a for v in nodes for a in range(3)])
+        ref[np.ix_(idx, idx)] += Kc[c]
+    inner = [3 * 13 + a for a in range(3)]  # coarse node (1,1,1)
+    free = np.setdiff1d(np.arange(3 * s ** 3), np.concatenate([3 * bnd + a for a in range(3)]))
+    assert np.allclose(dense[np.ix_(inner, free)], ref[np.ix_(inner, free)], rtol=1e-13,
+                       atol=1e-13 * np.abs(ref).max())
+    rep = mfs.Solver(A, tol=1e-12).run(b, x)
+    uc = x.download()
+    assert np.abs(uc.reshape(-1, 3) - cx @ G.T).max() < 1e-12
+    for c in range(nc ** 3):
+        xf, _ = mfs.mesh_box(n, origin=tuple(p["origin"][c]), h=(p["h"] / n,) * 3)
+        uf = B.downscale(uc, c).reshape(-1, 3)
+        assert np.abs(uf - xf @ G.T).max() < 1e-9
+
+
+def test_coarse_traction_load_total(ctx):
+    nc = 2
+    p = problems.msfem_cells(nc, n_lo=3, n_hi=5)
+    B = make_batch(ctx, p)
+    load = B.top_traction([0.0, 0.0, -1e6], (nc + 1) ** 3)
+    assert load[2::3].sum() == pytest.approx(-1e6, rel=1e-12)
+    assert np.abs(load[0::3]).max() == 0 and np.abs(load[1::3]).max() == 0

@@ -39,6 +39,9 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="mesh hexes per edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-msfem", action="store_true", help="skip the MsFEM batch measurement")
+    ap.add_argument("--no-large", action="store_true", help="skip the 128^3 elastic measurement")
+    ap.add_argument("--large-n", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -136,6 +139,9 @@ def main():
                          "serial dots/sweeps) on the same system: " + sample}
         log(f"[bench] cpu baseline {value:.0f} ms ({sample})")
 
+    msfem = None if args.no_msfem else bench_msfem(mfs, problems, ctx)
+    large = None if args.no_large else bench_large(mfs, problems, ctx, args.large_n)
+
     out = {
         "metric": METRIC,
         "value": round(ms_mixed, 3),
@@ -164,9 +170,129 @@ def main():
                      "kernel": "k_sell_spmv<float,1> (FP32 SELL-64 SpMV)",
                      "bytes_per_launch": spmv_bytes, "avg_us": round(spmv_ms * 1e3, 3)},
         "cpu_baseline": cpu,
+        "msfem_config2": msfem,
+        "elastic_large": large,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_msfem(mfs, problems, ctx):
+    """16^3 coarse cells, n ~ U{6..10}, lognormal E (sigma 1):
+    98,304 local problems to 1e-8.  Mixed batched persistent launch (hipGraph)
+    vs the same launch in plain FP64 vs one-problem-at-a-time FP64 Jacobi-PCG
+    through mfs_solve (sampled cells, extrapolated per problem)."""
+    import numpy as np
+    p = problems.msfem_cells(16)
+    B = mfs.Batch(ctx, p["cell_n"], p["origin"], p["h"], p["E"], p["nu"])
+    B.assemble()
+    B.solve(precision=mfs.PREC_MIXED, tol=1e-8)  # warm-up (and autotuned shape default)
+    res = {}
+    for name, prec in (("mixed_batched", mfs.PREC_MIXED), ("fp64_batched", mfs.PREC_F64)):
+        reps = [B.solve(precision=prec, tol=1e-8, use_graph=True) for _ in range(3)]
+        r = min(reps, key=lambda x: x.seconds)
+        res[name] = {"ms": round(r.seconds * 1e3, 2), "mean_iterations": round(r.mean_iterations, 1),
+                     "max_iterations": r.max_iterations, "max_true_relres": r.max_true_relres,
+                     "not_converged": r.n_not_converged, "fp32_to_fp64_fallbacks": r.n_precision_fallback}
+    # plain FP64 GPU CG, one local problem at a time (sample of cells)
+    sample = [int(np.nonzero(p["cell_n"] == n)[0][0]) for n in (6, 8, 10)]
+    per_problem = []
+    for c in sample:
+        n = int(p["cell_n"][c])
+        xyz, tets = mfs.mesh_box(n, origin=tuple(p["origin"][c]), h=(p["h"] / n,) * 3)
+        s = n + 1
+        ijk = np.array([[i, j, k] for k in range(s) for j in range(s) for i in range(s)])
+        bnd = np.nonzero(((ijk == 0) | (ijk == n)).any(1))[0]
+        X = ijk / n
+        times = []
+        for q in range(24):
+            cc, d = q // 3, q % 3
+            hat = ((X[:, 0] if cc & 1 else 1 - X[:, 0]) * (X[:, 1] if cc & 2 else 1 - X[:, 1]) *
+                   (X[:, 2] if cc & 4 else 1 - X[:, 2]))
+            g = np.zeros((len(bnd), 3))
+            g[:, d] = hat[bnd]
+            S = mfs.Assembly(ctx, mfs.PROBLEM_ELASTIC, xyz, tets, fixed_nodes=bnd, fixed_values=g)
+            S.run(problems.cell_E(p, c), p["nu"])
+            x = mfs.Vec(ctx, S.A.n)
+            x.copy_from(S.x0)
+            rep = mfs.Solver(S.A, precision=mfs.PREC_F64, tol=1e-8).run(S.b, x)
+            times.append(rep.seconds)
+            S.close()
+        per_problem.append((n, float(np.mean(times))))
+    # extrapolate with the batch's cell-size histogram
+    counts = {int(n): int((p["cell_n"] == n).sum()) for n in np.unique(p["cell_n"])}
+    t_of = dict(per_problem)
+    ns = sorted(t_of)
+    est = sum(24 * cnt * float(np.interp(n, ns, [t_of[k] for k in ns])) for n, cnt in counts.items())
+    res["fp64_one_at_a_time_ms_extrapolated"] = round(est * 1e3, 1)
+    res["fp64_one_at_a_time_sample"] = {f"n={n}": round(t * 1e3, 3) for n, t in per_problem}
+    res["speedup_mixed_batched_vs_fp64_one_at_a_time"] = round(est * 1e3 / res["mixed_batched"]["ms"], 1)
+    res["speedup_mixed_batched_vs_fp64_batched"] = round(res["fp64_batched"]["ms"] / res["mixed_batched"]["ms"], 3)
+    res["problems"] = 24 * len(p["cell_n"])
+    B.close()
+    log(f"[bench] msfem: {res}")
+    return res
+
+
+def bench_large(mfs, problems, ctx, n):
+    """n^3 elastic cube, two-phase E in {1e9, 2e11}; SpMV GB/s at
+    HBM scale (FP32 BSR3 working set ~1.3 GB), assembly time, mixed vs FP64
+    Jacobi-PCG, GMRES(30) (textbook convention: restarted GMRES stagnates on
+    the reference convention's indefinite operator)."""
+    import numpy as np
+    t0 = time.time()
+    p = problems.elastic_cube(n, field="grf")
+    S = mfs.Assembly(ctx, mfs.PROBLEM_ELASTIC, p["xyz"], p["tets"], material=p["material"], renumber=True,
+                     fixed_nodes=p["fixed_nodes"], fixed_values=p["fixed_values"], tris=p["tris"],
+                     traction=p["traction"])
+    setup = time.time() - t0
+    S.run(p["E"], p["nu"])
+    asm_ms = S.time(5)
+    A = S.A
+    res = {"n": n, "dofs": A.n, "nnz": A.nnz, "block_nnz": A.nnzb, "host_setup_s": round(setup, 2),
+           "assembly_ms": round(asm_ms, 3)}
+    nt, nn = len(p["tets"]), len(p["xyz"])
+    asm_bytes = nt * (16 + 8) + nn * 24 + A.nnz * 8
+    res["assembly_gbs_algorithmic"] = round(asm_bytes / (asm_ms * 1e-3) / 1e9, 1)
+    for prec, name in ((mfs.F32, "f32"), (mfs.F64, "f64")):
+        X = mfs.Vec(ctx, A.n, prec, np.ones(A.n))
+        Y = mfs.Vec(ctx, A.n, prec)
+        ms = A.spmv_time(X, Y, 50, prec)
+        gbs = A.spmv_bytes(prec) / (ms * 1e-3) / 1e9
+        res[f"spmv_{name}_us"] = round(ms * 1e3, 1)
+        res[f"spmv_{name}_gbs"] = round(gbs, 1)
+        res[f"spmv_{name}_frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
+        res[f"spmv_{name}_bytes"] = A.spmv_bytes(prec)
+    x = mfs.Vec(ctx, A.n)
+    # reference element convention: symmetric indefinite -> MINRES; CG shown
+    # for contrast (capped)
+    for name, kw in (("minres_fp64", dict(method=mfs.MINRES, precision=mfs.PREC_F64, maxit=30000)),
+                     ("minres_mixed", dict(method=mfs.MINRES, precision=mfs.PREC_MIXED, maxit=30000)),
+                     ("pcg_fp64_capped", dict(precision=mfs.PREC_F64, maxit=3000))):
+        x.copy_from(S.x0)
+        rep = mfs.Solver(A, tol=1e-8, **kw).run(S.b, x, check=False)
+        res[name] = {"ms": round(rep.seconds * 1e3, 1), "iterations": rep.iterations, "outer": rep.outer_steps,
+                     "true_relres": rep.true_relres, "status": rep.status}
+        log(f"[bench] large {name}: {res[name]}")
+    S.close()
+    # GMRES(30) on the textbook (SPD) operator of the same mesh
+    S = mfs.Assembly(ctx, mfs.PROBLEM_ELASTIC, p["xyz"], p["tets"], material=p["material"], renumber=True,
+                     convention=mfs.ELEM_TEXTBOOK, fixed_nodes=p["fixed_nodes"], fixed_values=p["fixed_values"],
+                     tris=p["tris"], traction=p["traction"])
+    S.run(p["E"], p["nu"])
+    for name, kw in (("pcg_fp64_textbook", dict(precision=mfs.PREC_F64, maxit=40000)),
+                     ("pcg_mixed_textbook", dict(precision=mfs.PREC_MIXED, maxit=40000)),
+                     ("gmres30_mixed_textbook", dict(method=mfs.GMRES, precision=mfs.PREC_MIXED, restart=30,
+                                                     maxit=6000))):
+        x = mfs.Vec(ctx, S.A.n)
+        x.copy_from(S.x0)
+        rep = mfs.Solver(S.A, tol=1e-8, **kw).run(S.b, x, check=False)
+        res[name] = {"ms": round(rep.seconds * 1e3, 1), "iterations": rep.iterations, "outer": rep.outer_steps,
+                     "true_relres": rep.true_relres, "status": rep.status}
+        log(f"[bench] large {name}: {res[name]}")
+    S.close()
+    log(f"[bench] large: {res}")
+    return res
 
 
 if __name__ == "__main__":

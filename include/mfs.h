@@ -39,7 +39,7 @@ enum {
 /* precisions / formats / methods */
 enum { MFS_F64 = 0, MFS_F32 = 1 };
 enum { MFS_FMT_CSR = 0, MFS_FMT_BSR3 = 1 };
-enum { MFS_PCG_JACOBI = 0, MFS_CG = 1, MFS_GMRES = 2 };
+enum { MFS_PCG_JACOBI = 0, MFS_CG = 1, MFS_GMRES = 2, MFS_MINRES = 3 };
 enum { MFS_PREC_F64 = 0, MFS_PREC_MIXED = 1 };
 enum { MFS_PROBLEM_SCALAR = 0, MFS_PROBLEM_ELASTIC = 1 };
 /* Elastic element block convention.  REFERENCE: block(I,J) = V B_J^T D B_I,
@@ -122,7 +122,7 @@ int mfs_spmv_time(mfs_mat A, int prec, mfs_vec x, mfs_vec y, int reps, double* a
 
 /* ---- solvers ----------------------------------------------------------- */
 typedef struct {
-  int method;       /* MFS_PCG_JACOBI | MFS_CG | MFS_GMRES                   */
+  int method;       /* MFS_PCG_JACOBI | MFS_CG | MFS_GMRES | MFS_MINRES      */
   int precision;    /* MFS_PREC_F64 | MFS_PREC_MIXED                          */
   double tol;       /* stop when ||b - A x||_2 / ||b||_2 <= tol (FP64, true)  */
   int maxit;        /* total inner iterations cap                             */
@@ -144,7 +144,9 @@ typedef struct {
  * restart 30, inner_tol 1e-4. */
 void mfs_solve_desc_default(mfs_solve_desc* desc);
 /* Plan a solve on A: allocates work vectors and captures the iteration
- * chunk into HIP graphs.  Replaces the solver family of CSSD_Matrix
+ * chunk into HIP graphs.  MFS_MINRES (Jacobi-preconditioned, symmetric
+ * indefinite A) serves the reference element convention's elastic operator,
+ * which is indefinite for heterogeneous or free-surface problems.  Replaces the solver family of CSSD_Matrix
  * (MSG :1496-1663, BiCG_Diag :1181-1300, MSG_PreconditioningSSOR
  * :1302-1495, GMRES :1774-1989) together with the drivers' continuation
  * loops (FEM/FEM.h:626-695, :2399-2485). */

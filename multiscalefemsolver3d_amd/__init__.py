@@ -26,7 +26,7 @@ MFS_OK = 0
 ERRORS = {-1: "ARG", -2: "OOM", -3: "NOT_CONVERGED", -4: "NONFINITE", -5: "HIP", -6: "UNSUPPORTED",
           -7: "NO_DEVICE"}
 F64, F32 = 0, 1
-PCG_JACOBI, CG, GMRES = 0, 1, 2
+PCG_JACOBI, CG, GMRES, MINRES = 0, 1, 2, 3
 PREC_F64, PREC_MIXED = 0, 1
 PROBLEM_SCALAR, PROBLEM_ELASTIC = 0, 1
 ELEM_REFERENCE, ELEM_TEXTBOOK = 0, 1

@@ -226,3 +226,24 @@ int spmv_residual_launch(mfs_mat A, int prec, const void* x, void* r, const void
                          unsigned* counter, double* out, hipStream_t s);
 int64_t spmv_grid(mfs_mat A);
 }  // namespace mfs
+
+// Fence-free two-stage reductions: a kernel writes one partial per block and
+// value (part[k*G + b]); k_finish_sums, one block, adds them in block order.
+template <typename T, int NV>
+__device__ __forceinline__ void write_block_partials(T (&v)[NV], double* __restrict__ part) {
+  __shared__ double red_bp[NV][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const double s = wave_sum((double)v[k]);
+    if (lane == 0) red_bp[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0;
+    for (int w = 0; w < nw; w++) s += red_bp[threadIdx.x][w];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ void k_finish_sums(const double* __restrict__ part, unsigned G, int nv, double* __restrict__ out);

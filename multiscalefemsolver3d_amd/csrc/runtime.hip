@@ -553,12 +553,21 @@ __global__ __launch_bounds__(256) void k_sell_spmv(const int64_t* __restrict__ s
       }
     }
   }
-  if constexpr (MODE != 0) {
-    double tot[2];
-    if (grid_sum<T, 2, 256>(red, partials, counter, tot) && threadIdx.x == 0) {
-      out[0] = tot[0];
-      out[1] = tot[1];
-    }
+  if constexpr (MODE != 0) write_block_partials<T, 2>(red, partials);
+}
+
+// One block: out[k] = sum over b of part[k*G + b], k < nv (block order).
+__global__ void k_finish_sums(const double* __restrict__ part, unsigned G, int nv, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = 0; k < nv; k++) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < G; b += blockDim.x) s += part[(size_t)k * G + b];
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
   }
 }
 
@@ -589,6 +598,7 @@ static int spmv_mode_launch(mfs_mat A, int prec, const void* x, void* y, const v
                                                        partials, counter, out);
   }
   MFS_HIP(hipGetLastError());
+  if (MODE != 0) k_finish_sums<<<1, 256, 0, s>>>(partials, grid, 2, out);
   return MFS_OK;
 }
 
@@ -642,8 +652,7 @@ __global__ __launch_bounds__(256) void k_dot(const T* __restrict__ x, const T* _
   T v[1] = {0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     v[0] += x[i] * y[i];
-  double tot[1];
-  if (grid_sum<T, 1, 256>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+  write_block_partials<T, 1>(v, partials);
 }
 
 template <typename T>
@@ -665,6 +674,7 @@ extern "C" int mfs_dot(mfs_vec x, mfs_vec y, double* result) {
     k_dot<double><<<g, 256, 0, s>>>((const double*)x->d, (const double*)y->d, x->n, rs.partials, rs.counter, out);
   else
     k_dot<float><<<g, 256, 0, s>>>((const float*)x->d, (const float*)y->d, x->n, rs.partials, rs.counter, out);
+  k_finish_sums<<<1, 256, 0, s>>>(rs.partials, g, 1, out);
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, s));
   MFS_HIP(hipStreamSynchronize(s));

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -20,9 +21,12 @@ using namespace mfs;
 
 namespace {
 
+// Two kernels per iteration.  K1 reads the fields K2 wrote (rz2, iter2) and
+// K2 reads the fields K1 wrote (rz1, iter1); each is written by block 0 only,
+// so no kernel reads a field it writes.
 struct CgState {
-  double rz, pq, alpha, beta, rr, bb, tol2, rz0;
-  int done, iter, maxit, pad;
+  double rz1, rz2, rr, bb, tol2, pq;
+  int done, iter1, iter2, maxit;
 };
 // done codes
 enum { CG_RUN = 0, CG_CONVERGED = 1, CG_BREAKDOWN = 2, CG_MAXIT = 3 };
@@ -32,21 +36,115 @@ inline unsigned grid_for(int64_t n, int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
-// q = A p, grid p.q; last block: alpha = rz / pq.
+// Sum of the G per-block partials part[k*G + b] (k < NV) in block order, done
+// redundantly by every block of the consuming kernel: the kernel boundary
+// makes the producer's partials visible, so no fences or atomics are needed,
+// and every block obtains bit-identical totals.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, unsigned G, double (&tot)[NV]) {
+  __shared__ double red[NV][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < G; b += 256) s += part[(size_t)k * G + b];
+    s = wave_sum(s);
+    if (lane == 0) red[k][wave] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) tot[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+  __syncthreads();
+}
+
+// Block partial of NV values into part[k*gridDim + blockIdx].
+template <typename T, int NV>
+__device__ __forceinline__ void block_partials(T (&v)[NV], double* __restrict__ part) {
+  __shared__ double red[NV][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const double s = wave_sum((double)v[k]);
+    if (lane == 0) red[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV)
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] =
+        (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
+}
+
+// SELL-64 row product with the operand formed on the fly:
+// x_j = z_j + beta * p_j (the new search direction).
 template <typename T, int BS>
-__global__ __launch_bounds__(256) void k_cg_spmv(const int64_t* __restrict__ slice_base,
-                                                 const int32_t* __restrict__ slice_width,
-                                                 const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                 const T* __restrict__ p, T* __restrict__ q, int64_t nb,
-                                                 int64_t n_slices, CgState* st, double* partials,
-                                                 unsigned* counter) {
+__device__ __forceinline__ void sell_row_dir(const int64_t* __restrict__ slice_base,
+                                             const int32_t* __restrict__ slice_width,
+                                             const int32_t* __restrict__ col, const T* __restrict__ val,
+                                             const T* __restrict__ z, const T* __restrict__ p, T beta,
+                                             int64_t slice, int lane, T (&acc)[BS]) {
+  constexpr int C = 64;
+#pragma unroll
+  for (int a = 0; a < BS; a++) acc[a] = 0;
+  const int64_t sb = slice_base[slice];
+  const int w = slice_width[slice];
+  const int32_t* cp = col + sb + lane;
+  const T* vp = val + sb * (BS * BS) + lane;
+  for (int j = 0; j < w; j++) {
+    const int32_t c = __builtin_nontemporal_load(cp + (int64_t)j * C);
+    if constexpr (BS == 1) {
+      const T xv = z[c] + beta * p[c];
+      acc[0] += __builtin_nontemporal_load(vp + (int64_t)j * C) * xv;
+    } else {
+      const int64_t c3 = 3 * (int64_t)c;
+      const T x0 = z[c3] + beta * p[c3], x1 = z[c3 + 1] + beta * p[c3 + 1], x2 = z[c3 + 2] + beta * p[c3 + 2];
+      const T* vj = vp + (int64_t)j * (BS * BS) * C;
+#pragma unroll
+      for (int a = 0; a < BS; a++) {
+        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 0) * C) * x0;
+        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 1) * C) * x1;
+        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 2) * C) * x2;
+      }
+    }
+  }
+}
+
+// K1: from K2's partials (r.z, r.r): stop test, beta = rz / rz_prev;
+// p_new = z + beta p_old (own rows written, gathered rows formed on the
+// fly), q = A p_new, per-block partial of p_new.q.
+template <typename T, int BS>
+__global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__ slice_base,
+                                                     const int32_t* __restrict__ slice_width,
+                                                     const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                     const T* __restrict__ z, const T* __restrict__ p_old,
+                                                     T* __restrict__ p_new, T* __restrict__ q, int64_t nb,
+                                                     int64_t n_slices, CgState* st, const double* __restrict__ part_rz,
+                                                     unsigned g_rz, double* __restrict__ part_pq) {
   if (st->done) return;
+  double t[2];
+  sum_partials<2>(part_rz, g_rz, t);
+  const double rz = t[0], rr = t[1];
+  const int iter = st->iter2;
+  int done = CG_RUN;
+  if (!isfinite(rz) || !isfinite(rr) || (iter > 0 && !(st->pq != 0.0 && isfinite(st->pq))))
+    done = CG_BREAKDOWN;
+  else if (rr <= st->tol2)
+    done = CG_CONVERGED;
+  else if (iter >= st->maxit)
+    done = CG_MAXIT;
+  const T beta = iter > 0 ? (T)(rz / st->rz2) : (T)0;
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz1 = rz;
+    st->rr = rr;
+    st->iter1 = iter;
+    st->done = done;
+  }
+  if (done != CG_RUN) return;
   const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t r = slice * 64 + lane;
   T acc[BS];
   if (slice < n_slices) {
-    sell_row<T, BS>(slice_base, slice_width, col, val, p, slice, lane, acc);
+    sell_row_dir<T, BS>(slice_base, slice_width, col, val, z, p_old, beta, slice, lane, acc);
   } else {
 #pragma unroll
     for (int a = 0; a < BS; a++) acc[a] = 0;
@@ -55,95 +153,73 @@ __global__ __launch_bounds__(256) void k_cg_spmv(const int64_t* __restrict__ sli
   if (r < nb) {
 #pragma unroll
     for (int a = 0; a < BS; a++) {
-      q[BS * r + a] = acc[a];
-      red[0] += acc[a] * p[BS * r + a];
+      const int64_t i = BS * r + a;
+      const T pn = z[i] + beta * p_old[i];
+      p_new[i] = pn;
+      q[i] = acc[a];
+      red[0] += acc[a] * pn;
     }
   }
-  double tot[1];
-  if (grid_sum<T, 1, 256>(red, partials, counter, tot) && threadIdx.x == 0) {
-    st->pq = tot[0];
-    if (!(tot[0] != 0.0) || !isfinite(tot[0]))
-      st->done = CG_BREAKDOWN;
-    else
-      st->alpha = st->rz / tot[0];
-  }
+  block_partials<T, 1>(red, part_pq);
 }
 
-// x += alpha p; r -= alpha q; grid r.z and r.r with z = dinv r.
+// K2: alpha = rz / sum(part_pq); x += alpha p; r -= alpha q; z = dinv r;
+// per-block partials of r.z and r.r.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r,
-                                                   const T* __restrict__ q, const T* __restrict__ dinv, int64_t n,
-                                                   CgState* st, double* partials, unsigned* counter) {
+                                                   T* __restrict__ z, const T* __restrict__ q,
+                                                   const T* __restrict__ dinv, int64_t n, CgState* st,
+                                                   const double* __restrict__ part_pq, unsigned g_pq,
+                                                   double* __restrict__ part_rz) {
   if (st->done) return;
-  const T alpha = (T)st->alpha;
+  double pq[1];
+  sum_partials<1>(part_pq, g_pq, pq);
+  const double rz = st->rz1;
+  const bool ok = pq[0] != 0.0 && isfinite(pq[0]);
+  const T alpha = ok ? (T)(rz / pq[0]) : (T)0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz2 = rz;
+    st->pq = pq[0];
+    st->iter2 = st->iter1 + 1;
+  }
   T red[2] = {0, 0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
     const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
     red[0] += ri * zi;
     red[1] += ri * ri;
   }
-  double tot[2];
-  if (grid_sum<T, 2, 256>(red, partials, counter, tot) && threadIdx.x == 0) {
-    st->iter += 1;
-    st->rr = tot[1];
-    if (!isfinite(tot[0]) || !isfinite(tot[1])) {
-      st->done = CG_BREAKDOWN;
-    } else if (tot[1] <= st->tol2) {
-      st->done = CG_CONVERGED;
-    } else if (st->iter >= st->maxit) {
-      st->done = CG_MAXIT;
-    } else {
-      st->beta = tot[0] / st->rz;
-      st->rz = tot[0];
-    }
-  }
+  block_partials<T, 2>(red, part_rz);
 }
 
-// p = dinv r + beta p
+// Start: z = dinv r, p_old = 0, partials of r.z and r.r; scalar state.
 template <typename T>
-__global__ __launch_bounds__(256) void k_cg_direction(T* __restrict__ p, const T* __restrict__ r,
-                                                      const T* __restrict__ dinv, int64_t n, const CgState* st) {
-  if (st->done) return;
-  const T beta = (T)st->beta;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = (dinv ? dinv[i] * r[i] : r[i]) + beta * p[i];
-}
-
-// Start: p = dinv r, grid r.z, r.r -> rz, rr; tol2 = tol^2 * bb (bb given),
-// converged already if rr <= tol2.
-template <typename T>
-__global__ __launch_bounds__(256) void k_cg_start(T* __restrict__ p, const T* __restrict__ r,
-                                                  const T* __restrict__ dinv, int64_t n, CgState* st, double tol,
-                                                  double bb, int maxit, double* partials, unsigned* counter) {
+__global__ __launch_bounds__(256) void k_cg_start(T* __restrict__ z, T* __restrict__ p0, const T* __restrict__ r,
+                                                  const T* __restrict__ dinv, int64_t n, double* part_rz) {
   T red[2] = {0, 0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const T ri = r[i];
     const T zi = dinv ? dinv[i] * ri : ri;
-    p[i] = zi;
+    z[i] = zi;
+    p0[i] = 0;
     red[0] += ri * zi;
     red[1] += ri * ri;
   }
-  double tot[2];
-  if (grid_sum<T, 2, 256>(red, partials, counter, tot) && threadIdx.x == 0) {
-    st->rz = tot[0];
-    st->rz0 = tot[0];
-    st->rr = tot[1];
+  block_partials<T, 2>(red, part_rz);
+}
+__global__ void k_cg_start_state(CgState* st, double tol, double bb, int maxit) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->rz1 = st->rz2 = 1.0;
+    st->rr = 0.0;
     st->bb = bb;
     st->tol2 = tol * tol * bb;
-    st->iter = 0;
+    st->iter1 = st->iter2 = 0;
     st->maxit = maxit;
-    st->alpha = st->beta = 0.0;
-    if (!isfinite(tot[0]) || !isfinite(tot[1]))
-      st->done = CG_BREAKDOWN;
-    else if (tot[1] <= st->tol2)
-      st->done = CG_CONVERGED;
-    else if (maxit <= 0)
-      st->done = CG_MAXIT;
-    else
-      st->done = CG_RUN;
+    st->pq = 1.0;
+    st->done = CG_RUN;
   }
 }
 
@@ -172,83 +248,92 @@ template <typename T>
 struct PcgEngine {
   mfs_mat A = nullptr;
   int64_t n = 0;
-  T *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+  T *x = nullptr, *r = nullptr, *z = nullptr, *q = nullptr;
+  T* pbuf[2] = {nullptr, nullptr};  // ping-pong search directions
   const T* dinv = nullptr;
   const T* val = nullptr;
   CgState* st = nullptr;
   ReduceScratch red;
   unsigned g_spmv = 0, g_vec = 0;
   int chunk = 16;
+  bool use_graph = true;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+
+  double* part_pq() { return red.partials; }
+  double* part_rz() { return red.partials + g_spmv; }
 
   int init(mfs_mat A_, bool precond, int chunk_) {
     A = A_;
     n = A->n();
-    chunk = chunk_;
+    chunk = std::max(2, chunk_ + (chunk_ & 1));  // even: the ping-pong parity repeats
     const bool f64 = std::is_same<T, double>::value;
     if (!f64 && !A->val32_valid) MFS_TRY(mat_refresh_derived(A));
     val = f64 ? (const T*)A->d_val64 : (const T*)A->d_val32;
     dinv = precond ? (f64 ? (const T*)A->d_dinv64 : (const T*)A->d_dinv32) : nullptr;
-    MFS_TRY(dmalloc((void**)&x, sizeof(T) * n));
-    MFS_TRY(dmalloc((void**)&r, sizeof(T) * n));
-    MFS_TRY(dmalloc((void**)&p, sizeof(T) * n));
-    MFS_TRY(dmalloc((void**)&q, sizeof(T) * n));
+    for (T** b : {&x, &r, &z, &q, &pbuf[0], &pbuf[1]}) MFS_TRY(dmalloc((void**)b, sizeof(T) * n));
     MFS_TRY(dmalloc((void**)&st, sizeof(CgState)));
     MFS_HIP(hipMemset(st, 0, sizeof(CgState)));
     g_spmv = (unsigned)spmv_grid(A);
     g_vec = grid_for(n, 4 * std::max(A->ctx->cu_count, 64));
-    MFS_TRY(reduce_scratch_alloc(red, 2 * (int64_t)std::max(g_spmv, g_vec) + 64));
+    MFS_TRY(reduce_scratch_alloc(red, (int64_t)g_spmv + 2 * (int64_t)g_vec + 64));
     return capture();
   }
 
-  int launch_iteration(hipStream_t s) {
+  void launch_iteration(hipStream_t s, int parity) {
     const int64_t ns = A->sell.n_slices;
+    T* p_old = pbuf[parity];
+    T* p_new = pbuf[parity ^ 1];
     if (A->bs == 1)
-      k_cg_spmv<T, 1><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, p, q, A->nb, ns, st,
-                                             red.partials, red.counter);
+      k_cg_dir_spmv<T, 1><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z, p_old, p_new, q,
+                                                 A->nb, ns, st, part_rz(), g_vec, part_pq());
     else
-      k_cg_spmv<T, 3><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, p, q, A->nb, ns, st,
-                                             red.partials, red.counter);
-    k_cg_update<T><<<g_vec, 256, 0, s>>>(x, p, r, q, dinv, n, st, red.partials, red.counter);
-    k_cg_direction<T><<<g_vec, 256, 0, s>>>(p, r, dinv, n, st);
-    return MFS_OK;
+      k_cg_dir_spmv<T, 3><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z, p_old, p_new, q,
+                                                 A->nb, ns, st, part_rz(), g_vec, part_pq());
+    k_cg_update<T><<<g_vec, 256, 0, s>>>(x, p_new, r, z, q, dinv, n, st, part_pq(), g_spmv, part_rz());
   }
 
   int capture() {
+    const char* e = std::getenv("MFS_NO_GRAPH");
+    use_graph = !(e && e[0] == '1');
+    if (!use_graph) return MFS_OK;
     hipStream_t s = A->ctx->stream;
     MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < chunk; k++) launch_iteration(s);
+    for (int k = 0; k < chunk; k++) launch_iteration(s, k & 1);
     MFS_HIP(hipStreamEndCapture(s, &graph));
     MFS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     return MFS_OK;
   }
 
-  // Start from r (already holding the initial residual) and x; bb = ||b||^2
-  // of the system being solved.  Runs until converged / maxit / breakdown.
+  // Start from r (holding the initial residual) and x; bb = ||b||^2 of the
+  // system being solved.  Runs until converged / maxit / breakdown.  The
+  // last K1 to run records the final residual in the state.
   int run(double tol, double bb, int maxit, CgState* host_state) {
     hipStream_t s = A->ctx->stream;
-    k_cg_start<T><<<g_vec, 256, 0, s>>>(p, r, dinv, n, st, tol, bb, maxit, red.partials, red.counter);
+    k_cg_start<T><<<g_vec, 256, 0, s>>>(z, pbuf[0], r, dinv, n, part_rz());
+    k_cg_start_state<<<1, 64, 0, s>>>(st, tol, bb, maxit);
     MFS_HIP(hipGetLastError());
     CgState* hs = (CgState*)A->ctx->pinned;
-    MFS_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    MFS_HIP(hipStreamSynchronize(s));
+    hs->done = CG_RUN;
     while (hs->done == CG_RUN) {
-      MFS_HIP(hipGraphLaunch(exec, s));
+      if (use_graph) {
+        MFS_HIP(hipGraphLaunch(exec, s));
+      } else {
+        for (int k = 0; k < chunk; k++) launch_iteration(s, k & 1);
+        MFS_HIP(hipGetLastError());
+      }
       MFS_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
       MFS_HIP(hipStreamSynchronize(s));
     }
     *host_state = *hs;
+    host_state->iter2 = hs->iter1;  // iterations completed when K1 stopped
     return MFS_OK;
   }
 
   void destroy() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    dfree(x);
-    dfree(r);
-    dfree(p);
-    dfree(q);
+    for (T* b : {x, r, z, q, pbuf[0], pbuf[1]}) dfree(b);
     dfree(st);
     reduce_scratch_free(red);
   }
@@ -260,48 +345,37 @@ struct PcgEngine {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_multidot(const T* __restrict__ V, const T* __restrict__ w, int64_t n,
-                                                  int kcount, double* __restrict__ partials,
-                                                  unsigned* __restrict__ counter, double* __restrict__ h) {
-  __shared__ double lds[32][4];
-  __shared__ bool last;
+                                                  int kcount, double* __restrict__ partials) {
+  // partials[k*G + b] = sum over this block's rows of v_k . w, 8 bases per
+  // pass so w is read ceil(kcount/8) times
+  __shared__ double red[8][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = 0; k < kcount; k++) {
-    T s = 0;
-    const T* v = V + (int64_t)k * n;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-      s += v[i] * w[i];
-    s = wave_sum(s);
-    if (lane == 0) lds[k][wave] = (double)s;
+  for (int k0 = 0; k0 < kcount; k0 += 8) {
+    T s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+      const T wi = w[i];
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (k0 + k < kcount) s[k] += V[(int64_t)(k0 + k) * n + i] * wi;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const double t = wave_sum((double)s[k]);
+      if (lane == 0) red[k][wave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && k0 + threadIdx.x < kcount)
+      partials[(size_t)(k0 + threadIdx.x) * gridDim.x + blockIdx.x] =
+          (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
+    __syncthreads();
   }
-  __syncthreads();
-  if (threadIdx.x < kcount)
-    partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] =
-        lds[threadIdx.x][0] + lds[threadIdx.x][1] + lds[threadIdx.x][2] + lds[threadIdx.x][3];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  for (int k = wave; k < kcount; k += 4) {
-    double s = 0;
-    for (unsigned b = lane; b < gridDim.x; b += 64)
-      s += __hip_atomic_load(&partials[(size_t)k * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = wave_sum(s);
-    if (lane == 0) h[k] = s;
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// w -= sum_k h_k v_k (h in device memory, double); accumulate h_acc += h;
-// then grid ||w||^2 into out[0].
+// w -= sum_k h_k v_k (h in device memory, double); per-block partial ||w||^2.
 template <typename T>
 __global__ __launch_bounds__(256) void k_orth_update(const T* __restrict__ V, T* __restrict__ w, int64_t n,
                                                      int kcount, const double* __restrict__ h,
-                                                     double* partials, unsigned* counter, double* out) {
+                                                     double* __restrict__ partials) {
   __shared__ T hs[32];
   if (threadIdx.x < kcount) hs[threadIdx.x] = (T)h[threadIdx.x];
   __syncthreads();
@@ -312,8 +386,7 @@ __global__ __launch_bounds__(256) void k_orth_update(const T* __restrict__ V, T*
     w[i] = wi;
     red[0] += wi * wi;
   }
-  double tot[1];
-  if (grid_sum<T, 1, 256>(red, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+  write_block_partials<T, 1>(red, partials);
 }
 
 template <typename T>
@@ -344,6 +417,69 @@ __global__ void k_combine(const T* __restrict__ V, const double* __restrict__ y,
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Preconditioned MINRES for symmetric (possibly indefinite) A with SPD M =
+// diag(A) (Elman, Silvester & Wathen, "Finite Elements and Fast Iterative
+// Solvers", Algorithm 2.4).  Scalar recurrences on the host, one 16-byte
+// read-back per iteration.
+// ---------------------------------------------------------------------------
+// v_new = q - (delta/g) v - (g/g_prev) v_old; z_new = dinv v_new;
+// per-block partial of z_new . v_new.   delta from device memory.
+template <typename T>
+__global__ __launch_bounds__(256) void k_minres_lanczos(const T* __restrict__ q, const T* __restrict__ v,
+                                                        const T* __restrict__ v_old, T* __restrict__ v_new,
+                                                        T* __restrict__ z_new, const T* __restrict__ dinv,
+                                                        const double* __restrict__ delta, double g, double g_prev,
+                                                        int64_t n, double* __restrict__ partials) {
+  const T c1 = (T)(delta[0] / g), c2 = g_prev > 0 ? (T)(g / g_prev) : (T)0;
+  T red[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T vn = q[i] - c1 * v[i] - c2 * v_old[i];
+    v_new[i] = vn;
+    const T zn = dinv ? dinv[i] * vn : vn;
+    z_new[i] = zn;
+    red[0] += zn * vn;
+  }
+  write_block_partials<T, 1>(red, partials);
+}
+// w_new = (z - a3 w_old - a2 w) / a1; x += (c eta) w_new; z_next *= 1/g_next
+template <typename T>
+__global__ __launch_bounds__(256) void k_minres_update(const T* __restrict__ z, const T* __restrict__ w,
+                                                       const T* __restrict__ w_old, T* __restrict__ w_new,
+                                                       T* __restrict__ x, T* __restrict__ z_next, double a1, double a2,
+                                                       double a3, double ceta, double inv_g_next, int64_t n) {
+  const T ia1 = (T)(1.0 / a1), ta2 = (T)a2, ta3 = (T)a3, tc = (T)ceta, tg = (T)inv_g_next;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T wn = (z[i] - ta3 * w_old[i] - ta2 * w[i]) * ia1;
+    w_new[i] = wn;
+    x[i] += tc * wn;
+    z_next[i] *= tg;
+  }
+}
+template <typename T>
+__global__ void k_scale_vec(T* __restrict__ a, double s, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] *= (T)s;
+}
+template <typename T>
+__global__ void k_minres_start(const T* __restrict__ r, T* __restrict__ v, T* __restrict__ z, const T* __restrict__ dinv,
+                               T* __restrict__ x, T* __restrict__ v_old, T* __restrict__ w, T* __restrict__ w_old,
+                               int64_t n, double* __restrict__ partials) {
+  T red[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = r[i];
+    v[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
+    x[i] = 0;
+    v_old[i] = 0;
+    w[i] = 0;
+    w_old[i] = 0;
+    red[0] += zi * ri;
+  }
+  write_block_partials<T, 1>(red, partials);
+}
+
+// ---------------------------------------------------------------------------
 // Solver object
 // ---------------------------------------------------------------------------
 struct mfs_solver_s {
@@ -360,6 +496,7 @@ struct mfs_solver_s {
   PcgEngine<float> cg32;
   bool has64 = false, has32 = false;
   // GMRES
+  void* mr[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // MINRES vectors
   void* V = nullptr;   // (m+1) * n in the inner precision
   void* w = nullptr;
   void* z = nullptr;
@@ -388,6 +525,7 @@ extern "C" int mfs_solver_destroy(mfs_solver S) {
   dfree(S->b64);
   dfree(S->r64);
   dfree(S->dout);
+  for (void* v : S->mr) dfree(v);
   dfree(S->V);
   dfree(S->w);
   dfree(S->z);
@@ -401,7 +539,8 @@ extern "C" int mfs_solver_destroy(mfs_solver S) {
 
 extern "C" int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solver* out) {
   MFS_CHECK_ARG(A && desc && out, "mfs_solver_create: bad argument");
-  MFS_CHECK_ARG(desc->method == MFS_PCG_JACOBI || desc->method == MFS_CG || desc->method == MFS_GMRES,
+  MFS_CHECK_ARG(desc->method == MFS_PCG_JACOBI || desc->method == MFS_CG || desc->method == MFS_GMRES ||
+                    desc->method == MFS_MINRES,
                 "mfs_solver_create: unknown method");
   MFS_CHECK_ARG(desc->precision == MFS_PREC_F64 || desc->precision == MFS_PREC_MIXED,
                 "mfs_solver_create: unknown precision");
@@ -427,7 +566,13 @@ extern "C" int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solv
   tryit(dmalloc((void**)&S->dout, 8 * 64));
   if (s == MFS_OK) tryit(reduce_scratch_alloc(S->red64, 2 * spmv_grid(A) + 64));
   const bool precond = desc->method != MFS_CG;
-  if (desc->method != MFS_GMRES) {
+  if (desc->method == MFS_MINRES) {
+    const size_t sv = desc->precision == MFS_PREC_F64 ? 8 : 4;
+    if (desc->precision == MFS_PREC_MIXED && !A->val32_valid) tryit(mat_refresh_derived(A));
+    for (void*& v : S->mr) tryit(dmalloc(&v, sv * n));
+    tryit(dmalloc((void**)&S->dh, 8 * 64));
+    if (s == MFS_OK) tryit(reduce_scratch_alloc(S->redg, 2 * 1024 + 64));
+  } else if (desc->method != MFS_GMRES) {
     if (s == MFS_OK && desc->precision == MFS_PREC_F64) {
       S->has64 = true;
       tryit(S->cg64.init(A, precond, chunk));
@@ -492,9 +637,10 @@ static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double
     MFS_TRY(spmv_launch(A, prec, z, w, s));
     // CGS2: two passes of h = V^T w, w -= V h
     for (int pass = 0; pass < 2; pass++) {
-      k_multidot<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->redg.partials, S->redg.counter, S->dh + pass * 32);
-      k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->dh + pass * 32, S->redg.partials, S->redg.counter,
-                                          S->dh + 62 + pass * 0);
+      k_multidot<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->redg.partials);
+      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, j + 1, S->dh + pass * 32);
+      k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->dh + pass * 32, S->redg.partials);
+      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh + 62);
     }
     MFS_HIP(hipGetLastError());
     MFS_HIP(hipMemcpyAsync(hp, S->dh, 64 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -539,6 +685,87 @@ static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double
   return MFS_OK;
 }
 
+// One MINRES run on A d = rhs (rhs in mr[0]; d returned in mr[1]), to
+// relative (M^-1-norm) tolerance rtol or maxsteps.  Returns the steps taken.
+template <typename T>
+static int minres_run(mfs_solver S, const T* dinv, double rtol, int maxsteps, int* steps, double* est_rel) {
+  hipStream_t s = S->A->ctx->stream;
+  mfs_mat A = S->A;
+  const int64_t n = S->n;
+  const int prec = std::is_same<T, double>::value ? MFS_F64 : MFS_F32;
+  T* rhs = (T*)S->mr[0];
+  T* x = (T*)S->mr[1];
+  T* v[3] = {(T*)S->mr[2], (T*)S->mr[3], (T*)S->mr[4]};  // v_{j-1}, v_j, v_{j+1}
+  T* w[3] = {(T*)S->mr[5], (T*)S->mr[6], (T*)S->mr[7]};  // w_{j-1}, w_j, w_{j+1}
+  T* zbuf[2] = {(T*)S->mr[8], nullptr};
+  T* q = (T*)S->w;  // scratch: A z
+  (void)q;
+  const unsigned gv = grid_for(n, 1024);
+  double* hp = (double*)A->ctx->pinned;
+  // z buffers: z_j in zbuf[0]; z_{j+1} in the spare v slot is not possible,
+  // so z_{j+1} goes to rhs (rhs is consumed by the start kernel).
+  T* zj = zbuf[0];
+  T* znext = rhs;
+  k_minres_start<T><<<gv, 256, 0, s>>>(rhs, v[1], zj, dinv, x, v[0], w[1], w[0], n, S->redg.partials);
+  k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh);
+  MFS_HIP(hipMemcpyAsync(hp, S->dh, 8, hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  double g = std::sqrt(std::max(hp[0], 0.0)), g_prev = 0.0;
+  if (!(g > 0) || !std::isfinite(g)) {
+    *est_rel = std::isfinite(g) ? 0.0 : 1.0;
+    return std::isfinite(g) ? MFS_OK : MFS_ERR_NONFINITE;
+  }
+  const double eta0 = g;
+  double eta = g, c_prev = 1, c = 1, s_prev = 0, s_cur = 0;
+  k_scale_vec<T><<<gv, 256, 0, s>>>(zj, 1.0 / g, n);
+  T* Az = (T*)S->w;
+  int j = 0;
+  for (; j < maxsteps; j++) {
+    MFS_TRY(spmv_launch(A, prec, zj, Az, s));
+    // delta = <A z_j, z_j>
+    k_multidot<T><<<gv, 256, 0, s>>>(zj, Az, n, 1, S->redg.partials);
+    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh + 1);
+    k_minres_lanczos<T><<<gv, 256, 0, s>>>(Az, v[1], v[0], v[2], znext, dinv, S->dh + 1, g, g_prev, n,
+                                           S->redg.partials);
+    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh + 2);
+    MFS_HIP(hipMemcpyAsync(hp, S->dh + 1, 16, hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipStreamSynchronize(s));
+    const double delta = hp[0];
+    const double g_next = std::sqrt(std::max(hp[1], 0.0));
+    if (!std::isfinite(delta) || !std::isfinite(g_next)) return MFS_ERR_NONFINITE;
+    const double a0 = c * delta - c_prev * s_cur * g;
+    const double a1 = std::sqrt(a0 * a0 + g_next * g_next);
+    const double a2 = s_cur * delta + c_prev * c * g;
+    const double a3 = s_prev * g;
+    if (!(a1 > 0)) return MFS_ERR_NONFINITE;
+    const double c_next = a0 / a1, s_next = g_next / a1;
+    k_minres_update<T><<<gv, 256, 0, s>>>(zj, w[1], w[0], w[2], x, znext, a1, a2, a3, c_next * eta,
+                                          g_next > 0 ? 1.0 / g_next : 0.0, n);
+    MFS_HIP(hipGetLastError());
+    eta = -s_next * eta;
+    // shift the recurrences and rotate buffers
+    c_prev = c;
+    c = c_next;
+    s_prev = s_cur;
+    s_cur = s_next;
+    g_prev = g;
+    g = g_next;
+    std::swap(v[0], v[1]);
+    std::swap(v[1], v[2]);  // v_{j-1} <- v_j, v_j <- v_{j+1}
+    std::swap(w[0], w[1]);
+    std::swap(w[1], w[2]);
+    std::swap(zj, znext);
+    if (std::fabs(eta) <= rtol * eta0 || g_next == 0) {
+      j++;
+      break;
+    }
+  }
+  // the correction must end in mr[1]: x never moves
+  *steps += j;
+  *est_rel = std::fabs(eta) / eta0;
+  return MFS_OK;
+}
+
 __global__ void k_add_f64(double* __restrict__ x, const double* __restrict__ u, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] += u[i];
@@ -577,7 +804,7 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
     MFS_HIP(hipMemcpyAsync(E.x, S->x64, 8 * n, hipMemcpyDeviceToDevice, s));
     CgState hs;
     MFS_TRY(E.run(d.tol, bb, d.maxit, &hs));
-    iters = hs.iter;
+    iters = hs.iter1;
     outer = 1;
     MFS_HIP(hipMemcpyAsync(S->x64, E.x, 8 * n, hipMemcpyDeviceToDevice, s));
     rep->recursive_relres = std::sqrt(hs.rr / bb);
@@ -595,7 +822,7 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
       const double itol = std::min(0.5, std::max(d.inner_tol, 0.5 * d.tol / rel));
       CgState hs;
       MFS_TRY(E.run(itol, 1.0, d.maxit - iters, &hs));
-      iters += hs.iter;
+      iters += hs.iter1;
       outer++;
       k_promote_add<<<gv, 256, 0, s>>>(S->x64, E.x, nullptr, rn, n);
       const double prev = rel;
@@ -606,7 +833,45 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
         status = MFS_ERR_NONFINITE;
         break;
       }
-      if (rel > 0.99 * prev && hs.iter <= 1) {  // no progress possible
+      if (rel > 0.99 * prev && hs.iter1 <= 1) {  // no progress possible
+        status = MFS_ERR_NOT_CONVERGED;
+        break;
+      }
+    }
+    if (status == MFS_OK && rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
+  } else if (d.method == MFS_MINRES) {
+    // MINRES runs, each from the current FP64 residual (refinement steps)
+    while (rel > d.tol && iters < d.maxit) {
+      const double rn = std::sqrt(rr);
+      double est = 1;
+      int steps = 0, st2;
+      if (d.precision == MFS_PREC_F64) {
+        MFS_HIP(hipMemcpyAsync(S->mr[0], S->r64, 8 * n, hipMemcpyDeviceToDevice, s));
+        st2 = minres_run<double>(S, A->d_dinv64, 0.5 * d.tol * std::sqrt(bb) / rn, d.maxit - iters, &steps, &est);
+        if (st2 == MFS_OK) k_add_f64<<<gv, 256, 0, s>>>(S->x64, (const double*)S->mr[1], n);
+      } else {
+        float* dummy = (float*)S->mr[2];
+        k_demote_scaled<<<gv, 256, 0, s>>>(S->r64, 1.0 / rn, (float*)S->mr[0], dummy, n);
+        const double itol = std::min(0.5, std::max(d.inner_tol, 0.5 * d.tol / rel));
+        st2 = minres_run<float>(S, A->d_dinv32, itol, d.maxit - iters, &steps, &est);
+        if (st2 == MFS_OK) k_promote_add<<<gv, 256, 0, s>>>(S->x64, (const float*)S->mr[1], nullptr, rn, n);
+      }
+      if (st2 != MFS_OK) {
+        status = st2;
+        break;
+      }
+      MFS_HIP(hipGetLastError());
+      iters += steps;
+      outer++;
+      const double prev = rel;
+      MFS_TRY(true_residual(S, &rr, &bb));
+      rel = std::sqrt(rr / bb);
+      rep->recursive_relres = est * prev;
+      if (!std::isfinite(rel)) {
+        status = MFS_ERR_NONFINITE;
+        break;
+      }
+      if (rel > 0.99 * prev && steps <= 1) {
         status = MFS_ERR_NOT_CONVERGED;
         break;
       }

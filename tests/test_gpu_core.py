@@ -244,3 +244,29 @@ def test_config1_full_size_mixed_vs_fp64(ctx):
     r = b - orc.csr_mv(rp, col, val, xs[mfs.PREC_MIXED])
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-8
     assert rel(xs[mfs.PREC_MIXED], xs[mfs.PREC_F64]) < 1e-6
+
+
+@pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
+def test_minres_reference_elastic_operator(ctx, precision):
+    """MINRES (Jacobi) on the indefinite reference-convention elastic system
+    (two-phase E, clamped bottom, free sides) matches dense LU."""
+    p = problems.elastic_cube(5, field="grf")
+    S = assemble(ctx, p)
+    O = oracle_elastic(p)
+    xd = orc.dense_solve(O.rowptr, O.col, O.val, O.rhs)
+    x = mfs.Vec(ctx, S.A.n)
+    x.copy_from(S.x0)
+    rep = mfs.Solver(S.A, method=mfs.MINRES, precision=precision, tol=1e-9, maxit=20000).run(S.b, x)
+    assert rep.true_relres <= 1e-9
+    assert rel(x.download(), xd) < 1e-6
+
+
+def test_minres_scalar_matches_pcg(ctx):
+    p = problems.laplace_inclusions(8, count=20, rmin=0.1, rmax=0.2)
+    S = assemble(ctx, p)
+    O = oracle_scalar(p)
+    xo, _, _ = orc.pcg(O.rowptr, O.col, O.val, O.rhs, O.x0, tol=1e-13)
+    x = mfs.Vec(ctx, S.A.n)
+    x.copy_from(S.x0)
+    rep = mfs.Solver(S.A, method=mfs.MINRES, tol=1e-10).run(S.b, x)
+    assert rep.true_relres <= 1e-10 and rel(x.download(), xo) < 1e-8

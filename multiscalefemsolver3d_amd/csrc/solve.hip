@@ -496,7 +496,8 @@ struct mfs_solver_s {
   PcgEngine<float> cg32;
   bool has64 = false, has32 = false;
   // GMRES
-  void* mr[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // MINRES vectors
+  // MINRES vectors: rhs / z_next, x, v[3], w[3], z, A z
+  void* mr[10] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   void* V = nullptr;   // (m+1) * n in the inner precision
   void* w = nullptr;
   void* z = nullptr;
@@ -698,8 +699,16 @@ static int minres_run(mfs_solver S, const T* dinv, double rtol, int maxsteps, in
   T* v[3] = {(T*)S->mr[2], (T*)S->mr[3], (T*)S->mr[4]};  // v_{j-1}, v_j, v_{j+1}
   T* w[3] = {(T*)S->mr[5], (T*)S->mr[6], (T*)S->mr[7]};  // w_{j-1}, w_j, w_{j+1}
   T* zbuf[2] = {(T*)S->mr[8], nullptr};
-  T* q = (T*)S->w;  // scratch: A z
-  (void)q;
+  T* Az = (T*)S->mr[9];
+  for (void* b : S->mr)
+    if (!b) {
+      set_error("minres: work vector not allocated");
+      return MFS_ERR_ARG;
+    }
+  if (!S->dh || !S->redg.partials) {
+    set_error("minres: reduction scratch not allocated");
+    return MFS_ERR_ARG;
+  }
   const unsigned gv = grid_for(n, 1024);
   double* hp = (double*)A->ctx->pinned;
   // z buffers: z_j in zbuf[0]; z_{j+1} in the spare v slot is not possible,
@@ -718,7 +727,6 @@ static int minres_run(mfs_solver S, const T* dinv, double rtol, int maxsteps, in
   const double eta0 = g;
   double eta = g, c_prev = 1, c = 1, s_prev = 0, s_cur = 0;
   k_scale_vec<T><<<gv, 256, 0, s>>>(zj, 1.0 / g, n);
-  T* Az = (T*)S->w;
   int j = 0;
   for (; j < maxsteps; j++) {
     MFS_TRY(spmv_launch(A, prec, zj, Az, s));
@@ -797,7 +805,8 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
   int iters = 0, outer = 0;
   int status = MFS_OK;
   const unsigned gv = grid_for(n, 1024);
-  if (d.method != MFS_GMRES && d.precision == MFS_PREC_F64) {
+  const bool is_cg = d.method == MFS_PCG_JACOBI || d.method == MFS_CG;
+  if (is_cg && d.precision == MFS_PREC_F64) {
     // plain FP64 (P)CG on b - A x0: r = r64 already; x correction starts at x0
     PcgEngine<double>& E = S->cg64;
     MFS_HIP(hipMemcpyAsync(E.r, S->r64, 8 * n, hipMemcpyDeviceToDevice, s));
@@ -812,7 +821,7 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
     rel = std::sqrt(rr / bb);
     if (hs.done == CG_BREAKDOWN) status = MFS_ERR_NONFINITE;
     else if (rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
-  } else if (d.method != MFS_GMRES) {
+  } else if (is_cg) {
     // FP64 outer refinement around FP32 (P)CG
     PcgEngine<float>& E = S->cg32;
     while (rel > d.tol && iters < d.maxit) {

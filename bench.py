@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-1e-8 solve & SpMV HBM GB/s: mixed+batched+learned vs FP64 GPU CG"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_cg_dir_spmv<float,1> on this workload from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+# calibration (None until measured)
+TRAFFIC_K1 = None
 
 
 def log(*a):
@@ -41,7 +45,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-msfem", action="store_true", help="skip the MsFEM batch measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the 128^3 elastic measurement")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess MsFEM measurement")
+    ap.add_argument("--learned-test-cells", type=int, default=32768)
     ap.add_argument("--large-n", type=int, default=128)
+    ap.add_argument("--large-spmv-only", action="store_true", help="128^3: assembly and SpMV only, no solves")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 512 MiB device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -100,8 +109,16 @@ def main():
         f"relres {r_m.true_relres:.2e}); fp64 {ms_fp64:.2f} ms ({r_d.iterations} it, relres {r_d.true_relres:.2e}); "
         f"|x_mixed-x_fp64|/|x_fp64| = {agree:.2e}")
 
-    # Dominant kernel: the FP32 SELL SpMV of the inner iterations, timed with
-    # HIP events on the library stream.
+    # Dominant kernel of the mixed solve: the FP32 fused direction + SpMV
+    # kernel of the PCG iteration (k_cg_dir_spmv<float,1>), timed with HIP
+    # events on the library stream; the update kernel beside it.
+    k1_ms, k1_bytes = mixed.kernel_time(0, 200)
+    k2_ms, k2_bytes = mixed.kernel_time(1, 200)
+    k1_gbs = k1_bytes / (k1_ms * 1e-3) / 1e9
+    k2_gbs = k2_bytes / (k2_ms * 1e-3) / 1e9
+    log(f"[bench] k_cg_dir_spmv<float,1> {k1_ms * 1e3:.2f} us = {k1_gbs:.0f} GB/s; k_cg_update<float> "
+        f"{k2_ms * 1e3:.2f} us = {k2_gbs:.0f} GB/s")
+    # plain SELL SpMV in both precisions
     X32, Y32 = mfs.Vec(ctx, A.n, mfs.F32, x_mixed), mfs.Vec(ctx, A.n, mfs.F32)
     spmv_ms = A.spmv_time(X32, Y32, 200, mfs.F32)
     spmv_bytes = A.spmv_bytes(mfs.F32)
@@ -139,8 +156,21 @@ def main():
                          "serial dots/sweeps) on the same system: " + sample}
         log(f"[bench] cpu baseline {value:.0f} ms ({sample})")
 
-    msfem = None if args.no_msfem else bench_msfem(mfs, problems, ctx)
-    large = None if args.no_large else bench_large(mfs, problems, ctx, args.large_n)
+    msfem = None if args.no_msfem else bench_msfem(mfs, problems, ctx, cpu=not args.no_cpu)
+    learned_res = None if args.no_learned else bench_learned(mfs, ctx, args.learned_test_cells)
+    large = None if args.no_large else bench_large(mfs, problems, ctx, args.large_n, not args.large_spmv_only)
+    if args.copy_calibration:
+        # one plain copy of 512 MiB: FETCH_SIZE and WRITE_SIZE of this kernel
+        # should each read 524,288 KiB; their ratio to that scales the counters
+        big = 64 * 1024 * 1024
+        src = mfs.Vec(ctx, big, mfs.F64)
+        dst = mfs.Vec(ctx, big, mfs.F64)
+        src.fill(1.0)
+        ctx.sync()
+        dst.copy_from(src)
+        ctx.sync()
+        log(f"[bench] copy calibration: {big * 8} bytes copied")
+        del src, dst
 
     out = {
         "metric": METRIC,
@@ -165,19 +195,26 @@ def main():
         "mixed_vs_fp64_solution_rel_diff": agree,
         "assembly_ms": round(asm_ms, 4),
         "spmv_f64_gbs": round(spmv64_gbs, 1),
-        "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_sell_spmv<float,1> (FP32 SELL-64 SpMV)",
-                     "bytes_per_launch": spmv_bytes, "avg_us": round(spmv_ms * 1e3, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_K1,
+                     "kernel": "k_cg_dir_spmv<float,1> (FP32 direction update fused with the SELL-64 SpMV)",
+                     "bytes_per_launch": k1_bytes, "avg_us": round(k1_ms * 1e3, 3),
+                     "note": "working set (35 MB) is resident in the 256 MB Infinity Cache; achieved is "
+                             "algorithmic bytes / launch time"},
+        "k_cg_update_f32": {"avg_us": round(k2_ms * 1e3, 3), "bytes_per_launch": k2_bytes,
+                            "gbs": round(k2_gbs, 1)},
+        "spmv_f32_gbs": round(spmv_gbs, 1),
+        "spmv_f32_us": round(spmv_ms * 1e3, 3),
         "cpu_baseline": cpu,
         "msfem_config2": msfem,
+        "learned_config3": learned_res,
         "elastic_large": large,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
 
 
-def bench_msfem(mfs, problems, ctx):
+def bench_msfem(mfs, problems, ctx, cpu=True):
     """16^3 coarse cells, n ~ U{6..10}, lognormal E (sigma 1):
     98,304 local problems to 1e-8.  Mixed batched persistent launch (hipGraph)
     vs the same launch in plain FP64 vs one-problem-at-a-time FP64 Jacobi-PCG
@@ -230,11 +267,63 @@ def bench_msfem(mfs, problems, ctx):
     res["speedup_mixed_batched_vs_fp64_batched"] = round(res["fp64_batched"]["ms"] / res["mixed_batched"]["ms"], 3)
     res["problems"] = 24 * len(p["cell_n"])
     B.close()
+    if cpu:
+        res["cpu_baseline"] = msfem_cpu_baseline(p, counts)
     log(f"[bench] msfem: {res}")
     return res
 
 
-def bench_large(mfs, problems, ctx, n):
+def msfem_cpu_baseline(p, counts, per_n=8):
+    """The reference's CPU path for the local problems, restated in the
+    oracle: per cell, CSSD assembly of the cell matrix, 24 Jacobi-PCG solves
+    to 1e-8 and the Galerkin product.  Cells run concurrently on the host
+    threads (one cell per thread, as an OpenMP loop over cells would); a
+    sample of `per_n` cells per subdivision is timed and the batch total is
+    extrapolated with the cell-size histogram."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle as orc
+    from multiscalefemsolver3d_amd import problems
+    threads = min(os.cpu_count() or 1, 16)
+    sample = []
+    for n in sorted(counts):
+        sample += [int(c) for c in np.nonzero(p["cell_n"] == n)[0][:per_n]]
+
+    def one(c):
+        n = int(p["cell_n"][c])
+        t0 = time.perf_counter()
+        orc.msfem_cell(n, tuple(p["origin"][c]), p["h"], problems.cell_E(p, c), p["nu"], tol=1e-8)
+        return n, time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, sample))
+    wall = time.perf_counter() - t0
+    per = {n: float(np.mean([t for m, t in res if m == n])) for n in counts}
+    total_cpu = sum(counts[n] * per[n] for n in counts)
+    est_ms = total_cpu / threads * 1e3
+    return {"value": round(est_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} cells ({per_n} per n) solved concurrently on {threads} threads in "
+                      f"{wall:.1f}s; per-cell seconds under that load {dict((f'n={n}', round(t, 3)) for n, t in per.items())}"
+                      f"; batch total = sum(count_n * t_n) / {threads}"}
+
+
+def bench_learned(mfs, ctx, n_test):
+    """learned initial guess for the MsFEM local problems at fine
+    n = 8: 20,000 training cells solved by the batched solver, a 4-layer MLP
+    of width 256 trained on the GPU (BF16 MFMA, FP32 accumulation), then the
+    32,768 cells of a 32^3 coarse grid solved to 1e-8 from the zero, the
+    trilinear-hat and the learned guess."""
+    from multiscalefemsolver3d_amd import learned
+    try:
+        return learned.run(ctx, n=8, n_train=20000, n_test=n_test, hidden=256, layers=4, epochs=20, batch=256,
+                           lr=1e-3, tol=1e-8, log=log)
+    except Exception as e:  # reported, not fatal for the primary measurement
+        log(f"[bench] learned: failed: {e}")
+        return {"error": str(e)}
+
+
+def bench_large(mfs, problems, ctx, n, solves=True):
     """n^3 elastic cube, two-phase E in {1e9, 2e11}; SpMV GB/s at
     HBM scale (FP32 BSR3 working set ~1.3 GB), assembly time, mixed vs FP64
     Jacobi-PCG, GMRES(30) (textbook convention: restarted GMRES stagnates on
@@ -263,11 +352,16 @@ def bench_large(mfs, problems, ctx, n):
         res[f"spmv_{name}_gbs"] = round(gbs, 1)
         res[f"spmv_{name}_frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
         res[f"spmv_{name}_bytes"] = A.spmv_bytes(prec)
+    if not solves:
+        S.close()
+        log(f"[bench] large: {res}")
+        return res
     x = mfs.Vec(ctx, A.n)
-    # reference element convention: symmetric indefinite -> MINRES; CG shown
-    # for contrast (capped)
-    for name, kw in (("minres_fp64", dict(method=mfs.MINRES, precision=mfs.PREC_F64, maxit=30000)),
-                     ("minres_mixed", dict(method=mfs.MINRES, precision=mfs.PREC_MIXED, maxit=30000)),
+    # reference element convention: symmetric indefinite -> MINRES (capped:
+    # it does not reach 1e-8 on this operator in tens of thousands of
+    # iterations); CG shown for contrast (capped)
+    for name, kw in (("minres_fp64_capped", dict(method=mfs.MINRES, precision=mfs.PREC_F64, maxit=4000)),
+                     ("minres_mixed_capped", dict(method=mfs.MINRES, precision=mfs.PREC_MIXED, maxit=4000)),
                      ("pcg_fp64_capped", dict(precision=mfs.PREC_F64, maxit=3000))):
         x.copy_from(S.x0)
         rep = mfs.Solver(A, tol=1e-8, **kw).run(S.b, x, check=False)
@@ -282,8 +376,8 @@ def bench_large(mfs, problems, ctx, n):
     S.run(p["E"], p["nu"])
     for name, kw in (("pcg_fp64_textbook", dict(precision=mfs.PREC_F64, maxit=40000)),
                      ("pcg_mixed_textbook", dict(precision=mfs.PREC_MIXED, maxit=40000)),
-                     ("gmres30_mixed_textbook", dict(method=mfs.GMRES, precision=mfs.PREC_MIXED, restart=30,
-                                                     maxit=6000))):
+                     ("gmres30_mixed_textbook_capped", dict(method=mfs.GMRES, precision=mfs.PREC_MIXED,
+                                                            restart=30, maxit=3000))):
         x = mfs.Vec(ctx, S.A.n)
         x.copy_from(S.x0)
         rep = mfs.Solver(S.A, tol=1e-8, **kw).run(S.b, x, check=False)

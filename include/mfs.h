@@ -154,6 +154,15 @@ int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solver* out);
 int mfs_solver_destroy(mfs_solver S);
 /* Solve A x = b; x holds the initial guess on entry (FP64 vectors). */
 int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_report* report);
+/* Time the two kernels of one (P)CG iteration of a CG-method solver in its
+ * inner precision (FP32 for MFS_PREC_MIXED): kernel 0 = direction update
+ * fused with the SpMV (q = A (z + beta p)), kernel 1 = the x/r/z update.
+ * Each kernel is launched reps times back to back on the context stream and
+ * timed with HIP events; *bytes receives its algorithmic HBM bytes per
+ * launch.  Overwrites the solver's work vectors (a later run starts afresh).
+ * No reference counterpart: measures the kernels replacing the inner loop of
+ * CSSD_Matrix::MSG / BiCG_Diag (Library/CSSD_Matrix.h:1496-1663). */
+int mfs_solver_kernel_time(mfs_solver S, int kernel, int reps, double* avg_ms, double* bytes);
 /* One-shot convenience: create + run + destroy. */
 int mfs_solve(mfs_mat A, const mfs_solve_desc* desc, mfs_vec b, mfs_vec x, mfs_solve_report* report);
 

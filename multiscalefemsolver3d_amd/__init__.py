@@ -125,6 +125,7 @@ _sig("mfs_solve_desc_default", None, [C.POINTER(SolveDesc)])
 _sig("mfs_solver_create", C.c_int, [vp, C.POINTER(SolveDesc), C.POINTER(vp)])
 _sig("mfs_solver_destroy", C.c_int, [vp])
 _sig("mfs_solver_run", C.c_int, [vp, vp, vp, C.POINTER(SolveReport)])
+_sig("mfs_solver_kernel_time", C.c_int, [vp, C.c_int, C.c_int, c_dp, c_dp])
 _sig("mfs_solve", C.c_int, [vp, C.POINTER(SolveDesc), vp, vp, C.POINTER(SolveReport)])
 _sig("mfs_assemble_create", C.c_int, [vp, C.POINTER(FemDesc), C.POINTER(vp)])
 _sig("mfs_assemble_run", C.c_int, [vp, c_dp, C.c_double])
@@ -449,6 +450,13 @@ class Solver:
             _chk(code, "mfs_solver_run")
         return Report(rep.iterations, rep.outer_steps, rep.true_relres, rep.recursive_relres, rep.seconds,
                       rep.status)
+
+    def kernel_time(self, kernel, reps=200):
+        """(avg ms, algorithmic bytes) of one PCG iteration kernel: 0 = fused
+        direction + SpMV, 1 = x/r/z update (inner precision)."""
+        ms, by = C.c_double(), C.c_double()
+        _chk(lib.mfs_solver_kernel_time(self.h, kernel, reps, C.byref(ms), C.byref(by)), "mfs_solver_kernel_time")
+        return ms.value, by.value
 
     def close(self):
         if self.h:

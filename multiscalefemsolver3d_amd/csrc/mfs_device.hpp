@@ -188,35 +188,65 @@ __device__ __forceinline__ bool grid_sum(T (&v)[NV], double* partials, unsigned*
 }
 
 // SELL-64 row product: thread `lane` of slice `slice` accumulates its block
-// row of A x into acc[BS].  Values of entry j, component e live at
-// val[(slice_base + j*64)*BS*BS + e*64 + lane]; matrix streams are read with
-// non-temporal loads (used once per product), x through the caches.
-template <typename T, int BS>
-__device__ __forceinline__ void sell_row(const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
-                                         const int32_t* __restrict__ col, const T* __restrict__ val,
-                                         const T* __restrict__ x, int64_t slice, int lane, T (&acc)[BS]) {
+// row of A X into acc[BS], where X(i) is supplied by the functor xl (x[i],
+// or z[i] + beta p[i] for the fused CG kernel).  Values of entry j,
+// component e live at val[(slice_base + j*64)*BS*BS + e*64 + lane]; matrix
+// streams are read with non-temporal loads (used once per product), X
+// through the caches.
+//
+// The row loop has a runtime trip count, so the compiler would issue one
+// index load, wait, gather, wait per entry (two memory latencies each).
+// Here the indices and values of U entries are loaded first and the U
+// gathers are then issued together: two latencies per U entries.  Entries
+// past the slice width re-read the last entry (a valid index) and use value
+// 0, so every load is unconditional.
+template <typename T, int BS, typename XL>
+__device__ __forceinline__ void sell_row_x(const int64_t* __restrict__ slice_base,
+                                           const int32_t* __restrict__ slice_width, const int32_t* __restrict__ col,
+                                           const T* __restrict__ val, int64_t slice, int lane, int64_t nb, XL xl,
+                                           T (&acc)[BS]) {
   constexpr int C = 64;
+  constexpr int U = BS == 1 ? 8 : 4;
+  constexpr int B2 = BS * BS;
 #pragma unroll
   for (int a = 0; a < BS; a++) acc[a] = 0;
+  (void)nb;
   const int64_t sb = slice_base[slice];
   const int w = slice_width[slice];
   const int32_t* cp = col + sb + lane;
-  const T* vp = val + sb * (BS * BS) + lane;
-  for (int j = 0; j < w; j++) {
-    const int32_t c = __builtin_nontemporal_load(cp + (int64_t)j * C);
-    if constexpr (BS == 1) {
-      acc[0] += __builtin_nontemporal_load(vp + (int64_t)j * C) * x[c];
-    } else {
-      const T x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
-      const T* vj = vp + (int64_t)j * (BS * BS) * C;
+  const T* vp = val + sb * B2 + lane;
+  for (int j0 = 0; j0 < w; j0 += U) {
+    int32_t c[U];
+    T v[U][B2];
 #pragma unroll
-      for (int a = 0; a < BS; a++) {
-        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 0) * C) * x0;
-        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 1) * C) * x1;
-        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 2) * C) * x2;
+    for (int u = 0; u < U; u++) {
+      const int j = min(j0 + u, w - 1);
+      const bool in = j0 + u < w;
+      c[u] = __builtin_nontemporal_load(cp + (int64_t)j * C);
+#pragma unroll
+      for (int e = 0; e < B2; e++) {
+        const T t = __builtin_nontemporal_load(vp + ((int64_t)j * B2 + e) * C);
+        v[u][e] = in ? t : (T)0;
       }
     }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      T xv[BS];
+#pragma unroll
+      for (int b = 0; b < BS; b++) xv[b] = xl((int64_t)BS * c[u] + b);
+#pragma unroll
+      for (int a = 0; a < BS; a++)
+#pragma unroll
+        for (int b = 0; b < BS; b++) acc[a] += v[u][BS * a + b] * xv[b];
+    }
   }
+}
+
+template <typename T, int BS>
+__device__ __forceinline__ void sell_row(const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
+                                         const int32_t* __restrict__ col, const T* __restrict__ val,
+                                         const T* __restrict__ x, int64_t slice, int lane, int64_t nb, T (&acc)[BS]) {
+  sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb, [&](int64_t i) { return x[i]; }, acc);
 }
 
 namespace mfs {

@@ -5,7 +5,7 @@
 // whose residual exceeds the zero guess's).
 //
 // GEMM: C[M][N] = sum_k A(m,k) B(k,n), A/B bf16 with arbitrary strides,
-// 64x64 output tile per 256-thread workgroup, 4 waves of 32x32, each a 2x2
+// 128x128 output tile per 256-thread workgroup, 4 waves of 64x64, each a 4x4
 // grid of v_mfma_f32_16x16x32_bf16, K staged through LDS in steps of 32.
 #include <hip/hip_runtime.h>
 
@@ -40,61 +40,121 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 
 enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_RELU = 2 };
 
-template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A, int64_t sam,
-                                                   int64_t sak, const uint16_t* __restrict__ B, int64_t sbk,
-                                                   int64_t sbn, float* __restrict__ C, int64_t ldc,
-                                                   const float* __restrict__ bias, uint16_t* __restrict__ Cbf) {
-  __shared__ __align__(16) uint16_t As[64][40];
-  __shared__ __align__(16) uint16_t Bs[64][40];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  f32x4 acc[2][2];
+// Load a TM x 32 (rows x k) bf16 tile into LDS dst[row][k] (row stride 40).
+// The operand element (row, k) is at src[row*s_row + k*s_k].  When k or row is
+// the contiguous dimension and the chunk is in bounds and 16-byte aligned, 8
+// elements move with one 16-byte load.
+template <int TM>
+__device__ __forceinline__ void load_tile(uint16_t (*dst)[40], const uint16_t* __restrict__ src, int64_t s_row,
+                                          int64_t s_k, int row0, int k0, int R, int K) {
+  const int tid = threadIdx.x;
+  if (s_k == 1) {
+    // chunks of 8 consecutive k: TM rows x 4 chunks
+    for (int c = tid; c < TM * 4; c += 256) {
+      const int row = c >> 2, kk = (c & 3) * 8;
+      const int gr = row0 + row, gk = k0 + kk;
+      const uint16_t* ptr = src + gr * s_row + gk;
+      if (gr < R && gk + 8 <= K && ((uintptr_t)ptr & 15) == 0) {
+        const uint4 v = *reinterpret_cast<const uint4*>(ptr);
+        *reinterpret_cast<uint4*>(&dst[row][kk]) = v;
+      } else {
 #pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0, 0, 0, 0};
-  const bool a_kfast = sak == 1, b_kfast = sbk == 1;
-  for (int k0 = 0; k0 < K; k0 += 32) {
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int idx = tid + 256 * i;
-      // map consecutive threads along the operand's contiguous dimension
-      const int mm = a_kfast ? idx / 32 : idx % 64, kk = a_kfast ? idx % 32 : idx / 64;
-      const int gm = m0 + mm, gk = k0 + kk;
-      As[mm][kk] = (gm < M && gk < K) ? A[gm * sam + gk * sak] : (uint16_t)0;
-      const int nn = b_kfast ? idx / 32 : idx % 64, kb = b_kfast ? idx % 32 : idx / 64;
-      const int gn = n0 + nn, gkb = k0 + kb;
-      Bs[nn][kb] = (gn < N && gkb < K) ? B[gkb * sbk + gn * sbn] : (uint16_t)0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int bi = 0; bi < 2; bi++) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + bi * 16 + (lane & 15)][8 * (lane >> 4)]);
-#pragma unroll
-      for (int bj = 0; bj < 2; bj++) {
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + bj * 16 + (lane & 15)][8 * (lane >> 4)]);
-        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[bi][bj], 0, 0, 0);
+        for (int e = 0; e < 8; e++) dst[row][kk + e] = (gr < R && gk + e < K) ? ptr[e] : (uint16_t)0;
       }
     }
+  } else {
+    // chunks of 8 consecutive rows (row contiguous, s_row == 1): 32 k x TM/8
+    for (int c = tid; c < TM * 4; c += 256) {
+      const int kk = c / (TM / 8), row = (c % (TM / 8)) * 8;
+      const int gr = row0 + row, gk = k0 + kk;
+      const uint16_t* ptr = src + gr * s_row + gk * s_k;
+      uint16_t v[8];
+      if (gk < K && gr + 8 <= R && s_row == 1 && ((uintptr_t)ptr & 15) == 0) {
+        *reinterpret_cast<uint4*>(v) = *reinterpret_cast<const uint4*>(ptr);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = (gk < K && gr + e < R) ? src[(gr + e) * s_row + gk * s_k] : (uint16_t)0;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e++) dst[row + e][kk] = v[e];
+    }
+  }
+}
+
+// C[M][N] = sum_k A(m,k) B(k,n): 128x128 tile per 256-thread workgroup, each
+// wave a 64x64 quadrant as 4x4 v_mfma_f32_16x16x32_bf16 tiles.  Split-K:
+// blockIdx.z covers k in [z*Kc, min(K, (z+1)*Kc)); with ws != nullptr the
+// raw partial tile goes to ws[z][M][N] (summed by k_splitk_reduce in split
+// order, so results do not depend on scheduling), else the epilogue writes C.
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int Kc, const uint16_t* __restrict__ A,
+                                                   int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
+                                                   int64_t sbk, int64_t sbn, float* __restrict__ C, int64_t ldc,
+                                                   const float* __restrict__ bias, uint16_t* __restrict__ Cbf,
+                                                   float* __restrict__ ws) {
+  __shared__ __align__(16) uint16_t As[128][40];
+  __shared__ __align__(16) uint16_t Bs[128][40];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+  const int kb = blockIdx.z * Kc, ke = min(K, kb + Kc);
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    load_tile<128>(As, A, sam, sak, m0, k0, M, ke);
+    load_tile<128>(Bs, B, sbn, sbk, n0, k0, N, ke);
+    __syncthreads();
+    bf16x8 af[4], bfr[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      af[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 64 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
 #pragma unroll
-  for (int bi = 0; bi < 2; bi++)
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int bj = 0; bj < 2; bj++)
+    for (int j = 0; j < 4; j++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int row = m0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + i;
-        const int col = n0 + wn * 32 + bj * 16 + (lane & 15);
+      for (int e = 0; e < 4; e++) {
+        const int row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + e;
+        const int col = n0 + wn * 64 + j * 16 + (lane & 15);
         if (row >= M || col >= N) continue;
-        float v = acc[bi][bj][i];
+        float v = acc[i][j][e];
+        if (ws) {
+          ws[((int64_t)blockIdx.z * M + row) * N + col] = v;
+          continue;
+        }
         if (EPI != EPI_NONE) v += bias[col];
         if (EPI == EPI_BIAS_RELU) v = v > 0.f ? v : 0.f;
         C[(int64_t)row * ldc + col] = v;
         if (Cbf) Cbf[(int64_t)row * ldc + col] = f2bf(v);
       }
+}
+
+// C = sum over the splits of ws[z] (in split order), then the epilogue
+template <int EPI>
+__global__ void k_splitk_reduce(const float* __restrict__ ws, int splits, int M, int N, float* __restrict__ C,
+                                int64_t ldc, const float* __restrict__ bias, uint16_t* __restrict__ Cbf) {
+  const int64_t MN = (int64_t)M * N;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < MN; q += (int64_t)gridDim.x * blockDim.x) {
+    float v = 0.f;
+    for (int z = 0; z < splits; z++) v += ws[z * MN + q];
+    const int64_t row = q / N, col = q % N;
+    if (EPI != EPI_NONE) v += bias[col];
+    if (EPI == EPI_BIAS_RELU) v = v > 0.f ? v : 0.f;
+    C[row * ldc + col] = v;
+    if (Cbf) Cbf[row * ldc + col] = f2bf(v);
+  }
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
@@ -182,6 +242,8 @@ struct mfs_mlp_s {
   std::vector<uint16_t*> actbf;  // bf16 [cap][dims[l]] (l = 0..L-1)
   std::vector<float*> dact;      // FP32 grads [cap][dims[l]] (l = 1..L)
   std::vector<uint16_t*> dactbf;
+  float* ws = nullptr;  // split-K partial tiles
+  size_t ws_bytes = 0;
   int* d_idx = nullptr;
   double* d_loss = nullptr;
 };
@@ -195,6 +257,7 @@ static int mlp_free(mfs_mlp M) {
     for (uint16_t* p : *v) dfree(p);
   dfree(M->d_idx);
   dfree(M->d_loss);
+  dfree(M->ws);
   mfs_ctx c = M->ctx;
   delete M;
   ctx_release(c);
@@ -202,6 +265,8 @@ static int mlp_free(mfs_mlp M) {
 }
 
 extern "C" int mfs_mlp_destroy(mfs_mlp M) { return mlp_free(M); }
+
+static size_t gemm_ws_bytes(mfs_mlp M, int B);
 
 static int mlp_capacity(mfs_mlp M, int batch) {
   if (batch <= M->cap) return MFS_OK;
@@ -231,6 +296,14 @@ static int mlp_capacity(mfs_mlp M, int batch) {
   }
   MFS_TRY(dmalloc((void**)&M->d_idx, sizeof(int) * batch));
   MFS_TRY(dmalloc((void**)&M->d_loss, sizeof(double) * 65536));
+  const size_t wsb = gemm_ws_bytes(M, batch);
+  if (wsb > M->ws_bytes) {
+    dfree(M->ws);
+    M->ws = nullptr;
+    M->ws_bytes = 0;
+    MFS_TRY(dmalloc((void**)&M->ws, wsb));
+    M->ws_bytes = wsb;
+  }
   M->cap = batch;
   return MFS_OK;
 }
@@ -314,25 +387,75 @@ extern "C" int mfs_mlp_set_params(mfs_mlp M, int l, const float* W, const float*
   return MFS_OK;
 }
 
-static void gemm(hipStream_t s, int epi, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak,
+// Split-K factor: enough workgroups to cover the device when the output has
+// few 128x128 tiles and K is long; every split keeps at least 256 of K.
+static int gemm_splits(int M, int N, int K, int* Kc) {
+  const int64_t tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+  int sp = (int)std::min<int64_t>(std::max<int64_t>(1, 512 / tiles), std::max(1, K / 256));
+  sp = std::min(sp, 64);
+  int kc = (K + sp - 1) / sp;
+  kc = (kc + 31) / 32 * 32;
+  sp = (K + kc - 1) / kc;
+  *Kc = kc;
+  return sp;
+}
+
+template <int EPI>
+static void gemm_epi(hipStream_t s, float* ws, size_t ws_bytes, int M, int N, int K, const uint16_t* A, int64_t sam,
+                     int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc,
+                     const float* bias, uint16_t* Cbf) {
+  int Kc;
+  int sp = gemm_splits(M, N, K, &Kc);
+  if (sp > 1 && (!ws || (size_t)sp * M * N * sizeof(float) > ws_bytes)) {
+    sp = 1;  // no room for the partial tiles: one pass over K
+    Kc = K;
+  }
+  dim3 grid((N + 127) / 128, (M + 127) / 128, sp);
+  if (sp == 1) {
+    k_gemm_bf16<EPI><<<grid, 256, 0, s>>>(M, N, K, Kc, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf, nullptr);
+  } else {
+    k_gemm_bf16<EPI_NONE><<<grid, 256, 0, s>>>(M, N, K, Kc, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf, ws);
+    const int64_t mn = (int64_t)M * N;
+    k_splitk_reduce<EPI><<<(unsigned)std::min<int64_t>(4096, (mn + 255) / 256), 256, 0, s>>>(ws, sp, M, N, C, ldc,
+                                                                                             bias, Cbf);
+  }
+}
+
+static void gemm(mfs_mlp Mp, int epi, int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak,
                  const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, const float* bias, uint16_t* Cbf) {
-  dim3 grid((N + 63) / 64, (M + 63) / 64);
+  hipStream_t s = Mp->ctx->stream;
   if (epi == EPI_NONE)
-    k_gemm_bf16<EPI_NONE><<<grid, 256, 0, s>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
+    gemm_epi<EPI_NONE>(s, Mp->ws, Mp->ws_bytes, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
   else if (epi == EPI_BIAS)
-    k_gemm_bf16<EPI_BIAS><<<grid, 256, 0, s>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
+    gemm_epi<EPI_BIAS>(s, Mp->ws, Mp->ws_bytes, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
   else
-    k_gemm_bf16<EPI_BIAS_RELU><<<grid, 256, 0, s>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
+    gemm_epi<EPI_BIAS_RELU>(s, Mp->ws, Mp->ws_bytes, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf);
+}
+
+// workspace for the split-K GEMMs of a batch size (forward and backward)
+static size_t gemm_ws_bytes(mfs_mlp M, int B) {
+  size_t need = 0;
+  auto one = [&](int m, int n, int k) {
+    int kc;
+    const int sp = gemm_splits(m, n, k, &kc);
+    if (sp > 1) need = std::max(need, (size_t)sp * m * n * sizeof(float));
+  };
+  for (int l = 0; l < M->L; l++) {
+    const int nin = M->dims[l], nout = M->dims[l + 1];
+    one(B, nout, nin);
+    one(nout, nin, B);
+    if (l > 0) one(B, nin, nout);
+  }
+  return need;
 }
 
 // forward on actbf[0] (already filled, B rows): act[L] holds the output
 static void mlp_forward_dev(mfs_mlp M, int B) {
-  hipStream_t s = M->ctx->stream;
   for (int l = 0; l < M->L; l++) {
     const int nin = M->dims[l], nout = M->dims[l + 1];
     const bool last = l == M->L - 1;
     // Y[B][out] = X[B][in] W[out][in]^T + b
-    gemm(s, last ? EPI_BIAS : EPI_BIAS_RELU, B, nout, nin, M->actbf[l], nin, 1, M->Wbf[l], 1, nin, M->act[l + 1], nout,
+    gemm(M, last ? EPI_BIAS : EPI_BIAS_RELU, B, nout, nin, M->actbf[l], nin, 1, M->Wbf[l], 1, nin, M->act[l + 1], nout,
          M->b[l], last ? nullptr : M->actbf[l + 1]);
   }
 }
@@ -343,11 +466,11 @@ static void mlp_backward_dev(mfs_mlp M, int B) {
   for (int l = M->L - 1; l >= 0; l--) {
     const int nin = M->dims[l], nout = M->dims[l + 1];
     // gW[out][in] = sum_m dY[m][out] X[m][in]
-    gemm(s, EPI_NONE, nout, nin, B, M->dactbf[l + 1], 1, nout, M->actbf[l], nin, 1, M->gW[l], nin, nullptr, nullptr);
+    gemm(M, EPI_NONE, nout, nin, B, M->dactbf[l + 1], 1, nout, M->actbf[l], nin, 1, M->gW[l], nin, nullptr, nullptr);
     k_colsum<<<g1d(nout), 256, 0, s>>>(M->dact[l + 1], B, nout, M->gb[l]);
     if (l > 0) {
       // dX[m][in] = sum_o dY[m][o] W[o][in], then ReLU mask
-      gemm(s, EPI_NONE, B, nin, nout, M->dactbf[l + 1], nout, 1, M->Wbf[l], nin, 1, M->dact[l], nin, nullptr, nullptr);
+      gemm(M, EPI_NONE, B, nin, nout, M->dactbf[l + 1], nout, 1, M->Wbf[l], nin, 1, M->dact[l], nin, nullptr, nullptr);
       k_relu_back<<<g1d((int64_t)B * nin), 256, 0, s>>>(M->dact[l], M->act[l], M->dactbf[l], (int64_t)B * nin);
     }
   }

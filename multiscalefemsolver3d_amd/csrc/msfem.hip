@@ -45,6 +45,10 @@ static const int h_stencil[NS][3] = {{-1, -1, -1}, {0, -1, -1}, {-1, 0, -1}, {0,
                                      {0, -1, 0},   {-1, 0, 0},  {0, 0, 0},   {1, 0, 0},   {0, 1, 0},
                                      {1, 1, 0},    {0, 0, 1},   {1, 0, 1},   {0, 1, 1},   {1, 1, 1}};
 constexpr int S_CENTER = 7;
+// the same table as a compile-time constant for the unrolled device loops
+__device__ constexpr int kStencil[NS][3] = {{-1, -1, -1}, {0, -1, -1}, {-1, 0, -1}, {0, 0, -1}, {-1, -1, 0},
+                                            {0, -1, 0},   {-1, 0, 0},  {0, 0, 0},   {1, 0, 0},   {0, 1, 0},
+                                            {1, 1, 0},    {0, 0, 1},   {1, 0, 1},   {0, 1, 1},   {1, 1, 1}};
 
 struct Cell {
   int n;        // fine hexes per edge
@@ -225,6 +229,14 @@ __global__ __launch_bounds__(256) void k_cell_assemble(const Cell* __restrict__ 
 // Stencil SpMM for one interior node and RG right-hand sides:
 // acc[a][r] = sum_s sum_b K[s][a][b] * P(nbr_s)[b][r]
 // ---------------------------------------------------------------------------
+// Stencil offsets packed two bits per point ((d + 1) << 2s) for the loop below.
+constexpr uint32_t pack_offsets(int dim) {
+  uint32_t w = 0;
+  for (int s = 0; s < NS; s++) w |= (uint32_t)(kStencil[s][dim] + 1) << (2 * s);
+  return w;
+}
+constexpr uint32_t kOffI = pack_offsets(0), kOffJ = pack_offsets(1), kOffK = pack_offsets(2);
+
 template <typename KT, typename VT, typename Load>
 __device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint, int v, int i, int j, int k, int m,
                                             Load load, VT (&acc)[3][RG]) {
@@ -232,14 +244,29 @@ __device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint,
   for (int a = 0; a < 3; a++)
 #pragma unroll
     for (int r = 0; r < RG; r++) acc[a][r] = 0;
+  // The 9 values of stencil point s+1 are loaded while point s is applied
+  // (one point of prefetch keeps the register count at two waves per SIMD).
+  const KT* kp = Kc + v;
+  KT kn[9];
 #pragma unroll
+  for (int e = 0; e < 9; e++) kn[e] = kp[(int64_t)e * nint];
+#pragma unroll 1
   for (int s = 0; s < NS; s++) {
-    const int ni = i + c_stencil[s][0], nj = j + c_stencil[s][1], nk = k + c_stencil[s][2];
-    if (ni < 1 || ni > m || nj < 1 || nj > m || nk < 1 || nk > m) continue;
-    const int u = interior_index(ni, nj, nk, m);
     VT kv[9];
 #pragma unroll
-    for (int e = 0; e < 9; e++) kv[e] = (VT)Kc[(int64_t)(s * 9 + e) * nint + v];
+    for (int e = 0; e < 9; e++) kv[e] = (VT)kn[e];
+    if (s + 1 < NS) {
+      kp += (int64_t)9 * nint;
+#pragma unroll
+      for (int e = 0; e < 9; e++) kn[e] = kp[(int64_t)e * nint];
+    }
+    // Blocks coupling to boundary nodes are stored as zeros; their column
+    // index is replaced by the row itself, so no lane skips a point.
+    const int ni = i + (int)((kOffI >> (2 * s)) & 3u) - 1;
+    const int nj = j + (int)((kOffJ >> (2 * s)) & 3u) - 1;
+    const int nk = k + (int)((kOffK >> (2 * s)) & 3u) - 1;
+    const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
+    const int u = in ? interior_index(ni, nj, nk, m) : v;
 #pragma unroll
     for (int b = 0; b < 3; b++) {
       VT pv[RG];
@@ -255,46 +282,84 @@ __device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint,
 }
 
 // ---------------------------------------------------------------------------
+// Workgroup all-reduce of N per-thread values (N a power of two, 8..32):
+// a butterfly reduce-scatter inside the wave (log2 N + remaining steps of
+// shuffles instead of N full wave reductions), one barrier, then lane k sums
+// value k over the waves and every thread reads all N totals with readlane.
+// The summation order is fixed, so every thread obtains identical totals.
+// part: [waves][N] doubles in LDS; its previous contents must no longer be
+// needed by any thread (a barrier separates two uses of one buffer).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+template <int N>
+__device__ __forceinline__ void wg_allreduce(double (&v)[N], double* part, double (&tot)[N]) {
+  static_assert(N >= 2 && N <= 32 && (N & (N - 1)) == 0, "N must be a power of two in 2..32");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  constexpr int LOG = N == 2 ? 1 : N == 4 ? 2 : N == 8 ? 3 : N == 16 ? 4 : 5;
+  double a[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) a[i] = v[i];
+  // after step t (offset 32 >> t) a lane keeps the half selected by its bit
+#pragma unroll
+  for (int t = 0; t < LOG; t++) {
+    const int o = 32 >> t, h = N >> (t + 1);
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < h; i++) {
+      const double send = up ? a[i] : a[h + i];
+      const double keep = up ? a[h + i] : a[i];
+      a[i] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+#pragma unroll
+  for (int o = 32 >> LOG; o > 0; o >>= 1) a[0] += __shfl_xor(a[0], o, 64);
+  // value index held by this lane: its top LOG bits (bit 5 most significant)
+  const int idx = lane >> (6 - LOG);
+  if ((lane & ((64 >> LOG) - 1)) == 0) part[wave * N + idx] = a[0];
+  __syncthreads();
+  double s = 0.0;
+  if (lane < N)
+    for (int w = 0; w < nw; w++) s += part[w * N + lane];
+#pragma unroll
+  for (int k = 0; k < N; k++) tot[k] = readlane_f64(s, k);
+}
+
+// ---------------------------------------------------------------------------
 // Persistent batched solve.  T = inner precision (float: FP32 inner with FP64
 // refinement; double: plain FP64 PCG).  Dynamic LDS: p [nint][3][RG] of T.
-// Per-RHS scalars (norms, alpha, beta, stop flags) are uniform across the
-// workgroup and live in LDS; threads own interior nodes (all RG RHS each).
+// Threads own interior nodes (all RG right-hand sides each).  Per-iteration
+// scalars (alpha, beta, the active set) are computed redundantly by every
+// thread from the all-reduced totals, so an inner iteration has three
+// barriers (two reductions and the p update).  Per-RHS values that persist
+// across iterations are written by thread 0 only, double-buffered where they
+// are read in the same phase.
+//
+// After an FP64 residual replacement the FP32 iteration is continued, not
+// restarted: the search direction kept in LDS is rescaled to the new units
+// and the first beta uses the previous iteration's r.z.
 // ---------------------------------------------------------------------------
 struct RhsState {
-  double bb[RG], rr64[RG], prev[RG], scale[RG], itol2[RG], rz[RG], rr[RG];
-  double alpha[RG], beta[RG];
-  int done[RG], failed[RG], act[RG], it[RG];
-  int all_done, any_act, guess_any;
+  double bb[RG], rr64[RG], prev[RG], scale[RG], itol2[RG];
+  double rz[2][RG];   // r.z of the current iterate, by iteration parity
+  double rzkeep[RG];  // r.z of the iteration before the last (continuation)
+  double rrpred[RG];  // recursive r.r at the end of the inner solve (scaled)
+  double rrmin[2][RG];  // stagnation watch: last r.r that was 4x below the
+  int itmin[2][RG];     // previous mark, and the inner iteration it was seen
+  double ratio[RG];   // previous scale / new scale
+  int done[RG], failed[RG], warm[RG], itbase[RG], it[RG];
 };
-
-// Sum NV*RG per-thread partials v[k][r] over the workgroup into tot[k][r]
-// (LDS, valid after the call in every thread).
-template <int NV>
-__device__ __forceinline__ void wg_sum_rg(double (&v)[NV][RG], double* part /* [16][NV*RG] */,
-                                          double* tot /* [NV*RG] */) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; k++)
-#pragma unroll
-    for (int r = 0; r < RG; r++) {
-      const double s = wave_sum(v[k][r]);
-      if (lane == 0) part[wave * NV * RG + k * RG + r] = s;
-    }
-  __syncthreads();
-  if (threadIdx.x < NV * RG) {
-    double s = 0;
-    for (int w = 0; w < nw; w++) s += part[w * NV * RG + threadIdx.x];
-    tot[threadIdx.x] = s;
-  }
-  __syncthreads();
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* P = reinterpret_cast<T*>(smem_raw);
-  __shared__ double part[16 * 2 * RG];
-  __shared__ double tot[2 * RG];
+  __shared__ double partA[4 * 2 * RG];
+  __shared__ double partB[4 * 2 * RG];
   __shared__ RhsState S;
   __shared__ int task_s;
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -316,27 +381,36 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
     double* x64 = A.x64 + C.int0 * (3 * NB);
     const T* dinv = fp64 ? (const T*)(A.dinv64 + C.int0 * 3) : (const T*)(A.dinv32 + C.int0 * 3);
     const int q0 = grp * RG;
+    // FP32 inner solves that need more than a few times the FP64 count are
+    // handed to the FP64 pass (CG iterations grow like n for these cells)
+    const int inner_cap = min(A.inner_cap, fp64 ? A.inner_cap : 20 * n + 60);
+    // an FP32 inner solve whose recursive r.r has not dropped 4x within this
+    // many iterations is stagnating (FP32 cannot resolve the problem)
+    constexpr int kStagWindow = 60;
 
     // ||b||^2 per RHS
     {
-      double w[1][RG];
+      double w[RG], tot[RG];
 #pragma unroll
-      for (int r = 0; r < RG; r++) w[0][r] = 0;
+      for (int r = 0; r < RG; r++) w[r] = 0;
       for (int v = tid; v < nint; v += nth)
         for (int a = 0; a < 3; a++)
 #pragma unroll
           for (int r = 0; r < RG; r++) {
             const double bv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
-            w[0][r] += bv * bv;
+            w[r] += bv * bv;
           }
-      wg_sum_rg<1>(w, part, tot);
-      if (tid < RG) {
-        S.bb[tid] = tot[tid] > 0 ? tot[tid] : 1.0;
-        S.done[tid] = 0;
-        S.failed[tid] = 0;
-        S.it[tid] = 0;
-        S.prev[tid] = 1e300;
-      }
+      wg_allreduce<RG>(w, partA, tot);
+      if (tid == 0)
+        for (int r = 0; r < RG; r++) {
+          S.bb[r] = tot[r] > 0 ? tot[r] : 1.0;
+          S.done[r] = 0;
+          S.failed[r] = 0;
+          S.warm[r] = 0;
+          S.it[r] = 0;
+          S.prev[r] = 1e300;
+          S.scale[r] = 1.0;
+        }
     }
     if (!A.use_guess)
       for (int v = tid; v < nint; v += nth)
@@ -347,10 +421,11 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
     bool guess_checked = !A.use_guess;
     for (int outer = 0; outer < 64; outer++) {
       // true residual r64 = b - K64 x64 (FP64) into R (unscaled)
+      double rr64[RG];
       {
-        double w[1][RG];
+        double w[RG];
 #pragma unroll
-        for (int r = 0; r < RG; r++) w[0][r] = 0;
+        for (int r = 0; r < RG; r++) w[r] = 0;
         for (int v = tid; v < nint; v += nth) {
           const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
           double acc[3][RG];
@@ -364,114 +439,128 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
 #pragma unroll
             for (int r = 0; r < RG; r++) {
               const double rv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r] - acc[a][r];
-              w[0][r] += rv * rv;
+              w[r] += rv * rv;
               R[((int64_t)v * 3 + a) * RG + r] = (T)rv;
             }
         }
-        wg_sum_rg<1>(w, part, tot);
+        wg_allreduce<RG>(w, partB, rr64);
       }
       if (!guess_checked) {
         // a guess whose residual exceeds the zero guess's (= ||b||) is dropped
         guess_checked = true;
-        if (tid == 0) {
-          int any = 0;
-          for (int r = 0; r < RG; r++) any |= !(tot[r] <= S.bb[r]);
-          S.guess_any = any;
-        }
-        __syncthreads();
-        if (S.guess_any) {
+        int bad = 0;
+#pragma unroll
+        for (int r = 0; r < RG; r++) bad |= (!(rr64[r] <= S.bb[r])) << r;
+        if (bad) {
           for (int v = tid; v < nint; v += nth)
             for (int a = 0; a < 3; a++)
 #pragma unroll
               for (int r = 0; r < RG; r++)
-                if (!(tot[r] <= S.bb[r])) {
+                if ((bad >> r) & 1) {
                   x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
                   R[((int64_t)v * 3 + a) * RG + r] = (T)b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
                 }
-          __syncthreads();
-          if (tid < RG && !(tot[tid] <= S.bb[tid])) {
-            A.flags[cell * NB + q0 + tid] |= 2;
-            tot[tid] = S.bb[tid];
-          }
-          __syncthreads();
-        }
-      }
-      if (tid < RG) {
-        const int r = tid;
-        const double rr64 = tot[r];
-        S.rr64[r] = rr64;
-        // an outer step that did not reduce the true residual means the inner
-        // precision cannot solve this problem: stop it and report it
-        if (!S.done[r] && outer > 0 && !(rr64 < 0.9 * S.prev[r])) S.failed[r] = 1;
-        S.prev[r] = rr64;
-        S.done[r] = S.done[r] || S.failed[r] || (rr64 <= A.tol * A.tol * S.bb[r]) || !(rr64 == rr64);
-        S.scale[r] = rr64 > 0 ? sqrt(rr64) : 1.0;
-        const double rel = sqrt(rr64 / S.bb[r]);
-        double it = fp64 ? A.tol / rel : fmax(A.inner_tol, 0.5 * A.tol / rel);
-        it = fmin(it, 0.5);
-        S.itol2[r] = it * it;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        int all = 1, left = 0;
-        for (int r = 0; r < RG; r++) {
-          all &= S.done[r];
-          if (!S.done[r]) left = max(left, A.maxit - S.it[r]);
-        }
-        S.all_done = all || left <= 0;
-      }
-      __syncthreads();
-      if (S.all_done) break;
-      // start: R <- R/scale, D = 0, P = dinv R, rz, rr
-      {
-        double w[2][RG];
+          if (tid < RG && ((bad >> tid) & 1)) A.flags[cell * NB + q0 + tid] |= 2;
 #pragma unroll
-        for (int r = 0; r < RG; r++) w[0][r] = w[1][r] = 0;
+          for (int r = 0; r < RG; r++)
+            if ((bad >> r) & 1) rr64[r] = S.bb[r];
+        }
+      }
+      if (tid == 0)
+        for (int r = 0; r < RG; r++) {
+          const double rr = rr64[r];
+          S.rr64[r] = rr;
+          // an outer step that did not reduce the true residual means the
+          // inner precision cannot solve this problem: stop it and report it
+          if (!S.done[r] && outer > 0 && !(rr < 0.9 * S.prev[r])) S.failed[r] = 1;
+          S.prev[r] = rr;
+          // continue the FP32 iteration only if its recursion stayed
+          // consistent with the true residual (within 2x in norm)
+          if (S.warm[r] && !(rr <= 4.0 * S.rrpred[r] * S.scale[r] * S.scale[r])) S.warm[r] = 0;
+          S.done[r] = S.done[r] || S.failed[r] || (rr <= A.tol * A.tol * S.bb[r]) || !(rr == rr);
+          const double sc = rr > 0 ? sqrt(rr) : 1.0;
+          S.ratio[r] = S.scale[r] / sc;
+          S.scale[r] = sc;
+          const double rel = sqrt(rr / S.bb[r]);
+          double it = fp64 ? A.tol / rel : fmax(A.inner_tol, 0.5 * A.tol / rel);
+          it = fmin(it, 0.5);
+          S.itol2[r] = it * it;
+          S.itbase[r] = S.it[r];
+        }
+      __syncthreads();
+      int live = 0, left = 0;
+#pragma unroll
+      for (int r = 0; r < RG; r++)
+        if (!S.done[r]) {
+          live |= 1 << r;
+          left = max(left, A.maxit - S.it[r]);
+        }
+      if (!live || left <= 0) break;
+      // start: R <- R/scale, D = 0, z = dinv R; the direction is z (fresh)
+      // or z + beta ratio p_kept (continuation)
+      int act = 0;
+      int par = 0;
+      {
+        double w[2 * RG], tot[2 * RG];
+#pragma unroll
+        for (int r = 0; r < 2 * RG; r++) w[r] = 0;
+        T inv[RG];
+#pragma unroll
+        for (int r = 0; r < RG; r++) inv[r] = ((live >> r) & 1) ? (T)(1.0 / S.scale[r]) : (T)0;
         for (int v = tid; v < nint; v += nth)
           for (int a = 0; a < 3; a++) {
             const T dv = dinv[v * 3 + a];
 #pragma unroll
             for (int r = 0; r < RG; r++) {
               const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              const T rv = S.done[r] ? (T)0 : (T)((double)R[e] / S.scale[r]);
+              const T rv = ((live >> r) & 1) ? (T)((double)R[e] * (double)inv[r]) : (T)0;
               R[e] = rv;
               D[e] = 0;
               const T z = dv * rv;
-              P[e] = z;
-              w[0][r] += (double)(rv * z);
-              w[1][r] += (double)(rv * rv);
+              w[r] += (double)(rv * z);
+              w[RG + r] += (double)(rv * rv);
             }
           }
-        wg_sum_rg<2>(w, part, tot);
-        if (tid < RG) {
-          S.rz[tid] = tot[tid];
-          S.rr[tid] = tot[RG + tid];
-          S.act[tid] = !S.done[tid] && tot[RG + tid] > S.itol2[tid];
+        wg_allreduce<2 * RG>(w, partA, tot);
+        T beta[RG];
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          const bool on = ((live >> r) & 1) && tot[RG + r] > S.itol2[r];
+          act |= on << r;
+          const bool keep = on && S.warm[r] && S.rzkeep[r] > 0.0 && S.ratio[r] > 0.0;
+          beta[r] = keep ? (T)(tot[r] / (S.rzkeep[r] * S.ratio[r])) : (T)0;
         }
+        if (tid == 0)
+          for (int r = 0; r < RG; r++) {
+            S.rz[0][r] = tot[r];
+            S.rrmin[0][r] = tot[RG + r];
+            S.itmin[0][r] = 0;
+          }
+        for (int v = tid; v < nint; v += nth)
+          for (int a = 0; a < 3; a++) {
+            const T dv = dinv[v * 3 + a];
+#pragma unroll
+            for (int r = 0; r < RG; r++) {
+              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+              const T z = dv * R[e];
+              P[e] = beta[r] != (T)0 ? z + beta[r] * P[e] : z;
+            }
+          }
         __syncthreads();
       }
-      int inner_it = 0;
-      for (;;) {
-        if (tid == 0) {
-          int any = 0;
-          for (int r = 0; r < RG; r++) any |= S.act[r];
-          if (any && inner_it >= A.inner_cap) {
-            for (int r = 0; r < RG; r++)
-              if (S.act[r]) {
-                S.failed[r] = 1;
-                S.act[r] = 0;
-              }
-            any = 0;
-          }
-          S.any_act = any;
+      int inner_it = 0, capfail = 0;
+      while (act) {
+        if (inner_it >= inner_cap) {
+          capfail |= act;  // recorded by thread 0 after the x update
+          act = 0;
+          break;
         }
-        __syncthreads();
-        if (!S.any_act) break;
-        // q = K p, pq
+        // q = K p, p.q
+        T alpha[RG];
         {
-          double w[1][RG];
+          double w[RG], pq[RG];
 #pragma unroll
-          for (int r = 0; r < RG; r++) w[0][r] = 0;
+          for (int r = 0; r < RG; r++) w[r] = 0;
           for (int v = tid; v < nint; v += nth) {
             const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
             T acc[3][RG];
@@ -489,66 +578,87 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
               for (int r = 0; r < RG; r++) {
                 const int64_t e = ((int64_t)v * 3 + a) * RG + r;
                 Q[e] = acc[a][r];
-                w[0][r] += (double)(acc[a][r] * P[e]);
+                w[r] += (double)(acc[a][r] * P[e]);
               }
           }
-          wg_sum_rg<1>(w, part, tot);
-          if (tid < RG) {
-            const double pq = tot[tid];
-            S.alpha[tid] = (S.act[tid] && pq != 0.0) ? S.rz[tid] / pq : 0.0;
-          }
-          __syncthreads();
-        }
-        // x += alpha p, r -= alpha q, rz', rr'
-        {
-          double w[2][RG];
+          wg_allreduce<RG>(w, partB, pq);
 #pragma unroll
-          for (int r = 0; r < RG; r++) w[0][r] = w[1][r] = 0;
+          for (int r = 0; r < RG; r++)
+            alpha[r] = (((act >> r) & 1) && pq[r] != 0.0) ? (T)(S.rz[par][r] / pq[r]) : (T)0;
+        }
+        // x += alpha p, r -= alpha q, z = dinv r; r.z, r.r
+        int act2 = act;
+        T beta[RG];
+        {
+          double w[2 * RG], tot[2 * RG];
+#pragma unroll
+          for (int r = 0; r < 2 * RG; r++) w[r] = 0;
           for (int v = tid; v < nint; v += nth)
             for (int a = 0; a < 3; a++) {
               const T dv = dinv[v * 3 + a];
 #pragma unroll
               for (int r = 0; r < RG; r++) {
                 const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-                const T al = (T)S.alpha[r];
-                D[e] += al * P[e];
-                const T rv = R[e] - al * Q[e];
+                const T pe = P[e];
+                D[e] += alpha[r] * pe;
+                const T rv = R[e] - alpha[r] * Q[e];
                 R[e] = rv;
                 const T z = dv * rv;
-                w[0][r] += (double)(rv * z);
-                w[1][r] += (double)(rv * rv);
+                w[r] += (double)(rv * z);
+                w[RG + r] += (double)(rv * rv);
               }
             }
-          wg_sum_rg<2>(w, part, tot);
-          if (tid < RG) {
-            const int r = tid;
-            S.beta[r] = 0.0;
-            if (S.act[r]) {
-              S.it[r]++;
-              const double rz2 = tot[r], rr2 = tot[RG + r];
-              if (!(rr2 == rr2) || rr2 <= S.itol2[r] || S.it[r] >= A.maxit) {
-                S.act[r] = 0;
-              } else {
-                S.beta[r] = rz2 / S.rz[r];
-                S.rz[r] = rz2;
-              }
-              S.rr[r] = rr2;
+          wg_allreduce<2 * RG>(w, partA, tot);
+#pragma unroll
+          for (int r = 0; r < RG; r++) {
+            beta[r] = 0;
+            if (!((act >> r) & 1)) continue;
+            const double rz2 = tot[r], rr2 = tot[RG + r];
+            const bool stop = !(rr2 == rr2) || rr2 <= S.itol2[r] || S.itbase[r] + inner_it + 1 >= A.maxit;
+            bool stag = false;
+            if (!fp64 && !stop) {
+              const int itm = rr2 < 0.25 * S.rrmin[par][r] ? inner_it + 1 : S.itmin[par][r];
+              stag = inner_it + 1 - itm > kStagWindow;
             }
+            if (stop || stag) act2 &= ~(1 << r);
+            else beta[r] = (T)(rz2 / S.rz[par][r]);
+            if (stag) capfail |= 1 << r;  // recorded by thread 0 after the x update
           }
-          __syncthreads();
+          if (tid == 0)
+            for (int r = 0; r < RG; r++) {
+              const bool mark = ((act >> r) & 1) && tot[RG + r] < 0.25 * S.rrmin[par][r];
+              S.rrmin[par ^ 1][r] = mark ? tot[RG + r] : S.rrmin[par][r];
+              S.itmin[par ^ 1][r] = mark ? inner_it + 1 : S.itmin[par][r];
+              if (!((act >> r) & 1)) {
+                S.rz[par ^ 1][r] = S.rz[par][r];
+                continue;
+              }
+              if ((act2 >> r) & 1) {
+                S.rz[par ^ 1][r] = tot[r];
+              } else {
+                // stopped: keep (p, r.z) of this iteration for the next step
+                S.rz[par ^ 1][r] = S.rz[par][r];
+                S.rzkeep[r] = S.rz[par][r];
+                S.it[r] = S.itbase[r] + inner_it + 1;
+                S.warm[r] = tot[RG + r] <= S.itol2[r];
+                S.rrpred[r] = tot[RG + r];
+              }
+            }
         }
-        // p = z + beta p (active RHS only)
+        // p = z + beta p (continuing RHS only)
         for (int v = tid; v < nint; v += nth)
           for (int a = 0; a < 3; a++) {
             const T dv = dinv[v * 3 + a];
 #pragma unroll
             for (int r = 0; r < RG; r++) {
-              if (!S.act[r]) continue;
+              if (!((act2 >> r) & 1)) continue;
               const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              P[e] = dv * R[e] + (T)S.beta[r] * P[e];
+              P[e] = dv * R[e] + beta[r] * P[e];
             }
           }
         __syncthreads();
+        act = act2;
+        par ^= 1;
         inner_it++;
       }
       // x64 += scale * d (not for problems whose inner solve failed)
@@ -556,13 +666,20 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
         for (int a = 0; a < 3; a++)
 #pragma unroll
           for (int r = 0; r < RG; r++)
-            if (!S.done[r] && !S.failed[r]) {
+            if (((live >> r) & 1) && !((capfail >> r) & 1) && !S.failed[r]) {
               const int64_t e = ((int64_t)v * 3 + a) * RG + r;
               x64[(int64_t)v * 3 * NB + a * NB + q0 + r] += S.scale[r] * (double)D[e];
             }
       __syncthreads();
+      if (tid == 0)
+        for (int r = 0; r < RG; r++)
+          if ((capfail >> r) & 1) {
+            S.failed[r] = 1;
+            S.it[r] = S.itbase[r] + inner_it;
+          }
       if (inner_it == 0) break;  // nothing more can be done
     }
+    __syncthreads();
     if (tid < RG) {
       const int r = tid;
       A.iters[cell * NB + q0 + r] = S.it[r];

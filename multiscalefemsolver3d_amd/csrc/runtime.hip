@@ -529,7 +529,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(const int64_t* __restrict__ s
   T acc[BS];
   const int64_t r = slice * C + lane;
   if (slice < n_slices) {
-    sell_row<T, BS>(slice_base, slice_width, col, val, x, slice, lane, acc);
+    sell_row<T, BS>(slice_base, slice_width, col, val, x, slice, lane, nb, acc);
   } else {
 #pragma unroll
     for (int a = 0; a < BS; a++) acc[a] = 0;
@@ -561,8 +561,16 @@ __global__ void k_finish_sums(const double* __restrict__ part, unsigned G, int n
   __shared__ double red[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int k = 0; k < nv; k++) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < G; b += blockDim.x) s += part[(size_t)k * G + b];
+    // eight independent loads in flight per thread
+    double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned b0 = threadIdx.x; b0 < G; b0 += 8 * blockDim.x) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const unsigned b = b0 + u * blockDim.x;
+        if (b < G) s8[u] += part[(size_t)k * G + b];
+      }
+    }
+    double s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
     s = wave_sum(s);
     if (lane == 0) red[wave] = s;
     __syncthreads();

@@ -27,6 +27,7 @@ namespace {
 struct CgState {
   double rz1, rz2, rr, bb, tol2, pq;
   int done, iter1, iter2, maxit;
+  int warm;  // 1: the first iteration continues a kept search direction
 };
 // done codes
 enum { CG_RUN = 0, CG_CONVERGED = 1, CG_BREAKDOWN = 2, CG_MAXIT = 3 };
@@ -46,8 +47,17 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, un
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < G; b += 256) s += part[(size_t)k * G + b];
+    // eight independent loads in flight per thread (a plain strided loop
+    // would wait for each load before issuing the next)
+    double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (unsigned b0 = threadIdx.x; b0 < G; b0 += 8 * 256) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const unsigned b = b0 + u * 256;
+        if (b < G) s8[u] += part[(size_t)k * G + b];
+      }
+    }
+    double s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
     s = wave_sum(s);
     if (lane == 0) red[k][wave] = s;
   }
@@ -73,40 +83,6 @@ __device__ __forceinline__ void block_partials(T (&v)[NV], double* __restrict__ 
         (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
 }
 
-// SELL-64 row product with the operand formed on the fly:
-// x_j = z_j + beta * p_j (the new search direction).
-template <typename T, int BS>
-__device__ __forceinline__ void sell_row_dir(const int64_t* __restrict__ slice_base,
-                                             const int32_t* __restrict__ slice_width,
-                                             const int32_t* __restrict__ col, const T* __restrict__ val,
-                                             const T* __restrict__ z, const T* __restrict__ p, T beta,
-                                             int64_t slice, int lane, T (&acc)[BS]) {
-  constexpr int C = 64;
-#pragma unroll
-  for (int a = 0; a < BS; a++) acc[a] = 0;
-  const int64_t sb = slice_base[slice];
-  const int w = slice_width[slice];
-  const int32_t* cp = col + sb + lane;
-  const T* vp = val + sb * (BS * BS) + lane;
-  for (int j = 0; j < w; j++) {
-    const int32_t c = __builtin_nontemporal_load(cp + (int64_t)j * C);
-    if constexpr (BS == 1) {
-      const T xv = z[c] + beta * p[c];
-      acc[0] += __builtin_nontemporal_load(vp + (int64_t)j * C) * xv;
-    } else {
-      const int64_t c3 = 3 * (int64_t)c;
-      const T x0 = z[c3] + beta * p[c3], x1 = z[c3 + 1] + beta * p[c3 + 1], x2 = z[c3 + 2] + beta * p[c3 + 2];
-      const T* vj = vp + (int64_t)j * (BS * BS) * C;
-#pragma unroll
-      for (int a = 0; a < BS; a++) {
-        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 0) * C) * x0;
-        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 1) * C) * x1;
-        acc[a] += __builtin_nontemporal_load(vj + (BS * a + 2) * C) * x2;
-      }
-    }
-  }
-}
-
 // K1: from K2's partials (r.z, r.r): stop test, beta = rz / rz_prev;
 // p_new = z + beta p_old (own rows written, gathered rows formed on the
 // fly), q = A p_new, per-block partial of p_new.q.
@@ -130,7 +106,7 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
     done = CG_CONVERGED;
   else if (iter >= st->maxit)
     done = CG_MAXIT;
-  const T beta = iter > 0 ? (T)(rz / st->rz2) : (T)0;
+  const T beta = (iter > 0 || st->warm) ? (T)(rz / st->rz2) : (T)0;
   __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rz1 = rz;
@@ -144,7 +120,9 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
   const int64_t r = slice * 64 + lane;
   T acc[BS];
   if (slice < n_slices) {
-    sell_row_dir<T, BS>(slice_base, slice_width, col, val, z, p_old, beta, slice, lane, acc);
+    // operand x_j = z_j + beta p_j (the new search direction), formed on the fly
+    sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb,
+                      [&](int64_t i) { return z[i] + beta * p_old[i]; }, acc);
   } else {
 #pragma unroll
     for (int a = 0; a < BS; a++) acc[a] = 0;
@@ -210,9 +188,32 @@ __global__ __launch_bounds__(256) void k_cg_start(T* __restrict__ z, T* __restri
   }
   block_partials<T, 2>(red, part_rz);
 }
-__global__ void k_cg_start_state(CgState* st, double tol, double bb, int maxit) {
+// Continuation after an FP64 residual replacement: the new residual is the
+// old one times `ratio` (new units), so the kept direction is rescaled the
+// same way; p_src may alias p0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_start_keep(T* __restrict__ z, T* p0, const T* p_src,
+                                                       const T* __restrict__ r, const T* __restrict__ dinv, T ratio,
+                                                       int64_t n, double* part_rz) {
+  T red[2] = {0, 0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = r[i];
+    const T zi = dinv ? dinv[i] * ri : ri;
+    z[i] = zi;
+    p0[i] = ratio * p_src[i];
+    red[0] += ri * zi;
+    red[1] += ri * ri;
+  }
+  block_partials<T, 2>(red, part_rz);
+}
+// warm = 0: fresh start (beta = 0 in the first iteration); warm = 1: the
+// first beta is r.z / rz_prev with rz_prev the previous iteration's r.z in
+// the new units.
+__global__ void k_cg_start_state(CgState* st, double tol, double bb, int maxit, int warm, double rz_prev) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->rz1 = st->rz2 = 1.0;
+    st->warm = warm;
+    st->rz1 = 1.0;
+    st->rz2 = warm ? rz_prev : 1.0;
     st->rr = 0.0;
     st->bb = bb;
     st->tol2 = tol * tol * bb;
@@ -308,10 +309,25 @@ struct PcgEngine {
   // Start from r (holding the initial residual) and x; bb = ||b||^2 of the
   // system being solved.  Runs until converged / maxit / breakdown.  The
   // last K1 to run records the final residual in the state.
-  int run(double tol, double bb, int maxit, CgState* host_state) {
+  //
+  // keep_ratio > 0 continues the previous run's search direction (residual
+  // replacement instead of a restart): the previous run must have stopped
+  // on convergence after at least one iteration, and r now holds the new
+  // residual, equal to the old recursive one times keep_ratio.
+  int last_iter = 0;
+  double last_rz_prev = 0.0;
+  bool can_keep = false;
+
+  int run(double tol, double bb, int maxit, CgState* host_state, double keep_ratio = 0.0) {
     hipStream_t s = A->ctx->stream;
-    k_cg_start<T><<<g_vec, 256, 0, s>>>(z, pbuf[0], r, dinv, n, part_rz());
-    k_cg_start_state<<<1, 64, 0, s>>>(st, tol, bb, maxit);
+    const bool warm = keep_ratio > 0.0 && can_keep;
+    if (warm) {
+      k_cg_start_keep<T><<<g_vec, 256, 0, s>>>(z, pbuf[0], pbuf[last_iter & 1], r, dinv, (T)keep_ratio, n, part_rz());
+      k_cg_start_state<<<1, 64, 0, s>>>(st, tol, bb, maxit, 1, last_rz_prev * keep_ratio * keep_ratio);
+    } else {
+      k_cg_start<T><<<g_vec, 256, 0, s>>>(z, pbuf[0], r, dinv, n, part_rz());
+      k_cg_start_state<<<1, 64, 0, s>>>(st, tol, bb, maxit, 0, 0.0);
+    }
     MFS_HIP(hipGetLastError());
     CgState* hs = (CgState*)A->ctx->pinned;
     hs->done = CG_RUN;
@@ -327,6 +343,11 @@ struct PcgEngine {
     }
     *host_state = *hs;
     host_state->iter2 = hs->iter1;  // iterations completed when K1 stopped
+    // the newest direction is in pbuf[iter & 1]; rz2 holds r.z of the
+    // iteration before the last (the denominator of the next beta)
+    last_iter = hs->iter1;
+    last_rz_prev = hs->rz2;
+    can_keep = hs->done == CG_CONVERGED && hs->iter1 > 0 && std::isfinite(hs->rz2) && hs->rz2 > 0.0;
     return MFS_OK;
   }
 
@@ -822,15 +843,29 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
     if (hs.done == CG_BREAKDOWN) status = MFS_ERR_NONFINITE;
     else if (rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
   } else if (is_cg) {
-    // FP64 outer refinement around FP32 (P)CG
+    // FP64 outer refinement around FP32 (P)CG.  When the FP32 recursion
+    // stayed consistent with the true residual (the FP64 residual after the
+    // step is within 2x of what the recursion predicted), the next step
+    // continues the FP32 iteration instead of restarting it: the recursive
+    // residual is replaced by the true FP64 one and the search direction is
+    // kept (rescaled to the new units), which preserves the Krylov
+    // information.  When the two have drifted apart the kept direction would
+    // no longer be conjugate, so the step restarts.  A continued step that
+    // takes more than twice the first step's iterations is cut short and the
+    // next one restarts.
     PcgEngine<float>& E = S->cg32;
+    E.can_keep = false;
+    double keep_ratio = 0.0;
+    int first_inner = 0;
     while (rel > d.tol && iters < d.maxit) {
       const double rn = std::sqrt(rr);
       k_demote_scaled<<<gv, 256, 0, s>>>(S->r64, 1.0 / rn, E.r, E.x, n);
       // inner tolerance: no tighter than needed to reach tol in this step
       const double itol = std::min(0.5, std::max(d.inner_tol, 0.5 * d.tol / rel));
       CgState hs;
-      MFS_TRY(E.run(itol, 1.0, d.maxit - iters, &hs));
+      const int cap = keep_ratio > 0.0 ? std::min(d.maxit - iters, 2 * first_inner + 50) : d.maxit - iters;
+      MFS_TRY(E.run(itol, 1.0, cap, &hs, keep_ratio));
+      if (outer == 0) first_inner = hs.iter1;
       iters += hs.iter1;
       outer++;
       k_promote_add<<<gv, 256, 0, s>>>(S->x64, E.x, nullptr, rn, n);
@@ -838,6 +873,8 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
       MFS_TRY(true_residual(S, &rr, &bb));
       rel = std::sqrt(rr / bb);
       rep->recursive_relres = std::sqrt(hs.rr) * prev;
+      const double predicted = std::sqrt(hs.rr) * prev;
+      keep_ratio = (E.can_keep && rel > 0.0 && rel <= 2.0 * predicted) ? rn / std::sqrt(rr) : 0.0;
       if (!std::isfinite(rel) || hs.done == CG_BREAKDOWN) {
         status = MFS_ERR_NONFINITE;
         break;
@@ -950,6 +987,83 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
   if (status == MFS_ERR_NOT_CONVERGED) mfs::set_error("mfs_solver_run: tolerance not reached within maxit");
   if (status == MFS_ERR_NONFINITE) mfs::set_error("mfs_solver_run: non-finite value or breakdown in the iteration");
   return status;
+}
+
+namespace {
+__global__ void k_fill(float* a, double* b, double v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (a) a[i] = (float)v;
+    if (b) b[i] = v;
+  }
+}
+
+template <typename T>
+int time_cg_kernel(PcgEngine<T>& E, int kernel, int reps, double* avg_ms, double* bytes) {
+  mfs_mat A = E.A;
+  hipStream_t s = A->ctx->stream;
+  const int64_t n = E.n;
+  const bool f64 = std::is_same<T, double>::value;
+  const unsigned gv = grid_for(n, 1024);
+  // r = 1, x = 0, then a fresh start with tol 0 so no launch stops early
+  k_fill<<<gv, 256, 0, s>>>(f64 ? nullptr : (float*)E.r, f64 ? (double*)E.r : nullptr, 1.0, n);
+  k_fill<<<gv, 256, 0, s>>>(f64 ? nullptr : (float*)E.x, f64 ? (double*)E.x : nullptr, 0.0, n);
+  k_cg_start<T><<<E.g_vec, 256, 0, s>>>(E.z, E.pbuf[0], E.r, E.dinv, n, E.part_rz());
+  k_cg_start_state<<<1, 64, 0, s>>>(E.st, 0.0, 1.0, 1 << 30, 0, 0.0);
+  // one K1 so that K2 has partials to read
+  E.launch_iteration(s, 0);
+  MFS_HIP(hipGetLastError());
+  hipEvent_t e0, e1;
+  MFS_HIP(hipEventCreate(&e0));
+  MFS_HIP(hipEventCreate(&e1));
+  const int64_t ns = A->sell.n_slices;
+  MFS_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) {
+    if (kernel == 0) {
+      if (A->bs == 1)
+        k_cg_dir_spmv<T, 1><<<E.g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, E.val, E.z,
+                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.part_rz(),
+                                                     E.g_vec, E.part_pq());
+      else
+        k_cg_dir_spmv<T, 3><<<E.g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, E.val, E.z,
+                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.part_rz(),
+                                                     E.g_vec, E.part_pq());
+    } else {
+      k_cg_update<T><<<E.g_vec, 256, 0, s>>>(E.x, E.pbuf[1], E.r, E.z, E.q, E.dinv, n, E.st, E.part_pq(), E.g_spmv,
+                                             E.part_rz());
+    }
+  }
+  MFS_HIP(hipEventRecord(e1, s));
+  MFS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the probe must not have stopped (every launch did the full work)
+  CgState hs;
+  MFS_HIP(hipMemcpy(&hs, E.st, sizeof(hs), hipMemcpyDeviceToHost));
+  if (hs.done != CG_RUN) {
+    set_error("mfs_solver_kernel_time: the timed iteration stopped early");
+    return MFS_ERR_NONFINITE;
+  }
+  *avg_ms = ms / reps;
+  const double sv = f64 ? 8.0 : 4.0;
+  // K1: matrix + z and p gathered (+ own rows) + p_new and q written;
+  // K2: x, p, r, q (+ dinv) read, x, r, z written
+  *bytes = kernel == 0 ? spmv_bytes(A, f64 ? MFS_F64 : MFS_F32) + 2.0 * (double)n * sv
+                       : (double)n * sv * (E.dinv ? 8.0 : 7.0);
+  return MFS_OK;
+}
+}  // namespace
+
+extern "C" int mfs_solver_kernel_time(mfs_solver S, int kernel, int reps, double* avg_ms, double* bytes) {
+  MFS_CHECK_ARG(S && avg_ms && bytes && reps > 0 && (kernel == 0 || kernel == 1),
+                "mfs_solver_kernel_time: bad argument");
+  const mfs_solve_desc& d = S->desc;
+  MFS_CHECK_ARG(d.method == MFS_PCG_JACOBI || d.method == MFS_CG,
+                "mfs_solver_kernel_time: only the CG methods have these kernels");
+  MFS_HIP(hipSetDevice(S->A->ctx->device));
+  if (d.precision == MFS_PREC_MIXED) return time_cg_kernel(S->cg32, kernel, reps, avg_ms, bytes);
+  return time_cg_kernel(S->cg64, kernel, reps, avg_ms, bytes);
 }
 
 extern "C" int mfs_solve(mfs_mat A, const mfs_solve_desc* desc, mfs_vec b, mfs_vec x, mfs_solve_report* report) {

@@ -17,7 +17,7 @@ import time
 
 import numpy as np
 
-from . import (MLP, PREC_MIXED, Batch, F32, Vec, field_lognormal)
+from . import (MLP, PREC_F64, PREC_MIXED, Batch, F32, Vec, field_lognormal)
 
 MU_LOGE = math.log(1e9)
 SIGMA_LOGE = 1.0
@@ -71,6 +71,8 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     B = Batch(ctx, cn, org, h, E)
     B.assemble()
     res = {}
+    # the plain FP64 batched solve from the zero guess (the comparison point)
+    res["fp64_zero"] = B.solve(precision=PREC_F64, tol=tol)
     r0 = B.solve(precision=PREC_MIXED, tol=tol)
     it0, rr0, _ = B.status()
     res["zero"] = r0
@@ -105,6 +107,8 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
         "hat": round(1 - float(it_hat.mean()) / float(it0.mean()), 4),
         "learned": round(1 - float(it_l.mean()) / float(it0.mean()), 4)}
     out["iterations_saved_learned_vs_hat"] = round(1 - float(it_l.mean()) / float(it_hat.mean()), 4)
+    out["speedup_mixed_learned_vs_fp64_zero"] = round(res["fp64_zero"].seconds / r_l.seconds, 3)
+    out["speedup_mixed_zero_vs_fp64_zero"] = round(res["fp64_zero"].seconds / r0.seconds, 3)
     out["seconds_total"] = round(time.time() - t0, 1)
     B.close()
     log(f"[learned] {out}")

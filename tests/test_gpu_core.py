@@ -151,6 +151,26 @@ def test_mixed_refinement_reaches_fp64_tolerance(ctx, method):
     assert rel(x.download(), xo) < 1e-8
 
 
+def test_mixed_cg_keeps_krylov_information(ctx):
+    """The FP32 iteration is continued across FP64 residual replacements (the
+    search direction is kept), so the mixed solve needs about as many
+    iterations as FP64 PCG.  On this system (24^3, inner reduction 1e-3,
+    tol 1e-9) FP64 PCG takes 381 iterations; restarting the FP32 iteration at
+    each of the 3 replacements takes 485 (numpy emulation of the same
+    algorithm), continuing it 427."""
+    p = problems.laplace_inclusions(24)
+    S = assemble(ctx, p)
+    its = {}
+    for prec in (mfs.PREC_F64, mfs.PREC_MIXED):
+        x = mfs.Vec(ctx, S.A.n)
+        x.copy_from(S.x0)
+        rep = mfs.Solver(S.A, precision=prec, tol=1e-9, inner_tol=1e-3).run(S.b, x)
+        assert rep.status == 0 and rep.true_relres <= 1e-9
+        its[prec] = (rep.iterations, rep.outer_steps)
+    assert its[mfs.PREC_MIXED][1] >= 3
+    assert its[mfs.PREC_MIXED][0] <= 1.15 * its[mfs.PREC_F64][0] + 5, its
+
+
 @pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
 def test_gmres_elastic_textbook_operator(ctx, precision):
     """GMRES(30) on the two-phase elastic system with the textbook (SPD)

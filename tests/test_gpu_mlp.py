@@ -8,7 +8,7 @@ import oracle as orc
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(70, 33, 17), (64, 128, 64), (3, 200, 129)])
+@pytest.mark.parametrize("shape", [(70, 33, 17), (64, 128, 64), (3, 200, 129), (300, 264, 260), (70, 3000, 130)])
 def test_gemm_forward_matches_fp64_on_bf16_operands(ctx, shape):
     """A single linear layer: output = X W^T + b with BF16 operands and FP32
     accumulation equals the FP64 product of the BF16-rounded operands to FP32
@@ -25,19 +25,24 @@ def test_gemm_forward_matches_fp64_on_bf16_operands(ctx, shape):
     assert np.abs(Y - ref).max() < 1e-5 * np.abs(ref).max() * np.sqrt(din)
 
 
-def test_gradients_match_oracle(ctx):
+@pytest.mark.parametrize("dims,batch", [([96, 64, 64, 80], 48), ([520, 256, 264], 200), ([64, 128, 1300], 200)])
+def test_gradients_match_oracle(ctx, dims, batch):
+    """Forward, loss and the three GEMM forms of the backward pass (X W^T,
+    dY^T X, dY W) against the FP64 oracle MLP; the second case has tiles
+    with 16-byte operand loads on both the row- and the k-contiguous side and
+    ragged edges in every dimension, the second and third split K over
+    several workgroups (forward first layer; backward dY W with K = 1300)."""
     rng = np.random.default_rng(2)
-    dims = [96, 64, 64, 80]
     net = mfs.MLP(ctx, dims, seed=5)
     params = net.get_params()
-    X = rng.normal(size=(48, dims[0])).astype(np.float32)
-    T = rng.normal(size=(48, dims[-1])).astype(np.float32)
+    X = rng.normal(size=(batch, dims[0])).astype(np.float32)
+    T = rng.normal(size=(batch, dims[-1])).astype(np.float32)
     loss, gW, gb = net.loss_grad(X, T)
     Ws = [mfs.bf16_round(W).astype(np.float64) for W, _ in params]
     bs = [b.astype(np.float64) for _, b in params]
     loss_o, gW_o, gb_o = orc.mlp_loss_grad(Ws, bs, mfs.bf16_round(X), T)
     assert loss == pytest.approx(loss_o, rel=2e-2)
-    for l in range(3):
+    for l in range(len(dims) - 1):
         for g, go in ((gW[l], gW_o[l]), (gb[l], gb_o[l])):
             cos = float((g.ravel() @ go.ravel()) / (np.linalg.norm(g) * np.linalg.norm(go)))
             assert cos > 0.999, (l, cos)

@@ -154,13 +154,14 @@ int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solver* out);
 int mfs_solver_destroy(mfs_solver S);
 /* Solve A x = b; x holds the initial guess on entry (FP64 vectors). */
 int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_report* report);
-/* Time the two kernels of one (P)CG iteration of a CG-method solver in its
- * inner precision (FP32 for MFS_PREC_MIXED): kernel 0 = direction update
- * fused with the SpMV (q = A (z + beta p)), kernel 1 = the x/r/z update.
- * Each kernel is launched reps times back to back on the context stream and
- * timed with HIP events; *bytes receives its algorithmic HBM bytes per
- * launch.  Overwrites the solver's work vectors (a later run starts afresh).
- * No reference counterpart: measures the kernels replacing the inner loop of
+/* Time the iteration kernels of a CG-method solver in its inner precision
+ * (FP32 for MFS_PREC_MIXED): kernel 0 = direction update fused with the
+ * SpMV (q = A (z + beta p)), kernel 1 = the x/r/z update.  The kernel is
+ * launched
+ * reps times back to back on the context stream and timed with HIP events;
+ * *bytes receives its algorithmic HBM bytes per launch.  Overwrites the
+ * solver's work vectors (a later run starts afresh).  No reference
+ * counterpart: measures the kernels replacing the inner loop of
  * CSSD_Matrix::MSG / BiCG_Diag (Library/CSSD_Matrix.h:1496-1663). */
 int mfs_solver_kernel_time(mfs_solver S, int kernel, int reps, double* avg_ms, double* bytes);
 /* One-shot convenience: create + run + destroy. */
@@ -261,6 +262,8 @@ typedef struct {
   int use_graph;   /* capture the persistent launch in a hipGraph             */
   int wg_size;     /* workgroup size (0 = autotuned)                          */
   int use_guess;   /* start from mfs_batch_set_guess values                   */
+  int variant;     /* kernel variant: 0 = autotuned/default, 1 or 2 = fewer or
+                      more rows carried per thread                          */
 } mfs_batch_solve_desc;
 
 typedef struct {
@@ -316,7 +319,13 @@ int mfs_msfem_top_traction(mfs_batch B, const double* t, double* load);
 /* Fine displacement of one cell, u = sum Phi u_coarse (MechanicalMain.cpp:
  * 2176-2181), full fine vector 3*(n+1)^3. */
 int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u_fine);
-/* Candidate launch shapes tried by the autotuner and the time of each. */
+/* Autotuner for mfs_batch_solve in desc->precision: times the whole batched
+ * solve for every kernel variant (rows carried per thread), workgroup size
+ * (64..256) and workgroups per CU (as many as the kernel's registers and LDS
+ * allow, or one fewer), keeps the fastest for later solves of this
+ * precision and returns it as *best_wg = wg*100 + per_cu*10 + variant with
+ * its time in *best_ms.  No reference counterpart (the reference solves
+ * cells one at a time, MsFEM/MsFEM_Grid.h:4795-5760). */
 int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* desc, int* best_wg, double* best_ms);
 
 /* ---- learned initial guess (BF16 MFMA MLP) --------------------------------

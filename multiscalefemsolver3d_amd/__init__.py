@@ -74,7 +74,7 @@ class BatchDesc(C.Structure):
 
 class BatchSolveDesc(C.Structure):
     _fields_ = [("precision", C.c_int), ("tol", C.c_double), ("maxit", C.c_int), ("inner_tol", C.c_double),
-                ("use_graph", C.c_int), ("wg_size", C.c_int), ("use_guess", C.c_int)]
+                ("use_graph", C.c_int), ("wg_size", C.c_int), ("use_guess", C.c_int), ("variant", C.c_int)]
 
 
 class BatchReport(C.Structure):
@@ -560,10 +560,10 @@ class BatchResult:
 
 
 def batch_solve_desc(precision=PREC_MIXED, tol=1e-8, maxit=20000, inner_tol=1e-4, use_graph=True, wg_size=0,
-                     use_guess=False):
+                     use_guess=False, variant=0):
     d = BatchSolveDesc()
     d.precision, d.tol, d.maxit, d.inner_tol = precision, tol, maxit, inner_tol
-    d.use_graph, d.wg_size, d.use_guess = int(use_graph), wg_size, int(use_guess)
+    d.use_graph, d.wg_size, d.use_guess, d.variant = int(use_graph), wg_size, int(use_guess), variant
     return d
 
 
@@ -602,7 +602,8 @@ class Batch:
         d = batch_solve_desc(**kw)
         best, ms = C.c_int(), C.c_double()
         _chk(lib.mfs_batch_autotune(self.h, C.byref(d), C.byref(best), C.byref(ms)), "mfs_batch_autotune")
-        return best.value // 100, best.value % 100, ms.value
+        v = best.value  # workgroup * 100 + workgroups per CU * 10 + kernel variant
+        return v // 100, (v // 10) % 10, v % 10, ms.value
 
     def n_interior(self, cell):
         ni, nt = C.c_int64(), C.c_int64()

@@ -107,14 +107,33 @@ __global__ __launch_bounds__(256) void k_tet_geometry(const double* __restrict__
 __global__ __launch_bounds__(256) void k_gather_scalar(const int64_t* __restrict__ gptr, const uint32_t* __restrict__ gent,
                                                        const double* __restrict__ tet, const int64_t* __restrict__ pos,
                                                        int64_t nnzb, double* __restrict__ sell) {
+  constexpr int U = 4;  // entries whose loads are in flight together
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nnzb; s += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int64_t k = gptr[s]; k < gptr[s + 1]; k++) {
-      const uint32_t c = gent[k];
-      const int64_t e = c >> 4;
-      const int i = (c >> 2) & 3, j = c & 3;
-      const double* o = tet + 16 * e;
-      acc += o[12] * (o[3 * i] * o[3 * j] + o[3 * i + 1] * o[3 * j + 1] + o[3 * i + 2] * o[3 * j + 2]);
+    const int64_t k0 = gptr[s], k1 = gptr[s + 1];
+    for (int64_t kb = k0; kb < k1; kb += U) {
+      // indices first, then the records they point at, then the sums (the
+      // order of the additions stays ascending); entries past the list end
+      // re-read the last one and are skipped
+      uint32_t c[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) c[u] = gent[min(kb + u, k1 - 1)];
+      double g[U][7];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const double* o = tet + 16 * (int64_t)(c[u] >> 4);
+        const int i = (c[u] >> 2) & 3, j = c[u] & 3;
+        g[u][0] = o[3 * i];
+        g[u][1] = o[3 * i + 1];
+        g[u][2] = o[3 * i + 2];
+        g[u][3] = o[3 * j];
+        g[u][4] = o[3 * j + 1];
+        g[u][5] = o[3 * j + 2];
+        g[u][6] = o[12];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (kb + u < k1) acc += g[u][6] * (g[u][0] * g[u][3] + g[u][1] * g[u][4] + g[u][2] * g[u][5]);
     }
     const int64_t p = pos[s];
     sell[p] = acc;  // bs = 1: entry index == value index
@@ -132,29 +151,50 @@ __global__ __launch_bounds__(256) void k_gather_elastic(const int64_t* __restric
                                                         const int64_t* __restrict__ pos, int64_t nnzb,
                                                         double* __restrict__ sell) {
   constexpr int C = 64;
+  constexpr int U = 4;  // entries whose loads are in flight together
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nnzb; s += (int64_t)gridDim.x * blockDim.x) {
     double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int64_t k = gptr[s]; k < gptr[s + 1]; k++) {
-      const uint32_t c = gent[k];
-      const int64_t e = c >> 4;
-      const int i = (c >> 2) & 3, j = c & 3;
-      const double* o = tet + 16 * e;
-      const double dI[3] = {o[3 * i], o[3 * i + 1], o[3 * i + 2]};
-      const double dJ[3] = {o[3 * j], o[3 * j + 1], o[3 * j + 2]};
-      const double vl = o[12], vm = o[13];
-      const double dd = dI[0] * dJ[0] + dI[1] * dJ[1] + dI[2] * dJ[2];
+    const int64_t k0 = gptr[s], k1 = gptr[s + 1];
+    for (int64_t kb = k0; kb < k1; kb += U) {
+      // indices first, then the records, then the sums in ascending entry
+      // order; entries past the list end re-read the last one and are skipped
+      uint32_t c[U];
 #pragma unroll
-      for (int a = 0; a < 3; a++)
+      for (int u = 0; u < U; u++) c[u] = gent[min(kb + u, k1 - 1)];
+      double g[U][8];
 #pragma unroll
-        for (int b = 0; b < 3; b++) {
-          double v;
-          if (CONV == MFS_ELEM_TEXTBOOK)
-            v = vl * dI[a] * dJ[b] + vm * dI[b] * dJ[a];
-          else
-            v = vl * dI[b] * dJ[a] + vm * dI[a] * dJ[b];
-          if (a == b) v += vm * dd;
-          K[a][b] += v;
-        }
+      for (int u = 0; u < U; u++) {
+        const double* o = tet + 16 * (int64_t)(c[u] >> 4);
+        const int i = (c[u] >> 2) & 3, j = c[u] & 3;
+        g[u][0] = o[3 * i];
+        g[u][1] = o[3 * i + 1];
+        g[u][2] = o[3 * i + 2];
+        g[u][3] = o[3 * j];
+        g[u][4] = o[3 * j + 1];
+        g[u][5] = o[3 * j + 2];
+        g[u][6] = o[12];
+        g[u][7] = o[13];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (kb + u >= k1) continue;
+        const double* dI = g[u];
+        const double* dJ = g[u] + 3;
+        const double vl = g[u][6], vm = g[u][7];
+        const double dd = dI[0] * dJ[0] + dI[1] * dJ[1] + dI[2] * dJ[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int b = 0; b < 3; b++) {
+            double v;
+            if (CONV == MFS_ELEM_TEXTBOOK)
+              v = vl * dI[a] * dJ[b] + vm * dI[b] * dJ[a];
+            else
+              v = vl * dI[b] * dJ[a] + vm * dI[a] * dJ[b];
+            if (a == b) v += vm * dd;
+            K[a][b] += v;
+          }
+      }
     }
     const int64_t p = pos[s];
     const int64_t base = (p / C) * C * 9 + (p % C);

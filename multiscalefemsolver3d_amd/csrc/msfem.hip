@@ -5,7 +5,7 @@
 // d the local problem is: K u = 0 inside the cell, u = hat_c * e_d on the
 // cell boundary (hat_c = trilinear coarse hat).  This replaces the per-cell
 // volume solves of Grid_forMech_Poly_Order1::CreateFunctionalSpaces
-// (MsFEM/MsFEM_Grid.h:4795-5760 via FEM/FEM.h:2969-3795) in the configs'
+// (MsFEM/MsFEM_Grid.h:4795-5760 via FEM/FEM.h:2969-3795) in the
 // three-direction formulation.
 //
 // Layout (device):
@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -38,6 +39,11 @@ constexpr int NS = 15;   // stencil points
 constexpr int RG = 8;    // right-hand sides per task
 constexpr int NB = 24;   // bases per cell
 constexpr int MAXN = 12; // largest supported cell subdivision (FP32 inner; FP64 inner: 10)
+// rows a thread carries through the inner-iteration phases at once: two
+// kernel variants per precision (the autotuner picks one; more rows keep
+// more loads in flight per wave, fewer rows leave registers for two waves
+// per SIMD)
+constexpr int kRptF32[2] = {2, 3}, kRptF64[2] = {1, 2};
 
 // Stencil offsets (di, dj, dk), sorted by linear offset (dk, dj, di lexicographic)
 __constant__ int c_stencil[NS][3];
@@ -281,6 +287,164 @@ __device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint,
   }
 }
 
+// The same product for RPT rows of one thread at once (rows v[k]; rows with
+// ok[k] false are computed on row 0 and discarded by the caller).  The K
+// values of all RPT rows for the next stencil point are loaded while the
+// current point is applied, so a wave keeps RPT*9 loads in flight and the
+// row passes of one thread do not queue behind each other.
+template <typename KT, typename VT, int RPT, typename Load>
+__device__ __forceinline__ void stencil_rows(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
+                                             const bool (&ok)[RPT], int m, Load load, VT (&acc)[RPT][3][RG]) {
+  int vi[RPT], ci[RPT], cj[RPT], ck[RPT];
+  const KT* kp[RPT];
+  KT kn[RPT][9];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    vi[q] = ok[q] ? v[q] : 0;
+    ci[q] = vi[q] % m + 1;
+    cj[q] = (vi[q] / m) % m + 1;
+    ck[q] = vi[q] / (m * m) + 1;
+    kp[q] = Kc + vi[q];
+#pragma unroll
+    for (int e = 0; e < 9; e++) kn[q][e] = kp[q][(int64_t)e * nint];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < RG; r++) acc[q][a][r] = 0;
+  }
+#pragma unroll 1
+  for (int s = 0; s < NS; s++) {
+    VT kv[RPT][9];
+#pragma unroll
+    for (int q = 0; q < RPT; q++)
+#pragma unroll
+      for (int e = 0; e < 9; e++) kv[q][e] = (VT)kn[q][e];
+    if (s + 1 < NS) {
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        kp[q] += (int64_t)9 * nint;
+#pragma unroll
+        for (int e = 0; e < 9; e++) kn[q][e] = kp[q][(int64_t)e * nint];
+      }
+    }
+    const int di = (int)((kOffI >> (2 * s)) & 3u) - 1;
+    const int dj = (int)((kOffJ >> (2 * s)) & 3u) - 1;
+    const int dk = (int)((kOffK >> (2 * s)) & 3u) - 1;
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int ni = ci[q] + di, nj = cj[q] + dj, nk = ck[q] + dk;
+      const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
+      const int u = in ? interior_index(ni, nj, nk, m) : vi[q];
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        VT pv[RG];
+        load(u, b, pv);
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          acc[q][0][r] += kv[q][0 * 3 + b] * pv[r];
+          acc[q][1][r] += kv[q][1 * 3 + b] * pv[r];
+          acc[q][2][r] += kv[q][2 * 3 + b] * pv[r];
+        }
+      }
+    }
+  }
+}
+
+// Inner-iteration vector phases over RPT rows of one thread at a time
+// (separate restrict-qualified arrays, so the loads of the RPT rows are
+// issued together).
+// q = K p into Q, partial p.q
+template <typename T, int RPT>
+__device__ __forceinline__ void phase_spmv(const float* __restrict__ K32, const double* __restrict__ K64, int nint,
+                                           int m, const T* P, T* __restrict__ Q, double (&w)[RG]) {
+  constexpr bool fp64 = std::is_same<T, double>::value;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  auto ld = [&](int u, int b, T (&pv)[RG]) {
+#pragma unroll
+    for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
+  };
+  for (int v0 = tid; v0 < nint; v0 += RPT * nth) {
+    int vr[RPT];
+    bool ok[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      vr[q] = v0 + q * nth;
+      ok[q] = vr[q] < nint;
+    }
+    T acc[RPT][3][RG];
+    if constexpr (fp64)
+      stencil_rows<double, double, RPT>(K64, nint, vr, ok, m, ld, acc);
+    else
+      stencil_rows<float, float, RPT>(K32, nint, vr, ok, m, ld, acc);
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      if (!ok[q]) continue;
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          const int64_t e = ((int64_t)vr[q] * 3 + a) * RG + r;
+          Q[e] = acc[q][a][r];
+          w[r] += (double)(acc[q][a][r] * P[e]);
+        }
+    }
+  }
+}
+
+// D += alpha p, R -= alpha q, z = dinv R; partials r.z, r.r
+template <typename T, int RPT>
+__device__ __forceinline__ void phase_update(int nint, const T* __restrict__ dinv, const T* P, const T* __restrict__ Q,
+                                             T* __restrict__ R, T* __restrict__ D, const T (&alpha)[RG],
+                                             double (&w)[2 * RG]) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int v0 = tid; v0 < nint; v0 += RPT * nth) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int v = v0 + q * nth;
+      if (v >= nint) continue;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const T dv = dinv[v * 3 + a];
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+          const T pe = P[e];
+          D[e] += alpha[r] * pe;
+          const T rv = R[e] - alpha[r] * Q[e];
+          R[e] = rv;
+          const T z = dv * rv;
+          w[r] += (double)(rv * z);
+          w[RG + r] += (double)(rv * rv);
+        }
+      }
+    }
+  }
+}
+
+// p = z + beta p for the right-hand sides still iterating (mask `on`)
+template <typename T, int RPT>
+__device__ __forceinline__ void phase_direction(int nint, const T* __restrict__ dinv, T* P, const T* __restrict__ R,
+                                                const T (&beta)[RG], int on) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int v0 = tid; v0 < nint; v0 += RPT * nth) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int v = v0 + q * nth;
+      if (v >= nint) continue;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const T dv = dinv[v * 3 + a];
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          if (!((on >> r) & 1)) continue;
+          const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+          P[e] = dv * R[e] + beta[r] * P[e];
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Workgroup all-reduce of N per-thread values (N a power of two, 8..32):
 // a butterfly reduce-scatter inside the wave (log2 N + remaining steps of
@@ -354,7 +518,7 @@ struct RhsState {
   int done[RG], failed[RG], warm[RG], itbase[RG], it[RG];
 };
 
-template <typename T>
+template <typename T, int RPT>
 __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* P = reinterpret_cast<T*>(smem_raw);
@@ -561,26 +725,7 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
           double w[RG], pq[RG];
 #pragma unroll
           for (int r = 0; r < RG; r++) w[r] = 0;
-          for (int v = tid; v < nint; v += nth) {
-            const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
-            T acc[3][RG];
-            auto ld = [&](int u, int b, T (&pv)[RG]) {
-#pragma unroll
-              for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
-            };
-            if constexpr (fp64)
-              stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
-            else
-              stencil_row<float, float>(K32, nint, v, i, j, k, m, ld, acc);
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-              for (int r = 0; r < RG; r++) {
-                const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-                Q[e] = acc[a][r];
-                w[r] += (double)(acc[a][r] * P[e]);
-              }
-          }
+          phase_spmv<T, RPT>(K32, K64, nint, m, P, Q, w);
           wg_allreduce<RG>(w, partB, pq);
 #pragma unroll
           for (int r = 0; r < RG; r++)
@@ -593,21 +738,7 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
           double w[2 * RG], tot[2 * RG];
 #pragma unroll
           for (int r = 0; r < 2 * RG; r++) w[r] = 0;
-          for (int v = tid; v < nint; v += nth)
-            for (int a = 0; a < 3; a++) {
-              const T dv = dinv[v * 3 + a];
-#pragma unroll
-              for (int r = 0; r < RG; r++) {
-                const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-                const T pe = P[e];
-                D[e] += alpha[r] * pe;
-                const T rv = R[e] - alpha[r] * Q[e];
-                R[e] = rv;
-                const T z = dv * rv;
-                w[r] += (double)(rv * z);
-                w[RG + r] += (double)(rv * rv);
-              }
-            }
+          phase_update<T, RPT>(nint, dinv, P, Q, R, D, alpha, w);
           wg_allreduce<2 * RG>(w, partA, tot);
 #pragma unroll
           for (int r = 0; r < RG; r++) {
@@ -646,16 +777,7 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
             }
         }
         // p = z + beta p (continuing RHS only)
-        for (int v = tid; v < nint; v += nth)
-          for (int a = 0; a < 3; a++) {
-            const T dv = dinv[v * 3 + a];
-#pragma unroll
-            for (int r = 0; r < RG; r++) {
-              if (!((act2 >> r) & 1)) continue;
-              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-              P[e] = dv * R[e] + beta[r] * P[e];
-            }
-          }
+        phase_direction<T, RPT>(nint, dinv, P, R, beta, act2);
         __syncthreads();
         act = act2;
         par ^= 1;
@@ -782,7 +904,8 @@ struct mfs_batch_s {
   int max_n = 0, max_nint = 0;
   int ntasks = 0;
   bool assembled = false;
-  int tuned_wg = 0, tuned_per_cu = 0;
+  int tuned_wg[2] = {0, 0}, tuned_per_cu[2] = {0, 0};  // per precision (0 mixed, 1 FP64)
+  int tuned_var[2] = {-1, -1};                        // kernel variant (rows per thread)
   // device
   Cell* d_cells = nullptr;
   int64_t* d_int0 = nullptr;
@@ -1006,8 +1129,15 @@ static int batch_scratch(mfs_batch B, int precision, int per_cu) {
 
 // Launch reset + persistent solve on the context stream (capturable: no
 // allocation, no memset; the queue and flags are cleared by a kernel).
-static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu, const int* task_cell = nullptr,
-                        const int* task_group = nullptr, int ntasks = -1, bool keep_flags = false) {
+static const void* batch_kernel(int precision, int var) {
+  if (precision == MFS_PREC_F64)
+    return var == 0 ? (const void*)k_batch_solve<double, kRptF64[0]> : (const void*)k_batch_solve<double, kRptF64[1]>;
+  return var == 0 ? (const void*)k_batch_solve<float, kRptF32[0]> : (const void*)k_batch_solve<float, kRptF32[1]>;
+}
+
+static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu, int var,
+                        const int* task_cell = nullptr, const int* task_group = nullptr, int ntasks = -1,
+                        bool keep_flags = false) {
   hipStream_t s = B->ctx->stream;
   const int grid = B->ctx->cu_count * per_cu;
   const size_t lds = lds_bytes(B, d->precision);
@@ -1042,10 +1172,17 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   A.use_guess = d->use_guess;
   k_batch_reset<<<std::max(1, (B->ncells * NB + 255) / 256), 256, 0, s>>>(B->d_queue, B->d_flags,
                                                                             keep_flags ? 0 : B->ncells * NB);
-  if (d->precision == MFS_PREC_F64)
-    k_batch_solve<double><<<grid, wg, lds, s>>>(A);
-  else
-    k_batch_solve<float><<<grid, wg, lds, s>>>(A);
+  if (d->precision == MFS_PREC_F64) {
+    if (var == 0)
+      k_batch_solve<double, kRptF64[0]><<<grid, wg, lds, s>>>(A);
+    else
+      k_batch_solve<double, kRptF64[1]><<<grid, wg, lds, s>>>(A);
+  } else {
+    if (var == 0)
+      k_batch_solve<float, kRptF32[0]><<<grid, wg, lds, s>>>(A);
+    else
+      k_batch_solve<float, kRptF32[1]><<<grid, wg, lds, s>>>(A);
+  }
   MFS_HIP(hipGetLastError());
   return MFS_OK;
 }
@@ -1057,6 +1194,7 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
   MFS_CHECK_ARG(d->tol > 0, "mfs_batch_solve: tol must be positive");
   MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 256),
                 "mfs_batch_solve: wg_size must be a multiple of 64 in 64..256");
+  MFS_CHECK_ARG(d->variant >= 0 && d->variant <= 2, "mfs_batch_solve: variant must be 0, 1 or 2");
   if (lds_bytes(B, d->precision) > 160 * 1024 - 8192) {
     set_error("mfs_batch_solve: cells too large for the LDS-resident search directions (FP64 inner needs n <= 10)");
     return MFS_ERR_UNSUPPORTED;
@@ -1064,20 +1202,24 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
   return MFS_OK;
 }
 
-static int pick_shape(mfs_batch B, const mfs_batch_solve_desc* d, int* wg, int* per_cu) {
-  const size_t lds = lds_bytes(B, d->precision) + 4096;
-  int wgs = d->wg_size ? d->wg_size : (B->tuned_wg ? B->tuned_wg : 256);
-  int pc = B->tuned_per_cu ? B->tuned_per_cu : 2;
-  if (d->wg_size) pc = 1;
-  while (pc > 1 && (size_t)pc * lds > 160 * 1024) pc--;
+// Workgroups of size wg that fit one CU at once (registers and LDS), at
+// least 1.
+static int resident_per_cu(mfs_batch B, int precision, int wg, int var) {
+  const size_t lds = lds_bytes(B, precision);
+  const void* fn = batch_kernel(precision, var);
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, wg, lds) != hipSuccess) nb = 1;
+  (void)hipGetLastError();
+  return std::max(1, std::min(nb, 8));
+}
+
+static int pick_shape(mfs_batch B, const mfs_batch_solve_desc* d, int* wg, int* per_cu, int* var) {
+  const int t = d->precision == MFS_PREC_F64 ? 1 : 0;
+  const int wgs = d->wg_size ? d->wg_size : (B->tuned_wg[t] ? B->tuned_wg[t] : 256);
+  *var = d->variant > 0 ? d->variant - 1 : (B->tuned_var[t] >= 0 ? B->tuned_var[t] : 1);
   *wg = wgs;
-  *per_cu = pc;
-  const hipError_t e = d->precision == MFS_PREC_F64
-                           ? hipFuncSetAttribute((const void*)k_batch_solve<double>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(B, d->precision))
-                           : hipFuncSetAttribute((const void*)k_batch_solve<float>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(B, d->precision));
-  (void)e;
+  *per_cu = (!d->wg_size && B->tuned_per_cu[t]) ? B->tuned_per_cu[t] : resident_per_cu(B, d->precision, wgs, *var);
   return MFS_OK;
 }
 
@@ -1109,18 +1251,21 @@ static int batch_collect(mfs_batch B, mfs_batch_report* rep) {
 extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_batch_report* rep) {
   MFS_TRY(batch_check_desc(B, d));
   MFS_CHECK_ARG(rep, "mfs_batch_solve: report is NULL");
-  int wg, pc;
-  MFS_TRY(pick_shape(B, d, &wg, &pc));
+  int wg, pc, var;
+  MFS_TRY(pick_shape(B, d, &wg, &pc, &var));
   MFS_TRY(batch_scratch(B, d->precision, pc));
   hipStream_t s = B->ctx->stream;
   hipEvent_t e0, e1;
   MFS_HIP(hipEventCreate(&e0));
   MFS_HIP(hipEventCreate(&e1));
-  if (d->use_graph) {
+  // MFS_NO_GRAPH=1 launches the same kernels directly (kernel tracing)
+  const char* ng = std::getenv("MFS_NO_GRAPH");
+  const bool graph = d->use_graph && !(ng && ng[0] == '1');
+  if (graph) {
     hipGraph_t g;
     hipGraphExec_t ge;
     MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = batch_launch(B, d, wg, pc);
+    int st = batch_launch(B, d, wg, pc, var);
     MFS_HIP(hipStreamEndCapture(s, &g));
     MFS_TRY(st);
     MFS_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -1132,7 +1277,7 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
     (void)hipGraphDestroy(g);
   } else {
     MFS_HIP(hipEventRecord(e0, s));
-    MFS_TRY(batch_launch(B, d, wg, pc));
+    MFS_TRY(batch_launch(B, d, wg, pc, var));
     MFS_HIP(hipEventRecord(e1, s));
     MFS_HIP(hipEventSynchronize(e1));
   }
@@ -1168,8 +1313,9 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
       mfs_batch_solve_desc d64 = *d;
       d64.precision = MFS_PREC_F64;
       d64.use_guess = 0;
-      int wg64, pc64;
-      MFS_TRY(pick_shape(B, &d64, &wg64, &pc64));
+      d64.variant = 0;
+      int wg64, pc64, var64;
+      MFS_TRY(pick_shape(B, &d64, &wg64, &pc64, &var64));
       MFS_TRY(batch_scratch(B, MFS_PREC_F64, pc64));
       // clear the per-problem flags of the re-solved tasks only
       std::vector<int> fl2(fl);
@@ -1180,7 +1326,7 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
       MFS_HIP(hipEventCreate(&f0));
       MFS_HIP(hipEventCreate(&f1));
       MFS_HIP(hipEventRecord(f0, s));
-      MFS_TRY(batch_launch(B, &d64, wg64, pc64, dtc, dtg, (int)tc.size(), true));
+      MFS_TRY(batch_launch(B, &d64, wg64, pc64, var64, dtc, dtg, (int)tc.size(), true));
       MFS_HIP(hipEventRecord(f1, s));
       MFS_HIP(hipEventSynchronize(f1));
       float ms2 = 0;
@@ -1205,31 +1351,42 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
   return MFS_OK;
 }
 
+// Autotuner: times the whole batched solve for each kernel variant (rows
+// carried per thread), each candidate workgroup size (64..256 threads) and
+// as many workgroups per CU as fit at once (from the kernel's register and
+// LDS use) or one fewer, and keeps the fastest shape for this precision.
 extern "C" int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* d, int* best_wg, double* best_ms) {
   MFS_TRY(batch_check_desc(B, d));
-  const int shapes[][2] = {{128, 4}, {128, 2}, {256, 2}, {256, 1}, {192, 2}};
+  const int t = d->precision == MFS_PREC_F64 ? 1 : 0;
   double best = 1e30;
-  int bw = 256, bp = 2;
-  const size_t lds = lds_bytes(B, d->precision) + 4096;
-  for (auto& sh : shapes) {
-    if ((size_t)sh[1] * lds > 160 * 1024) continue;
-    B->tuned_wg = sh[0];
-    B->tuned_per_cu = sh[1];
-    mfs_batch_solve_desc dd = *d;
-    dd.wg_size = 0;
-    dd.use_guess = 0;
-    mfs_batch_report rep;
-    int st = mfs_batch_solve(B, &dd, &rep);
-    if (st != MFS_OK && st != MFS_ERR_NOT_CONVERGED) return st;
-    if (rep.seconds < best) {
-      best = rep.seconds;
-      bw = sh[0];
-      bp = sh[1];
+  int bw = 256, bp = 1, bv = 1;
+  for (int var : {0, 1})
+  for (int wg : {64, 128, 192, 256}) {
+    const int res = resident_per_cu(B, d->precision, wg, var);
+    for (int pc : {res, res - 1}) {
+      if (pc < 1 || (pc == res - 1 && res == 1)) continue;
+      B->tuned_wg[t] = wg;
+      B->tuned_per_cu[t] = pc;
+      B->tuned_var[t] = var;
+      mfs_batch_solve_desc dd = *d;
+      dd.wg_size = 0;
+      dd.use_guess = 0;
+      mfs_batch_report rep;
+      int st = mfs_batch_solve(B, &dd, &rep);
+      if (st != MFS_OK && st != MFS_ERR_NOT_CONVERGED) return st;
+      if (rep.seconds < best) {
+        best = rep.seconds;
+        bw = wg;
+        bp = pc;
+        bv = var;
+      }
     }
   }
-  B->tuned_wg = bw;
-  B->tuned_per_cu = bp;
-  if (best_wg) *best_wg = bw * 100 + bp;
+  B->tuned_wg[t] = bw;
+  B->tuned_per_cu[t] = bp;
+  B->tuned_var[t] = bv;
+  // encoded as wg*100 + per_cu*10 + variant
+  if (best_wg) *best_wg = bw * 100 + bp * 10 + bv;
   if (best_ms) *best_ms = best * 1e3;
   return MFS_OK;
 }

@@ -161,14 +161,21 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* _
     st->iter2 = st->iter1 + 1;
   }
   T red[2] = {0, 0};
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
-    const T ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const T zi = dinv ? dinv[i] * ri : ri;
-    z[i] = zi;
-    red[0] += ri * zi;
-    red[1] += ri * ri;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // four elements per pass so that their loads are in flight together
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int64_t i = i0 + k * stride;
+      if (i >= n) continue;
+      x[i] += alpha * p[i];
+      const T ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      const T zi = dinv ? dinv[i] * ri : ri;
+      z[i] = zi;
+      red[0] += ri * zi;
+      red[1] += ri * ri;
+    }
   }
   block_partials<T, 2>(red, part_rz);
 }
@@ -276,7 +283,11 @@ struct PcgEngine {
     MFS_TRY(dmalloc((void**)&st, sizeof(CgState)));
     MFS_HIP(hipMemset(st, 0, sizeof(CgState)));
     g_spmv = (unsigned)spmv_grid(A);
-    g_vec = grid_for(n, 4 * std::max(A->ctx->cu_count, 64));
+    // update-kernel grid: about 4 elements per thread (its loop keeps 4 in
+    // flight) but at least one block per CU; fewer blocks also means fewer
+    // partial sums for every block of the next kernel to add up
+    const int64_t cus = std::max(A->ctx->cu_count, 64);
+    g_vec = (unsigned)std::min<int64_t>(4 * cus, std::max<int64_t>(cus, (n + 1023) / 1024));
     MFS_TRY(reduce_scratch_alloc(red, (int64_t)g_spmv + 2 * (int64_t)g_vec + 64));
     return capture();
   }
@@ -827,9 +838,8 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
   int status = MFS_OK;
   const unsigned gv = grid_for(n, 1024);
   const bool is_cg = d.method == MFS_PCG_JACOBI || d.method == MFS_CG;
-  if (is_cg && d.precision == MFS_PREC_F64) {
-    // plain FP64 (P)CG on b - A x0: r = r64 already; x correction starts at x0
-    PcgEngine<double>& E = S->cg64;
+  // plain FP64 (P)CG on b - A x0: r = r64 already; x correction starts at x0
+  auto run_f64 = [&](auto& E) -> int {
     MFS_HIP(hipMemcpyAsync(E.r, S->r64, 8 * n, hipMemcpyDeviceToDevice, s));
     MFS_HIP(hipMemcpyAsync(E.x, S->x64, 8 * n, hipMemcpyDeviceToDevice, s));
     CgState hs;
@@ -842,18 +852,19 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
     rel = std::sqrt(rr / bb);
     if (hs.done == CG_BREAKDOWN) status = MFS_ERR_NONFINITE;
     else if (rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
-  } else if (is_cg) {
-    // FP64 outer refinement around FP32 (P)CG.  When the FP32 recursion
-    // stayed consistent with the true residual (the FP64 residual after the
-    // step is within 2x of what the recursion predicted), the next step
-    // continues the FP32 iteration instead of restarting it: the recursive
-    // residual is replaced by the true FP64 one and the search direction is
-    // kept (rescaled to the new units), which preserves the Krylov
-    // information.  When the two have drifted apart the kept direction would
-    // no longer be conjugate, so the step restarts.  A continued step that
-    // takes more than twice the first step's iterations is cut short and the
-    // next one restarts.
-    PcgEngine<float>& E = S->cg32;
+    return MFS_OK;
+  };
+  // FP64 outer refinement around FP32 (P)CG.  When the FP32 recursion
+  // stayed consistent with the true residual (the FP64 residual after the
+  // step is within 2x of what the recursion predicted), the next step
+  // continues the FP32 iteration instead of restarting it: the recursive
+  // residual is replaced by the true FP64 one and the search directions are
+  // kept (rescaled to the new units), which preserves the Krylov
+  // information.  When the two have drifted apart the kept directions would
+  // no longer be conjugate, so the step restarts.  A continued step that
+  // takes more than twice the first step's iterations is cut short and the
+  // next one restarts.
+  auto run_mixed = [&](auto& E) -> int {
     E.can_keep = false;
     double keep_ratio = 0.0;
     int first_inner = 0;
@@ -885,6 +896,12 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
       }
     }
     if (status == MFS_OK && rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
+    return MFS_OK;
+  };
+  if (is_cg && d.precision == MFS_PREC_F64) {
+    MFS_TRY(run_f64(S->cg64));
+  } else if (is_cg) {
+    MFS_TRY(run_mixed(S->cg32));
   } else if (d.method == MFS_MINRES) {
     // MINRES runs, each from the current FP64 residual (refinement steps)
     while (rel > d.tol && iters < d.maxit) {

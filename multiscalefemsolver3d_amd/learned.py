@@ -34,7 +34,7 @@ def cells(count, n, seed, h=1.0 / 32):
 
 
 def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, batch=256, lr=1e-3, tol=1e-8,
-        seed_train=4, seed_test=5, log=print):
+        seed_train=4, seed_test=5, log=print, tune=False):
     t0 = time.time()
     nt = 6 * n ** 3
     nout = 3 * (n - 1) ** 3 * 24
@@ -70,6 +70,9 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     cn, org, h, E = cells(n_test, n, seed_test)
     B = Batch(ctx, cn, org, h, E)
     B.assemble()
+    if tune:
+        out["autotuned_shape"] = {"mixed": B.autotune(precision=PREC_MIXED, tol=tol)[:3],
+                                  "fp64": B.autotune(precision=PREC_F64, tol=tol)[:3]}
     res = {}
     # the plain FP64 batched solve from the zero guess (the comparison point)
     res["fp64_zero"] = B.solve(precision=PREC_F64, tol=tol)

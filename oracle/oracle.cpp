@@ -960,8 +960,8 @@ EXPORT double orc_bicg_diag(int n, const int* rp, const int* ci, const double* v
 }
 
 // ---------------------------------------------------------------------------
-// MsFEM local problems on one coarse hex cell (the configs' formulation: three
-// unit directions per coarse node; the boundary trace is the trilinear hat of
+// MsFEM local problems on one coarse hex cell (three-direction formulation:
+// three unit directions per coarse node; the boundary trace is the trilinear hat of
 // the coarse node times e_d on the fine boundary nodes; zero body force).  The
 // reference solves one vector basis per node with edge->face->volume traces
 // (MsFEM_Grid.h:4795-5760, FEM.h:2969-3795); by linearity its (1,1,1) basis is

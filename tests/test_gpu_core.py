@@ -246,7 +246,7 @@ def test_solver_reuse_same_results(ctx):
 
 
 @pytest.mark.slow
-def test_config1_full_size_mixed_vs_fp64(ctx):
+def test_laplace64_full_size_mixed_vs_fp64(ctx):
     """64^3 Laplace with 200 inclusions: mixed reaches the FP64 tolerance and
     agrees with the FP64 GPU PCG solution; the residual is re-checked in numpy
     from the exported matrix."""

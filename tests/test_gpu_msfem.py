@@ -37,6 +37,33 @@ def test_bases_match_oracle(ctx, small, precision):
         assert np.abs(Kc[c] - Kc_o).max() < 1e-7 * np.abs(Kc_o).max()
 
 
+@pytest.mark.parametrize("precision", [mfs.PREC_MIXED, mfs.PREC_F64])
+@pytest.mark.parametrize("variant,wg", [(1, 64), (1, 256), (2, 64), (2, 192)])
+def test_kernel_variants_and_shapes_match_oracle(ctx, small, precision, variant, wg):
+    """Every kernel variant (rows carried per thread) at small and large
+    workgroups, where rows per thread exceed or fall short of the variant's
+    interleave, gives the oracle's bases."""
+    p, B = small
+    rep = B.solve(precision=precision, tol=1e-10, use_graph=False, variant=variant, wg_size=wg)
+    assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-10
+    for c in (0, B.ncells - 1):
+        n = int(p["cell_n"][c])
+        basis_o, _, _ = orc.msfem_cell(n, p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"], tol=1e-13)
+        err = np.linalg.norm(B.basis(c) - basis_o) / np.linalg.norm(basis_o)
+        assert err < 1e-8, (c, err)
+
+
+def test_autotuner_shape_is_used_and_consistent(ctx, small):
+    p, B = small
+    wg, per_cu, var, ms = B.autotune(precision=mfs.PREC_MIXED, tol=1e-10)
+    assert wg in (64, 128, 192, 256) and per_cu >= 1 and var in (0, 1) and ms > 0
+    B.solve(precision=mfs.PREC_MIXED, tol=1e-10)
+    tuned = np.concatenate([B.basis(c).ravel() for c in range(B.ncells)])
+    B.solve(precision=mfs.PREC_F64, tol=1e-10, variant=1, wg_size=256)
+    ref = np.concatenate([B.basis(c).ravel() for c in range(B.ncells)])
+    assert np.linalg.norm(tuned - ref) / np.linalg.norm(ref) < 1e-8
+
+
 def test_partition_of_unity_and_coarse_invariants(ctx, small):
     p, B = small
     B.solve(precision=mfs.PREC_MIXED, tol=1e-9)
@@ -91,12 +118,12 @@ def test_guess_exact_and_bad_guess_fallback(ctx, small):
 
 
 @pytest.mark.slow
-def test_config2_full_batch(ctx):
+def test_msfem_16cubed_cells_full_batch(ctx):
     """16^3 coarse cells, n ~ U{6..10}, lognormal E: 98,304 local problems."""
     p = problems.msfem_cells(16)
     B = make_batch(ctx, p)
     rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-8)
-    print("config2 mixed:", rep)
+    print("16^3-cell batch, mixed:", rep)
     assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-8
     # spot-check one cell of each size against the oracle
     for n in np.unique(p["cell_n"])[:2]:

@@ -102,7 +102,7 @@ def test_elastic_dof_indexing_and_pattern_bit_exact_vs_oracle():
     assert np.array_equal(srp, S.rowptr) and np.array_equal(scol, S.col)
 
 
-def test_config0_sizes():
+def test_elastic20_sizes():
     p = problems.elastic_cube(20)
     rp, col = mfs.portrait(len(p["xyz"]), p["tets"])
     assert len(p["xyz"]) == 9261 and len(p["tets"]) == 48000

@@ -178,7 +178,7 @@ def test_portrait_brute_force_and_counts(n):
     assert S.nnz == nodes + 2 * edges
 
 
-def test_config0_pattern_count_closed_form():
+def test_elastic20_pattern_count_closed_form():
     """20^3 elastic cube: nnz = 9 (nodes + 2 edges) = 1,157,229."""
     nodes, edges = freudenthal_counts(20)
     assert nodes + 2 * edges == 128581

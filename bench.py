@@ -25,10 +25,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-1e-8 solve & SpMV HBM GB/s: mixed+batched+learned vs FP64 GPU CG"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of k_cg_dir_spmv<float,1> on this workload from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
-# calibration (None until measured)
-TRAFFIC_K1 = None
+# Bytes per launch of k_cg_dir_spmv<float,1> on this workload between the L2
+# caches and the memory fabric, from rocprofv3 --pmc FETCH_SIZE (19,011.4
+# KiB, x2.000 from a 512 MiB copy) and WRITE_SIZE (2,179.2 KiB, x1.000),
+# medians over 1,194 dispatches
+TRAFFIC_K1 = 41167104
 
 
 def log(*a):
@@ -322,11 +323,18 @@ def bench_learned(mfs, ctx, n_test):
     trilinear-hat and the learned guess."""
     from multiscalefemsolver3d_amd import learned
     try:
-        return learned.run(ctx, n=8, n_train=20000, n_test=n_test, hidden=256, layers=4, epochs=20, batch=256,
-                           lr=1e-3, tol=1e-8, log=log)
+        out = learned.run(ctx, n=8, n_train=20000, n_test=n_test, hidden=256, layers=4, epochs=20, batch=256,
+                          lr=1e-3, tol=1e-8, log=log)
     except Exception as e:  # reported, not fatal for the primary measurement
         log(f"[bench] learned: failed: {e}")
         return {"error": str(e)}
+    # the MLP's GEMM kernel alone on a large product (MFMA utilisation of the
+    # kernel, separate from the small-batch launch overheads of training)
+    ms, tf = mfs.gemm_bf16_time(ctx, 8192, 8192, 8192, reps=5)
+    out["gemm_bf16_kernel_8192cubed"] = {"ms": round(ms, 3), "tflops": round(tf, 1),
+                                         "frac_dense_bf16_peak": round(tf / 2500.0, 4)}
+    log(f"[bench] k_gemm_bf16 8192^3: {ms:.2f} ms = {tf:.0f} TFLOP/s")
+    return out
 
 
 def bench_large(mfs, problems, ctx, n, solves=True):

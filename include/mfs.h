@@ -360,6 +360,11 @@ int mfs_mlp_train(mfs_mlp M, int64_t n_samples, const float* X, const float* T, 
 int mfs_mlp_train_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_vec T, int batch, int epochs, uint64_t seed,
                       double* final_loss);
 int mfs_mlp_forward_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_vec Y, int batch);
+/* Time the BF16 MFMA GEMM kernel alone on an M x N x K product (A row-major
+ * [M][K], B [N][K], FP32 output; split-K when the output has few tiles),
+ * reps launches back to back: average ms and TFLOP/s (2MNK per launch).
+ * No reference counterpart (the learned guess is specific to this build). */
+int mfs_gemm_bf16_time(mfs_ctx ctx, int M, int N, int K, int reps, double* avg_ms, double* tflops);
 /* Average device time (ms) and achieved TFLOP/s of the forward+backward GEMMs
  * for one minibatch of `batch`. */
 int mfs_mlp_time_step(mfs_mlp M, int batch, int reps, double* avg_ms, double* tflops);

@@ -126,6 +126,14 @@ _sig("mfs_solver_create", C.c_int, [vp, C.POINTER(SolveDesc), C.POINTER(vp)])
 _sig("mfs_solver_destroy", C.c_int, [vp])
 _sig("mfs_solver_run", C.c_int, [vp, vp, vp, C.POINTER(SolveReport)])
 _sig("mfs_solver_kernel_time", C.c_int, [vp, C.c_int, C.c_int, c_dp, c_dp])
+_sig("mfs_gemm_bf16_time", C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, c_dp, c_dp])
+
+
+def gemm_bf16_time(ctx, M, N, K, reps=10):
+    """(avg ms, TFLOP/s) of the BF16 MFMA GEMM kernel on an M x N x K product."""
+    ms, tf = C.c_double(), C.c_double()
+    _chk(lib.mfs_gemm_bf16_time(ctx.h, M, N, K, reps, C.byref(ms), C.byref(tf)), "mfs_gemm_bf16_time")
+    return ms.value, tf.value
 _sig("mfs_solve", C.c_int, [vp, C.POINTER(SolveDesc), vp, vp, C.POINTER(SolveReport)])
 _sig("mfs_assemble_create", C.c_int, [vp, C.POINTER(FemDesc), C.POINTER(vp)])
 _sig("mfs_assemble_run", C.c_int, [vp, c_dp, C.c_double])

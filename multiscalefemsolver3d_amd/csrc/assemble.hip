@@ -215,33 +215,58 @@ __global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ b
                                                    const double* __restrict__ fbase, double* __restrict__ rhs,
                                                    double* __restrict__ x0, int64_t nb) {
   constexpr int C = 64;
+  constexpr int U = 4;  // entries whose index / flag loads are in flight together
   for (int64_t R = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; R < nb; R += (int64_t)gridDim.x * blockDim.x) {
     const int64_t q = R / C, lane = R % C;
     const int64_t sb = slice_base[q];
     const int len = browptr[R + 1] - browptr[R];
+    double f[BS];
+    bool rf[BS];
+#pragma unroll
+    for (int a = 0; a < BS; a++) {
+      f[a] = fbase[BS * R + a];
+      rf[a] = fixed[BS * R + a] != 0;
+    }
+    for (int j0 = 0; j0 < len; j0 += U) {
+      // column indices first, then the fixed flags and values they select;
+      // entries past the row end re-read the last one and are skipped
+      int32_t c[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) c[u] = col[sb + (int64_t)min(j0 + u, len - 1) * C + lane];
+      uint8_t fx[U][BS];
+      double gv[U][BS];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int b = 0; b < BS; b++) {
+          const int64_t cj = (int64_t)BS * c[u] + b;
+          fx[u][b] = fixed[cj];
+          gv[u][b] = g[cj];
+        }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (j0 + u >= len) continue;
+        const int64_t vbase = (sb + (int64_t)(j0 + u) * C) * (BS * BS) + lane;
+#pragma unroll
+        for (int a = 0; a < BS; a++)
+#pragma unroll
+          for (int b = 0; b < BS; b++) {
+            const int64_t vi = vbase + (int64_t)(BS * a + b) * C;
+            const int64_t cj = (int64_t)BS * c[u] + b;
+            if (rf[a]) {
+              sell[vi] = (cj == BS * R + a) ? 1.0 : 0.0;
+            } else if (fx[u][b]) {
+              f[a] -= sell[vi] * gv[u][b];
+              sell[vi] = 0.0;
+            }
+          }
+      }
+    }
 #pragma unroll
     for (int a = 0; a < BS; a++) {
       const int64_t i = BS * R + a;
-      double f = fbase[i];
-      const bool rf = fixed[i] != 0;
-      for (int j = 0; j < len; j++) {
-        const int64_t ent = sb + (int64_t)j * C + lane;
-        const int32_t c = col[ent];
-        const int64_t vbase = (sb + (int64_t)j * C) * (BS * BS) + lane;
-#pragma unroll
-        for (int b = 0; b < BS; b++) {
-          const int64_t vi = vbase + (int64_t)(BS * a + b) * C;
-          const int64_t cj = (int64_t)BS * c + b;
-          if (rf) {
-            sell[vi] = (cj == i) ? 1.0 : 0.0;
-          } else if (fixed[cj]) {
-            f -= sell[vi] * g[cj];
-            sell[vi] = 0.0;
-          }
-        }
-      }
-      rhs[i] = rf ? g[i] : f;
-      x0[i] = rf ? g[i] : 0.0;
+      rhs[i] = rf[a] ? g[i] : f[a];
+      x0[i] = rf[a] ? g[i] : 0.0;
     }
   }
 }

@@ -40,85 +40,131 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 
 enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_RELU = 2 };
 
-// Load a TM x 32 (rows x k) bf16 tile into LDS dst[row][k] (row stride 40).
-// The operand element (row, k) is at src[row*s_row + k*s_k].  When k or row is
-// the contiguous dimension and the chunk is in bounds and 16-byte aligned, 8
-// elements move with one 16-byte load.
-template <int TM>
-__device__ __forceinline__ void load_tile(uint16_t (*dst)[40], const uint16_t* __restrict__ src, int64_t s_row,
-                                          int64_t s_k, int row0, int k0, int R, int K) {
-  const int tid = threadIdx.x;
-  if (s_k == 1) {
-    // chunks of 8 consecutive k: TM rows x 4 chunks
-    for (int c = tid; c < TM * 4; c += 256) {
+// A 128 x 32 (rows x k) bf16 operand tile is moved global -> registers ->
+// LDS dst[row][k] (row stride 40 halves) as 512 chunks of 8 halves, two per
+// thread.  The operand element (row, k) is at src[row*s_row + k*s_k].  When k
+// is contiguous (s_k == 1) a chunk is 8 consecutive k of one row, else 8
+// consecutive rows of one k; a chunk that is in bounds and 16-byte aligned
+// moves with one 16-byte load, otherwise element by element (zero padded).
+struct TileRegs {
+  uint4 v[2];
+};
+
+__device__ __forceinline__ uint4 pack8(const uint16_t (&e)[8]) {
+  uint4 r;
+  r.x = (uint32_t)e[0] | ((uint32_t)e[1] << 16);
+  r.y = (uint32_t)e[2] | ((uint32_t)e[3] << 16);
+  r.z = (uint32_t)e[4] | ((uint32_t)e[5] << 16);
+  r.w = (uint32_t)e[6] | ((uint32_t)e[7] << 16);
+  return r;
+}
+
+__device__ __forceinline__ void tile_load(TileRegs& t, const uint16_t* __restrict__ src, int64_t s_row, int64_t s_k,
+                                          int row0, int k0, int R, int K) {
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int c = threadIdx.x + q * 256;
+    if (s_k == 1) {
       const int row = c >> 2, kk = (c & 3) * 8;
       const int gr = row0 + row, gk = k0 + kk;
       const uint16_t* ptr = src + gr * s_row + gk;
       if (gr < R && gk + 8 <= K && ((uintptr_t)ptr & 15) == 0) {
-        const uint4 v = *reinterpret_cast<const uint4*>(ptr);
-        *reinterpret_cast<uint4*>(&dst[row][kk]) = v;
+        t.v[q] = *reinterpret_cast<const uint4*>(ptr);
       } else {
+        uint16_t e8[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) dst[row][kk + e] = (gr < R && gk + e < K) ? ptr[e] : (uint16_t)0;
+        for (int e = 0; e < 8; e++) e8[e] = (gr < R && gk + e < K) ? ptr[e] : (uint16_t)0;
+        t.v[q] = pack8(e8);
       }
-    }
-  } else {
-    // chunks of 8 consecutive rows (row contiguous, s_row == 1): 32 k x TM/8
-    for (int c = tid; c < TM * 4; c += 256) {
-      const int kk = c / (TM / 8), row = (c % (TM / 8)) * 8;
+    } else {
+      const int kk = c >> 4, row = (c & 15) * 8;
       const int gr = row0 + row, gk = k0 + kk;
       const uint16_t* ptr = src + gr * s_row + gk * s_k;
-      uint16_t v[8];
       if (gk < K && gr + 8 <= R && s_row == 1 && ((uintptr_t)ptr & 15) == 0) {
-        *reinterpret_cast<uint4*>(v) = *reinterpret_cast<const uint4*>(ptr);
+        t.v[q] = *reinterpret_cast<const uint4*>(ptr);
       } else {
+        uint16_t e8[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) v[e] = (gk < K && gr + e < R) ? src[(gr + e) * s_row + gk * s_k] : (uint16_t)0;
+        for (int e = 0; e < 8; e++) e8[e] = (gk < K && gr + e < R) ? src[(gr + e) * s_row + gk * s_k] : (uint16_t)0;
+        t.v[q] = pack8(e8);
       }
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_store(uint16_t (*dst)[40], const TileRegs& t, bool kcontig) {
 #pragma unroll
-      for (int e = 0; e < 8; e++) dst[row + e][kk] = v[e];
+  for (int q = 0; q < 2; q++) {
+    const int c = threadIdx.x + q * 256;
+    if (kcontig) {
+      *reinterpret_cast<uint4*>(&dst[c >> 2][(c & 3) * 8]) = t.v[q];
+    } else {
+      const int kk = c >> 4, row = (c & 15) * 8;
+      const uint32_t w[4] = {t.v[q].x, t.v[q].y, t.v[q].z, t.v[q].w};
+#pragma unroll
+      for (int e = 0; e < 8; e++) dst[row + e][kk] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
     }
   }
 }
 
 // C[M][N] = sum_k A(m,k) B(k,n): 128x128 tile per 256-thread workgroup, each
-// wave a 64x64 quadrant as 4x4 v_mfma_f32_16x16x32_bf16 tiles.  Split-K:
-// blockIdx.z covers k in [z*Kc, min(K, (z+1)*Kc)); with ws != nullptr the
-// raw partial tile goes to ws[z][M][N] (summed by k_splitk_reduce in split
-// order, so results do not depend on scheduling), else the epilogue writes C.
+// wave a 64x64 quadrant as 4x4 v_mfma_f32_16x16x32_bf16 tiles.  The next K
+// step's operands are loaded into registers while the MFMAs run on the
+// current LDS buffer, then stored to the other buffer: one barrier per K
+// step.  Split-K: blockIdx.z covers k in [z*Kc, min(K, (z+1)*Kc)); with
+// ws != nullptr the raw partial tile goes to ws[z][M][N] (summed by
+// k_splitk_reduce in split order, so results do not depend on scheduling),
+// else the epilogue writes C.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int Kc, const uint16_t* __restrict__ A,
                                                    int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
                                                    int64_t sbk, int64_t sbn, float* __restrict__ C, int64_t ldc,
                                                    const float* __restrict__ bias, uint16_t* __restrict__ Cbf,
                                                    float* __restrict__ ws) {
-  __shared__ __align__(16) uint16_t As[128][40];
-  __shared__ __align__(16) uint16_t Bs[128][40];
+  __shared__ __align__(16) uint16_t As[2][128][40];
+  __shared__ __align__(16) uint16_t Bs[2][128][40];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
+  const bool a_kc = sak == 1, b_kc = sbk == 1;
   f32x4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0, 0, 0, 0};
   const int kb = blockIdx.z * Kc, ke = min(K, kb + Kc);
+  TileRegs ra, rb;
+  if (kb < ke) {
+    tile_load(ra, A, sam, sak, m0, kb, M, ke);
+    tile_load(rb, B, sbn, sbk, n0, kb, N, ke);
+    tile_store(As[0], ra, a_kc);
+    tile_store(Bs[0], rb, b_kc);
+  }
+  __syncthreads();
+  int buf = 0;
   for (int k0 = kb; k0 < ke; k0 += 32) {
-    load_tile<128>(As, A, sam, sak, m0, k0, M, ke);
-    load_tile<128>(Bs, B, sbn, sbk, n0, k0, N, ke);
-    __syncthreads();
+    const bool more = k0 + 32 < ke;
+    if (more) {
+      tile_load(ra, A, sam, sak, m0, k0 + 32, M, ke);
+      tile_load(rb, B, sbn, sbk, n0, k0 + 32, N, ke);
+    }
     bf16x8 af[4], bfr[4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
-      af[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+      af[i] = *reinterpret_cast<const bf16x8*>(&As[buf][wm * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
 #pragma unroll
     for (int j = 0; j < 4; j++)
-      bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 64 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+      bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][wn * 64 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    if (more) {
+      tile_store(As[buf ^ 1], ra, a_kc);
+      tile_store(Bs[buf ^ 1], rb, b_kc);
+    }
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int i = 0; i < 4; i++)
@@ -635,6 +681,49 @@ extern "C" int mfs_mlp_forward_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_
   }
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipStreamSynchronize(s));
+  return MFS_OK;
+}
+
+extern "C" int mfs_gemm_bf16_time(mfs_ctx ctx, int M, int N, int K, int reps, double* avg_ms, double* tflops) {
+  MFS_CHECK_ARG(ctx && M > 0 && N > 0 && K > 0 && reps > 0 && avg_ms, "mfs_gemm_bf16_time: bad argument");
+  MFS_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  uint16_t *A = nullptr, *B = nullptr;
+  float *C = nullptr, *ws = nullptr;
+  int st = dmalloc((void**)&A, 2 * (size_t)M * K);
+  if (st == MFS_OK) st = dmalloc((void**)&B, 2 * (size_t)N * K);
+  if (st == MFS_OK) st = dmalloc((void**)&C, 4 * (size_t)M * N);
+  int kc;
+  const int sp = gemm_splits(M, N, K, &kc);
+  const size_t wsb = sp > 1 ? (size_t)sp * M * N * sizeof(float) : 0;
+  if (st == MFS_OK && wsb) st = dmalloc((void**)&ws, wsb);
+  if (st != MFS_OK) {
+    dfree(A);
+    dfree(B);
+    dfree(C);
+    return st;
+  }
+  // operands: X W^T layout of the forward pass (A row-major [M][K], B [N][K])
+  MFS_HIP(hipMemsetAsync(A, 0x3c, 2 * (size_t)M * K, s));
+  MFS_HIP(hipMemsetAsync(B, 0x3c, 2 * (size_t)N * K, s));
+  gemm_epi<EPI_NONE>(s, ws, wsb, M, N, K, A, K, 1, B, 1, K, C, N, nullptr, nullptr);
+  hipEvent_t e0, e1;
+  MFS_HIP(hipEventCreate(&e0));
+  MFS_HIP(hipEventCreate(&e1));
+  MFS_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; r++) gemm_epi<EPI_NONE>(s, ws, wsb, M, N, K, A, K, 1, B, 1, K, C, N, nullptr, nullptr);
+  MFS_HIP(hipEventRecord(e1, s));
+  MFS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(A);
+  dfree(B);
+  dfree(C);
+  dfree(ws);
+  *avg_ms = ms / reps;
+  if (tflops) *tflops = 2.0 * M * N * (double)K / (*avg_ms * 1e-3) / 1e12;
   return MFS_OK;
 }
 

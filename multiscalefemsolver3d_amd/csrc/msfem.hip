@@ -590,22 +590,36 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
         double w[RG];
 #pragma unroll
         for (int r = 0; r < RG; r++) w[r] = 0;
-        for (int v = tid; v < nint; v += nth) {
-          const int i = v % m + 1, j = (v / m) % m + 1, k = v / (m * m) + 1;
-          double acc[3][RG];
-          auto ld = [&](int u, int b, double (&pv)[RG]) {
+        // two rows per thread interleaved in the variants that already use
+        // the whole register file (one wave per SIMD); the others keep one
+        // row here so that they stay at two waves per SIMD
+        constexpr int RR = (RPT >= 3 || (fp64 && RPT >= 2)) ? 2 : 1;
+        auto ld = [&](int u, int b, double (&pv)[RG]) {
 #pragma unroll
-            for (int r = 0; r < RG; r++) pv[r] = x64[(int64_t)u * 3 * NB + b * NB + q0 + r];
-          };
-          stencil_row<double, double>(K64, nint, v, i, j, k, m, ld, acc);
+          for (int r = 0; r < RG; r++) pv[r] = x64[(int64_t)u * 3 * NB + b * NB + q0 + r];
+        };
+        for (int v0 = tid; v0 < nint; v0 += RR * nth) {
+          int vr[RR];
+          bool ok[RR];
 #pragma unroll
-          for (int a = 0; a < 3; a++)
+          for (int q = 0; q < RR; q++) {
+            vr[q] = v0 + q * nth;
+            ok[q] = vr[q] < nint;
+          }
+          double acc[RR][3][RG];
+          stencil_rows<double, double, RR>(K64, nint, vr, ok, m, ld, acc);
 #pragma unroll
-            for (int r = 0; r < RG; r++) {
-              const double rv = b64[(int64_t)v * 3 * NB + a * NB + q0 + r] - acc[a][r];
-              w[r] += rv * rv;
-              R[((int64_t)v * 3 + a) * RG + r] = (T)rv;
-            }
+          for (int q = 0; q < RR; q++) {
+            if (!ok[q]) continue;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+              for (int r = 0; r < RG; r++) {
+                const double rv = b64[(int64_t)vr[q] * 3 * NB + a * NB + q0 + r] - acc[q][a][r];
+                w[r] += rv * rv;
+                R[((int64_t)vr[q] * 3 + a) * RG + r] = (T)rv;
+              }
+          }
         }
         wg_allreduce<RG>(w, partB, rr64);
       }

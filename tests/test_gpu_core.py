@@ -266,6 +266,45 @@ def test_laplace64_full_size_mixed_vs_fp64(ctx):
     assert rel(xs[mfs.PREC_MIXED], xs[mfs.PREC_F64]) < 1e-6
 
 
+@pytest.mark.parametrize("method,precision,tol,bound", [(mfs.PCG_JACOBI, mfs.PREC_F64, 1e-12, 1e-8),
+                                                         (mfs.MINRES, mfs.PREC_F64, 1e-12, 1e-8),
+                                                         (mfs.MINRES, mfs.PREC_MIXED, 1e-10, 1e-6)])
+def test_elastic20_full_size_vs_oracle(ctx, method, precision, tol, bound):
+    """20^3 elastic cube (27,783 DOFs; clamped bottom, pressure on top,
+    reference element convention): the GPU solution matches the oracle's FP64
+    CPU PCG solution on the identical seeded mesh (relative L2 error).  The
+    operator is symmetric indefinite: FP64 PCG still converges, FP32 CG does
+    not (the mixed path uses MINRES here)."""
+    p = problems.elastic_cube(20)
+    S = assemble(ctx, p)
+    O = oracle_elastic(p)
+    xo, _, _ = orc.pcg(O.rowptr, O.col, O.val, O.rhs, O.x0, tol=1e-13)
+    x = mfs.Vec(ctx, S.A.n)
+    x.copy_from(S.x0)
+    rep = mfs.Solver(S.A, method=method, precision=precision, tol=tol, maxit=200000).run(S.b, x)
+    assert rep.status == 0 and rep.true_relres <= tol
+    err = rel(x.download(), xo)
+    print(f"20^3 elastic, method {method} precision {precision}: relative L2 error vs oracle {err:.2e}")
+    assert err < bound
+
+
+@pytest.mark.slow
+def test_laplace64_vs_oracle_solution(ctx):
+    """64^3 Laplace, 200 inclusions: the mixed-precision GPU solution matches
+    the oracle's FP64 CPU PCG solution to a relative L2 error below 1e-8."""
+    p = problems.laplace_inclusions(64)
+    S = assemble(ctx, p)
+    O = oracle_scalar(p)
+    xo, _, _ = orc.pcg(O.rowptr, O.col, O.val, O.rhs, O.x0, tol=1e-12)
+    x = mfs.Vec(ctx, S.A.n)
+    x.copy_from(S.x0)
+    rep = mfs.Solver(S.A, precision=mfs.PREC_MIXED, tol=1e-11).run(S.b, x)
+    assert rep.status == 0 and rep.true_relres <= 1e-11
+    err = rel(x.download(), xo)
+    print(f"64^3 Laplace, mixed: relative L2 error vs oracle {err:.2e}")
+    assert err < 1e-8
+
+
 @pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
 def test_minres_reference_elastic_operator(ctx, precision):
     """MINRES (Jacobi) on the indefinite reference-convention elastic system

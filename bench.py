@@ -324,7 +324,7 @@ def bench_learned(mfs, ctx, n_test):
     from multiscalefemsolver3d_amd import learned
     try:
         out = learned.run(ctx, n=8, n_train=20000, n_test=n_test, hidden=256, layers=4, epochs=20, batch=256,
-                          lr=1e-3, tol=1e-8, log=log)
+                          lr=1e-3, tol=1e-8, log=log, tune=True)
     except Exception as e:  # reported, not fatal for the primary measurement
         log(f"[bench] learned: failed: {e}")
         return {"error": str(e)}

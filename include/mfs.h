@@ -262,8 +262,9 @@ typedef struct {
   int use_graph;   /* capture the persistent launch in a hipGraph             */
   int wg_size;     /* workgroup size (0 = autotuned)                          */
   int use_guess;   /* start from mfs_batch_set_guess values                   */
-  int variant;     /* kernel variant: 0 = autotuned/default, 1 or 2 = fewer or
-                      more rows carried per thread                          */
+  int variant;     /* kernel variant: 0 = autotuned/default; FP32 inner:
+                      1 = 2 rows per thread, 2 = 3 rows, 3 = one row with
+                      chunked K prefetch; FP64: 1 = 1 row, 2 = 2 rows       */
 } mfs_batch_solve_desc;
 
 typedef struct {
@@ -323,7 +324,8 @@ int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u
  * solve for every kernel variant (rows carried per thread), workgroup size
  * (64..256) and workgroups per CU (as many as the kernel's registers and LDS
  * allow, or one fewer), keeps the fastest for later solves of this
- * precision and returns it as *best_wg = wg*100 + per_cu*10 + variant with
+ * precision and returns it as *best_wg = wg*100 + per_cu*10 + variant
+ * (0-based; desc->variant takes it plus one) with
  * its time in *best_ms.  No reference counterpart (the reference solves
  * cells one at a time, MsFEM/MsFEM_Grid.h:4795-5760). */
 int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* desc, int* best_wg, double* best_ms);

@@ -39,11 +39,14 @@ constexpr int NS = 15;   // stencil points
 constexpr int RG = 8;    // right-hand sides per task
 constexpr int NB = 24;   // bases per cell
 constexpr int MAXN = 12; // largest supported cell subdivision (FP32 inner; FP64 inner: 10)
-// rows a thread carries through the inner-iteration phases at once: two
-// kernel variants per precision (the autotuner picks one; more rows keep
-// more loads in flight per wave, fewer rows leave registers for two waves
-// per SIMD)
-constexpr int kRptF32[2] = {2, 3}, kRptF64[2] = {1, 2};
+// Kernel variants per precision, picked by the autotuner: the number of
+// rows a thread carries through the inner-iteration phases at once (more
+// rows keep more loads in flight per wave, fewer rows leave registers for
+// two waves per SIMD), or 0: one row at a time with its K values loaded in
+// three chunks, each chunk's 45 loads issued while the previous chunk is
+// applied (FP32 only)
+constexpr int kRptF32[3] = {2, 3, 0}, kRptF64[2] = {1, 2};
+constexpr int kVariants[2] = {3, 2};  // per precision: FP32 inner, FP64 inner
 
 // Stencil offsets (di, dj, dk), sorted by linear offset (dk, dj, di lexicographic)
 __constant__ int c_stencil[NS][3];
@@ -350,6 +353,54 @@ __device__ __forceinline__ void stencil_rows(const KT* __restrict__ Kc, int nint
   }
 }
 
+// One row at a time with the K values loaded in three chunks of 5 stencil
+// points, the next chunk's 45 loads issued before the current chunk is
+// applied (a wave keeps 45-90 loads in flight without a second row).
+template <typename Load>
+__device__ __forceinline__ void stencil_row_chunked(const float* __restrict__ Kc, int nint, int v, int m, Load load,
+                                                    float (&acc)[3][RG]) {
+  constexpr int CH = 5;  // stencil points per chunk (NS = 3 chunks)
+  const float* kp = Kc + v;
+  float kn[CH * 9];
+#pragma unroll
+  for (int q = 0; q < CH * 9; q++) kn[q] = kp[(int64_t)q * nint];
+  const int ci = v % m + 1, cj = (v / m) % m + 1, ck = v / (m * m) + 1;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int r = 0; r < RG; r++) acc[a][r] = 0;
+#pragma unroll 1
+  for (int c = 0; c < NS / CH; c++) {
+    float kv[CH * 9];
+#pragma unroll
+    for (int q = 0; q < CH * 9; q++) kv[q] = kn[q];
+    if (c + 1 < NS / CH) {
+#pragma unroll
+      for (int q = 0; q < CH * 9; q++) kn[q] = kp[(int64_t)((c + 1) * CH * 9 + q) * nint];
+    }
+#pragma unroll
+    for (int sp = 0; sp < CH; sp++) {
+      const int s = c * CH + sp;
+      const int ni = ci + kStencil[s][0], nj = cj + kStencil[s][1], nk = ck + kStencil[s][2];
+      const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
+      const int u = in ? interior_index(ni, nj, nk, m) : v;
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        float pv[RG];
+        load(u, b, pv);
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          acc[0][r] += kv[sp * 9 + 0 * 3 + b] * pv[r];
+          acc[1][r] += kv[sp * 9 + 1 * 3 + b] * pv[r];
+          acc[2][r] += kv[sp * 9 + 2 * 3 + b] * pv[r];
+        }
+      }
+      // keep the scheduler from hoisting the LDS reads of later points
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
 // Inner-iteration vector phases over RPT rows of one thread at a time
 // (separate restrict-qualified arrays, so the loads of the RPT rows are
 // issued together).
@@ -363,21 +414,39 @@ __device__ __forceinline__ void phase_spmv(const float* __restrict__ K32, const 
 #pragma unroll
     for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
   };
-  for (int v0 = tid; v0 < nint; v0 += RPT * nth) {
-    int vr[RPT];
-    bool ok[RPT];
+  if constexpr (RPT == 0) {
+    static_assert(!fp64, "chunked K loads are an FP32 variant");
+#pragma unroll 1
+    for (int v = tid; v < nint; v += nth) {
+      float acc[3][RG];
+      stencil_row_chunked(K32, nint, v, m, ld, acc);
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+          Q[e] = acc[a][r];
+          w[r] += (double)(acc[a][r] * P[e]);
+        }
+    }
+    return;
+  }
+  constexpr int RP = RPT > 0 ? RPT : 1;
+  for (int v0 = tid; v0 < nint; v0 += RP * nth) {
+    int vr[RP];
+    bool ok[RP];
+#pragma unroll
+    for (int q = 0; q < RP; q++) {
       vr[q] = v0 + q * nth;
       ok[q] = vr[q] < nint;
     }
-    T acc[RPT][3][RG];
+    T acc[RP][3][RG];
     if constexpr (fp64)
-      stencil_rows<double, double, RPT>(K64, nint, vr, ok, m, ld, acc);
+      stencil_rows<double, double, RP>(K64, nint, vr, ok, m, ld, acc);
     else
-      stencil_rows<float, float, RPT>(K32, nint, vr, ok, m, ld, acc);
+      stencil_rows<float, float, RP>(K32, nint, vr, ok, m, ld, acc);
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
+    for (int q = 0; q < RP; q++) {
       if (!ok[q]) continue;
 #pragma unroll
       for (int a = 0; a < 3; a++)
@@ -397,9 +466,10 @@ __device__ __forceinline__ void phase_update(int nint, const T* __restrict__ din
                                              T* __restrict__ R, T* __restrict__ D, const T (&alpha)[RG],
                                              double (&w)[2 * RG]) {
   const int tid = threadIdx.x, nth = blockDim.x;
-  for (int v0 = tid; v0 < nint; v0 += RPT * nth) {
+  constexpr int RP = RPT > 0 ? RPT : 1;
+  for (int v0 = tid; v0 < nint; v0 += RP * nth) {
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
+    for (int q = 0; q < RP; q++) {
       const int v = v0 + q * nth;
       if (v >= nint) continue;
 #pragma unroll
@@ -426,9 +496,10 @@ template <typename T, int RPT>
 __device__ __forceinline__ void phase_direction(int nint, const T* __restrict__ dinv, T* P, const T* __restrict__ R,
                                                 const T (&beta)[RG], int on) {
   const int tid = threadIdx.x, nth = blockDim.x;
-  for (int v0 = tid; v0 < nint; v0 += RPT * nth) {
+  constexpr int RP = RPT > 0 ? RPT : 1;
+  for (int v0 = tid; v0 < nint; v0 += RP * nth) {
 #pragma unroll
-    for (int q = 0; q < RPT; q++) {
+    for (int q = 0; q < RP; q++) {
       const int v = v0 + q * nth;
       if (v >= nint) continue;
 #pragma unroll
@@ -534,9 +605,12 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
   for (;;) {
     if (tid == 0) task_s = (int)__hip_atomic_fetch_add(A.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const int task = task_s;
+    // uniform across the workgroup: keep the task, its cell record and every
+    // pointer derived from them in scalar registers
+    const int task = __builtin_amdgcn_readfirstlane(task_s);
     if (task >= A.ntasks) break;
-    const int cell = A.task_cell[task], grp = A.task_group[task];
+    const int cell = __builtin_amdgcn_readfirstlane(A.task_cell[task]);
+    const int grp = __builtin_amdgcn_readfirstlane(A.task_group[task]);
     const Cell C = A.cells[cell];
     const int n = C.n, m = n - 1, nint = C.nint;
     const float* K32 = A.K32 + C.int0 * (NS * 9);
@@ -1146,7 +1220,9 @@ static int batch_scratch(mfs_batch B, int precision, int per_cu) {
 static const void* batch_kernel(int precision, int var) {
   if (precision == MFS_PREC_F64)
     return var == 0 ? (const void*)k_batch_solve<double, kRptF64[0]> : (const void*)k_batch_solve<double, kRptF64[1]>;
-  return var == 0 ? (const void*)k_batch_solve<float, kRptF32[0]> : (const void*)k_batch_solve<float, kRptF32[1]>;
+  return var == 0   ? (const void*)k_batch_solve<float, kRptF32[0]>
+         : var == 1 ? (const void*)k_batch_solve<float, kRptF32[1]>
+                    : (const void*)k_batch_solve<float, kRptF32[2]>;
 }
 
 static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu, int var,
@@ -1194,8 +1270,10 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   } else {
     if (var == 0)
       k_batch_solve<float, kRptF32[0]><<<grid, wg, lds, s>>>(A);
-    else
+    else if (var == 1)
       k_batch_solve<float, kRptF32[1]><<<grid, wg, lds, s>>>(A);
+    else
+      k_batch_solve<float, kRptF32[2]><<<grid, wg, lds, s>>>(A);
   }
   MFS_HIP(hipGetLastError());
   return MFS_OK;
@@ -1208,7 +1286,8 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
   MFS_CHECK_ARG(d->tol > 0, "mfs_batch_solve: tol must be positive");
   MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 256),
                 "mfs_batch_solve: wg_size must be a multiple of 64 in 64..256");
-  MFS_CHECK_ARG(d->variant >= 0 && d->variant <= 2, "mfs_batch_solve: variant must be 0, 1 or 2");
+  MFS_CHECK_ARG(d->variant >= 0 && d->variant <= kVariants[d->precision == MFS_PREC_F64 ? 1 : 0],
+                "mfs_batch_solve: variant must be 0 (autotuned) or 1..3 (FP32 inner) / 1..2 (FP64)");
   if (lds_bytes(B, d->precision) > 160 * 1024 - 8192) {
     set_error("mfs_batch_solve: cells too large for the LDS-resident search directions (FP64 inner needs n <= 10)");
     return MFS_ERR_UNSUPPORTED;
@@ -1231,7 +1310,7 @@ static int resident_per_cu(mfs_batch B, int precision, int wg, int var) {
 static int pick_shape(mfs_batch B, const mfs_batch_solve_desc* d, int* wg, int* per_cu, int* var) {
   const int t = d->precision == MFS_PREC_F64 ? 1 : 0;
   const int wgs = d->wg_size ? d->wg_size : (B->tuned_wg[t] ? B->tuned_wg[t] : 256);
-  *var = d->variant > 0 ? d->variant - 1 : (B->tuned_var[t] >= 0 ? B->tuned_var[t] : 1);
+  *var = d->variant > 0 ? d->variant - 1 : (B->tuned_var[t] >= 0 ? B->tuned_var[t] : 0);
   *wg = wgs;
   *per_cu = (!d->wg_size && B->tuned_per_cu[t]) ? B->tuned_per_cu[t] : resident_per_cu(B, d->precision, wgs, *var);
   return MFS_OK;
@@ -1374,7 +1453,7 @@ extern "C" int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* d, in
   const int t = d->precision == MFS_PREC_F64 ? 1 : 0;
   double best = 1e30;
   int bw = 256, bp = 1, bv = 1;
-  for (int var : {0, 1})
+  for (int var = 0; var < kVariants[t]; var++)
   for (int wg : {64, 128, 192, 256}) {
     const int res = resident_per_cu(B, d->precision, wg, var);
     for (int pc : {res, res - 1}) {

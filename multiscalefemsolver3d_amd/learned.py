@@ -44,7 +44,15 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     cn, org, h, E = cells(n_train, n, seed_train)
     Btr = Batch(ctx, cn, org, h, E)
     Btr.assemble()
-    rep = Btr.solve(precision=PREC_MIXED, tol=tol)
+    shape = {PREC_MIXED: {}, PREC_F64: {}}
+    if tune:
+        # launch shapes tuned on the training cells (same subdivision as the
+        # test cells) and reused for the test batch
+        for prec in (PREC_MIXED, PREC_F64):
+            wg, _, var, _ = Btr.autotune(precision=prec, tol=tol)
+            shape[prec] = {"wg_size": wg, "variant": var + 1}
+        out["autotuned_shape"] = {"mixed": shape[PREC_MIXED], "fp64": shape[PREC_F64]}
+    rep = Btr.solve(precision=PREC_MIXED, tol=tol, **shape[PREC_MIXED])
     out["train_solve_ms"] = round(rep.seconds * 1e3, 1)
     X = Vec(ctx, n_train * nt, F32)
     T = Vec(ctx, n_train * nout, F32)
@@ -70,13 +78,10 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     cn, org, h, E = cells(n_test, n, seed_test)
     B = Batch(ctx, cn, org, h, E)
     B.assemble()
-    if tune:
-        out["autotuned_shape"] = {"mixed": B.autotune(precision=PREC_MIXED, tol=tol)[:3],
-                                  "fp64": B.autotune(precision=PREC_F64, tol=tol)[:3]}
     res = {}
     # the plain FP64 batched solve from the zero guess (the comparison point)
-    res["fp64_zero"] = B.solve(precision=PREC_F64, tol=tol)
-    r0 = B.solve(precision=PREC_MIXED, tol=tol)
+    res["fp64_zero"] = B.solve(precision=PREC_F64, tol=tol, **shape[PREC_F64])
+    r0 = B.solve(precision=PREC_MIXED, tol=tol, **shape[PREC_MIXED])
     it0, rr0, _ = B.status()
     res["zero"] = r0
     # reference solution for guess-quality figures
@@ -84,7 +89,7 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     B.targets(Tref)
     corr = Tref.download()
     B.guess_from(None)
-    r_hat = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True)
+    r_hat = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True, **shape[PREC_MIXED])
     it_hat, _, _ = B.status()
     res["hat"] = r_hat
     Xt = Vec(ctx, n_test * nt, F32)
@@ -93,7 +98,7 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     net.forward_vec(n_test, Xt, Y, batch)
     pred = Y.download()
     B.guess_from(Y)
-    r_l = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True)
+    r_l = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True, **shape[PREC_MIXED])
     it_l, rr_l, fl_l = B.status()
     res["learned"] = r_l
     # guess quality: relative error of the interior correction (basis - hat)

@@ -131,6 +131,38 @@ def main():
     log(f"[bench] SpMV f32 {spmv_ms * 1e3:.1f} us = {spmv_gbs:.0f} GB/s; f64 {spmv64_ms * 1e3:.1f} us = "
         f"{spmv64_gbs:.0f} GB/s (algorithmic bytes; working set is cache-resident)")
 
+    # six right-hand sides with this matrix (the homogenisation driver's load
+    # cases): one iteration stream with one SpMM per iteration vs six solves
+    rng = np.random.default_rng(6)
+    b0 = S.b.download()
+    bs6 = [mfs.Vec(ctx, A.n, mfs.F64, b0 * (1.0 + 0.2 * j) + rng.normal(size=A.n) * np.abs(b0).max() * 0.05)
+           for j in range(6)]
+    xs6 = [mfs.Vec(ctx, A.n) for _ in range(6)]
+
+    def reset6():
+        for v in xs6:
+            v.copy_from(S.x0)
+
+    reset6()
+    mfs.solve_multi(A, bs6, xs6, precision=mfs.PREC_MIXED, tol=1e-8, inner_tol=1e-4)  # warm-up
+    reset6()
+    ctx.sync()
+    t6 = time.perf_counter()
+    reps6 = mfs.solve_multi(A, bs6, xs6, precision=mfs.PREC_MIXED, tol=1e-8, inner_tol=1e-4)
+    ctx.sync()
+    joint_ms = 1000.0 * (time.perf_counter() - t6)
+    reset6()
+    ctx.sync()
+    t6 = time.perf_counter()
+    for j in range(6):
+        mixed.run(bs6[j], xs6[j])
+    ctx.sync()
+    seq_ms = 1000.0 * (time.perf_counter() - t6)
+    multi6 = {"systems": 6, "joint_ms": round(joint_ms, 2), "sequential_ms": round(seq_ms, 2),
+              "speedup": round(seq_ms / joint_ms, 3), "max_true_relres": max(r.true_relres for r in reps6),
+              "iterations": [r.iterations for r in reps6], "precision": "f32 inner / f64 outer"}
+    log(f"[bench] 6 right-hand sides: joint {joint_ms:.1f} ms vs sequential {seq_ms:.1f} ms")
+
     cpu = None
     if not args.no_cpu:
         import oracle as orc
@@ -205,6 +237,7 @@ def main():
                              "algorithmic bytes / launch time"},
         "k_cg_update_f32": {"avg_us": round(k2_ms * 1e3, 3), "bytes_per_launch": k2_bytes,
                             "gbs": round(k2_gbs, 1)},
+        "multi_rhs_6": multi6,
         "spmv_f32_gbs": round(spmv_gbs, 1),
         "spmv_f32_us": round(spmv_ms * 1e3, 3),
         "cpu_baseline": cpu,
@@ -388,6 +421,32 @@ def bench_large(mfs, problems, ctx, n, solves=True):
                      convention=mfs.ELEM_TEXTBOOK, fixed_nodes=p["fixed_nodes"], fixed_values=p["fixed_values"],
                      tris=p["tris"], traction=p["traction"])
     S.run(p["E"], p["nu"])
+    # six right-hand sides: iteration throughput of the joint stream (matrix
+    # read once per iteration for all six) vs one system at a time, 300
+    # iterations each (HBM-bound regime)
+    rng = np.random.default_rng(7)
+    b0 = S.b.download()
+    bs6 = [mfs.Vec(ctx, S.A.n, mfs.F64, b0 * (1.0 + 0.2 * j) + rng.normal(size=S.A.n) * np.abs(b0).max() * 0.05)
+           for j in range(6)]
+    xs6 = [mfs.Vec(ctx, S.A.n) for _ in range(6)]
+    for v in xs6:
+        v.copy_from(S.x0)
+    ctx.sync()
+    t6 = time.perf_counter()
+    mfs.solve_multi(S.A, bs6, xs6, check=False, precision=mfs.PREC_MIXED, tol=1e-12, maxit=300)
+    ctx.sync()
+    joint = time.perf_counter() - t6
+    one = mfs.Solver(S.A, precision=mfs.PREC_MIXED, tol=1e-12, maxit=300)
+    xs6[0].copy_from(S.x0)
+    ctx.sync()
+    t6 = time.perf_counter()
+    one.run(bs6[0], xs6[0], check=False)
+    ctx.sync()
+    single = time.perf_counter() - t6
+    res["multi_rhs_6_throughput"] = {"iterations": 300, "joint_six_s": round(joint, 3), "one_system_s": round(single, 3),
+                                     "speedup_vs_six_sequential": round(6 * single / joint, 3)}
+    log(f"[bench] large 6 right-hand sides, 300 iterations: joint {joint:.2f} s vs one system {single:.2f} s")
+    del bs6, xs6
     for name, kw in (("pcg_fp64_textbook", dict(precision=mfs.PREC_F64, maxit=40000)),
                      ("pcg_mixed_textbook", dict(precision=mfs.PREC_MIXED, maxit=40000)),
                      ("gmres30_mixed_textbook_capped", dict(method=mfs.GMRES, precision=mfs.PREC_MIXED,

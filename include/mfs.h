@@ -164,6 +164,18 @@ int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_report* report)
  * counterpart: measures the kernels replacing the inner loop of
  * CSSD_Matrix::MSG / BiCG_Diag (Library/CSSD_Matrix.h:1496-1663). */
 int mfs_solver_kernel_time(mfs_solver S, int kernel, int reps, double* avg_ms, double* bytes);
+/* Solve k <= 8 systems A x_j = b_j that share A (FP64 vectors; x_j holds
+ * the initial guess on entry) as one iteration stream: each iteration reads
+ * A once for all systems (SELL SpMM over interleaved vectors), per-system
+ * alpha, beta and stop tests, a converged system is frozen while the others
+ * continue.  desc->method: MFS_PCG_JACOBI or MFS_CG; precision MFS_PREC_F64
+ * or MFS_PREC_MIXED (FP64 refinement per system around the FP32
+ * iteration).  reports[j] receives system j's outcome (seconds: the whole
+ * call).  Replaces the homogenisation driver's six sequential solves with
+ * the same matrix (Mechanical/MechanicalMain.cpp:243-420, one
+ * CSSD_Matrix::MSG_PreconditioningSSOR per STAGE). */
+int mfs_solve_multi(mfs_mat A, const mfs_solve_desc* desc, int k, mfs_vec* b, mfs_vec* x,
+                    mfs_solve_report* reports);
 /* One-shot convenience: create + run + destroy. */
 int mfs_solve(mfs_mat A, const mfs_solve_desc* desc, mfs_vec b, mfs_vec x, mfs_solve_report* report);
 
@@ -262,9 +274,9 @@ typedef struct {
   int use_graph;   /* capture the persistent launch in a hipGraph             */
   int wg_size;     /* workgroup size (0 = autotuned)                          */
   int use_guess;   /* start from mfs_batch_set_guess values                   */
-  int variant;     /* kernel variant: 0 = autotuned/default; FP32 inner:
-                      1 = 2 rows per thread, 2 = 3 rows, 3 = one row with
-                      chunked K prefetch; FP64: 1 = 1 row, 2 = 2 rows       */
+  int variant;     /* kernel variant: 0 = autotuned/default; rows carried
+                      per thread: FP32 inner 1 -> 2 rows, 2 -> 3 rows;
+                      FP64 inner 1 -> 1 row, 2 -> 2 rows                    */
 } mfs_batch_solve_desc;
 
 typedef struct {

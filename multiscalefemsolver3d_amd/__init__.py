@@ -126,6 +126,24 @@ _sig("mfs_solver_create", C.c_int, [vp, C.POINTER(SolveDesc), C.POINTER(vp)])
 _sig("mfs_solver_destroy", C.c_int, [vp])
 _sig("mfs_solver_run", C.c_int, [vp, vp, vp, C.POINTER(SolveReport)])
 _sig("mfs_solver_kernel_time", C.c_int, [vp, C.c_int, C.c_int, c_dp, c_dp])
+_sig("mfs_solve_multi", C.c_int, [vp, C.POINTER(SolveDesc), C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                  C.POINTER(SolveReport)])
+
+
+def solve_multi(A, bs, xs, check=True, **kw):
+    """Solve A x_j = b_j for the k <= 8 FP64 vector pairs in one iteration
+    stream (the matrix is read once per iteration for all of them); x_j hold
+    the initial guesses.  Returns one Report per system."""
+    k = len(bs)
+    assert k == len(xs) and 1 <= k <= 8
+    d = solve_desc(**kw)
+    hb = (vp * k)(*[v.h for v in bs])
+    hx = (vp * k)(*[v.h for v in xs])
+    reps = (SolveReport * k)()
+    code = lib.mfs_solve_multi(A.h, C.byref(d), k, hb, hx, reps)
+    if check:
+        _chk(code, "mfs_solve_multi")
+    return [Report(r.iterations, r.outer_steps, r.true_relres, r.recursive_relres, r.seconds, r.status) for r in reps]
 _sig("mfs_gemm_bf16_time", C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, c_dp, c_dp])
 
 

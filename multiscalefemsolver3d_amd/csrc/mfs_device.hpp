@@ -277,3 +277,4 @@ __device__ __forceinline__ void write_block_partials(T (&v)[NV], double* __restr
 }
 
 __global__ void k_finish_sums(const double* __restrict__ part, unsigned G, int nv, double* __restrict__ out);
+__global__ void k_finish_sums_multi(const double* __restrict__ part, unsigned G, double* __restrict__ out);

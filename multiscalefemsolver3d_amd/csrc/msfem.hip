@@ -42,11 +42,11 @@ constexpr int MAXN = 12; // largest supported cell subdivision (FP32 inner; FP64
 // Kernel variants per precision, picked by the autotuner: the number of
 // rows a thread carries through the inner-iteration phases at once (more
 // rows keep more loads in flight per wave, fewer rows leave registers for
-// two waves per SIMD), or 0: one row at a time with its K values loaded in
-// three chunks, each chunk's 45 loads issued while the previous chunk is
-// applied (FP32 only)
-constexpr int kRptF32[3] = {2, 3, 0}, kRptF64[2] = {1, 2};
-constexpr int kVariants[2] = {3, 2};  // per precision: FP32 inner, FP64 inner
+// two waves per SIMD).  (A one-row variant with the row's K values loaded
+// in chunks of 5 stencil points was measured and never won: it needs one
+// wave per SIMD.)
+constexpr int kRptF32[2] = {2, 3}, kRptF64[2] = {1, 2};
+constexpr int kVariants[2] = {2, 2};  // per precision: FP32 inner, FP64 inner
 
 // Stencil offsets (di, dj, dk), sorted by linear offset (dk, dj, di lexicographic)
 __constant__ int c_stencil[NS][3];
@@ -353,54 +353,6 @@ __device__ __forceinline__ void stencil_rows(const KT* __restrict__ Kc, int nint
   }
 }
 
-// One row at a time with the K values loaded in three chunks of 5 stencil
-// points, the next chunk's 45 loads issued before the current chunk is
-// applied (a wave keeps 45-90 loads in flight without a second row).
-template <typename Load>
-__device__ __forceinline__ void stencil_row_chunked(const float* __restrict__ Kc, int nint, int v, int m, Load load,
-                                                    float (&acc)[3][RG]) {
-  constexpr int CH = 5;  // stencil points per chunk (NS = 3 chunks)
-  const float* kp = Kc + v;
-  float kn[CH * 9];
-#pragma unroll
-  for (int q = 0; q < CH * 9; q++) kn[q] = kp[(int64_t)q * nint];
-  const int ci = v % m + 1, cj = (v / m) % m + 1, ck = v / (m * m) + 1;
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int r = 0; r < RG; r++) acc[a][r] = 0;
-#pragma unroll 1
-  for (int c = 0; c < NS / CH; c++) {
-    float kv[CH * 9];
-#pragma unroll
-    for (int q = 0; q < CH * 9; q++) kv[q] = kn[q];
-    if (c + 1 < NS / CH) {
-#pragma unroll
-      for (int q = 0; q < CH * 9; q++) kn[q] = kp[(int64_t)((c + 1) * CH * 9 + q) * nint];
-    }
-#pragma unroll
-    for (int sp = 0; sp < CH; sp++) {
-      const int s = c * CH + sp;
-      const int ni = ci + kStencil[s][0], nj = cj + kStencil[s][1], nk = ck + kStencil[s][2];
-      const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
-      const int u = in ? interior_index(ni, nj, nk, m) : v;
-#pragma unroll
-      for (int b = 0; b < 3; b++) {
-        float pv[RG];
-        load(u, b, pv);
-#pragma unroll
-        for (int r = 0; r < RG; r++) {
-          acc[0][r] += kv[sp * 9 + 0 * 3 + b] * pv[r];
-          acc[1][r] += kv[sp * 9 + 1 * 3 + b] * pv[r];
-          acc[2][r] += kv[sp * 9 + 2 * 3 + b] * pv[r];
-        }
-      }
-      // keep the scheduler from hoisting the LDS reads of later points
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-}
-
 // Inner-iteration vector phases over RPT rows of one thread at a time
 // (separate restrict-qualified arrays, so the loads of the RPT rows are
 // issued together).
@@ -414,23 +366,6 @@ __device__ __forceinline__ void phase_spmv(const float* __restrict__ K32, const 
 #pragma unroll
     for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
   };
-  if constexpr (RPT == 0) {
-    static_assert(!fp64, "chunked K loads are an FP32 variant");
-#pragma unroll 1
-    for (int v = tid; v < nint; v += nth) {
-      float acc[3][RG];
-      stencil_row_chunked(K32, nint, v, m, ld, acc);
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int r = 0; r < RG; r++) {
-          const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-          Q[e] = acc[a][r];
-          w[r] += (double)(acc[a][r] * P[e]);
-        }
-    }
-    return;
-  }
   constexpr int RP = RPT > 0 ? RPT : 1;
   for (int v0 = tid; v0 < nint; v0 += RP * nth) {
     int vr[RP];
@@ -1220,9 +1155,7 @@ static int batch_scratch(mfs_batch B, int precision, int per_cu) {
 static const void* batch_kernel(int precision, int var) {
   if (precision == MFS_PREC_F64)
     return var == 0 ? (const void*)k_batch_solve<double, kRptF64[0]> : (const void*)k_batch_solve<double, kRptF64[1]>;
-  return var == 0   ? (const void*)k_batch_solve<float, kRptF32[0]>
-         : var == 1 ? (const void*)k_batch_solve<float, kRptF32[1]>
-                    : (const void*)k_batch_solve<float, kRptF32[2]>;
+  return var == 0 ? (const void*)k_batch_solve<float, kRptF32[0]> : (const void*)k_batch_solve<float, kRptF32[1]>;
 }
 
 static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu, int var,
@@ -1270,10 +1203,8 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   } else {
     if (var == 0)
       k_batch_solve<float, kRptF32[0]><<<grid, wg, lds, s>>>(A);
-    else if (var == 1)
-      k_batch_solve<float, kRptF32[1]><<<grid, wg, lds, s>>>(A);
     else
-      k_batch_solve<float, kRptF32[2]><<<grid, wg, lds, s>>>(A);
+      k_batch_solve<float, kRptF32[1]><<<grid, wg, lds, s>>>(A);
   }
   MFS_HIP(hipGetLastError());
   return MFS_OK;
@@ -1287,7 +1218,7 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
   MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 256),
                 "mfs_batch_solve: wg_size must be a multiple of 64 in 64..256");
   MFS_CHECK_ARG(d->variant >= 0 && d->variant <= kVariants[d->precision == MFS_PREC_F64 ? 1 : 0],
-                "mfs_batch_solve: variant must be 0 (autotuned) or 1..3 (FP32 inner) / 1..2 (FP64)");
+                "mfs_batch_solve: variant must be 0 (autotuned), 1 or 2");
   if (lds_bytes(B, d->precision) > 160 * 1024 - 8192) {
     set_error("mfs_batch_solve: cells too large for the LDS-resident search directions (FP64 inner needs n <= 10)");
     return MFS_ERR_UNSUPPORTED;

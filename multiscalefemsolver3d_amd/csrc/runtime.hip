@@ -579,6 +579,30 @@ __global__ void k_finish_sums(const double* __restrict__ part, unsigned G, int n
   }
 }
 
+// Block k adds up value k's G partials part[k*G + b] in a fixed order and
+// writes out[k] (grid = number of values).  Used between a producer with a
+// large grid and a consumer that would otherwise have every block re-add
+// all partials.
+__global__ __launch_bounds__(256) void k_finish_sums_multi(const double* __restrict__ part, unsigned G,
+                                                           double* __restrict__ out) {
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = blockIdx.x;
+  double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (unsigned b0 = threadIdx.x; b0 < G; b0 += 8 * blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const unsigned b = b0 + u * blockDim.x;
+      if (b < G) s8[u] += part[(size_t)k * G + b];
+    }
+  }
+  double s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
+  s = wave_sum(s);
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[k] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 namespace mfs {
 template <int MODE>
 static int spmv_mode_launch(mfs_mat A, int prec, const void* x, void* y, const void* b, double* partials,

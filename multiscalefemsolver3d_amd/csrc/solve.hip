@@ -268,8 +268,23 @@ struct PcgEngine {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
 
+  // Large systems: one extra kernel per reduction adds the producer's
+  // partials once (tot), instead of every consumer block re-adding them all
+  // (their cost grows with the product of the two grids).
+  bool fin = false;
+  double* tot = nullptr;  // [pq, rz, rr]
   double* part_pq() { return red.partials; }
   double* part_rz() { return red.partials + g_spmv; }
+  const double* rz_src() { return fin ? tot + 1 : part_rz(); }
+  unsigned rz_g() { return fin ? 1u : g_vec; }
+  const double* pq_src() { return fin ? tot : part_pq(); }
+  unsigned pq_g() { return fin ? 1u : g_spmv; }
+  void finish_rz(hipStream_t s) {
+    if (fin) k_finish_sums_multi<<<2, 256, 0, s>>>(part_rz(), g_vec, tot + 1);
+  }
+  void finish_pq(hipStream_t s) {
+    if (fin) k_finish_sums_multi<<<1, 256, 0, s>>>(part_pq(), g_spmv, tot);
+  }
 
   int init(mfs_mat A_, bool precond, int chunk_) {
     A = A_;
@@ -289,6 +304,9 @@ struct PcgEngine {
     const int64_t cus = std::max(A->ctx->cu_count, 64);
     g_vec = (unsigned)std::min<int64_t>(4 * cus, std::max<int64_t>(cus, (n + 1023) / 1024));
     MFS_TRY(reduce_scratch_alloc(red, (int64_t)g_spmv + 2 * (int64_t)g_vec + 64));
+    fin = (uint64_t)g_spmv * g_vec > 2000000ull;
+    if (const char* e = std::getenv("MFS_FINISH_SUMS")) fin = e[0] == '1';  // force either way (tests)
+    MFS_TRY(dmalloc((void**)&tot, sizeof(double) * 4));
     return capture();
   }
 
@@ -298,11 +316,13 @@ struct PcgEngine {
     T* p_new = pbuf[parity ^ 1];
     if (A->bs == 1)
       k_cg_dir_spmv<T, 1><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z, p_old, p_new, q,
-                                                 A->nb, ns, st, part_rz(), g_vec, part_pq());
+                                                 A->nb, ns, st, rz_src(), rz_g(), part_pq());
     else
       k_cg_dir_spmv<T, 3><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z, p_old, p_new, q,
-                                                 A->nb, ns, st, part_rz(), g_vec, part_pq());
-    k_cg_update<T><<<g_vec, 256, 0, s>>>(x, p_new, r, z, q, dinv, n, st, part_pq(), g_spmv, part_rz());
+                                                 A->nb, ns, st, rz_src(), rz_g(), part_pq());
+    finish_pq(s);
+    k_cg_update<T><<<g_vec, 256, 0, s>>>(x, p_new, r, z, q, dinv, n, st, pq_src(), pq_g(), part_rz());
+    finish_rz(s);
   }
 
   int capture() {
@@ -339,6 +359,7 @@ struct PcgEngine {
       k_cg_start<T><<<g_vec, 256, 0, s>>>(z, pbuf[0], r, dinv, n, part_rz());
       k_cg_start_state<<<1, 64, 0, s>>>(st, tol, bb, maxit, 0, 0.0);
     }
+    finish_rz(s);
     MFS_HIP(hipGetLastError());
     CgState* hs = (CgState*)A->ctx->pinned;
     hs->done = CG_RUN;
@@ -367,6 +388,7 @@ struct PcgEngine {
     if (graph) (void)hipGraphDestroy(graph);
     for (T* b : {x, r, z, q, pbuf[0], pbuf[1]}) dfree(b);
     dfree(st);
+    dfree(tot);
     reduce_scratch_free(red);
   }
 };
@@ -1026,6 +1048,7 @@ int time_cg_kernel(PcgEngine<T>& E, int kernel, int reps, double* avg_ms, double
   k_fill<<<gv, 256, 0, s>>>(f64 ? nullptr : (float*)E.x, f64 ? (double*)E.x : nullptr, 0.0, n);
   k_cg_start<T><<<E.g_vec, 256, 0, s>>>(E.z, E.pbuf[0], E.r, E.dinv, n, E.part_rz());
   k_cg_start_state<<<1, 64, 0, s>>>(E.st, 0.0, 1.0, 1 << 30, 0, 0.0);
+  E.finish_rz(s);
   // one K1 so that K2 has partials to read
   E.launch_iteration(s, 0);
   MFS_HIP(hipGetLastError());
@@ -1038,14 +1061,14 @@ int time_cg_kernel(PcgEngine<T>& E, int kernel, int reps, double* avg_ms, double
     if (kernel == 0) {
       if (A->bs == 1)
         k_cg_dir_spmv<T, 1><<<E.g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, E.val, E.z,
-                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.part_rz(),
-                                                     E.g_vec, E.part_pq());
+                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.rz_src(),
+                                                     E.rz_g(), E.part_pq());
       else
         k_cg_dir_spmv<T, 3><<<E.g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, E.val, E.z,
-                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.part_rz(),
-                                                     E.g_vec, E.part_pq());
+                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.rz_src(),
+                                                     E.rz_g(), E.part_pq());
     } else {
-      k_cg_update<T><<<E.g_vec, 256, 0, s>>>(E.x, E.pbuf[1], E.r, E.z, E.q, E.dinv, n, E.st, E.part_pq(), E.g_spmv,
+      k_cg_update<T><<<E.g_vec, 256, 0, s>>>(E.x, E.pbuf[1], E.r, E.z, E.q, E.dinv, n, E.st, E.pq_src(), E.pq_g(),
                                              E.part_rz());
     }
   }

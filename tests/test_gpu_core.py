@@ -151,6 +151,32 @@ def test_mixed_refinement_reaches_fp64_tolerance(ctx, method):
     assert rel(x.download(), xo) < 1e-8
 
 
+@pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
+def test_finish_sum_kernels_give_the_same_solve(ctx, monkeypatch, precision):
+    """Large systems add the per-block partial sums once in a separate
+    kernel instead of in every consumer block (MFS_FINISH_SUMS forces either
+    path): both reach the tolerance with the same iterations and solution."""
+    p = problems.laplace_inclusions(16)
+    S = assemble(ctx, p)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MFS_FINISH_SUMS", mode)
+        x = mfs.Vec(ctx, S.A.n)
+        x.copy_from(S.x0)
+        rep = mfs.Solver(S.A, precision=precision, tol=1e-10).run(S.b, x)
+        out[mode] = (rep, x.download())
+        xs = [mfs.Vec(ctx, S.A.n) for _ in range(3)]
+        for v in xs:
+            v.copy_from(S.x0)
+        reps = mfs.solve_multi(S.A, [S.b, S.b, S.b], xs, precision=precision, tol=1e-10)
+        assert all(r.true_relres <= 1e-10 for r in reps)
+        assert rel(xs[2].download(), out[mode][1]) < 1e-8
+    r0, r1 = out["0"][0], out["1"][0]
+    assert r0.true_relres <= 1e-10 and r1.true_relres <= 1e-10
+    assert abs(r0.iterations - r1.iterations) <= 2
+    assert rel(out["0"][1], out["1"][1]) < 1e-9
+
+
 def test_mixed_cg_keeps_krylov_information(ctx):
     """The FP32 iteration is continued across FP64 residual replacements (the
     search direction is kept), so the mixed solve needs about as many
@@ -303,6 +329,44 @@ def test_laplace64_vs_oracle_solution(ctx):
     err = rel(x.download(), xo)
     print(f"64^3 Laplace, mixed: relative L2 error vs oracle {err:.2e}")
     assert err < 1e-8
+
+
+@pytest.mark.parametrize("kind", ["scalar", "elastic"])
+@pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
+@pytest.mark.parametrize("k", [1, 3, 6])
+def test_multi_rhs_matches_single_solves(ctx, kind, precision, k):
+    """k systems sharing one matrix, solved as one iteration stream (one
+    SpMM per iteration), give the solutions of k separate solves; the padding
+    systems of the interleaved layout (k = 1, 3 -> 4, 6 -> 8) stay inert."""
+    if kind == "scalar":
+        p = problems.laplace_inclusions(10, count=30, rmin=0.05, rmax=0.15)
+        S = assemble(ctx, p)
+    else:
+        p = problems.elastic_cube(5, field="grf")
+        S = assemble(ctx, p, mfs.ELEM_TEXTBOOK)
+    n = S.A.n
+    rng = np.random.default_rng(40 + k)
+    b0 = S.b.download()
+    bs = [mfs.Vec(ctx, n, mfs.F64, b0 * (1.0 + 0.5 * j) + rng.normal(size=n) * np.abs(b0).max() * 0.1)
+          for j in range(k)]
+    xs = []
+    for j in range(k):
+        x = mfs.Vec(ctx, n)
+        x.copy_from(S.x0)
+        xs.append(x)
+    reps = mfs.solve_multi(S.A, bs, xs, precision=precision, tol=1e-10)
+    rp, col, val = S.A.export_csr()
+    for j in range(k):
+        assert reps[j].status == 0 and reps[j].true_relres <= 1e-10, (j, reps[j])
+        xm = xs[j].download()
+        bj = bs[j].download()
+        r = bj - orc.csr_mv(rp, col, val, xm)
+        assert np.linalg.norm(r) / np.linalg.norm(bj) <= 1.01e-10
+        xsgl = mfs.Vec(ctx, n)
+        xsgl.copy_from(S.x0)
+        ref = mfs.Solver(S.A, precision=mfs.PREC_F64, tol=1e-11).run(bs[j], xsgl, check=False)
+        assert ref.true_relres <= 1e-10
+        assert rel(xm, xsgl.download()) < 1e-7, j
 
 
 @pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])

@@ -38,8 +38,7 @@ def test_bases_match_oracle(ctx, small, precision):
 
 
 @pytest.mark.parametrize("precision,variant,wg", [
-    (p, v, w) for p in (mfs.PREC_MIXED, mfs.PREC_F64) for v, w in ((1, 64), (1, 256), (2, 64), (2, 192))
-] + [(mfs.PREC_MIXED, 3, 64), (mfs.PREC_MIXED, 3, 256)])
+    (p, v, w) for p in (mfs.PREC_MIXED, mfs.PREC_F64) for v, w in ((1, 64), (1, 256), (2, 64), (2, 192))])
 def test_kernel_variants_and_shapes_match_oracle(ctx, small, precision, variant, wg):
     """Every kernel variant (rows carried per thread) at small and large
     workgroups, where rows per thread exceed or fall short of the variant's
@@ -57,7 +56,7 @@ def test_kernel_variants_and_shapes_match_oracle(ctx, small, precision, variant,
 def test_autotuner_shape_is_used_and_consistent(ctx, small):
     p, B = small
     wg, per_cu, var, ms = B.autotune(precision=mfs.PREC_MIXED, tol=1e-10)
-    assert wg in (64, 128, 192, 256) and per_cu >= 1 and var in (0, 1, 2) and ms > 0
+    assert wg in (64, 128, 192, 256) and per_cu >= 1 and var in (0, 1) and ms > 0
     B.solve(precision=mfs.PREC_MIXED, tol=1e-10)
     tuned = np.concatenate([B.basis(c).ravel() for c in range(B.ncells)])
     B.solve(precision=mfs.PREC_F64, tol=1e-10, variant=1, wg_size=256)

@@ -47,6 +47,14 @@ enum { MFS_PROBLEM_SCALAR = 0, MFS_PROBLEM_ELASTIC = 1 };
  * textbook block; identical rows for nodes off the Neumann/free boundary).
  * TEXTBOOK: block(I,J) = V B_I^T D B_J. */
 enum { MFS_ELEM_REFERENCE = 0, MFS_ELEM_TEXTBOOK = 1 };
+/* MsFEM boundary data of the local problems: the trilinear coarse hat times
+ * e_d, or the reference's edge -> face -> volume hierarchy
+ * (MsFEM/MsFEM_Grid.h:5026-5506) driven by e_d. */
+enum { MFS_TRACE_Q1 = 0, MFS_TRACE_REFERENCE = 1 };
+/* MsFEM coarse functions: the 24 bases phi_{c,d} themselves, or the
+ * reference's component-wise use of one vector basis per coarse node,
+ * Psi_{c,a} = e_a (sum_d phi_{c,d})_a (MsFEM/MsFEM_Grid.h:3872-3903). */
+enum { MFS_BASIS_PER_DIRECTION = 0, MFS_BASIS_COMPONENTWISE = 1 };
 
 typedef struct mfs_ctx_s* mfs_ctx;
 typedef struct mfs_vec_s* mfs_vec;
@@ -264,6 +272,10 @@ typedef struct {
   const double* E;           /* per fine tet, cells concatenated in order     */
   double nu;
   int convention;            /* MFS_ELEM_*                                    */
+  int trace;                 /* MFS_TRACE_* (0: Q1 hats)                      */
+  int coarse_basis;          /* MFS_BASIS_* (0: per direction)                */
+  double trace_tol;          /* reference traces: face solves to ||r||/||b||  */
+                             /* (0: 1e-13)                                    */
 } mfs_batch_desc;
 
 typedef struct {
@@ -294,8 +306,18 @@ int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* desc, mfs_batch* out);
 int mfs_batch_destroy(mfs_batch B);
 /* Sizes: interior DOFs of cell c, total fine DOFs of cell c. */
 int mfs_batch_cell_dofs(mfs_batch B, int cell, int64_t* n_interior, int64_t* n_total);
-/* Assemble every cell matrix on the GPU (batched element + gather). */
+/* Assemble every cell matrix on the GPU (batched element + gather).  In
+ * the reference-trace mode this first runs the batched 1-D edge solves
+ * (MsFEM_Grid.h:5026-5175 via FEM/FEM.h:3796-4365) and 2-D face solves
+ * (MsFEM_Grid.h:5176-5325 via FEM/FEM.h:4367-5178) that give the boundary
+ * data of the volume problems. */
 int mfs_batch_assemble(mfs_batch B);
+/* Reference-trace mode, after mfs_batch_assemble: largest iteration count
+ * of the face solves, the number of face problems (of 12 per face) that
+ * missed trace_tol, and the device time of the edge + face launches (ms).
+ * No reference counterpart (the reference prints a warning per solve,
+ * FEM/FEM.h:4343-4351).  Any pointer may be NULL. */
+int mfs_batch_trace_status(mfs_batch B, int* max_face_iterations, int* n_face_failed, double* trace_ms);
 /* Solve the 24 local problems of every cell in one persistent launch. */
 int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* desc, mfs_batch_report* report);
 /* Initial guesses for the interior DOFs of every (cell, basis), FP32 device
@@ -332,6 +354,12 @@ int mfs_msfem_top_traction(mfs_batch B, const double* t, double* load);
 /* Fine displacement of one cell, u = sum Phi u_coarse (MechanicalMain.cpp:
  * 2176-2181), full fine vector 3*(n+1)^3. */
 int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u_fine);
+/* MsFEM-vs-FEM error of MechanicalMain.cpp:2112-2221: per component a,
+ * sqrt(sum_tets V (u_ms - u_fem)_a^2 (centroid)) / sqrt(sum_tets V
+ * u_fem,a^2) (denominator 1 when below 1e-10).  All cells share one n;
+ * u_fem is the fine FEM solution on the whole box's fine grid
+ * ((ncx*n+1) x (ncy*n+1) x (ncz*n+1) nodes, x fastest, 3 per node). */
+int mfs_msfem_error(mfs_batch B, const double* u_coarse, const double* u_fem, double* err3);
 /* Autotuner for mfs_batch_solve in desc->precision: times the whole batched
  * solve for every kernel variant (rows carried per thread), workgroup size
  * (64..256) and workgroups per CU (as many as the kernel's registers and LDS

@@ -30,6 +30,8 @@ PCG_JACOBI, CG, GMRES, MINRES = 0, 1, 2, 3
 PREC_F64, PREC_MIXED = 0, 1
 PROBLEM_SCALAR, PROBLEM_ELASTIC = 0, 1
 ELEM_REFERENCE, ELEM_TEXTBOOK = 0, 1
+TRACE_Q1, TRACE_REFERENCE = 0, 1
+BASIS_PER_DIRECTION, BASIS_COMPONENTWISE = 0, 1
 
 c_i64p = C.POINTER(C.c_int64)
 c_i32p = C.POINTER(C.c_int32)
@@ -69,7 +71,8 @@ class FemDesc(C.Structure):
 
 class BatchDesc(C.Structure):
     _fields_ = [("n_cells", C.c_int), ("cell_n", c_i32p), ("cell_origin", c_dp), ("cell_h", C.c_double),
-                ("E", c_dp), ("nu", C.c_double), ("convention", C.c_int)]
+                ("E", c_dp), ("nu", C.c_double), ("convention", C.c_int), ("trace", C.c_int),
+                ("coarse_basis", C.c_int), ("trace_tol", C.c_double)]
 
 
 class BatchSolveDesc(C.Structure):
@@ -202,6 +205,8 @@ _sig("mfs_msfem_coarse_system", C.c_int, [vp, c_dp, c_dp, C.c_int64, c_i32p, c_d
                                           C.POINTER(vp)])
 _sig("mfs_msfem_top_traction", C.c_int, [vp, c_dp, c_dp])
 _sig("mfs_msfem_downscale", C.c_int, [vp, c_dp, C.c_int, c_dp])
+_sig("mfs_msfem_error", C.c_int, [vp, c_dp, c_dp, c_dp])
+_sig("mfs_batch_trace_status", C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)])
 
 
 def ptr(a, ctype):
@@ -596,7 +601,8 @@ def batch_solve_desc(precision=PREC_MIXED, tol=1e-8, maxit=20000, inner_tol=1e-4
 class Batch:
     """All local basis problems of a set of coarse cells (mfs_batch_*)."""
 
-    def __init__(self, ctx, cell_n, cell_origin, cell_h, E, nu=0.3, convention=ELEM_REFERENCE):
+    def __init__(self, ctx, cell_n, cell_origin, cell_h, E, nu=0.3, convention=ELEM_REFERENCE, trace=TRACE_Q1,
+                 coarse_basis=BASIS_PER_DIRECTION, trace_tol=0.0):
         self.ctx = ctx
         self.cell_n = i32(cell_n)
         self.cell_origin = f64(cell_origin).reshape(-1, 3)
@@ -607,6 +613,7 @@ class Batch:
         d.cell_n = ptr(self.cell_n, C.c_int32)
         d.cell_origin = ptr(self.cell_origin, C.c_double)
         d.cell_h, d.E, d.nu, d.convention = cell_h, ptr(E, C.c_double), nu, convention
+        d.trace, d.coarse_basis, d.trace_tol = trace, coarse_basis, trace_tol
         h = vp()
         _chk(lib.mfs_batch_create(ctx.h, C.byref(d), C.byref(h)), "mfs_batch_create")
         self.h = h
@@ -683,6 +690,24 @@ class Batch:
         out = np.empty(nt)
         _chk(lib.mfs_msfem_downscale(self.h, ptr(u_coarse, C.c_double), cell, ptr(out, C.c_double)), "downscale")
         return out
+
+    def msfem_error(self, u_coarse, u_fem):
+        """Per-component relative L2 error of the downscaled MsFEM field
+        against a fine FEM field on the whole box's fine grid."""
+        u_coarse = f64(u_coarse).ravel()
+        u_fem = f64(u_fem).ravel()
+        out = np.empty(3)
+        _chk(lib.mfs_msfem_error(self.h, ptr(u_coarse, C.c_double), ptr(u_fem, C.c_double), ptr(out, C.c_double)),
+             "mfs_msfem_error")
+        return out
+
+    def trace_status(self):
+        """Reference-trace mode: (largest face-solve iteration count, face
+        problems that missed the trace tolerance, device ms of the edge + face
+        launches in the last assemble)."""
+        it, bad, ms = C.c_int(), C.c_int(), C.c_double()
+        _chk(lib.mfs_batch_trace_status(self.h, C.byref(it), C.byref(bad), C.byref(ms)), "mfs_batch_trace_status")
+        return it.value, bad.value, ms.value
 
     def status(self):
         it = np.empty((self.ncells, 24), dtype=np.int32)

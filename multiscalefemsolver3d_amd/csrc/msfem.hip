@@ -18,6 +18,22 @@
 //  * one task = (cell, group of 8 bases); a persistent kernel pulls tasks
 //    from an atomic queue (largest cells first) and runs FP32 Jacobi-PCG with
 //    FP64 refinement for its 8 right-hand sides, search directions in LDS.
+//
+// Reference-trace mode (MFS_TRACE_REFERENCE) replaces the boundary data
+// hat_c * e_d by the reference's edge -> face hierarchy (MsFEM_Grid.h:
+// 5026-5506): per coarse edge a 1-D elastic solve with the coarse-node
+// value at one end and 0 at the other, per coarse face and face corner a
+// 2-D elastic solve with the two edge solutions that are 1 at that corner,
+// then the volume solve with the face solutions of the three faces meeting
+// at the node.  Basis 3c+d is driven with e_d; the reference drives (1,1,1)
+// and its basis of node c is the sum over d (each step is linear).
+//  * edge profiles ET[cell.et0 + (edge*(n+1) + t)*2 + end]: the 1-D chain
+//    solution is e_d times a scalar profile (axis-aligned segments give
+//    diagonal 3x3 blocks, so the components decouple);
+//  * face traces FT[cell.ft0 + ((face*(n+1)^2 + u + (n+1)v)*3 + a)*12 +
+//    3*lc + d] for the 4 face corners lc and 3 drives d, interior face nodes
+//    from a batched FP64 Jacobi-PCG (one workgroup per face, 12 right-hand
+//    sides), face-boundary nodes copied from the edge profiles.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,7 +82,52 @@ struct Cell {
   int64_t int0; // first interior node (global numbering over cells)
   int tmpl;     // template id
   double ox, oy, oz;
+  int64_t et0;  // reference-trace mode: first edge-profile entry (12*(n+1)*2 per cell)
+  int64_t ft0;  // reference-trace mode: first face-trace entry (6*(n+1)^2*36 per cell)
 };
+
+// Coarse faces f = 2*ax + side (side 1: coordinate n); face coordinates
+// (u, v) run along the two other axes in increasing order.  Coarse edges
+// e = 4*dir + b1 + 2*b2, b1 / b2 the sides on the lower / higher other axis.
+__host__ __device__ __forceinline__ void face_axes(int ax, int& A1, int& A2) {
+  A1 = ax == 0 ? 1 : 0;
+  A2 = ax == 2 ? 1 : 2;
+}
+__host__ __device__ __forceinline__ int edge_id(int dir, int axa, int sa, int axb, int sb) {
+  // sides sa on axis axa and sb on axis axb (the two axes other than dir)
+  return axa < axb ? 4 * dir + sa + 2 * sb : 4 * dir + sb + 2 * sa;
+}
+constexpr int FT_NODE = 36;  // face-trace values per face node: [a][lc][d]
+
+// Boundary value (3 components) of basis q = 3c + d at boundary node
+// (i, j, k) of an n-cell from its face traces: any face holding both the
+// node and corner c (they agree on shared edges), 0 when none does.
+__host__ __device__ __forceinline__ void face_trace_at(const double* ft_cell, int n, int i, int j, int k, int q,
+                                                       double (&g)[3]) {
+  const int c = q / 3, d = q % 3;
+  const int idx[3] = {i, j, k};
+  g[0] = g[1] = g[2] = 0.0;
+  for (int ax = 0; ax < 3; ax++) {
+    const int side = idx[ax] == 0 ? 0 : (idx[ax] == n ? 1 : -1);
+    if (side < 0 || ((c >> ax) & 1) != side) continue;
+    int A1, A2;
+    face_axes(ax, A1, A2);
+    const int lc = ((c >> A1) & 1) + 2 * ((c >> A2) & 1);
+    const int64_t w = idx[A1] + (int64_t)(n + 1) * idx[A2];
+    const double* base = ft_cell + (((int64_t)(2 * ax + side) * (n + 1) * (n + 1) + w) * FT_NODE) + 3 * lc + d;
+    for (int a = 0; a < 3; a++) g[a] = base[a * 12];
+    return;
+  }
+}
+
+// Q1 boundary value hat_c(x) e_d.
+__host__ __device__ __forceinline__ void hat_trace_at(int n, int i, int j, int k, int q, double (&g)[3]) {
+  const int c = q / 3, d = q % 3;
+  const double X = (double)i / n, Y = (double)j / n, Z = (double)k / n;
+  const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
+  g[0] = g[1] = g[2] = 0.0;
+  g[d] = hat;
+}
 
 struct Tmpl {  // per distinct n: gather lists of interior rows x stencil
   int n;
@@ -167,7 +228,8 @@ __global__ __launch_bounds__(256) void k_cell_assemble(const Cell* __restrict__ 
                                                        const int64_t* __restrict__ g_ptr,
                                                        const uint32_t* __restrict__ g_ent,
                                                        const double* __restrict__ E, double nu, double h, int conv,
-                                                       int64_t total_int, float* __restrict__ K32,
+                                                       const double* __restrict__ ft, int64_t total_int,
+                                                       float* __restrict__ K32,
                                                        double* __restrict__ K64, float* __restrict__ dinv32,
                                                        double* __restrict__ dinv64, double* __restrict__ b64) {
   for (int64_t gv = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gv < total_int;
@@ -213,13 +275,23 @@ __global__ __launch_bounds__(256) void k_cell_assemble(const Cell* __restrict__ 
           K32c[at] = interior ? (float)Kb[a][b] : 0.0f;
         }
       if (!interior) {
-        // g for basis (c, d) at this boundary node: hat_c(x) e_d
-        const double X = (double)ni / n, Y = (double)nj / n, Z = (double)nk / n;
-        for (int c = 0; c < 8; c++) {
-          const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
-          if (hat == 0.0) continue;
-          for (int d = 0; d < 3; d++)
-            for (int a = 0; a < 3; a++) bacc[a][3 * c + d] -= Kb[a][d] * hat;
+        if (ft == nullptr) {
+          // g for basis (c, d) at this boundary node: hat_c(x) e_d
+          const double X = (double)ni / n, Y = (double)nj / n, Z = (double)nk / n;
+          for (int c = 0; c < 8; c++) {
+            const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
+            if (hat == 0.0) continue;
+            for (int d = 0; d < 3; d++)
+              for (int a = 0; a < 3; a++) bacc[a][3 * c + d] -= Kb[a][d] * hat;
+          }
+        } else {
+          // g from the face traces (reference-trace mode): full vectors
+#pragma unroll
+          for (int q = 0; q < NB; q++) {
+            double g[3];
+            face_trace_at(ft + C.ft0, n, ni, nj, nk, q, g);
+            for (int a = 0; a < 3; a++) bacc[a][q] -= Kb[a][0] * g[0] + Kb[a][1] * g[1] + Kb[a][2] * g[2];
+          }
         }
       }
       if (s == S_CENTER)
@@ -844,8 +916,16 @@ __global__ void k_batch_reset(unsigned* queue, int* flags, int nflags) {
 
 // Coarse Galerkin matrices Kc = Phi^T K Phi (24x24 per cell), FP64, by an
 // element loop: one workgroup per cell, tets processed one at a time.
+// form MFS_BASIS_COMPONENTWISE: the coarse functions are Psi_{c,a} =
+// e_a (sum_d phi_{c,d})_a (the reference's use of one vector basis per
+// coarse node, component by component) and the coarse element is the
+// reference's block(I,J) = int B_J^T D B_I of their gradients
+// (MsFEM_Grid.h:3837-3975), integrated exactly over the fine tets: the
+// textbook Galerkin product, blockwise transposed in the reference
+// convention.
 __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells, const double* __restrict__ E,
                                                   double nu, double h, int conv, const double* __restrict__ x64,
+                                                  const double* __restrict__ ft, int form,
                                                   double* __restrict__ Kc) {
   __shared__ double phi[12][NB];   // element values of the 24 bases
   __shared__ double Ke[12][12];
@@ -870,12 +950,23 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
       if (interior) {
         val = x64[(C.int0 + interior_index(i, j, k, m)) * 3 * NB + a * NB + bq];
       } else {
-        const int c = bq / 3, d = bq % 3;
-        const double X = (double)i / n, Y = (double)j / n, Z = (double)k / n;
-        const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
-        val = (a == d) ? hat : 0.0;
+        double g[3];
+        if (ft)
+          face_trace_at(ft + C.ft0, n, i, j, k, bq, g);
+        else
+          hat_trace_at(n, i, j, k, bq, g);
+        val = g[a];
       }
       phi[p][bq] = val;
+    }
+    if (form == MFS_BASIS_COMPONENTWISE) {
+      __syncthreads();
+      double sum = 0.0;
+      const int p = tid / 8, c = tid % 8;  // 12 x 8 (element dof, coarse node)
+      if (tid < 96) sum = phi[p][3 * c] + phi[p][3 * c + 1] + phi[p][3 * c + 2];
+      __syncthreads();
+      if (tid < 96)
+        for (int a2 = 0; a2 < 3; a2++) phi[p][3 * c + a2] = (p % 3 == a2) ? sum : 0.0;
     }
     if (tid < 16) {
       double g[4][3], V;
@@ -883,7 +974,8 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
       const int I = tid / 4, J = tid % 4;
       const double Et = E[C.tet0 + t];
       double Kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      elastic_block(g[I], g[J], V * Et * lamf, V * Et * muf, conv, Kb);
+      elastic_block(g[I], g[J], V * Et * lamf, V * Et * muf,
+                    form == MFS_BASIS_COMPONENTWISE ? (int)MFS_ELEM_TEXTBOOK : conv, Kb);
       for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) Ke[3 * I + a][3 * J + b] = Kb[a][b];
     }
@@ -905,9 +997,325 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
     }
     __syncthreads();
   }
+  const bool bt = form == MFS_BASIS_COMPONENTWISE && conv == MFS_ELEM_REFERENCE;
   for (int w = 0; w < 3; w++) {
     const int pr = tid + 256 * w;
-    if (pr < NB * NB) Kc[(int64_t)cell * NB * NB + pr] = acc[w];
+    if (pr >= NB * NB) continue;
+    const int bi = pr / NB, bj = pr % NB;
+    const int out = bt ? (3 * (bi / 3) + bj % 3) * NB + 3 * (bj / 3) + bi % 3 : pr;
+    Kc[(int64_t)cell * NB * NB + out] = acc[w];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reference-trace mode, step 1: coarse-edge profiles.
+// The tet owning the boundary triangle of square (a, b) of face (ax, side):
+// the lower triangle steps A1 then A2 from the square's low corner, the
+// upper one A2 then A1; the tet steps the face axis last (side 0) or first
+// (side 1).  Perm index in the order xyz,xzy,yxz,yzx,zxy,zyx = 2*o0 + (o1>o2).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int face_tri_tet(int n, int ax, int side, int a, int b, int upper) {
+  int A1, A2;
+  face_axes(ax, A1, A2);
+  int hx[3];
+  hx[ax] = side ? n - 1 : 0;
+  hx[A1] = a;
+  hx[A2] = b;
+  const int s1 = upper ? A2 : A1, s2 = upper ? A1 : A2;
+  const int o0 = side ? ax : s1, o1 = side ? s1 : s2, o2 = side ? s2 : ax;
+  return 6 * (hx[0] + n * (hx[1] + n * hx[2])) + 2 * o0 + (o1 > o2 ? 1 : 0);
+}
+
+// One thread per (cell, edge).  A segment's modulus is that of the boundary
+// triangle holding it in the first face (lower face axis) that contains the
+// edge (the reference reads it from the first neighbouring face,
+// MsFEM_Grid.h:4880-4920).  The chain of equal segments with diagonal blocks
+// (E_s/L)(mu + (lam+mu) t t^T) is solved exactly per component: the profile
+// from end 0 is 1 - C(t)/C(n) with C(t) = sum_{s<t} 1/E_s (FEM_1D with
+// Dirichlet 1 / 0 ends, FEM.h:3796-4365), the same for every component.
+__global__ void k_edge_profiles(const Cell* __restrict__ cells, int ncells, const double* __restrict__ E,
+                                double* __restrict__ et) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncells * 12) return;
+  const int cell = g / 12, e = g % 12;
+  const Cell C = cells[cell];
+  const int n = C.n;
+  const int dir = e / 4, b1 = e & 1, b2 = (e >> 1) & 1;
+  const int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;  // other axes, increasing
+  // first face: axis o1, side b1; the edge's fixed coordinate on o2 is b2*n
+  int A1, A2;
+  face_axes(o1, A1, A2);
+  double* out = et + C.et0 + (int64_t)e * (n + 1) * 2;
+  double acc = 0.0;
+  out[0] = 0.0;  // C(0)
+  for (int t = 0; t < n; t++) {
+    int tet;
+    if (dir == A1)  // edge along u at v = b2*n
+      tet = b2 ? face_tri_tet(n, o1, b1, t, n - 1, 1) : face_tri_tet(n, o1, b1, t, 0, 0);
+    else  // edge along v at u = b2*n
+      tet = b2 ? face_tri_tet(n, o1, b1, n - 1, t, 0) : face_tri_tet(n, o1, b1, 0, t, 1);
+    acc += 1.0 / E[C.tet0 + tet];
+    out[2 * (t + 1)] = acc;
+  }
+  for (int t = 0; t <= n; t++) {
+    const double ct = out[2 * t];
+    out[2 * t] = (acc - ct) / acc;  // end 0 carries the coarse-node value
+    out[2 * t + 1] = ct / acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reference-trace mode, step 2: face solves.  One workgroup of 192 threads
+// per (cell, face): the 2-D elastic problem on the face's (n-1)^2 interior
+// nodes (FEM_2D_forElasticDeformation, FEM.h:4367-5178) for 12 right-hand
+// sides (4 face corners x 3 drives) with FP64 Jacobi-PCG.  Element: per
+// boundary triangle area * block(gI, gJ) with the in-plane hat gradients
+// (FEM_Grid.h:4429-4500) and the owning tet's modulus.  Structured 7-point
+// face stencil, blocks in LDS; 16 lanes per right-hand side, so every
+// reduction is a 16-lane shuffle.
+// ---------------------------------------------------------------------------
+constexpr int FS = 7;       // face stencil points
+constexpr int F_CENTER = 3;
+constexpr int FRHS = 12;    // face bases: 4 corners x 3 drives
+constexpr int FSEG = 16;    // lanes per face basis
+constexpr int FJ = 8;       // interior face nodes per lane: (MAXN-1)^2 <= FSEG*FJ
+constexpr int F_THREADS = FRHS * FSEG;
+static_assert((MAXN - 1) * (MAXN - 1) <= FSEG * FJ, "face nodes per lane");
+__device__ constexpr int kFace[FS][2] = {{-1, -1}, {0, -1}, {-1, 0}, {0, 0}, {1, 0}, {0, 1}, {1, 1}};
+
+__device__ __forceinline__ int face_slot(int du, int dv) {
+  switch ((du + 1) + 3 * (dv + 1)) {
+    case 0: return 0;
+    case 1: return 1;
+    case 3: return 2;
+    case 4: return 3;
+    case 5: return 4;
+    case 7: return 5;
+    default: return 6;  // (1, 1)
+  }
+}
+
+__device__ __forceinline__ double seg_sum(double v) {
+  v += __shfl_xor(v, 8, FSEG);
+  v += __shfl_xor(v, 4, FSEG);
+  v += __shfl_xor(v, 2, FSEG);
+  v += __shfl_xor(v, 1, FSEG);
+  return v;
+}
+
+size_t face_lds_bytes(int max_n) {
+  const int nf = (max_n - 1) * (max_n - 1);
+  return (size_t)nf * (FS * 9 + 3 * FRHS + 3) * sizeof(double) + FRHS * sizeof(int);
+}
+
+__global__ __launch_bounds__(F_THREADS) void k_face_traces(const Cell* __restrict__ cells,
+                                                           const int* __restrict__ order,
+                                                           const double* __restrict__ E, double nu, double h,
+                                                           int conv, const double* __restrict__ et,
+                                                           double* __restrict__ ft, double tol2, int* stats) {
+  extern __shared__ double lds[];
+  const int cell = order[blockIdx.x / 6], f = blockIdx.x % 6;
+  const Cell C = cells[cell];
+  const int n = C.n, m = n - 1, nf = m * m, s = n + 1;
+  const int ax = f >> 1, side = f & 1;
+  int A1, A2;
+  face_axes(ax, A1, A2);
+  double* K = lds;                      // [nf][FS][9]
+  double* P = K + (size_t)nf * FS * 9;  // [nf][3][FRHS]
+  double* dinv = P + (size_t)nf * 3 * FRHS;
+  int* flags = (int*)(dinv + 3 * nf);
+  double* FT = ft + C.ft0 + (int64_t)f * s * s * FT_NODE;
+  const double* ET = et + C.et0;
+  const double hf = h / n;
+  const double lamf = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), muf = 1.0 / (2.0 * (1.0 + nu));
+  const int tid = threadIdx.x;
+
+  // (a) face-boundary nodes: edge profiles times e_d
+  for (int w = tid; w < s * s; w += blockDim.x) {
+    const int u = w % s, v = w / s;
+    if (u > 0 && u < n && v > 0 && v < n) continue;
+    double val[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool cu = u == 0 || u == n, cv = v == 0 || v == n;
+    if (cu && cv) {
+      val[(u == n) + 2 * (v == n)] = 1.0;
+    } else if (cv) {  // edge along A1 at v = 0 or n
+      const int e = edge_id(A1, ax, side, A2, v == n);
+      const int hi = 2 * (v == n);
+      val[hi] = ET[((int64_t)e * s + u) * 2];
+      val[hi + 1] = ET[((int64_t)e * s + u) * 2 + 1];
+    } else {  // edge along A2 at u = 0 or n
+      const int e = edge_id(A2, ax, side, A1, u == n);
+      const int lo = (u == n);
+      val[lo] = ET[((int64_t)e * s + v) * 2];
+      val[lo + 2] = ET[((int64_t)e * s + v) * 2 + 1];
+    }
+    double* o = FT + (int64_t)w * FT_NODE;
+    for (int a = 0; a < 3; a++)
+      for (int lc = 0; lc < 4; lc++)
+        for (int d = 0; d < 3; d++) o[a * 12 + 3 * lc + d] = a == d ? val[lc] : 0.0;
+  }
+  // (b) stencil blocks of the interior face nodes from their 6 triangles:
+  // squares (0,0) lower/upper, (-1,-1) lower/upper, (-1,0) lower, (0,-1) upper
+  // (offsets of the square's low corner, centre's vertex index in the triangle)
+  for (int w = tid; w < nf; w += blockDim.x) {
+    const int u = w % m + 1, v = w / m + 1;
+    double* Kw = K + (size_t)w * FS * 9;  // this thread's node only: accumulate in place
+    for (int q = 0; q < FS * 9; q++) Kw[q] = 0.0;
+    const int sq[6][3] = {{0, 0, 0}, {0, 0, 1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, 0}, {0, -1, 1}};
+    for (int tr = 0; tr < 6; tr++) {
+      const int a0 = u + sq[tr][0], b0 = v + sq[tr][1], up = sq[tr][2];
+      // vertices in face coordinates: lower (a,b),(a+1,b),(a+1,b+1); upper (a,b),(a,b+1),(a+1,b+1)
+      const int vu[3] = {a0, up ? a0 : a0 + 1, a0 + 1};
+      const int vv[3] = {b0, up ? b0 + 1 : b0, b0 + 1};
+      double X[3][3];
+      int ic = 0;
+      for (int k = 0; k < 3; k++) {
+        X[k][ax] = hf * (side ? n : 0);
+        X[k][A1] = hf * vu[k];
+        X[k][A2] = hf * vv[k];
+        if (vu[k] == u && vv[k] == v) ic = k;
+      }
+      // in-plane hat gradients: g_i = N x (X_{i+2} - X_{i+1}) / |N|^2
+      double e1[3], e2[3], N[3];
+      for (int a = 0; a < 3; a++) {
+        e1[a] = X[1][a] - X[0][a];
+        e2[a] = X[2][a] - X[0][a];
+      }
+      N[0] = e1[1] * e2[2] - e1[2] * e2[1];
+      N[1] = e1[2] * e2[0] - e1[0] * e2[2];
+      N[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      const double nn2 = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
+      double gr[3][3];
+      for (int k = 0; k < 3; k++) {
+        const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+        double ed[3];
+        for (int a = 0; a < 3; a++) ed[a] = X[k2][a] - X[k1][a];
+        gr[k][0] = (N[1] * ed[2] - N[2] * ed[1]) / nn2;
+        gr[k][1] = (N[2] * ed[0] - N[0] * ed[2]) / nn2;
+        gr[k][2] = (N[0] * ed[1] - N[1] * ed[0]) / nn2;
+      }
+      const double area = 0.5 * sqrt(nn2);
+      const double Et = E[C.tet0 + face_tri_tet(n, ax, side, a0, b0, up)];
+      for (int k = 0; k < 3; k++) {
+        double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        elastic_block(gr[ic], gr[k], area * Et * lamf, area * Et * muf, conv, blk);
+        const int q = face_slot(vu[k] - u, vv[k] - v);
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) Kw[q * 9 + 3 * a + b] += blk[a][b];
+      }
+    }
+    for (int a = 0; a < 3; a++) dinv[3 * w + a] = 1.0 / Kw[F_CENTER * 9 + 4 * a];
+  }
+  __syncthreads();  // K, dinv in LDS; boundary traces written (visible to the workgroup)
+
+  // (c) Jacobi-PCG, right-hand side r = tid / FSEG, nodes l + FSEG*j
+  const int r = tid / FSEG, l = tid % FSEG;
+  double x[FJ][3], res[FJ][3];
+  double bb = 0.0, rz = 0.0;
+#pragma unroll
+  for (int j = 0; j < FJ; j++) {
+    const int w = l + FSEG * j;
+    for (int a = 0; a < 3; a++) x[j][a] = res[j][a] = 0.0;
+    if (w >= nf) continue;
+    const int u = w % m + 1, v = w / m + 1;
+    for (int q = 0; q < FS; q++) {
+      const int nu_ = u + kFace[q][0], nv = v + kFace[q][1];
+      if (nu_ > 0 && nu_ < n && nv > 0 && nv < n) continue;
+      const double* g = FT + ((int64_t)nu_ + s * nv) * FT_NODE + r;
+      const double* kq = K + ((size_t)w * FS + q) * 9;
+      for (int a = 0; a < 3; a++) res[j][a] -= kq[3 * a] * g[0] + kq[3 * a + 1] * g[12] + kq[3 * a + 2] * g[24];
+    }
+    for (int a = 0; a < 3; a++) {
+      const double z = dinv[3 * w + a] * res[j][a];
+      P[((size_t)w * 3 + a) * FRHS + r] = z;
+      bb += res[j][a] * res[j][a];
+      rz += res[j][a] * z;
+    }
+  }
+  bb = seg_sum(bb);
+  rz = seg_sum(rz);
+  const int cap = 4 * 3 * nf + 100;
+  bool done = !(bb > 0.0);
+  int it = 0;
+  bool ok = true;
+  if (l == 0) flags[r] = done;
+  __syncthreads();
+  for (;;) {
+    bool all = true;
+    for (int q = 0; q < FRHS; q++) all = all && flags[q];
+    if (all) break;
+    double qv[FJ][3];
+    double pq = 0.0;
+#pragma unroll
+    for (int j = 0; j < FJ; j++) {
+      const int w = l + FSEG * j;
+      qv[j][0] = qv[j][1] = qv[j][2] = 0.0;
+      if (w >= nf) continue;
+      const int u = w % m + 1, v = w / m + 1;
+      for (int q = 0; q < FS; q++) {
+        const int nu_ = u + kFace[q][0], nv = v + kFace[q][1];
+        if (nu_ <= 0 || nu_ >= n || nv <= 0 || nv >= n) continue;
+        const int wn = (nu_ - 1) + m * (nv - 1);
+        const double p0 = P[((size_t)wn * 3 + 0) * FRHS + r], p1 = P[((size_t)wn * 3 + 1) * FRHS + r],
+                     p2 = P[((size_t)wn * 3 + 2) * FRHS + r];
+        const double* kq = K + ((size_t)w * FS + q) * 9;
+        for (int a = 0; a < 3; a++) qv[j][a] += kq[3 * a] * p0 + kq[3 * a + 1] * p1 + kq[3 * a + 2] * p2;
+      }
+      for (int a = 0; a < 3; a++) pq += P[((size_t)w * 3 + a) * FRHS + r] * qv[j][a];
+    }
+    pq = seg_sum(pq);
+    const double alpha = done ? 0.0 : rz / pq;
+    double rzn = 0.0, rr = 0.0;
+#pragma unroll
+    for (int j = 0; j < FJ; j++) {
+      const int w = l + FSEG * j;
+      if (w >= nf) continue;
+      for (int a = 0; a < 3; a++) {
+        x[j][a] += alpha * P[((size_t)w * 3 + a) * FRHS + r];
+        res[j][a] -= alpha * qv[j][a];
+        qv[j][a] = dinv[3 * w + a] * res[j][a];  // z
+        rzn += res[j][a] * qv[j][a];
+        rr += res[j][a] * res[j][a];
+      }
+    }
+    rzn = seg_sum(rzn);
+    rr = seg_sum(rr);
+    const double beta = rzn / rz;
+    __syncthreads();  // every read of P done
+    if (!done) {
+#pragma unroll
+      for (int j = 0; j < FJ; j++) {
+        const int w = l + FSEG * j;
+        if (w >= nf) continue;
+        for (int a = 0; a < 3; a++) {
+          double* pp = P + ((size_t)w * 3 + a) * FRHS + r;
+          *pp = qv[j][a] + beta * *pp;
+        }
+      }
+      rz = rzn;
+      it++;
+      const bool conv_ = rr <= tol2 * bb;
+      const bool bad = !(rr == rr) || !(pq == pq);
+      if (conv_ || bad || it >= cap) {
+        done = true;
+        ok = conv_ && !bad;
+      }
+    }
+    if (l == 0) flags[r] = done;
+    __syncthreads();
+  }
+  // (d) interior face values
+#pragma unroll
+  for (int j = 0; j < FJ; j++) {
+    const int w = l + FSEG * j;
+    if (w >= nf) continue;
+    const int u = w % m + 1, v = w / m + 1;
+    double* o = FT + ((int64_t)u + s * v) * FT_NODE + r;
+    for (int a = 0; a < 3; a++) o[12 * a] = x[j][a];
+  }
+  if (l == 0) {
+    atomicMax(stats, it);
+    if (!ok) atomicAdd(stats + 1, 1);
   }
 }
 
@@ -929,6 +1337,11 @@ struct mfs_batch_s {
   bool assembled = false;
   int tuned_wg[2] = {0, 0}, tuned_per_cu[2] = {0, 0};  // per precision (0 mixed, 1 FP64)
   int tuned_var[2] = {-1, -1};                        // kernel variant (rows per thread)
+  int trace = MFS_TRACE_Q1, form = MFS_BASIS_PER_DIRECTION;
+  double trace_tol = 1e-13;
+  int64_t et_total = 0, ft_total = 0;
+  int face_stats[2] = {0, 0};  // max face iterations, failed face problems
+  double trace_ms = 0.0;       // device time of the edge + face launches
   // device
   Cell* d_cells = nullptr;
   int64_t* d_int0 = nullptr;
@@ -950,6 +1363,10 @@ struct mfs_batch_s {
   unsigned* d_queue = nullptr;
   void* d_scr = nullptr;
   size_t scr_bytes = 0;
+  double* d_et = nullptr;  // reference traces: edge profiles
+  double* d_ft = nullptr;  // reference traces: face traces
+  int* d_face_order = nullptr;
+  int* d_fstats = nullptr;
 };
 
 static int batch_free(mfs_batch B) {
@@ -957,7 +1374,8 @@ static int batch_free(mfs_batch B) {
   if (B->ctx) (void)hipStreamSynchronize(B->ctx->stream);
   void* ptrs[] = {B->d_cells, B->d_int0, B->d_tmpls, B->d_gptr, B->d_gent, B->d_E, B->d_K32, B->d_K64,
                   B->d_dinv32, B->d_dinv64, B->d_b64, B->d_x64, B->d_iters, B->d_relres, B->d_flags,
-                  B->d_task_cell, B->d_task_group, B->d_queue, B->d_scr};
+                  B->d_task_cell, B->d_task_group, B->d_queue, B->d_scr, B->d_et, B->d_ft, B->d_face_order,
+                  B->d_fstats};
   for (void* p : ptrs) dfree(p);
   mfs_ctx c = B->ctx;
   delete B;
@@ -973,6 +1391,10 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   MFS_CHECK_ARG(d->cell_h > 0 && d->nu > -1.0 && d->nu < 0.5, "mfs_batch_create: bad cell size or nu");
   MFS_CHECK_ARG(d->convention == MFS_ELEM_REFERENCE || d->convention == MFS_ELEM_TEXTBOOK,
                 "mfs_batch_create: unknown element convention");
+  MFS_CHECK_ARG(d->trace == MFS_TRACE_Q1 || d->trace == MFS_TRACE_REFERENCE, "mfs_batch_create: unknown trace mode");
+  MFS_CHECK_ARG(d->coarse_basis == MFS_BASIS_PER_DIRECTION || d->coarse_basis == MFS_BASIS_COMPONENTWISE,
+                "mfs_batch_create: unknown coarse basis form");
+  MFS_CHECK_ARG(d->trace_tol >= 0 && d->trace_tol < 1, "mfs_batch_create: trace_tol outside [0, 1)");
   for (int c = 0; c < d->n_cells; c++)
     MFS_CHECK_ARG(d->cell_n[c] >= 2 && d->cell_n[c] <= MAXN, "mfs_batch_create: cell subdivision outside 2..12");
   MFS_HIP(hipSetDevice(ctx->device));
@@ -984,6 +1406,9 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   B->h = d->cell_h;
   B->nu = d->nu;
   B->conv = d->convention;
+  B->trace = d->trace;
+  B->form = d->coarse_basis;
+  if (d->trace_tol > 0) B->trace_tol = d->trace_tol;
   // templates per distinct n
   std::map<int, int> tmpl_of;
   std::vector<Tmpl> tmpls;
@@ -1032,9 +1457,13 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   // cells
   B->cells.resize(d->n_cells);
   B->int0.resize(d->n_cells);
-  int64_t t0 = 0, i0 = 0;
+  int64_t t0 = 0, i0 = 0, e0 = 0, f0 = 0;
   for (int c = 0; c < d->n_cells; c++) {
     Cell& C = B->cells[c];
+    C.et0 = e0;
+    C.ft0 = f0;
+    e0 += 12ll * (d->cell_n[c] + 1) * 2;
+    f0 += 6ll * (d->cell_n[c] + 1) * (d->cell_n[c] + 1) * FT_NODE;
     C.n = d->cell_n[c];
     C.nint = (C.n - 1) * (C.n - 1) * (C.n - 1);
     C.tet0 = t0;
@@ -1051,6 +1480,8 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   }
   B->total_tets = t0;
   B->total_int = i0;
+  B->et_total = e0;
+  B->ft_total = f0;
   for (int c = 0; c < d->n_cells; c++)
     if (!(d->E[c] > 0) && false) break;
   // tasks: 3 per cell, largest cells first (stable by cell id)
@@ -1086,6 +1517,12 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   alloc((void**)&B->d_task_cell, sizeof(int) * B->ntasks);
   alloc((void**)&B->d_task_group, sizeof(int) * B->ntasks);
   alloc((void**)&B->d_queue, 64);
+  if (B->trace == MFS_TRACE_REFERENCE) {
+    alloc((void**)&B->d_et, sizeof(double) * B->et_total);
+    alloc((void**)&B->d_ft, sizeof(double) * B->ft_total);
+    alloc((void**)&B->d_face_order, sizeof(int) * d->n_cells);
+    alloc((void**)&B->d_fstats, sizeof(int) * 2);
+  }
   if (st != MFS_OK) {
     batch_free(B);
     return st;
@@ -1100,6 +1537,8 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   MFS_HIP(hipMemcpyAsync(B->d_E, d->E, sizeof(double) * t0, hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(B->d_task_cell, tc.data(), sizeof(int) * B->ntasks, hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(B->d_task_group, tg.data(), sizeof(int) * B->ntasks, hipMemcpyHostToDevice, s));
+  if (B->d_face_order)
+    MFS_HIP(hipMemcpyAsync(B->d_face_order, order.data(), sizeof(int) * d->n_cells, hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemsetAsync(B->d_x64, 0, sizeof(double) * 3 * NB * i0, s));
   MFS_HIP(hipMemsetAsync(B->d_iters, 0, sizeof(int) * NB * d->n_cells, s));
   MFS_HIP(hipMemsetAsync(B->d_relres, 0, sizeof(double) * NB * d->n_cells, s));
@@ -1120,14 +1559,79 @@ extern "C" int mfs_batch_cell_dofs(mfs_batch B, int cell, int64_t* n_interior, i
 extern "C" int mfs_batch_assemble(mfs_batch B) {
   MFS_CHECK_ARG(B, "mfs_batch_assemble: NULL batch");
   hipStream_t s = B->ctx->stream;
+  if (B->trace == MFS_TRACE_REFERENCE) {
+    // edge profiles, then the face solves (boundary data of the volume problems)
+    MFS_HIP(hipMemsetAsync(B->d_fstats, 0, sizeof(int) * 2, s));
+    hipEvent_t ev[2];
+    MFS_HIP(hipEventCreate(&ev[0]));
+    MFS_HIP(hipEventCreate(&ev[1]));
+    MFS_HIP(hipEventRecord(ev[0], s));
+    k_edge_profiles<<<(B->ncells * 12 + 63) / 64, 64, 0, s>>>(B->d_cells, B->ncells, B->d_E, B->d_et);
+    MFS_HIP(hipGetLastError());
+    k_face_traces<<<B->ncells * 6, F_THREADS, face_lds_bytes(B->max_n), s>>>(
+        B->d_cells, B->d_face_order, B->d_E, B->nu, B->h, B->conv, B->d_et, B->d_ft, B->trace_tol * B->trace_tol,
+        B->d_fstats);
+    MFS_HIP(hipGetLastError());
+    MFS_HIP(hipEventRecord(ev[1], s));
+    MFS_HIP(hipMemcpyAsync(B->face_stats, B->d_fstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    MFS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    B->trace_ms = ms;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
   const unsigned grid = (unsigned)std::min<int64_t>(65536, (B->total_int + 255) / 256);
   k_cell_assemble<<<grid, 256, 0, s>>>(B->d_cells, B->ncells, B->d_int0, B->d_tmpls, B->d_gptr, B->d_gent, B->d_E,
-                                       B->nu, B->h, B->conv, B->total_int, B->d_K32, B->d_K64, B->d_dinv32,
-                                       B->d_dinv64, B->d_b64);
+                                       B->nu, B->h, B->conv, B->d_ft, B->total_int, B->d_K32, B->d_K64,
+                                       B->d_dinv32, B->d_dinv64, B->d_b64);
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipStreamSynchronize(s));
   B->assembled = true;
   return MFS_OK;
+}
+
+extern "C" int mfs_batch_trace_status(mfs_batch B, int* max_face_iterations, int* n_face_failed,
+                                      double* trace_ms) {
+  MFS_CHECK_ARG(B, "mfs_batch_trace_status: NULL batch");
+  MFS_CHECK_ARG(B->trace == MFS_TRACE_REFERENCE, "mfs_batch_trace_status: batch is not in the reference-trace mode");
+  MFS_CHECK_ARG(B->assembled, "mfs_batch_trace_status: call mfs_batch_assemble first");
+  if (max_face_iterations) *max_face_iterations = B->face_stats[0];
+  if (n_face_failed) *n_face_failed = B->face_stats[1];
+  if (trace_ms) *trace_ms = B->trace_ms;
+  return MFS_OK;
+}
+
+// Host copy of one cell's face traces (reference-trace mode; empty otherwise).
+static int cell_face_traces(mfs_batch B, int cell, std::vector<double>& ft) {
+  ft.clear();
+  if (B->trace != MFS_TRACE_REFERENCE) return MFS_OK;
+  const Cell& C = B->cells[cell];
+  ft.resize(6ull * (C.n + 1) * (C.n + 1) * FT_NODE);
+  MFS_HIP(hipMemcpyAsync(ft.data(), B->d_ft + C.ft0, sizeof(double) * ft.size(), hipMemcpyDeviceToHost,
+                         B->ctx->stream));
+  MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+  return MFS_OK;
+}
+
+// The 24 coarse functions at boundary node (i,j,k) of a cell, in the batch's
+// coarse-basis form: out[q][a].
+static void boundary_functions(const mfs_batch_s* B, const std::vector<double>& ft, int n, int i, int j, int k,
+                               double (&out)[NB][3]) {
+  for (int q = 0; q < NB; q++) {
+    double g[3];
+    if (B->trace == MFS_TRACE_REFERENCE)
+      face_trace_at(ft.data(), n, i, j, k, q, g);
+    else
+      hat_trace_at(n, i, j, k, q, g);
+    for (int a = 0; a < 3; a++) out[q][a] = g[a];
+  }
+  if (B->form == MFS_BASIS_COMPONENTWISE)
+    for (int c = 0; c < 8; c++)
+      for (int a = 0; a < 3; a++) {
+        const double sum = out[3 * c][a] + out[3 * c + 1][a] + out[3 * c + 2][a];
+        for (int a2 = 0; a2 < 3; a2++) out[3 * c + a2][a] = a2 == a ? sum : 0.0;
+      }
 }
 
 static size_t lds_bytes(mfs_batch B, int prec) {
@@ -1431,27 +1935,30 @@ extern "C" int mfs_batch_get_basis(mfs_batch B, int cell, double* basis) {
   MFS_HIP(hipMemcpyAsync(xi.data(), B->d_x64 + C.int0 * 3 * NB, sizeof(double) * xi.size(), hipMemcpyDeviceToHost,
                          B->ctx->stream));
   MFS_HIP(hipStreamSynchronize(B->ctx->stream));
+  std::vector<double> ft;
+  MFS_TRY(cell_face_traces(B, cell, ft));
   const int64_t N = 3ll * s * s * s;
   for (int k = 0; k <= n; k++)
     for (int j = 0; j <= n; j++)
       for (int i = 0; i <= n; i++) {
         const int64_t node = i + s * (j + s * k);
         const bool interior = i >= 1 && i <= m && j >= 1 && j <= m && k >= 1 && k <= m;
-        const double X = (double)i / n, Y = (double)j / n, Z = (double)k / n;
-        for (int q = 0; q < NB; q++) {
-          const int c = q / 3, d = q % 3;
-          for (int a = 0; a < 3; a++) {
-            double v;
-            if (interior) {
-              const int u = (i - 1) + m * ((j - 1) + m * (k - 1));
-              v = xi[((size_t)u * 3 + a) * NB + q];
-            } else {
-              const double hat = ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
-              v = a == d ? hat : 0.0;
-            }
-            basis[q * N + 3 * node + a] = v;
-          }
+        double f[NB][3];
+        if (interior) {
+          const int u = (i - 1) + m * ((j - 1) + m * (k - 1));
+          for (int q = 0; q < NB; q++)
+            for (int a = 0; a < 3; a++) f[q][a] = xi[((size_t)u * 3 + a) * NB + q];
+          if (B->form == MFS_BASIS_COMPONENTWISE)
+            for (int c = 0; c < 8; c++)
+              for (int a = 0; a < 3; a++) {
+                const double sum = f[3 * c][a] + f[3 * c + 1][a] + f[3 * c + 2][a];
+                for (int a2 = 0; a2 < 3; a2++) f[3 * c + a2][a] = a2 == a ? sum : 0.0;
+              }
+        } else {
+          boundary_functions(B, ft, n, i, j, k, f);
         }
+        for (int q = 0; q < NB; q++)
+          for (int a = 0; a < 3; a++) basis[q * N + 3 * node + a] = f[q][a];
       }
   return MFS_OK;
 }
@@ -1462,7 +1969,7 @@ extern "C" int mfs_batch_coarse_matrices(mfs_batch B, double* Kc) {
   hipStream_t s = B->ctx->stream;
   double* d_kc;
   MFS_TRY(dmalloc((void**)&d_kc, sizeof(double) * NB * NB * B->ncells));
-  k_galerkin<<<B->ncells, 256, 0, s>>>(B->d_cells, B->d_E, B->nu, B->h, B->conv, B->d_x64, d_kc);
+  k_galerkin<<<B->ncells, 256, 0, s>>>(B->d_cells, B->d_E, B->nu, B->h, B->conv, B->d_x64, B->d_ft, B->form, d_kc);
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipMemcpyAsync(Kc, d_kc, sizeof(double) * NB * NB * B->ncells, hipMemcpyDeviceToHost, s));
   MFS_HIP(hipStreamSynchronize(s));
@@ -1531,6 +2038,7 @@ extern "C" int mfs_batch_features(mfs_batch B, double mu, double sigma, mfs_vec 
 }
 
 static int targets_guess(mfs_batch B, mfs_vec V, int mode) {
+  MFS_CHECK_ARG(B->trace == MFS_TRACE_Q1, "mfs_batch_targets/guess: the learned guess needs the Q1 trace mode");
   const int n = batch_uniform_n(B);
   MFS_CHECK_ARG(n > 0, "mfs_batch_targets/guess: all cells must have the same subdivision");
   const int64_t per_cell = (int64_t)(n - 1) * (n - 1) * (n - 1) * 3 * NB;
@@ -1695,6 +2203,8 @@ extern "C" int mfs_msfem_top_traction(mfs_batch B, const double* t, double* load
   for (int c = 0; c < B->ncells; c++) {
     if (gi[3 * c + 2] != nz - 1) continue;
     const int n = B->cells[c].n;
+    std::vector<double> ft;
+    MFS_TRY(cell_face_traces(B, c, ft));
     const double A = 0.5 * (B->h / n) * (B->h / n);  // fine triangle area
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
@@ -1702,12 +2212,14 @@ extern "C" int mfs_msfem_top_traction(mfs_batch B, const double* t, double* load
         const int tri[2][3][2] = {{{i, j}, {i + 1, j}, {i + 1, j + 1}}, {{i, j}, {i, j + 1}, {i + 1, j + 1}}};
         for (int tt = 0; tt < 2; tt++)
           for (int v = 0; v < 3; v++) {
-            const double X = (double)tri[tt][v][0] / n, Y = (double)tri[tt][v][1] / n;
+            double f[NB][3];
+            boundary_functions(B, ft, n, tri[tt][v][0], tri[tt][v][1], n, f);
             for (int q = 4; q < 8; q++) {  // top coarse nodes of the cell
-              const double hat = ((q & 1) ? X : 1 - X) * ((q & 2) ? Y : 1 - Y);
               const int64_t node =
                   (gi[3 * c] + (q & 1)) + sx * ((gi[3 * c + 1] + ((q >> 1) & 1)) + (int64_t)sy * nz);
-              for (int d = 0; d < 3; d++) load[3 * node + d] += t[d] * hat * A / 3.0;
+              for (int d = 0; d < 3; d++)
+                load[3 * node + d] +=
+                    (t[0] * f[3 * q + d][0] + t[1] * f[3 * q + d][1] + t[2] * f[3 * q + d][2]) * A / 3.0;
             }
           }
       }
@@ -1738,6 +2250,51 @@ extern "C" int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell
       const double* ph = basis.data() + (size_t)(3 * q + d) * N;
       for (int64_t e = 0; e < N; e++) u_fine[e] += w * ph[e];
     }
+  }
+  return MFS_OK;
+}
+
+// MsFEM-vs-FEM relative error per component (MechanicalMain.cpp:2112-2221):
+// one-point rule per fine tet (centroid, weight V), the MsFEM field from the
+// downscaled cell solution and the FEM field from the global fine grid.
+extern "C" int mfs_msfem_error(mfs_batch B, const double* u_coarse, const double* u_fem, double* err3) {
+  MFS_CHECK_ARG(B && u_coarse && u_fem && err3, "mfs_msfem_error: bad argument");
+  const int n = batch_uniform_n(B);
+  MFS_CHECK_ARG(n > 0, "mfs_msfem_error: all cells must have the same subdivision");
+  int nx, ny, nz;
+  std::vector<int> gi;
+  MFS_TRY(coarse_grid(B, &nx, &ny, &nz, gi));
+  const int64_t gx = (int64_t)nx * n + 1, gy = (int64_t)ny * n + 1;
+  const int s = n + 1;
+  const double hf = B->h / n, V = hf * hf * hf / 6.0;
+  double num[3] = {0, 0, 0}, den[3] = {0, 0, 0};
+  std::vector<double> uf(3ull * s * s * s);
+  for (int c = 0; c < B->ncells; c++) {
+    MFS_TRY(mfs_msfem_downscale(B, u_coarse, c, uf.data()));
+    for (int t = 0; t < 6 * n * n * n; t++) {
+      const int hex = t / 6, p = t % 6;
+      int cc[3] = {hex % n, (hex / n) % n, hex / (n * n)};
+      static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+      double ms[3] = {0, 0, 0}, fe[3] = {0, 0, 0};
+      for (int v = 0; v < 4; v++) {
+        if (v > 0) cc[perm[p][v - 1]]++;
+        const int64_t loc = cc[0] + s * (cc[1] + (int64_t)s * cc[2]);
+        const int64_t glob = (gi[3 * c] * (int64_t)n + cc[0]) +
+                             gx * ((gi[3 * c + 1] * (int64_t)n + cc[1]) + gy * (gi[3 * c + 2] * (int64_t)n + cc[2]));
+        for (int a = 0; a < 3; a++) {
+          ms[a] += 0.25 * uf[3 * loc + a];
+          fe[a] += 0.25 * u_fem[3 * glob + a];
+        }
+      }
+      for (int a = 0; a < 3; a++) {
+        num[a] += V * (ms[a] - fe[a]) * (ms[a] - fe[a]);
+        den[a] += V * fe[a] * fe[a];
+      }
+    }
+  }
+  for (int a = 0; a < 3; a++) {
+    const double sd = std::sqrt(den[a]);
+    err3[a] = std::sqrt(num[a]) / (sd < 1e-10 ? 1.0 : sd);
   }
   return MFS_OK;
 }

@@ -94,3 +94,19 @@ def msfem_cells(nc, n_lo=6, n_hi=10, n_fixed=None, seed_n=3, seed_E=1, mu=float(
 def cell_E(p, c):
     off = int((6 * p["cell_n"][:c].astype(np.int64) ** 3).sum())
     return p["E"][off:off + 6 * int(p["cell_n"][c]) ** 3]
+
+
+def msfem_global_E(p):
+    """Per-tet modulus of the whole box's fine mesh mesh_box(nc*n) when every
+    cell has the same n: global hex (I,J,K) lies in cell (I//n, J//n, K//n)
+    at local hex (I%n, J%n, K%n), with the same 6-tet split."""
+    nc = p["nc"]
+    n = int(p["cell_n"][0])
+    assert (p["cell_n"] == n).all(), "cells must share one subdivision"
+    N = nc * n
+    I, J, K = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    I, J, K = I.T.ravel(), J.T.ravel(), K.T.ravel()  # hex order x fastest
+    cell = (I // n) + nc * ((J // n) + nc * (K // n))
+    local = (I % n) + n * ((J % n) + n * (K % n))
+    idx = (cell[:, None] * 6 * n ** 3 + 6 * local[:, None] + np.arange(6)[None, :]).ravel()
+    return p["E"][idx]

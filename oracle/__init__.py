@@ -77,6 +77,11 @@ _sig("orc_refpath_solve", C.c_int, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c
 _sig("orc_bicg_diag", C.c_double, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp, ip])
 _sig("orc_msfem_cell", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, C.c_double,
                                  C.c_double, dp, dp, ip])
+_sig("orc_elem_seg", C.c_double, [dp, C.c_double, C.c_double, C.c_int, dp])
+_sig("orc_elem_tri", C.c_double, [dp, C.c_double, C.c_double, C.c_int, dp])
+_sig("orc_edge_1d", C.c_int, [C.c_int, dp, dp, C.c_double, C.c_int, dp, dp, dp])
+_sig("orc_msfem_cell_reftrace", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, C.c_double,
+                                          C.c_double, C.c_int, dp, dp, ip])
 _sig("orc_mlp_loss_grad", C.c_double, [C.c_int, ip, C.POINTER(dp), C.POINTER(dp), C.c_int, dp, dp, C.POINTER(dp),
                                        C.POINTER(dp)])
 
@@ -265,6 +270,41 @@ def msfem_cell(n, origin, h, E, nu, tol=1e-12):
     it = C.c_int()
     lib.orc_msfem_cell(n, *origin, h, _p(_f64(E), C.c_double), nu, tol, _p(basis, C.c_double), _p(Kc, C.c_double),
                        C.byref(it))
+    return basis, Kc, it.value
+
+
+def elem_seg(P, E, nu, convention=0):
+    K = np.empty((6, 6))
+    L = lib.orc_elem_seg(_p(_f64(P), C.c_double), E, nu, convention, _p(K, C.c_double))
+    return K, L
+
+
+def elem_tri(P, E, nu, convention=0):
+    K = np.empty((9, 9))
+    A = lib.orc_elem_tri(_p(_f64(P), C.c_double), E, nu, convention, _p(K, C.c_double))
+    return K, A
+
+
+def edge_1d(xyz, E, nu, u0, un, convention=0):
+    xyz = _f64(xyz).reshape(-1, 3)
+    U = np.empty(xyz.size)
+    rc = lib.orc_edge_1d(len(xyz) - 1, _p(xyz, C.c_double), _p(_f64(E), C.c_double), nu, convention,
+                         _p(_f64(u0), C.c_double), _p(_f64(un), C.c_double), _p(U, C.c_double))
+    assert rc == 0
+    return U.reshape(-1, 3)
+
+
+def msfem_cell_reftrace(n, origin, h, E, nu, tol=1e-12, form=0):
+    """Bases of one cell with the reference's edge -> face -> volume traces
+    (basis 3c+d driven by e_d), and Kc for the per-direction (form 0) or
+    component-wise (form 1) coarse functions."""
+    N = 3 * (n + 1) ** 3
+    basis = np.empty((24, N))
+    Kc = np.empty((24, 24))
+    it = C.c_int()
+    rc = lib.orc_msfem_cell_reftrace(n, *origin, h, _p(_f64(E), C.c_double), nu, tol, form, _p(basis, C.c_double),
+                                     _p(Kc, C.c_double), C.byref(it))
+    assert rc == N, rc
     return basis, Kc, it.value
 
 

@@ -1046,6 +1046,427 @@ EXPORT int orc_msfem_cell(int n, double x0, double y0, double z0, double h, cons
 }
 
 // ---------------------------------------------------------------------------
+// MsFEM reference-trace mode: the edge -> face -> volume hierarchy of
+// Grid_forMech_Poly_Order1::CreateFunctionalSpaces (MsFEM_Grid.h:5026-5506).
+// ---------------------------------------------------------------------------
+// Dense LU with partial pivoting, nrhs right-hand sides (columns of B,
+// row-major n x nrhs), overwritten by the solutions.
+static int gepp_multi(int n, std::vector<double>& A, int nrhs, std::vector<double>& B) {
+  for (int c = 0; c < n; c++) {
+    int p = c;
+    for (int r = c + 1; r < n; r++)
+      if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)p * n + c])) p = r;
+    if (A[(size_t)p * n + c] == 0) return -1;
+    if (p != c) {
+      for (int k = 0; k < n; k++) std::swap(A[(size_t)p * n + k], A[(size_t)c * n + k]);
+      for (int k = 0; k < nrhs; k++) std::swap(B[(size_t)p * nrhs + k], B[(size_t)c * nrhs + k]);
+    }
+    for (int r = c + 1; r < n; r++) {
+      double f = A[(size_t)r * n + c] / A[(size_t)c * n + c];
+      if (f == 0) continue;
+      for (int k = c; k < n; k++) A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+      for (int k = 0; k < nrhs; k++) B[(size_t)r * nrhs + k] -= f * B[(size_t)c * nrhs + k];
+    }
+  }
+  for (int i = n - 1; i >= 0; i--)
+    for (int k = 0; k < nrhs; k++) {
+      double s = B[(size_t)i * nrhs + k];
+      for (int j = i + 1; j < n; j++) s -= A[(size_t)i * n + j] * B[(size_t)j * nrhs + k];
+      B[(size_t)i * nrhs + k] = s / A[(size_t)i * n + i];
+    }
+  return 0;
+}
+
+// Block (I,J) of B_J^T D B_I (convention 0) or B_I^T D B_J (convention 1)
+// for gradients gI, gJ, times the weight w: the integrand of every
+// SolveLocalMatrix (FEM_Grid.h:3649-3735 segments, :4429-4500 triangles,
+// :2125-2191 tets) with the basis derivative x = y = z = grad phi.
+static void orc_block(const double gI[3], const double gJ[3], const double D[6][6], int convention, double w,
+                      double R[3][3]) {
+  double BI[6][3], BJ[6][3];
+  orc_B(gI, BI);
+  orc_B(gJ, BJ);
+  const double(*Bl)[3] = convention == 0 ? BJ : BI;
+  const double(*Br)[3] = convention == 0 ? BI : BJ;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      double s = 0;
+      for (int k = 0; k < 6; k++)
+        for (int l = 0; l < 6; l++) s += Bl[k][r] * D[k][l] * Br[l][c];
+      R[r][c] = s * w;
+    }
+}
+
+// Segment element (FiniteElement_1D_forMech, FEM_Grid.h:3649-3735 with the
+// basis of :4011-4101): local frame with X along node0 -> node1, hat
+// derivative -+1/len along X mapped back by the reverse basis, 4-point
+// rule whose weights add up to the length.  K is 6x6, index 3*I+a.
+EXPORT double orc_elem_seg(const double* P, double E, double nu, int convention, double* K) {
+  double t[3], L = 0;
+  for (int a = 0; a < 3; a++) {
+    t[a] = P[3 + a] - P[a];
+    L += t[a] * t[a];
+  }
+  L = std::sqrt(L);
+  double g[2][3];
+  for (int a = 0; a < 3; a++) {
+    g[0][a] = -t[a] / L / L;
+    g[1][a] = t[a] / L / L;
+  }
+  double D[6][6];
+  orc_isotropic_D(E, nu, D);
+  for (int I = 0; I < 2; I++)
+    for (int J = 0; J < 2; J++) {
+      double R[3][3];
+      orc_block(g[I], g[J], D, convention, L, R);
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) K[(3 * I + r) * 6 + 3 * J + c] = R[r][c];
+    }
+  return L;
+}
+
+// Triangle element (FiniteElement_2D_forMech, FEM_Grid.h:4429-4500 with the
+// basis of :4879-4950): orthonormal local frame (X along node0 -> node1, Y
+// in the plane, GeometryShape.h:979-1060), phi_i = alpha_i0 + alpha_i1 x +
+// alpha_i2 y from the inverse of [1 x y] in local coordinates, gradient
+// alpha_i1 X + alpha_i2 Y; area weight (GetVolume, GeometryShape.h:1337-1364).
+// K is 9x9, index 3*I+a.
+EXPORT double orc_elem_tri(const double* P, double E, double nu, int convention, double* K) {
+  double e1[3], e2[3], N[3];
+  for (int a = 0; a < 3; a++) {
+    e1[a] = P[3 + a] - P[a];
+    e2[a] = P[6 + a] - P[a];
+  }
+  N[0] = e1[1] * e2[2] - e1[2] * e2[1];
+  N[1] = e1[2] * e2[0] - e1[0] * e2[2];
+  N[2] = e1[0] * e2[1] - e1[1] * e2[0];
+  const double nN = std::sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+  const double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+  double X[3], Y[3];
+  for (int a = 0; a < 3; a++) X[a] = e1[a] / l1;
+  // Y = (N/|N|) x X
+  Y[0] = (N[1] * X[2] - N[2] * X[1]) / nN;
+  Y[1] = (N[2] * X[0] - N[0] * X[2]) / nN;
+  Y[2] = (N[0] * X[1] - N[1] * X[0]) / nN;
+  double loc[3][2];
+  for (int i = 0; i < 3; i++) {
+    double d[3];
+    for (int a = 0; a < 3; a++) d[a] = P[3 * i + a] - P[a];
+    loc[i][0] = d[0] * X[0] + d[1] * X[1] + d[2] * X[2];
+    loc[i][1] = d[0] * Y[0] + d[1] * Y[1] + d[2] * Y[2];
+  }
+  // alpha = inverse of M = [1 x_i y_i] (rows i); phi_i coefficients are column i
+  double M[3][3] = {{1, loc[0][0], loc[0][1]}, {1, loc[1][0], loc[1][1]}, {1, loc[2][0], loc[2][1]}};
+  double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+               M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+  double inv[3][3];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+      inv[r][c] = (M[r1][c1] * M[r2][c2] - M[r1][c2] * M[r2][c1]) / det;
+    }
+  double g[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int a = 0; a < 3; a++) g[i][a] = inv[1][i] * X[a] + inv[2][i] * Y[a];
+  const double area = 0.5 * nN;
+  double D[6][6];
+  orc_isotropic_D(E, nu, D);
+  for (int I = 0; I < 3; I++)
+    for (int J = 0; J < 3; J++) {
+      double R[3][3];
+      orc_block(g[I], g[J], D, convention, area, R);
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) K[(3 * I + r) * 9 + 3 * J + c] = R[r][c];
+    }
+  return area;
+}
+
+// Solve a small elastic system with 3 DOFs per node given element lists;
+// nodes with known[] set take the values in U (nrhs per DOF), the rest are
+// solved (Dirichlet rows replaced and moved to the right side as in
+// FEM.h:4126-4200).  U is [node][3][nrhs].
+static int orc_small_elastic(int nn, int ne, int npe, const int* elem, const double* Ke /*ne x (3npe)^2*/,
+                             const char* known, int nrhs, double* U) {
+  std::vector<int> loc(3 * nn, -1);
+  int nu_ = 0;
+  for (int v = 0; v < nn; v++)
+    if (!known[v])
+      for (int a = 0; a < 3; a++) loc[3 * v + a] = nu_++;
+  std::vector<double> A((size_t)nu_ * nu_, 0.0), B((size_t)nu_ * nrhs, 0.0);
+  const int m = 3 * npe;
+  for (int e = 0; e < ne; e++)
+    for (int I = 0; I < npe; I++)
+      for (int a = 0; a < 3; a++) {
+        const int row = loc[3 * elem[e * npe + I] + a];
+        if (row < 0) continue;
+        for (int J = 0; J < npe; J++)
+          for (int b = 0; b < 3; b++) {
+            const double k = Ke[(size_t)e * m * m + (3 * I + a) * m + 3 * J + b];
+            const int cj = 3 * elem[e * npe + J] + b;
+            if (loc[cj] >= 0)
+              A[(size_t)row * nu_ + loc[cj]] += k;
+            else
+              for (int r = 0; r < nrhs; r++) B[(size_t)row * nrhs + r] -= k * U[(size_t)cj * nrhs + r];
+          }
+      }
+  if (nu_ > 0 && gepp_multi(nu_, A, nrhs, B) != 0) return -1;
+  for (int d = 0; d < 3 * nn; d++)
+    if (loc[d] >= 0)
+      for (int r = 0; r < nrhs; r++) U[(size_t)d * nrhs + r] = B[(size_t)loc[d] * nrhs + r];
+  return 0;
+}
+
+// 1-D elastic chain (FEM_1D_forElasticDeformation, FEM.h:3796-4365):
+// nodes xyz[(nseg+1)*3] in order, segment moduli E[nseg], Dirichlet at both
+// ends with u0[3], un[3].  Writes U[(nseg+1)*3].
+EXPORT int orc_edge_1d(int nseg, const double* xyz, const double* E, double nu, int convention, const double* u0,
+                       const double* un, double* U) {
+  const int nn = nseg + 1;
+  std::vector<int> el(2 * nseg);
+  std::vector<double> Ke((size_t)nseg * 36);
+  for (int s = 0; s < nseg; s++) {
+    el[2 * s] = s;
+    el[2 * s + 1] = s + 1;
+    orc_elem_seg(xyz + 3 * s, E[s], nu, convention, Ke.data() + (size_t)s * 36);
+  }
+  std::vector<char> known(nn, 0);
+  known[0] = known[nn - 1] = 1;
+  for (int d = 0; d < 3 * nn; d++) U[d] = 0.0;
+  for (int a = 0; a < 3; a++) {
+    U[a] = u0[a];
+    U[3 * (nn - 1) + a] = un[a];
+  }
+  return orc_small_elastic(nn, nseg, 2, el.data(), Ke.data(), known.data(), 1, U);
+}
+
+// Local bases of one structured cell in the reference-trace mode.
+//   edges: per coarse edge a 1-D solve with the coarse-node value at one end
+//     and 0 at the other (MsFEM_Grid.h:5076-5175); segment modulus = that of
+//     the boundary triangle holding the segment in the first face (x, y, z
+//     faces in that order) containing the edge (:4880-4920 takes the
+//     triangle's material from the first neighbouring face);
+//   faces: per coarse face and face corner a 2-D solve with the two edge
+//     solutions that are 1 at the corner on the edges meeting there and 0 on
+//     the other two (:5213-5325); triangle modulus = that of the tet owning
+//     the triangle;
+//   volume: per coarse node a 3-D solve with the face solutions of the three
+//     faces meeting at the node and 0 on the others (:5421-5506).
+// The reference drives all three components together with (1,1,1); here
+// basis 3c+d is driven with e_d, so that the reference's basis of node c is
+// the sum over d (every step is linear in its boundary data).
+// form 0: Kc = Phi^T K Phi of these bases (K in the reference convention);
+// form 1: the component-wise coarse functions Psi_{c,a} = e_a (sum_d
+// phi_{c,d})_a, the reference's use of a vector basis per coarse DOF, with
+// the reference's coarse element block(I,J) = int B_J^T D B_I of their
+// gradients (MsFEM_Grid.h:3837-3975) integrated exactly over the fine tets:
+// Kc = blockwise transpose of Psi^T K_textbook Psi.  Returns N.
+EXPORT int orc_msfem_cell_reftrace(int n, double x0, double y0, double z0, double h, const double* E, double nu,
+                                   double tol, int form, double* basis, double* Kc, int* iters_out) {
+  const int s = n + 1;
+  const int nn = s * s * s;
+  const i64 nt = 6 * (i64)n * n * n;
+  std::vector<double> xyz(3 * (size_t)nn);
+  std::vector<int> tets(4 * nt);
+  orc_mesh_box(n, n, n, x0, y0, z0, h / n, h / n, h / n, xyz.data(), tets.data());
+  auto ijk = [&](int v, int a) { return a == 0 ? v % s : a == 1 ? (v / s) % s : v / (s * s); };
+  auto on_face = [&](int v, int f) { return ijk(v, f / 2) == (f % 2 ? n : 0); };
+  // boundary triangles of each face, with the modulus of the owning tet
+  struct Tri {
+    int v[3];
+    double E;
+  };
+  std::vector<Tri> ftri[6];
+  for (i64 t = 0; t < nt; t++)
+    for (int omit = 0; omit < 4; omit++) {
+      int v[3], q = 0;
+      for (int k = 0; k < 4; k++)
+        if (k != omit) v[q++] = tets[4 * t + k];
+      for (int f = 0; f < 6; f++)
+        if (on_face(v[0], f) && on_face(v[1], f) && on_face(v[2], f)) ftri[f].push_back({{v[0], v[1], v[2]}, E[t]});
+    }
+  std::vector<double> trace((size_t)3 * nn * 24, 0.0);  // [node][a][basis]
+  std::vector<char> traced(nn, 0);
+  // ---- edges: direction ax, other axes at sides; corners c0 (bit ax = 0), c1
+  std::vector<std::vector<double>> E_sol(12);  // [end][node][3][3 drives]
+  for (int e = 0; e < 12; e++) {
+    const int ax = e / 4, o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;
+    const int b1 = e & 1, b2 = (e >> 1) & 1;
+    std::vector<int> nodes(s);
+    for (int t = 0; t < s; t++) {
+      int c[3];
+      c[ax] = t;
+      c[o1] = b1 * n;
+      c[o2] = b2 * n;
+      nodes[t] = c[0] + s * (c[1] + s * c[2]);
+    }
+    // segment moduli from the first face containing the edge
+    const int f0 = 2 * o1 + b1;
+    std::vector<double> Es(n), pts(3 * s);
+    for (int t = 0; t < n; t++) {
+      bool found = false;
+      for (const Tri& T : ftri[f0]) {
+        int hit = 0;
+        for (int k = 0; k < 3; k++) hit += (T.v[k] == nodes[t]) + (T.v[k] == nodes[t + 1]);
+        if (hit == 2) {
+          Es[t] = T.E;
+          found = true;
+          break;
+        }
+      }
+      if (!found) return -1;
+    }
+    for (int t = 0; t < s; t++)
+      for (int a = 0; a < 3; a++) pts[3 * t + a] = xyz[3 * nodes[t] + a];
+    E_sol[e].assign((size_t)2 * s * 9, 0.0);
+    for (int end = 0; end < 2; end++)
+      for (int d = 0; d < 3; d++) {
+        double u0[3] = {0, 0, 0}, un[3] = {0, 0, 0};
+        (end == 0 ? u0 : un)[d] = 1.0;
+        std::vector<double> U(3 * s);
+        if (orc_edge_1d(n, pts.data(), Es.data(), nu, 0, u0, un, U.data()) != 0) return -1;
+        for (int t = 0; t < s; t++)
+          for (int a = 0; a < 3; a++) E_sol[e][((size_t)(end * s + t) * 3 + a) * 3 + d] = U[3 * t + a];
+      }
+  }
+  // ---- faces
+  std::vector<double> F_sol[6];  // [node][3][corner 8][3 drives], corners of the face only
+  for (int f = 0; f < 6; f++) {
+    const int ax = f / 2, side = f % 2;
+    std::vector<int> fnodes;
+    std::vector<int> floc(nn, -1);
+    for (int v = 0; v < nn; v++)
+      if (on_face(v, f)) {
+        floc[v] = (int)fnodes.size();
+        fnodes.push_back(v);
+      }
+    const int fn = (int)fnodes.size();
+    std::vector<int> el;
+    std::vector<double> Ke;
+    for (const Tri& T : ftri[f]) {
+      double P[9];
+      for (int k = 0; k < 3; k++) {
+        el.push_back(floc[T.v[k]]);
+        for (int a = 0; a < 3; a++) P[3 * k + a] = xyz[3 * T.v[k] + a];
+      }
+      Ke.resize(Ke.size() + 81);
+      orc_elem_tri(P, T.E, nu, 0, Ke.data() + Ke.size() - 81);
+    }
+    // Dirichlet data on the face boundary from the edge solutions: 8 corners x 3 drives
+    std::vector<char> known(fn, 0);
+    std::vector<double> U((size_t)fn * 3 * 24, 0.0);
+    for (int e = 0; e < 12; e++) {
+      const int eax = e / 4;
+      if (eax == ax) continue;
+      const int o1 = eax == 0 ? 1 : 0, o2 = eax == 2 ? 1 : 2;
+      const int b1 = e & 1, b2 = (e >> 1) & 1;
+      // the edge lies in face f when its fixed coordinate on axis ax equals the face side
+      const int bax = (o1 == ax) ? b1 : b2;
+      if (bax != side) continue;
+      for (int t = 0; t < s; t++) {
+        int c[3];
+        c[eax] = t;
+        c[o1] = b1 * n;
+        c[o2] = b2 * n;
+        const int v = c[0] + s * (c[1] + s * c[2]);
+        const int fl = floc[v];
+        known[fl] = 1;
+        for (int end = 0; end < 2; end++) {
+          int cc[3];
+          cc[eax] = end;
+          cc[o1] = b1;
+          cc[o2] = b2;
+          const int corner = cc[0] | (cc[1] << 1) | (cc[2] << 2);
+          for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++)
+              U[((size_t)fl * 3 + a) * 24 + 3 * corner + d] = E_sol[e][((size_t)(end * s + t) * 3 + a) * 3 + d];
+        }
+      }
+    }
+    if (orc_small_elastic(fn, (int)ftri[f].size(), 3, el.data(), Ke.data(), known.data(), 24, U.data()) != 0)
+      return -1;
+    // only corners of this face carry a trace on it
+    for (int fl = 0; fl < fn; fl++) {
+      const int v = fnodes[fl];
+      traced[v] = 1;
+      for (int corner = 0; corner < 8; corner++) {
+        if (((corner >> ax) & 1) != side) continue;
+        for (int a = 0; a < 3; a++)
+          for (int d = 0; d < 3; d++)
+            trace[((size_t)v * 3 + a) * 24 + 3 * corner + d] = U[((size_t)fl * 3 + a) * 24 + 3 * corner + d];
+      }
+    }
+  }
+  // ---- volume: reference-convention cell matrix, interior solve per basis
+  void* raw = orc_elastic_system(nn, nt, xyz.data(), tets.data(), nullptr, E, nu, 0, nullptr, nullptr, 0, nullptr,
+                                 nullptr, 0, 0);
+  OrcSystem* S = (OrcSystem*)raw;
+  std::vector<int> rp, ci;
+  std::vector<double> vv;
+  orc_to_csr(S->A, rp, ci, vv);
+  const int N = 3 * nn;
+  std::vector<int> loc(N, -1);
+  int ni = 0;
+  for (int d = 0; d < N; d++)
+    if (!traced[d / 3]) loc[d] = ni++;
+  std::vector<int> irp(ni + 1, 0), ici;
+  std::vector<double> iv;
+  for (int d = 0; d < N; d++) {
+    if (loc[d] < 0) continue;
+    for (int k = rp[d]; k < rp[d + 1]; k++)
+      if (loc[ci[k]] >= 0) {
+        ici.push_back(loc[ci[k]]);
+        iv.push_back(vv[k]);
+      }
+    irp[loc[d] + 1] = (int)ici.size();
+  }
+  int maxit_seen = 0;
+  for (int q = 0; q < 24; q++) {
+    double* phi = basis + (size_t)q * N;
+    for (int d = 0; d < N; d++) phi[d] = loc[d] < 0 ? trace[(size_t)d * 24 + q] : 0.0;
+    std::vector<double> rhs(ni, 0.0), xi(ni, 0.0);
+    for (int dd = 0; dd < N; dd++) {
+      if (loc[dd] < 0) continue;
+      double sacc = 0;
+      for (int k = rp[dd]; k < rp[dd + 1]; k++)
+        if (loc[ci[k]] < 0) sacc += vv[k] * phi[ci[k]];
+      rhs[loc[dd]] = -sacc;
+    }
+    double rr;
+    int it = orc_pcg(ni, irp.data(), ici.data(), iv.data(), rhs.data(), xi.data(), tol, 100000, 1, &rr);
+    maxit_seen = std::max(maxit_seen, it);
+    for (int dd = 0; dd < N; dd++)
+      if (loc[dd] >= 0) phi[dd] = xi[loc[dd]];
+  }
+  if (form == 1) {
+    // Psi_{c,a}[node][b] = delta_ab * sum_d phi_{c,d}[node][a]
+    std::vector<double> tmp((size_t)24 * N);
+    for (int c = 0; c < 8; c++)
+      for (int v = 0; v < nn; v++)
+        for (int a = 0; a < 3; a++) {
+          double sum = 0;
+          for (int d = 0; d < 3; d++) sum += basis[(size_t)(3 * c + d) * N + 3 * v + a];
+          for (int a2 = 0; a2 < 3; a2++) tmp[(size_t)(3 * c + a2) * N + 3 * v + a] = a2 == a ? sum : 0.0;
+        }
+    std::copy(tmp.begin(), tmp.end(), basis);
+    void* raw_tb = orc_elastic_system(nn, nt, xyz.data(), tets.data(), nullptr, E, nu, 0, nullptr, nullptr, 0,
+                                      nullptr, nullptr, 1, 0);
+    orc_to_csr(((OrcSystem*)raw_tb)->A, rp, ci, vv);
+    orc_sys_free(raw_tb);
+  }
+  std::vector<double> KP((size_t)24 * N);
+  for (int b2 = 0; b2 < 24; b2++)
+    csr_mv(N, rp.data(), ci.data(), vv.data(), basis + (size_t)b2 * N, KP.data() + (size_t)b2 * N, 1);
+  for (int a = 0; a < 24; a++)
+    for (int b2 = 0; b2 < 24; b2++) {
+      const int row = form == 1 ? 3 * (a / 3) + b2 % 3 : a, col = form == 1 ? 3 * (b2 / 3) + a % 3 : b2;
+      Kc[row * 24 + col] = dotp(N, basis + (size_t)a * N, KP.data() + (size_t)b2 * N);
+    }
+  orc_sys_free(raw);
+  if (iters_out) *iters_out = maxit_seen;
+  return N;
+}
+
+// ---------------------------------------------------------------------------
 // MLP oracle (FP64): layers L, sizes dims[0..L], ReLU between layers, linear
 // output, loss = mean over batch of 0.5*||y - t||^2 / out_dim.  Weights W_l are
 // row-major [out][in], biases b_l.  Computes loss and gradients (backprop).

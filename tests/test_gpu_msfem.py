@@ -181,3 +181,171 @@ def test_coarse_traction_load_total(ctx):
     load = B.top_traction([0.0, 0.0, -1e6], (nc + 1) ** 3)
     assert load[2::3].sum() == pytest.approx(-1e6, rel=1e-12)
     assert np.abs(load[0::3]).max() == 0 and np.abs(load[1::3]).max() == 0
+
+
+# --- reference-trace mode (edge -> face -> volume, MsFEM_Grid.h:5026-5506) ----
+
+def make_ref_batch(ctx, p, form=mfs.BASIS_PER_DIRECTION):
+    B = mfs.Batch(ctx, p["cell_n"], p["origin"], p["h"], p["E"], p["nu"], trace=mfs.TRACE_REFERENCE,
+                  coarse_basis=form)
+    B.assemble()
+    return B
+
+
+@pytest.fixture(scope="module")
+def small_ref(ctx):
+    p = problems.msfem_cells(2, n_lo=3, n_hi=5)
+    return p, make_ref_batch(ctx, p)
+
+
+@pytest.mark.parametrize("precision", [mfs.PREC_MIXED, mfs.PREC_F64])
+def test_reference_traces_match_oracle(ctx, small_ref, precision):
+    """Edge profiles, batched face solves and volume solves against the
+    oracle's per-cell hierarchy (dense edge / face solves, FP64 PCG volume)."""
+    p, B = small_ref
+    it, bad, ms = B.trace_status()
+    assert bad == 0 and it > 0 and ms > 0
+    rep = B.solve(precision=precision, tol=1e-10, use_graph=False)
+    assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-10
+    Kc = B.coarse_matrices()
+    for c in range(B.ncells):
+        n = int(p["cell_n"][c])
+        basis_o, Kc_o, _ = orc.msfem_cell_reftrace(n, p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"],
+                                                   tol=1e-13)
+        err = np.linalg.norm(B.basis(c) - basis_o) / np.linalg.norm(basis_o)
+        assert err < 1e-8, (c, n, err)
+        assert np.abs(Kc[c] - Kc_o).max() < 1e-7 * np.abs(Kc_o).max(), c
+
+
+def test_reference_traces_componentwise_coarse_form(ctx):
+    p = problems.msfem_cells(2, n_lo=3, n_hi=4, seed_E=7)
+    B = make_ref_batch(ctx, p, form=mfs.BASIS_COMPONENTWISE)
+    B.solve(precision=mfs.PREC_MIXED, tol=1e-10)
+    Kc = B.coarse_matrices()
+    for c in (0, B.ncells - 1):
+        n = int(p["cell_n"][c])
+        basis_o, Kc_o, _ = orc.msfem_cell_reftrace(n, p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"],
+                                                   tol=1e-13, form=1)
+        assert np.linalg.norm(B.basis(c) - basis_o) / np.linalg.norm(basis_o) < 1e-8
+        assert np.abs(Kc[c] - Kc_o).max() < 1e-7 * np.abs(Kc_o).max()
+
+
+@pytest.mark.parametrize("form", [mfs.BASIS_PER_DIRECTION, mfs.BASIS_COMPONENTWISE])
+def test_reference_traces_affine_patch(ctx, form):
+    """Homogeneous material, affine displacement on every coarse boundary
+    node: coarse solve + downscaling reproduces the affine field at every fine
+    node, with the reference traces and either coarse form."""
+    nc, n = 2, 4
+    p = problems.msfem_cells(nc, n_fixed=n)
+    p["E"][:] = 1e9
+    B = make_ref_batch(ctx, p, form)
+    B.solve(precision=mfs.PREC_MIXED, tol=1e-11)
+    Kc = B.coarse_matrices()
+    s = nc + 1
+    cx = np.array([[i, j, k] for k in range(s) for j in range(s) for i in range(s)], float) / nc
+    G = np.array([[0.01, 0.002, 0], [0.001, -0.02, 0.003], [0, 0.004, 0.01]])
+    bnd = np.nonzero((cx == 0).any(1) | (cx == 1).any(1))[0]
+    A, b, x = B.coarse_system(Kc, np.zeros(3 * s ** 3), bnd, cx[bnd] @ G.T)
+    mfs.Solver(A, tol=1e-12).run(b, x)
+    uc = x.download()
+    assert np.abs(uc.reshape(-1, 3) - cx @ G.T).max() < 1e-12
+    for c in range(nc ** 3):
+        xf, _ = mfs.mesh_box(n, origin=tuple(p["origin"][c]), h=(p["h"] / n,) * 3)
+        assert np.abs(B.downscale(uc, c).reshape(-1, 3) - xf @ G.T).max() < 1e-9
+    # the fine FEM solution of this problem is the affine field itself
+    xg, _ = mfs.mesh_box(nc * n)
+    assert np.abs(B.msfem_error(uc, xg @ G.T)).max() < 1e-9
+
+
+def numpy_msfem_error(B, p, uc, u_fem):
+    """MechanicalMain.cpp:2150-2221 restated: centroid value of each fine tet,
+    weight V, per component sqrt(sum V diff^2) / sqrt(sum V u_fem^2)."""
+    nc, n = p["nc"], int(p["cell_n"][0])
+    N = nc * n
+    num, den = np.zeros(3), np.zeros(3)
+    uf = u_fem.reshape(-1, 3)
+    for c in range(nc ** 3):
+        xl, tl = mfs.mesh_box(n, origin=tuple(p["origin"][c]), h=(p["h"] / n,) * 3)
+        ul = B.downscale(uc, c).reshape(-1, 3)
+        gidx = np.rint(xl * N).astype(int)
+        g = gidx[:, 0] + (N + 1) * (gidx[:, 1] + (N + 1) * gidx[:, 2])
+        V = np.abs(np.linalg.det(xl[tl[:, 1:]] - xl[tl[:, :1]])) / 6
+        ms = ul[tl].mean(1)
+        fe = uf[g[tl]].mean(1)
+        num += (V[:, None] * (ms - fe) ** 2).sum(0)
+        den += (V[:, None] * fe ** 2).sum(0)
+    return np.sqrt(num) / np.where(np.sqrt(den) < 1e-10, 1.0, np.sqrt(den))
+
+
+def oracle_coarse_solution(p, trace, form, t, fixed):
+    """The MsFEM coarse solve from the oracle's per-cell bases and coarse
+    matrices: scatter, top-traction load Phi^T f (P1, area/3 per vertex),
+    fixed coarse nodes eliminated, dense solve."""
+    nc, n, h = p["nc"], int(p["cell_n"][0]), p["h"]
+    s, sf = nc + 1, n + 1
+    K = np.zeros((3 * s ** 3, 3 * s ** 3))
+    load = np.zeros(3 * s ** 3)
+    A = 0.5 * (h / n) ** 2
+    for c in range(nc ** 3):
+        E = problems.cell_E(p, c)
+        if trace == mfs.TRACE_REFERENCE:
+            basis, Kc, _ = orc.msfem_cell_reftrace(n, p["origin"][c], h, E, p["nu"], tol=1e-13, form=form)
+        else:
+            basis, Kc, _ = orc.msfem_cell(n, p["origin"][c], h, E, p["nu"], tol=1e-13)
+        o = np.rint(p["origin"][c] * nc).astype(int)
+        nodes = [(o[0] + (q & 1)) + s * ((o[1] + ((q >> 1) & 1)) + s * (o[2] + (q >> 2))) for q in range(8)]
+        idx = np.array([3 * v + a for v in nodes for a in range(3)])
+        K[np.ix_(idx, idx)] += Kc
+        if o[2] != nc - 1:
+            continue
+        b = basis.reshape(24, -1, 3)
+        for j in range(n):
+            for i in range(n):
+                for tri in (((i, j), (i + 1, j), (i + 1, j + 1)), ((i, j), (i, j + 1), (i + 1, j + 1))):
+                    for (a, bb) in tri:
+                        load[idx] += (b[:, a + sf * (bb + sf * n)] @ t) * A / 3
+    free = np.setdiff1d(np.arange(3 * s ** 3), np.concatenate([3 * fixed + a for a in range(3)]))
+    uc = np.zeros(3 * s ** 3)
+    uc[free] = np.linalg.solve(K[np.ix_(free, free)], load[free])
+    return uc
+
+
+@pytest.mark.parametrize("trace,form", [(mfs.TRACE_Q1, mfs.BASIS_PER_DIRECTION),
+                                        (mfs.TRACE_REFERENCE, mfs.BASIS_PER_DIRECTION),
+                                        (mfs.TRACE_REFERENCE, mfs.BASIS_COMPONENTWISE)])
+def test_msfem_vs_fem_error_metric(ctx, trace, form):
+    """Clamped bottom, top pressure, lognormal field on 2^3 cells of 4^3
+    hexes: the GPU MsFEM coarse solution equals the oracle's, the library's
+    error equals the numpy restatement of the reference's metric, and with
+    per-direction bases the z error is a modest fraction.  (With the
+    component-wise coarse functions the coarse operator is nearly singular
+    on this problem, so that error is large; it is still the oracle's.)"""
+    nc, n = 2, 4
+    p = problems.msfem_cells(nc, n_fixed=n, seed_E=11, sigma=0.5)
+    B = mfs.Batch(ctx, p["cell_n"], p["origin"], p["h"], p["E"], p["nu"], trace=trace, coarse_basis=form)
+    B.assemble()
+    B.solve(precision=mfs.PREC_MIXED, tol=1e-11)
+    s = nc + 1
+    t = np.array([0.0, 0.0, -1e6])
+    load = B.top_traction(t, s ** 3)
+    bottom = np.arange(s * s)
+    A, b, x = B.coarse_system(B.coarse_matrices(), load, bottom)
+    mfs.Solver(A, tol=1e-13, method=mfs.MINRES).run(b, x)
+    uc = x.download()
+    uc_o = oracle_coarse_solution(p, trace, form, t, bottom)
+    assert np.linalg.norm(uc - uc_o) / np.linalg.norm(uc_o) < 1e-6
+    fine = problems.elastic_cube(nc * n)
+    S = mfs.Assembly(ctx, mfs.PROBLEM_ELASTIC, fine["xyz"], fine["tets"], fixed_nodes=fine["fixed_nodes"],
+                     fixed_values=fine["fixed_values"], tris=fine["tris"], traction=fine["traction"])
+    S.run(problems.msfem_global_E(p), nu=p["nu"])
+    xf = mfs.Vec(ctx, S.A.n)
+    xf.copy_from(S.x0)
+    mfs.Solver(S.A, tol=1e-12, method=mfs.MINRES).run(S.b, xf)
+    u_fem = xf.download()
+    err = B.msfem_error(uc, u_fem)
+    ref = numpy_msfem_error(B, p, uc, u_fem)
+    print("MsFEM vs FEM error", trace, form, err)
+    assert np.allclose(err, ref, rtol=1e-12, atol=0)
+    assert np.all(np.isfinite(err))
+    if form == mfs.BASIS_PER_DIRECTION:
+        assert err[2] < 0.2, err

@@ -384,6 +384,153 @@ def test_msfem_cell_homogeneous_affine_reproduction():
     assert abs(u.reshape(-1, 3) - xyz @ G.T).max() < 1e-10
 
 
+# --- MsFEM reference-trace mode (edge -> face -> volume) ---------------------
+
+def series_chain(xyz, E, nu, u0, un):
+    """1-D chain of segments with stiffness blocks (E/L)((lam+mu) t t^T + mu I):
+    the flux is the same in every segment, so u(t) = u0 + (sum_{s<t} S_s^-1) c
+    with c = (sum_s S_s^-1)^-1 (un - u0)."""
+    lam, mu = lame(1.0, nu)
+    comp = []
+    for s in range(len(E)):
+        d = xyz[s + 1] - xyz[s]
+        L = np.linalg.norm(d)
+        t = d / L
+        comp.append(np.linalg.inv((E[s] / L) * ((lam + mu) * np.outer(t, t) + mu * np.eye(3))))
+    c = np.linalg.solve(sum(comp), np.asarray(un, float) - np.asarray(u0, float))
+    U = [np.asarray(u0, float)]
+    for S in comp:
+        U.append(U[-1] + S @ c)
+    return np.array(U)
+
+
+@pytest.mark.parametrize("direction", [(1.0, 0, 0), (0.3, -0.5, 0.8)])
+def test_edge_1d_matches_series_springs(direction):
+    rng = np.random.default_rng(5)
+    nseg = 7
+    t = np.asarray(direction) / np.linalg.norm(direction)
+    steps = rng.uniform(0.5, 1.5, nseg)  # unequal segment lengths
+    xyz = np.vstack([np.zeros(3), np.cumsum(steps)[:, None] * t[None, :]]) + np.array([0.2, -0.1, 0.4])
+    E = np.exp(rng.normal(size=nseg))
+    u0, un = [1.0, -0.5, 0.25], [0.0, 0.3, -1.0]
+    U = orc.edge_1d(xyz, E, NU, u0, un)
+    assert abs(U - series_chain(xyz, E, NU, u0, un)).max() < 1e-13
+
+
+def cst_plane_strain(P2, E, nu):
+    """Plane-strain constant-strain triangle (textbook) for the in-plane
+    components plus mu * P1 Laplacian for the normal component, 9x9 in
+    node-major order with components (x, y, normal)."""
+    lam, mu = lame(E, nu)
+    M = np.c_[np.ones(3), P2]
+    g = np.linalg.inv(M)[1:, :].T
+    A = 0.5 * abs(np.linalg.det(M))
+    Dp = np.array([[lam + 2 * mu, lam, 0], [lam, lam + 2 * mu, 0], [0, 0, mu]])
+    K = np.zeros((9, 9))
+    for I in range(3):
+        for J in range(3):
+            BI = np.array([[g[I, 0], 0], [0, g[I, 1]], [g[I, 1], g[I, 0]]])
+            BJ = np.array([[g[J, 0], 0], [0, g[J, 1]], [g[J, 1], g[J, 0]]])
+            K[3 * I:3 * I + 2, 3 * J:3 * J + 2] = A * BI.T @ Dp @ BJ
+            K[3 * I + 2, 3 * J + 2] = A * mu * g[I] @ g[J]
+    return K, A
+
+
+def test_triangle_element_is_rotated_plane_strain():
+    rng = np.random.default_rng(9)
+    P2 = np.array([[0.0, 0.0], [0.7, 0.1], [0.2, 0.6]])
+    Kp, A = cst_plane_strain(P2, 2.5, NU)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))  # rows: the plane's x, y axes and its normal
+    P3 = np.c_[P2, np.zeros(3)] @ Q + np.array([0.3, -0.2, 0.5])
+    K, area = orc.elem_tri(P3, 2.5, NU, convention=1)
+    R = np.kron(np.eye(3), Q.T)
+    assert area == pytest.approx(A, rel=1e-14)
+    assert abs(K - R @ Kp @ R.T).max() < 1e-13 * abs(Kp).max()
+    K0, _ = orc.elem_tri(P3, 2.5, NU, convention=0)
+    for I, J in itertools.product(range(3), range(3)):
+        assert abs(K0[3 * I:3 * I + 3, 3 * J:3 * J + 3] - K[3 * I:3 * I + 3, 3 * J:3 * J + 3].T).max() < 1e-13
+
+
+def test_segment_element_closed_form():
+    P = np.array([[0.1, 0.2, 0.3], [0.4, -0.1, 0.9]])
+    K, L = orc.elem_seg(P, 3.0, NU, convention=0)
+    d = P[1] - P[0]
+    t = d / np.linalg.norm(d)
+    lam, mu = lame(3.0, NU)
+    S = ((lam + mu) * np.outer(t, t) + mu * np.eye(3)) / L
+    assert L == pytest.approx(np.linalg.norm(d), rel=1e-15)
+    assert abs(K - np.kron(np.array([[1, -1], [-1, 1]]), S)).max() < 1e-13 * abs(S).max()
+
+
+def reftrace_cell(n, E):
+    return orc.msfem_cell_reftrace(n, (0.0, 0.0, 0.0), 1.0, E, NU, tol=1e-13)
+
+
+def test_reftrace_edges_are_harmonic_mean_profiles():
+    """On coarse edge x (y = z = 0) the basis of corner 0 driven by e_d is
+    e_d (1 - C(t)/C(n)), C(t) = sum_{s<t} 1/E_s, with E_s the modulus of the
+    tet owning the boundary triangle on the face y = 0 that holds segment s
+    (found here by brute force over the tets)."""
+    n = 4
+    E = orc.field_lognormal(6 * n ** 3, 1, 3, 0, 0.0, 1.0)
+    basis, _, _ = reftrace_cell(n, E)
+    xyz, tets = orc.mesh_box(n)
+    s = n + 1
+    Es = []
+    for t in range(n):
+        a, b = t, t + 1  # nodes (t,0,0), (t+1,0,0)
+        owner = [k for k in range(len(tets)) if a in tets[k] and b in tets[k]
+                 and sum(xyz[v][1] == 0 for v in tets[k]) == 3]
+        assert len(owner) == 1
+        Es.append(E[owner[0]])
+    C = np.concatenate([[0.0], np.cumsum(1.0 / np.array(Es))])
+    prof = 1 - C / C[-1]
+    for d in range(3):
+        phi = basis[d].reshape(-1, 3)
+        e = np.zeros(3)
+        e[d] = 1
+        for t in range(s):
+            assert abs(phi[t] - prof[t] * e).max() < 1e-13
+
+
+def test_reftrace_partition_of_unity_and_coarse_invariants():
+    n = 4
+    E = orc.field_lognormal(6 * n ** 3, 1, 4, 0, np.log(1e9), 1.0)
+    for form in (0, 1):
+        basis, Kc, _ = orc.msfem_cell_reftrace(n, (0, 0, 0), 0.5, E, NU, tol=1e-13, form=form)
+        N = basis.shape[1]
+        for d in range(3):
+            e = np.zeros(N)
+            e[d::3] = 1
+            assert abs(basis[d::3].sum(0) - e).max() < 1e-9, (form, d)
+        assert np.allclose(Kc, Kc.T, atol=1e-9 * abs(Kc).max())
+        for d in range(3):
+            t = np.zeros(24)
+            t[d::3] = 1
+            assert abs(Kc @ t).max() < 1e-8 * abs(Kc).max()
+
+
+def test_reftrace_face_traces_differ_from_hats_but_agree_for_homogeneous_affine():
+    """Homogeneous material: every stage of the hierarchy reproduces affine
+    data (the edge profile is linear, the face and volume problems have the
+    affine field as their discrete solution), so the bases reproduce affine
+    fields in both coarse forms; with a heterogeneous field the traces are
+    not the hats."""
+    n = 3
+    xyz, _ = orc.mesh_box(n)
+    G = np.array([[0.01, 0.002, 0], [0.001, -0.02, 0.003], [0, 0.004, 0.01]])
+    corners = np.array([[c & 1, (c >> 1) & 1, (c >> 2) & 1] for c in range(8)], dtype=float)
+    coef = (corners @ G.T).ravel()
+    for form in (0, 1):
+        basis, _, _ = orc.msfem_cell_reftrace(n, (0, 0, 0), 1.0, np.full(6 * n ** 3, 1e9), NU, tol=1e-13, form=form)
+        u = coef @ basis
+        assert abs(u.reshape(-1, 3) - xyz @ G.T).max() < 1e-10, form
+    E = orc.field_lognormal(6 * n ** 3, 1, 5, 0, 0.0, 1.0)
+    b_ref, _, _ = reftrace_cell(n, E)
+    b_q1, _, _ = orc.msfem_cell(n, (0, 0, 0), 1.0, E, NU, tol=1e-13)
+    assert abs(b_ref - b_q1).max() > 1e-3
+
+
 # --- MLP oracle gradient --------------------------------------------------------
 
 def test_mlp_oracle_gradient_central_differences():

@@ -326,13 +326,13 @@ def bench_msfem_reference_trace(mfs, ctx, p):
     B.assemble()
     t_asm = time.perf_counter() - t0
     B.assemble()  # second assemble: warm
-    it, bad, ms = B.trace_status()
+    worst, bad, ms = B.trace_status()
     wg, per_cu, var, _ = B.autotune(precision=mfs.PREC_MIXED, tol=1e-8)
     reps = [B.solve(precision=mfs.PREC_MIXED, tol=1e-8, use_graph=True) for _ in range(3)]
     r = min(reps, key=lambda x: x.seconds)
     Kc = B.coarse_matrices()
     out = {"edge_face_solves_ms": round(ms, 2), "face_problems": 6 * 12 * len(p["cell_n"]),
-           "face_max_iterations": it, "face_failed": bad, "first_assemble_wall_s": round(t_asm, 3),
+           "face_solver": "banded LDL^T in LDS (direct)", "face_max_relres": worst, "face_failed": bad, "first_assemble_wall_s": round(t_asm, 3),
            "mixed_batched_ms": round(r.seconds * 1e3, 2), "mean_iterations": round(r.mean_iterations, 1),
            "max_iterations": r.max_iterations, "max_true_relres": r.max_true_relres,
            "not_converged": r.n_not_converged, "shape": {"workgroup": wg, "workgroups_per_cu": per_cu, "variant": var},

@@ -39,7 +39,7 @@ enum {
 /* precisions / formats / methods */
 enum { MFS_F64 = 0, MFS_F32 = 1 };
 enum { MFS_FMT_CSR = 0, MFS_FMT_BSR3 = 1 };
-enum { MFS_PCG_JACOBI = 0, MFS_CG = 1, MFS_GMRES = 2, MFS_MINRES = 3 };
+enum { MFS_PCG_JACOBI = 0, MFS_CG = 1, MFS_GMRES = 2, MFS_MINRES = 3, MFS_BICGSTAB = 4 };
 enum { MFS_PREC_F64 = 0, MFS_PREC_MIXED = 1 };
 enum { MFS_PROBLEM_SCALAR = 0, MFS_PROBLEM_ELASTIC = 1 };
 /* Elastic element block convention.  REFERENCE: block(I,J) = V B_J^T D B_I,
@@ -130,7 +130,8 @@ int mfs_spmv_time(mfs_mat A, int prec, mfs_vec x, mfs_vec y, int reps, double* a
 
 /* ---- solvers ----------------------------------------------------------- */
 typedef struct {
-  int method;       /* MFS_PCG_JACOBI | MFS_CG | MFS_GMRES | MFS_MINRES      */
+  int method;       /* MFS_PCG_JACOBI | MFS_CG | MFS_GMRES | MFS_MINRES |
+                       MFS_BICGSTAB                                            */
   int precision;    /* MFS_PREC_F64 | MFS_PREC_MIXED                          */
   double tol;       /* stop when ||b - A x||_2 / ||b||_2 <= tol (FP64, true)  */
   int maxit;        /* total inner iterations cap                             */
@@ -154,10 +155,13 @@ void mfs_solve_desc_default(mfs_solve_desc* desc);
 /* Plan a solve on A: allocates work vectors and captures the iteration
  * chunk into HIP graphs.  MFS_MINRES (Jacobi-preconditioned, symmetric
  * indefinite A) serves the reference element convention's elastic operator,
- * which is indefinite for heterogeneous or free-surface problems.  Replaces the solver family of CSSD_Matrix
- * (MSG :1496-1663, BiCG_Diag :1181-1300, MSG_PreconditioningSSOR
- * :1302-1495, GMRES :1774-1989) together with the drivers' continuation
- * loops (FEM/FEM.h:626-695, :2399-2485). */
+ * which is indefinite for heterogeneous or free-surface problems.
+ * MFS_BICGSTAB (right Jacobi preconditioning) serves non-symmetric
+ * systems.  Replaces the solver family of CSSD_Matrix (MSG :1496-1663,
+ * BiCG_Diag :1181-1300, MSG_PreconditioningSSOR :1302-1495, GMRES
+ * :1774-1989, BiCG_Stab :730-853 / BiCG_Stab_Preconditioning :1058-1163)
+ * together with the drivers' continuation loops (FEM/FEM.h:626-695,
+ * :2399-2485). */
 int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solver* out);
 int mfs_solver_destroy(mfs_solver S);
 /* Solve A x = b; x holds the initial guess on entry (FP64 vectors). */
@@ -274,8 +278,8 @@ typedef struct {
   int convention;            /* MFS_ELEM_*                                    */
   int trace;                 /* MFS_TRACE_* (0: Q1 hats)                      */
   int coarse_basis;          /* MFS_BASIS_* (0: per direction)                */
-  double trace_tol;          /* reference traces: face solves to ||r||/||b||  */
-                             /* (0: 1e-13)                                    */
+  double trace_tol;          /* reference traces: largest accepted face-solve */
+                             /* residual ||r||/||b|| (0: 1e-11)               */
 } mfs_batch_desc;
 
 typedef struct {
@@ -312,12 +316,12 @@ int mfs_batch_cell_dofs(mfs_batch B, int cell, int64_t* n_interior, int64_t* n_t
  * (MsFEM_Grid.h:5176-5325 via FEM/FEM.h:4367-5178) that give the boundary
  * data of the volume problems. */
 int mfs_batch_assemble(mfs_batch B);
-/* Reference-trace mode, after mfs_batch_assemble: largest iteration count
- * of the face solves, the number of face problems (of 12 per face) that
- * missed trace_tol, and the device time of the edge + face launches (ms).
- * No reference counterpart (the reference prints a warning per solve,
- * FEM/FEM.h:4343-4351).  Any pointer may be NULL. */
-int mfs_batch_trace_status(mfs_batch B, int* max_face_iterations, int* n_face_failed, double* trace_ms);
+/* Reference-trace mode, after mfs_batch_assemble: largest relative
+ * residual ||b - K x|| / ||b|| of the face solves, the number of face
+ * problems (12 per face) above trace_tol, and the device time of the edge +
+ * face launches (ms).  No reference counterpart (the reference prints its
+ * best residual per solve, FEM/FEM.h:4343-4351).  Any pointer may be NULL. */
+int mfs_batch_trace_status(mfs_batch B, double* max_face_relres, int* n_face_failed, double* trace_ms);
 /* Solve the 24 local problems of every cell in one persistent launch. */
 int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* desc, mfs_batch_report* report);
 /* Initial guesses for the interior DOFs of every (cell, basis), FP32 device

@@ -14,7 +14,8 @@
 //   MSG                      -> CG (FP64)
 //   MSG_PreconditioningSSOR  -> Jacobi-PCG (FP64); the SSOR factor is unused
 //   BiCG_Diag                -> Jacobi-PCG (the matrices are symmetric)
-//   GMRES, BiCG_Stab family  -> GMRES(30) with Jacobi, FP64
+//   GMRES                    -> GMRES(30) with Jacobi, FP64
+//   BiCG_Stab family         -> BiCGStab with right Jacobi preconditioning, FP64
 // Set MFS_CSSD_MIXED=1 in the environment to run the mixed-precision path.
 //
 // Include it where the reference includes Library/CSSD_Matrix.h, after the
@@ -204,9 +205,9 @@ class CSSD_Matrix {
     return precond_residual(diag);
   }
   double GMRES(int maxiter, double E) { return solve_inplace(MFS_GMRES, maxiter, E, X, F); }
-  double BiCG_Stab(int maxiter, double E) { return solve_inplace(MFS_GMRES, maxiter, E, X, F); }
-  double BCG_Stab2(int maxiter, double E) { return solve_inplace(MFS_GMRES, maxiter, E, X, F); }
-  double BiCG_Stab_blocks(int maxiter, double E) { return solve_inplace(MFS_GMRES, maxiter, E, X, F); }
+  double BiCG_Stab(int maxiter, double E) { return solve_inplace(MFS_BICGSTAB, maxiter, E, X, F); }
+  double BCG_Stab2(int maxiter, double E) { return solve_inplace(MFS_BICGSTAB, maxiter, E, X, F); }
+  double BiCG_Stab_blocks(int maxiter, double E) { return solve_inplace(MFS_BICGSTAB, maxiter, E, X, F); }
 
   // Diagnostics of the last GPU solve.
   mfs_solve_report last_report{};

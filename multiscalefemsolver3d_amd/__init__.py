@@ -26,7 +26,7 @@ MFS_OK = 0
 ERRORS = {-1: "ARG", -2: "OOM", -3: "NOT_CONVERGED", -4: "NONFINITE", -5: "HIP", -6: "UNSUPPORTED",
           -7: "NO_DEVICE"}
 F64, F32 = 0, 1
-PCG_JACOBI, CG, GMRES, MINRES = 0, 1, 2, 3
+PCG_JACOBI, CG, GMRES, MINRES, BICGSTAB = 0, 1, 2, 3, 4
 PREC_F64, PREC_MIXED = 0, 1
 PROBLEM_SCALAR, PROBLEM_ELASTIC = 0, 1
 ELEM_REFERENCE, ELEM_TEXTBOOK = 0, 1
@@ -206,7 +206,7 @@ _sig("mfs_msfem_coarse_system", C.c_int, [vp, c_dp, c_dp, C.c_int64, c_i32p, c_d
 _sig("mfs_msfem_top_traction", C.c_int, [vp, c_dp, c_dp])
 _sig("mfs_msfem_downscale", C.c_int, [vp, c_dp, C.c_int, c_dp])
 _sig("mfs_msfem_error", C.c_int, [vp, c_dp, c_dp, c_dp])
-_sig("mfs_batch_trace_status", C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)])
+_sig("mfs_batch_trace_status", C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)])
 
 
 def ptr(a, ctype):
@@ -702,10 +702,10 @@ class Batch:
         return out
 
     def trace_status(self):
-        """Reference-trace mode: (largest face-solve iteration count, face
-        problems that missed the trace tolerance, device ms of the edge + face
+        """Reference-trace mode: (largest face-solve relative residual, face
+        problems above the trace tolerance, device ms of the edge + face
         launches in the last assemble)."""
-        it, bad, ms = C.c_int(), C.c_int(), C.c_double()
+        it, bad, ms = C.c_double(), C.c_int(), C.c_double()
         _chk(lib.mfs_batch_trace_status(self.h, C.byref(it), C.byref(bad), C.byref(ms)), "mfs_batch_trace_status")
         return it.value, bad.value, ms.value
 

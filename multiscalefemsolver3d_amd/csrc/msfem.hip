@@ -1065,72 +1065,117 @@ __global__ void k_edge_profiles(const Cell* __restrict__ cells, int ncells, cons
 }
 
 // ---------------------------------------------------------------------------
-// Reference-trace mode, step 2: face solves.  One workgroup of 192 threads
-// per (cell, face): the 2-D elastic problem on the face's (n-1)^2 interior
-// nodes (FEM_2D_forElasticDeformation, FEM.h:4367-5178) for 12 right-hand
-// sides (4 face corners x 3 drives) with FP64 Jacobi-PCG.  Element: per
-// boundary triangle area * block(gI, gJ) with the in-plane hat gradients
-// (FEM_Grid.h:4429-4500) and the owning tet's modulus.  Structured 7-point
-// face stencil, blocks in LDS; 16 lanes per right-hand side, so every
-// reduction is a 16-lane shuffle.
+// Reference-trace mode, step 2: face solves.  One workgroup per (cell,
+// face): the 2-D elastic problem on the face's (n-1)^2 interior nodes
+// (FEM_2D_forElasticDeformation, FEM.h:4367-5178) for 12 right-hand sides
+// (4 face corners x 3 drives).  Element: per boundary triangle area *
+// block(gI, gJ) with the in-plane hat gradients (FEM_Grid.h:4429-4500) and
+// the owning tet's modulus.  The reference iterates (SSOR-PCG to 1e-15);
+// here the same discrete system is factored: node-major DOFs 3w+a with
+// w = (u-1) + m(v-1) give a half bandwidth of 3m+5, so a banded LDL^T of the
+// whole face fits in LDS (<= 113 KB at n = 12) and costs ~N bw^2 / 2 flops
+// instead of ~150 iterations of 12 stencil products.  One barrier per
+// elimination step: the trailing update and the forward elimination of the
+// 12 right-hand sides use the unscaled pivot column; the column is scaled to
+// L one step later, when nothing reads it.  A final residual pass (blocks
+// recomputed per node) checks every solve against the trace tolerance.
 // ---------------------------------------------------------------------------
 constexpr int FS = 7;       // face stencil points
 constexpr int F_CENTER = 3;
 constexpr int FRHS = 12;    // face bases: 4 corners x 3 drives
-constexpr int FSEG = 16;    // lanes per face basis
-constexpr int FJ = 8;       // interior face nodes per lane: (MAXN-1)^2 <= FSEG*FJ
-constexpr int F_THREADS = FRHS * FSEG;
-static_assert((MAXN - 1) * (MAXN - 1) <= FSEG * FJ, "face nodes per lane");
+constexpr int F_THREADS = 256;
+constexpr int kMaxFaceBw = 3 * MAXN + 2;  // face_bw(MAXN - 1)
+static_assert(kMaxFaceBw * (kMaxFaceBw + 1) / 2 <= 3 * F_THREADS, "trailing-update entries per thread");
 __device__ constexpr int kFace[FS][2] = {{-1, -1}, {0, -1}, {-1, 0}, {0, 0}, {1, 0}, {0, 1}, {1, 1}};
+// The 6 triangles around a face node: vertex offsets (du, dv) from the node,
+// and the low corner of their square with the upper-triangle flag.  Lower
+// triangle of square (a,b): (a,b),(a+1,b),(a+1,b+1); upper: (a,b),(a,b+1),(a+1,b+1).
+__device__ constexpr int kTriV[6][3][2] = {{{0, 0}, {1, 0}, {1, 1}},     {{0, 0}, {0, 1}, {1, 1}},
+                                           {{-1, -1}, {0, -1}, {0, 0}},  {{-1, -1}, {-1, 0}, {0, 0}},
+                                           {{-1, 0}, {0, 0}, {0, 1}},    {{0, -1}, {0, 0}, {1, 0}}};
+__device__ constexpr int kTriSq[6][3] = {{0, 0, 0}, {0, 0, 1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, 0}, {0, -1, 1}};
+__device__ constexpr int kTriC[6] = {0, 0, 2, 2, 1, 1};  // the node's vertex index
 
-__device__ __forceinline__ int face_slot(int du, int dv) {
-  switch ((du + 1) + 3 * (dv + 1)) {
-    case 0: return 0;
-    case 1: return 1;
-    case 3: return 2;
-    case 4: return 3;
-    case 5: return 4;
-    case 7: return 5;
-    default: return 6;  // (1, 1)
+__host__ __device__ constexpr int face_slot(int du, int dv) {
+  return du == -1 ? (dv == -1 ? 0 : 2) : du == 0 ? (dv == -1 ? 1 : dv == 0 ? 3 : 5) : (dv == 0 ? 4 : 6);
+}
+
+// Stencil blocks of interior face node (u, v): Kb[slot][3a+b].
+__device__ __forceinline__ void face_node_blocks(const double* __restrict__ Ecell, int n, int ax, int side, int A1,
+                                                 int A2, double hf, double lamf, double muf, int conv, int u, int v,
+                                                 double (&Kb)[FS][9]) {
+#pragma unroll
+  for (int q = 0; q < FS; q++)
+#pragma unroll
+    for (int e = 0; e < 9; e++) Kb[q][e] = 0.0;
+#pragma unroll
+  for (int tr = 0; tr < 6; tr++) {
+    // 2-D hat gradients in (u, v) units: g_i = perp(P_{i+2} - P_{i+1}) / (2A)
+    const double area2 = (double)((kTriV[tr][1][0] - kTriV[tr][0][0]) * (kTriV[tr][2][1] - kTriV[tr][0][1]) -
+                                  (kTriV[tr][1][1] - kTriV[tr][0][1]) * (kTriV[tr][2][0] - kTriV[tr][0][0]));
+    double g[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+      const double gu = -(double)(kTriV[tr][k2][1] - kTriV[tr][k1][1]) / (area2 * hf);
+      const double gv = (double)(kTriV[tr][k2][0] - kTriV[tr][k1][0]) / (area2 * hf);
+      // embed: component A1 <- gu, A2 <- gv, face axis <- 0
+      g[k][0] = A1 == 0 ? gu : (A2 == 0 ? gv : 0.0);
+      g[k][1] = A1 == 1 ? gu : (A2 == 1 ? gv : 0.0);
+      g[k][2] = A1 == 2 ? gu : (A2 == 2 ? gv : 0.0);
+    }
+    const double area = 0.5 * fabs(area2) * hf * hf;
+    const double Et =
+        Ecell[face_tri_tet(n, ax, side, u + kTriSq[tr][0], v + kTriSq[tr][1], kTriSq[tr][2])];
+    const int ic = kTriC[tr];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      elastic_block(g[ic], g[k], area * Et * lamf, area * Et * muf, conv, blk);
+      const int q = face_slot(kTriV[tr][k][0] - kTriV[tr][ic][0], kTriV[tr][k][1] - kTriV[tr][ic][1]);
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Kb[q][3 * a + b] += blk[a][b];
+    }
   }
 }
 
-__device__ __forceinline__ double seg_sum(double v) {
-  v += __shfl_xor(v, 8, FSEG);
-  v += __shfl_xor(v, 4, FSEG);
-  v += __shfl_xor(v, 2, FSEG);
-  v += __shfl_xor(v, 1, FSEG);
-  return v;
+__host__ __device__ __forceinline__ int face_bw(int m) { return 3 * (m + 1) + 2; }
+size_t face_lds_bytes(int max_n) {
+  const int m = max_n - 1, N = 3 * m * m, bw = face_bw(m);
+  return sizeof(double) * ((size_t)N * (bw + 1) + (size_t)N * FRHS + 2 * FRHS);
 }
 
-size_t face_lds_bytes(int max_n) {
-  const int nf = (max_n - 1) * (max_n - 1);
-  return (size_t)nf * (FS * 9 + 3 * FRHS + 3) * sizeof(double) + FRHS * sizeof(int);
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 __global__ __launch_bounds__(F_THREADS) void k_face_traces(const Cell* __restrict__ cells,
                                                            const int* __restrict__ order,
                                                            const double* __restrict__ E, double nu, double h,
                                                            int conv, const double* __restrict__ et,
-                                                           double* __restrict__ ft, double tol2, int* stats) {
+                                                           double* __restrict__ ft, double tol2,
+                                                           unsigned long long* worst, int* nfail) {
   extern __shared__ double lds[];
   const int cell = order[blockIdx.x / 6], f = blockIdx.x % 6;
   const Cell C = cells[cell];
-  const int n = C.n, m = n - 1, nf = m * m, s = n + 1;
+  const int n = C.n, m = n - 1, nf = m * m, s = n + 1, N = 3 * nf, bw = face_bw(m), ld = bw + 1;
   const int ax = f >> 1, side = f & 1;
   int A1, A2;
   face_axes(ax, A1, A2);
-  double* K = lds;                      // [nf][FS][9]
-  double* P = K + (size_t)nf * FS * 9;  // [nf][3][FRHS]
-  double* dinv = P + (size_t)nf * 3 * FRHS;
-  int* flags = (int*)(dinv + 3 * nf);
+  double* B = lds;                    // band, row i: columns i-bw .. i at B[i*ld + j-i+bw]
+  double* X = B + (size_t)N * ld;     // [N][FRHS]: right-hand sides, then solutions
+  double* red = X + (size_t)N * FRHS; // [2][FRHS]: r.r, b.b
   double* FT = ft + C.ft0 + (int64_t)f * s * s * FT_NODE;
   const double* ET = et + C.et0;
+  const double* Ecell = E + C.tet0;
   const double hf = h / n;
   const double lamf = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), muf = 1.0 / (2.0 * (1.0 + nu));
   const int tid = threadIdx.x;
 
-  // (a) face-boundary nodes: edge profiles times e_d
+  // (a) face-boundary nodes: edge profiles times e_d; zero the band
   for (int w = tid; w < s * s; w += blockDim.x) {
     const int u = w % s, v = w / s;
     if (u > 0 && u < n && v > 0 && v < n) continue;
@@ -1154,170 +1199,177 @@ __global__ __launch_bounds__(F_THREADS) void k_face_traces(const Cell* __restric
       for (int lc = 0; lc < 4; lc++)
         for (int d = 0; d < 3; d++) o[a * 12 + 3 * lc + d] = a == d ? val[lc] : 0.0;
   }
-  // (b) stencil blocks of the interior face nodes from their 6 triangles:
-  // squares (0,0) lower/upper, (-1,-1) lower/upper, (-1,0) lower, (0,-1) upper
-  // (offsets of the square's low corner, centre's vertex index in the triangle)
-  for (int w = tid; w < nf; w += blockDim.x) {
-    const int u = w % m + 1, v = w / m + 1;
-    double* Kw = K + (size_t)w * FS * 9;  // this thread's node only: accumulate in place
-    for (int q = 0; q < FS * 9; q++) Kw[q] = 0.0;
-    const int sq[6][3] = {{0, 0, 0}, {0, 0, 1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, 0}, {0, -1, 1}};
-    for (int tr = 0; tr < 6; tr++) {
-      const int a0 = u + sq[tr][0], b0 = v + sq[tr][1], up = sq[tr][2];
-      // vertices in face coordinates: lower (a,b),(a+1,b),(a+1,b+1); upper (a,b),(a,b+1),(a+1,b+1)
-      const int vu[3] = {a0, up ? a0 : a0 + 1, a0 + 1};
-      const int vv[3] = {b0, up ? b0 + 1 : b0, b0 + 1};
-      double X[3][3];
-      int ic = 0;
-      for (int k = 0; k < 3; k++) {
-        X[k][ax] = hf * (side ? n : 0);
-        X[k][A1] = hf * vu[k];
-        X[k][A2] = hf * vv[k];
-        if (vu[k] == u && vv[k] == v) ic = k;
-      }
-      // in-plane hat gradients: g_i = N x (X_{i+2} - X_{i+1}) / |N|^2
-      double e1[3], e2[3], N[3];
-      for (int a = 0; a < 3; a++) {
-        e1[a] = X[1][a] - X[0][a];
-        e2[a] = X[2][a] - X[0][a];
-      }
-      N[0] = e1[1] * e2[2] - e1[2] * e2[1];
-      N[1] = e1[2] * e2[0] - e1[0] * e2[2];
-      N[2] = e1[0] * e2[1] - e1[1] * e2[0];
-      const double nn2 = N[0] * N[0] + N[1] * N[1] + N[2] * N[2];
-      double gr[3][3];
-      for (int k = 0; k < 3; k++) {
-        const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
-        double ed[3];
-        for (int a = 0; a < 3; a++) ed[a] = X[k2][a] - X[k1][a];
-        gr[k][0] = (N[1] * ed[2] - N[2] * ed[1]) / nn2;
-        gr[k][1] = (N[2] * ed[0] - N[0] * ed[2]) / nn2;
-        gr[k][2] = (N[0] * ed[1] - N[1] * ed[0]) / nn2;
-      }
-      const double area = 0.5 * sqrt(nn2);
-      const double Et = E[C.tet0 + face_tri_tet(n, ax, side, a0, b0, up)];
-      for (int k = 0; k < 3; k++) {
-        double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        elastic_block(gr[ic], gr[k], area * Et * lamf, area * Et * muf, conv, blk);
-        const int q = face_slot(vu[k] - u, vv[k] - v);
-        for (int a = 0; a < 3; a++)
-          for (int b = 0; b < 3; b++) Kw[q * 9 + 3 * a + b] += blk[a][b];
-      }
-    }
-    for (int a = 0; a < 3; a++) dinv[3 * w + a] = 1.0 / Kw[F_CENTER * 9 + 4 * a];
-  }
-  __syncthreads();  // K, dinv in LDS; boundary traces written (visible to the workgroup)
+  for (int e = tid; e < N * ld; e += blockDim.x) B[e] = 0.0;
+  if (tid < 2 * FRHS) red[tid] = 0.0;
+  __syncthreads();  // boundary traces visible to the workgroup; band zeroed
 
-  // (c) Jacobi-PCG, right-hand side r = tid / FSEG, nodes l + FSEG*j
-  const int r = tid / FSEG, l = tid % FSEG;
-  double x[FJ][3], res[FJ][3];
-  double bb = 0.0, rz = 0.0;
+  // (b) band (lower triangle) and right-hand sides, one thread per node
+  static_assert((MAXN - 1) * (MAXN - 1) <= F_THREADS, "one face node per thread");
+  if (tid < nf) {
+    const int w = tid, u = w % m + 1, v = w / m + 1;
+    double Kb[FS][9];
+    face_node_blocks(Ecell, n, ax, side, A1, A2, hf, lamf, muf, conv, u, v, Kb);
 #pragma unroll
-  for (int j = 0; j < FJ; j++) {
-    const int w = l + FSEG * j;
-    for (int a = 0; a < 3; a++) x[j][a] = res[j][a] = 0.0;
-    if (w >= nf) continue;
-    const int u = w % m + 1, v = w / m + 1;
     for (int q = 0; q < FS; q++) {
       const int nu_ = u + kFace[q][0], nv = v + kFace[q][1];
-      if (nu_ > 0 && nu_ < n && nv > 0 && nv < n) continue;
-      const double* g = FT + ((int64_t)nu_ + s * nv) * FT_NODE + r;
-      const double* kq = K + ((size_t)w * FS + q) * 9;
-      for (int a = 0; a < 3; a++) res[j][a] -= kq[3 * a] * g[0] + kq[3 * a + 1] * g[12] + kq[3 * a + 2] * g[24];
+      if (!(nu_ > 0 && nu_ < n && nv > 0 && nv < n)) continue;
+      const int wn = (nu_ - 1) + m * (nv - 1);
+      if (wn > w) continue;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          const int i = 3 * w + a, j = 3 * wn + b;
+          if (j <= i) B[(size_t)i * ld + (j - i + bw)] = Kb[q][3 * a + b];
+        }
     }
-    for (int a = 0; a < 3; a++) {
-      const double z = dinv[3 * w + a] * res[j][a];
-      P[((size_t)w * 3 + a) * FRHS + r] = z;
-      bb += res[j][a] * res[j][a];
-      rz += res[j][a] * z;
-    }
-  }
-  bb = seg_sum(bb);
-  rz = seg_sum(rz);
-  const int cap = 4 * 3 * nf + 100;
-  bool done = !(bb > 0.0);
-  int it = 0;
-  bool ok = true;
-  if (l == 0) flags[r] = done;
-  __syncthreads();
-  for (;;) {
-    bool all = true;
-    for (int q = 0; q < FRHS; q++) all = all && flags[q];
-    if (all) break;
-    double qv[FJ][3];
-    double pq = 0.0;
+    for (int r0 = 0; r0 < FRHS; r0 += 4) {  // b = -K_IB g, four right-hand sides at a time
+      double rhs[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
-    for (int j = 0; j < FJ; j++) {
-      const int w = l + FSEG * j;
-      qv[j][0] = qv[j][1] = qv[j][2] = 0.0;
-      if (w >= nf) continue;
-      const int u = w % m + 1, v = w / m + 1;
       for (int q = 0; q < FS; q++) {
         const int nu_ = u + kFace[q][0], nv = v + kFace[q][1];
-        if (nu_ <= 0 || nu_ >= n || nv <= 0 || nv >= n) continue;
-        const int wn = (nu_ - 1) + m * (nv - 1);
-        const double p0 = P[((size_t)wn * 3 + 0) * FRHS + r], p1 = P[((size_t)wn * 3 + 1) * FRHS + r],
-                     p2 = P[((size_t)wn * 3 + 2) * FRHS + r];
-        const double* kq = K + ((size_t)w * FS + q) * 9;
-        for (int a = 0; a < 3; a++) qv[j][a] += kq[3 * a] * p0 + kq[3 * a + 1] * p1 + kq[3 * a + 2] * p2;
+        if (nu_ > 0 && nu_ < n && nv > 0 && nv < n) continue;
+        const double* gv = FT + ((int64_t)nu_ + s * nv) * FT_NODE + r0;
+        for (int a = 0; a < 3; a++)
+          for (int r = 0; r < 4; r++)
+            rhs[a][r] -= Kb[q][3 * a] * gv[r] + Kb[q][3 * a + 1] * gv[12 + r] + Kb[q][3 * a + 2] * gv[24 + r];
       }
-      for (int a = 0; a < 3; a++) pq += P[((size_t)w * 3 + a) * FRHS + r] * qv[j][a];
+      for (int a = 0; a < 3; a++)
+        for (int r = 0; r < 4; r++) X[(size_t)(3 * w + a) * FRHS + r0 + r] = rhs[a][r];
     }
-    pq = seg_sum(pq);
-    const double alpha = done ? 0.0 : rz / pq;
-    double rzn = 0.0, rr = 0.0;
-#pragma unroll
-    for (int j = 0; j < FJ; j++) {
-      const int w = l + FSEG * j;
-      if (w >= nf) continue;
-      for (int a = 0; a < 3; a++) {
-        x[j][a] += alpha * P[((size_t)w * 3 + a) * FRHS + r];
-        res[j][a] -= alpha * qv[j][a];
-        qv[j][a] = dinv[3 * w + a] * res[j][a];  // z
-        rzn += res[j][a] * qv[j][a];
-        rr += res[j][a] * res[j][a];
+  }
+  __syncthreads();
+
+  // (c) banded LDL^T with the forward elimination of the 12 right-hand sides
+  const int npairs = bw * (bw + 1) / 2;
+  int pdi[3], pdj[3];  // this thread's trailing-update entries (di >= dj >= 1)
+  for (int c = 0; c < 3; c++) {
+    const int p = tid + F_THREADS * c;
+    pdi[c] = pdj[c] = 0;
+    if (p < npairs) {
+      int di = 1, base = 0;
+      while (base + di <= p) {
+        base += di;
+        di++;
       }
+      pdi[c] = di;
+      pdj[c] = p - base + 1;
     }
-    rzn = seg_sum(rzn);
-    rr = seg_sum(rr);
-    const double beta = rzn / rz;
-    __syncthreads();  // every read of P done
-    if (!done) {
-#pragma unroll
-      for (int j = 0; j < FJ; j++) {
-        const int w = l + FSEG * j;
-        if (w >= nf) continue;
-        for (int a = 0; a < 3; a++) {
-          double* pp = P + ((size_t)w * 3 + a) * FRHS + r;
-          *pp = qv[j][a] + beta * *pp;
-        }
-      }
-      rz = rzn;
-      it++;
-      const bool conv_ = rr <= tol2 * bb;
-      const bool bad = !(rr == rr) || !(pq == pq);
-      if (conv_ || bad || it >= cap) {
-        done = true;
-        ok = conv_ && !bad;
-      }
+  }
+  for (int k = 0; k < N; k++) {
+    const double dk = B[(size_t)k * ld + bw];
+    const double inv = 1.0 / dk;
+    for (int c = 0; c < 3; c++) {
+      const int i = k + pdi[c], j = k + pdj[c];
+      if (pdi[c] == 0 || i >= N) continue;
+      B[(size_t)i * ld + (j - i + bw)] -= B[(size_t)i * ld + (k - i + bw)] * B[(size_t)j * ld + (k - j + bw)] * inv;
     }
-    if (l == 0) flags[r] = done;
+    for (int t = tid; t < bw * FRHS; t += F_THREADS) {
+      const int i = k + 1 + t / FRHS, r = t % FRHS;
+      if (i < N) X[(size_t)i * FRHS + r] -= B[(size_t)i * ld + (k - i + bw)] * inv * X[(size_t)k * FRHS + r];
+    }
+    if (k > 0 && tid < bw) {  // scale column k-1 to L (no one reads it in this step)
+      const int i = k + tid;
+      if (i < N) B[(size_t)i * ld + (k - 1 - i + bw)] /= B[(size_t)(k - 1) * ld + bw];
+    }
     __syncthreads();
   }
-  // (d) interior face values
-#pragma unroll
-  for (int j = 0; j < FJ; j++) {
-    const int w = l + FSEG * j;
-    if (w >= nf) continue;
-    const int u = w % m + 1, v = w / m + 1;
-    double* o = FT + ((int64_t)u + s * v) * FT_NODE + r;
-    for (int a = 0; a < 3; a++) o[12 * a] = x[j][a];
+  for (int t = tid; t < N * FRHS; t += F_THREADS) X[t] /= B[(size_t)(t / FRHS) * ld + bw];
+  __syncthreads();
+  for (int k = N - 1; k > 0; k--) {  // L^T x = y
+    for (int t = tid; t < bw * FRHS; t += F_THREADS) {
+      const int i = k - 1 - t / FRHS, r = t % FRHS;
+      if (i >= 0 && i >= k - bw) X[(size_t)i * FRHS + r] -= B[(size_t)k * ld + (i - k + bw)] * X[(size_t)k * FRHS + r];
+    }
+    __syncthreads();
   }
-  if (l == 0) {
-    atomicMax(stats, it);
-    if (!ok) atomicAdd(stats + 1, 1);
+
+  // (d) interior face values, residual check r = b - K x, and iterative
+  // refinement with the same factor for faces whose residual is above the
+  // tolerance (nf <= F_THREADS: one node per thread)
+  const bool own = tid < nf;
+  const int uo = own ? tid % m + 1 : 1, vo = own ? tid / m + 1 : 1;
+  double Kb[FS][9];
+  face_node_blocks(Ecell, n, ax, side, A1, A2, hf, lamf, muf, conv, uo, vo, Kb);
+  for (int pass = 0;; pass++) {
+    if (own) {
+      double* o = FT + ((int64_t)uo + s * vo) * FT_NODE;
+      for (int a = 0; a < 3; a++)
+        for (int r = 0; r < FRHS; r++)
+          o[12 * a + r] = (pass ? o[12 * a + r] : 0.0) + X[(size_t)(3 * tid + a) * FRHS + r];
+    }
+    if (tid < 2 * FRHS) red[tid] = 0.0;
+    __syncthreads();  // x complete in FT; X free for the residual
+    for (int r0 = 0; r0 < FRHS; r0 += 4) {
+      double rr[4] = {0, 0, 0, 0}, bb[4] = {0, 0, 0, 0};
+      if (own) {
+        double res[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+        double bvec[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+        for (int q = 0; q < FS; q++) {
+          const int nu_ = uo + kFace[q][0], nv = vo + kFace[q][1];
+          const bool in = nu_ > 0 && nu_ < n && nv > 0 && nv < n;
+          const double* xn = FT + ((int64_t)nu_ + s * nv) * FT_NODE + r0;
+          for (int b = 0; b < 3; b++)
+            for (int r = 0; r < 4; r++) {
+              const double xv = xn[12 * b + r];
+              for (int a = 0; a < 3; a++) {
+                res[a][r] -= Kb[q][3 * a + b] * xv;
+                if (!in) bvec[a][r] -= Kb[q][3 * a + b] * xv;
+              }
+            }
+        }
+        for (int a = 0; a < 3; a++)
+          for (int r = 0; r < 4; r++) {
+            // res = -K_II x - K_IB g = b - K_II x (b = -K_IB g = bvec)
+            X[(size_t)(3 * tid + a) * FRHS + r0 + r] = res[a][r];
+            rr[r] += res[a][r] * res[a][r];
+            bb[r] += bvec[a][r] * bvec[a][r];
+          }
+      }
+      for (int r = 0; r < 4; r++) {
+        rr[r] = wave_sum(rr[r]);
+        bb[r] = wave_sum(bb[r]);
+      }
+      if ((tid & 63) == 0)
+        for (int r = 0; r < 4; r++) {
+          atomicAdd(&red[r0 + r], rr[r]);
+          atomicAdd(&red[FRHS + r0 + r], bb[r]);
+        }
+    }
+    __syncthreads();
+    bool bad = false;
+    for (int r = 0; r < FRHS; r++) {
+      const double rel2 = red[FRHS + r] > 0.0 ? red[r] / red[FRHS + r] : red[r];
+      bad = bad || !(rel2 <= tol2);
+    }
+    if (!bad || pass == 2) break;
+    // L D L^T delta = r (X), then x += delta at the top of the next pass
+    for (int k = 0; k + 1 < N; k++) {
+      for (int t = tid; t < bw * FRHS; t += F_THREADS) {
+        const int i = k + 1 + t / FRHS, r = t % FRHS;
+        if (i < N) X[(size_t)i * FRHS + r] -= B[(size_t)i * ld + (k - i + bw)] * X[(size_t)k * FRHS + r];
+      }
+      __syncthreads();
+    }
+    for (int t = tid; t < N * FRHS; t += F_THREADS) X[t] /= B[(size_t)(t / FRHS) * ld + bw];
+    __syncthreads();
+    for (int k = N - 1; k > 0; k--) {
+      for (int t = tid; t < bw * FRHS; t += F_THREADS) {
+        const int i = k - 1 - t / FRHS, r = t % FRHS;
+        if (i >= 0 && i >= k - bw) X[(size_t)i * FRHS + r] -= B[(size_t)k * ld + (i - k + bw)] * X[(size_t)k * FRHS + r];
+      }
+      __syncthreads();
+    }
+  }
+  if (tid < FRHS) {
+    const double rr = red[tid], bb = red[FRHS + tid];
+    const double rel2 = bb > 0.0 ? rr / bb : rr;
+    const bool ok = rel2 == rel2 && rel2 <= tol2;
+    if (!ok) atomicAdd(nfail, 1);
+    const double rel = rel2 == rel2 ? sqrt(rel2) : 1e300;
+    atomicMax(worst, (unsigned long long)__double_as_longlong(rel));
   }
 }
+
 
 }  // namespace
 
@@ -1338,9 +1390,10 @@ struct mfs_batch_s {
   int tuned_wg[2] = {0, 0}, tuned_per_cu[2] = {0, 0};  // per precision (0 mixed, 1 FP64)
   int tuned_var[2] = {-1, -1};                        // kernel variant (rows per thread)
   int trace = MFS_TRACE_Q1, form = MFS_BASIS_PER_DIRECTION;
-  double trace_tol = 1e-13;
+  double trace_tol = 1e-11;  // face solves: FP64 residual floor of ill-conditioned faces is ~1e-12
   int64_t et_total = 0, ft_total = 0;
-  int face_stats[2] = {0, 0};  // max face iterations, failed face problems
+  double face_worst = 0.0;  // largest relative residual of the face solves
+  int face_failed = 0;       // face solves above trace_tol
   double trace_ms = 0.0;       // device time of the edge + face launches
   // device
   Cell* d_cells = nullptr;
@@ -1366,7 +1419,7 @@ struct mfs_batch_s {
   double* d_et = nullptr;  // reference traces: edge profiles
   double* d_ft = nullptr;  // reference traces: face traces
   int* d_face_order = nullptr;
-  int* d_fstats = nullptr;
+  unsigned long long* d_fstats = nullptr;  // [0] worst relative residual bits, [1] failures
 };
 
 static int batch_free(mfs_batch B) {
@@ -1521,7 +1574,7 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
     alloc((void**)&B->d_et, sizeof(double) * B->et_total);
     alloc((void**)&B->d_ft, sizeof(double) * B->ft_total);
     alloc((void**)&B->d_face_order, sizeof(int) * d->n_cells);
-    alloc((void**)&B->d_fstats, sizeof(int) * 2);
+    alloc((void**)&B->d_fstats, sizeof(unsigned long long) * 2);
   }
   if (st != MFS_OK) {
     batch_free(B);
@@ -1561,7 +1614,7 @@ extern "C" int mfs_batch_assemble(mfs_batch B) {
   hipStream_t s = B->ctx->stream;
   if (B->trace == MFS_TRACE_REFERENCE) {
     // edge profiles, then the face solves (boundary data of the volume problems)
-    MFS_HIP(hipMemsetAsync(B->d_fstats, 0, sizeof(int) * 2, s));
+    MFS_HIP(hipMemsetAsync(B->d_fstats, 0, sizeof(unsigned long long) * 2, s));
     hipEvent_t ev[2];
     MFS_HIP(hipEventCreate(&ev[0]));
     MFS_HIP(hipEventCreate(&ev[1]));
@@ -1570,14 +1623,19 @@ extern "C" int mfs_batch_assemble(mfs_batch B) {
     MFS_HIP(hipGetLastError());
     k_face_traces<<<B->ncells * 6, F_THREADS, face_lds_bytes(B->max_n), s>>>(
         B->d_cells, B->d_face_order, B->d_E, B->nu, B->h, B->conv, B->d_et, B->d_ft, B->trace_tol * B->trace_tol,
-        B->d_fstats);
+        B->d_fstats, (int*)(B->d_fstats + 1));
     MFS_HIP(hipGetLastError());
     MFS_HIP(hipEventRecord(ev[1], s));
-    MFS_HIP(hipMemcpyAsync(B->face_stats, B->d_fstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-    MFS_HIP(hipEventSynchronize(ev[1]));
+    unsigned long long hst[2];
+    MFS_HIP(hipMemcpyAsync(hst, B->d_fstats, sizeof(hst), hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     MFS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     B->trace_ms = ms;
+    double worst;
+    std::memcpy(&worst, &hst[0], sizeof(double));
+    B->face_worst = worst;
+    B->face_failed = (int)(hst[1] & 0xffffffffu);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
   }
@@ -1591,13 +1649,13 @@ extern "C" int mfs_batch_assemble(mfs_batch B) {
   return MFS_OK;
 }
 
-extern "C" int mfs_batch_trace_status(mfs_batch B, int* max_face_iterations, int* n_face_failed,
+extern "C" int mfs_batch_trace_status(mfs_batch B, double* max_face_relres, int* n_face_failed,
                                       double* trace_ms) {
   MFS_CHECK_ARG(B, "mfs_batch_trace_status: NULL batch");
   MFS_CHECK_ARG(B->trace == MFS_TRACE_REFERENCE, "mfs_batch_trace_status: batch is not in the reference-trace mode");
   MFS_CHECK_ARG(B->assembled, "mfs_batch_trace_status: call mfs_batch_assemble first");
-  if (max_face_iterations) *max_face_iterations = B->face_stats[0];
-  if (n_face_failed) *n_face_failed = B->face_stats[1];
+  if (max_face_relres) *max_face_relres = B->face_worst;
+  if (n_face_failed) *n_face_failed = B->face_failed;
   if (trace_ms) *trace_ms = B->trace_ms;
   return MFS_OK;
 }

@@ -534,6 +534,245 @@ __global__ void k_minres_start(const T* __restrict__ r, T* __restrict__ v, T* __
 }
 
 // ---------------------------------------------------------------------------
+// BiCGStab with right Jacobi preconditioning (van der Vorst 1992; the
+// reference's BiCG_Stab_Preconditioning, CSSD_Matrix.h:1058-1163, with its
+// inner M-solve replaced by M = diag(A)):
+//   p^ = M^-1 p, v = A p^, alpha = rho / (r0.v), s = r - alpha v,
+//   s^ = M^-1 s, t = A s^, omega = (t.s) / (t.t), x += alpha p^ + omega s^,
+//   r = s - omega t, rho' = r0.r, beta = (rho'/rho)(alpha/omega),
+//   p = r + beta (p - omega v).
+// Scalars stay on the device: each reduction ends in k_finish_sums (block
+// order, deterministic), a one-thread state kernel updates rho / beta and
+// the stop flag, and every kernel returns at once when the flag is set, so a
+// chunk of iterations is one hipGraph launch.
+// ---------------------------------------------------------------------------
+struct BsState {
+  double rho, tol2, rr, alpha, omega, beta;
+  int done, iter, maxit;
+};
+enum { BS_RUN = 0, BS_CONVERGED = 1, BS_BREAKDOWN = 2, BS_MAXIT = 3 };
+
+// y = A (dinv o u); per-block partials of w1.y (and y.w2 when W2)
+template <typename T, int BS, bool W2>
+__global__ __launch_bounds__(256) void k_bs_spmv(const int64_t* __restrict__ slice_base,
+                                                 const int32_t* __restrict__ slice_width,
+                                                 const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                 const T* __restrict__ u, const T* __restrict__ dinv,
+                                                 const T* __restrict__ w1, T* __restrict__ y, int64_t nb,
+                                                 int64_t n_slices, const BsState* st, double* __restrict__ part) {
+  if (st->done) return;
+  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t r = slice * 64 + lane;
+  T acc[BS];
+  if (slice < n_slices) {
+    sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb,
+                      [&](int64_t i) { return dinv ? dinv[i] * u[i] : u[i]; }, acc);
+  } else {
+#pragma unroll
+    for (int a = 0; a < BS; a++) acc[a] = 0;
+  }
+  T red[2] = {0, 0};
+  if (r < nb) {
+#pragma unroll
+    for (int a = 0; a < BS; a++) {
+      const int64_t i = BS * r + a;
+      y[i] = acc[a];
+      red[0] += w1[i] * acc[a];
+      if (W2) red[1] += acc[a] * acc[a];
+    }
+  }
+  if (W2)
+    block_partials<T, 2>(red, part);
+  else {
+    T one[1] = {red[0]};
+    block_partials<T, 1>(one, part);
+  }
+}
+
+// s = r - alpha v, alpha = rho / (r0.v)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bs_s(const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ s,
+                                              int64_t n, const BsState* st, const double* __restrict__ tot) {
+  if (st->done) return;
+  const T alpha = (T)(st->rho / tot[0]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s[i] = r[i] - alpha * v[i];
+}
+
+// omega = (t.s)/(t.t); x += alpha dinv p + omega dinv s; r = s - omega t;
+// per-block partials of r0.r and r.r
+template <typename T>
+__global__ __launch_bounds__(256) void k_bs_update(T* __restrict__ x, const T* __restrict__ p, const T* __restrict__ s,
+                                                   const T* __restrict__ t, T* __restrict__ r,
+                                                   const T* __restrict__ r0, const T* __restrict__ dinv, int64_t n,
+                                                   const BsState* st, const double* __restrict__ tot,
+                                                   double* __restrict__ part) {
+  if (st->done) return;
+  const T alpha = (T)(st->rho / tot[0]), omega = (T)(tot[1] / tot[2]);
+  T red[2] = {0, 0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T di = dinv ? dinv[i] : (T)1;
+    x[i] += di * (alpha * p[i] + omega * s[i]);
+    const T ri = s[i] - omega * t[i];
+    r[i] = ri;
+    red[0] += r0[i] * ri;
+    red[1] += ri * ri;
+  }
+  block_partials<T, 2>(red, part);
+}
+
+// tot: [0] r0.v, [1] t.s, [2] t.t, [3] r0.r, [4] r.r
+__global__ void k_bs_state(BsState* st, const double* __restrict__ tot) {
+  if (st->done) return;
+  const double alpha = st->rho / tot[0], omega = tot[1] / tot[2], rho = tot[3], rr = tot[4];
+  st->iter++;
+  st->rr = rr;
+  st->alpha = alpha;
+  st->omega = omega;
+  if (!isfinite(alpha) || !isfinite(omega) || !isfinite(rho) || !isfinite(rr))
+    st->done = BS_BREAKDOWN;
+  else if (rr <= st->tol2)
+    st->done = BS_CONVERGED;
+  else if (st->iter >= st->maxit)
+    st->done = BS_MAXIT;
+  else if (rho == 0.0 || omega == 0.0)
+    st->done = BS_BREAKDOWN;
+  if (st->done) return;
+  st->beta = (rho / st->rho) * (alpha / omega);
+  st->rho = rho;
+}
+
+// p = r + beta (p - omega v)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bs_p(T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ v,
+                                              int64_t n, const BsState* st) {
+  if (st->done) return;
+  const T beta = (T)st->beta, omega = (T)st->omega;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// r0 = p = r (the right-hand side; x starts at 0); partial of r.r
+template <typename T>
+__global__ __launch_bounds__(256) void k_bs_start(const T* __restrict__ r, T* __restrict__ r0, T* __restrict__ p,
+                                                  T* __restrict__ x, int64_t n, double* __restrict__ part) {
+  T red[1] = {0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = r[i];
+    r0[i] = ri;
+    p[i] = ri;
+    x[i] = 0;
+    red[0] += ri * ri;
+  }
+  block_partials<T, 1>(red, part);
+}
+
+__global__ void k_bs_init_state(BsState* st, const double* __restrict__ tot, double rtol, int maxit) {
+  st->rho = tot[0];
+  st->tol2 = rtol * rtol * tot[0];
+  st->rr = tot[0];
+  st->alpha = st->omega = st->beta = 0.0;
+  st->iter = 0;
+  st->maxit = maxit;
+  st->done = tot[0] > 0.0 ? (isfinite(tot[0]) ? BS_RUN : BS_BREAKDOWN) : BS_CONVERGED;
+}
+
+// Inner BiCGStab engine in precision T: solves A d = rhs from d = 0 (rhs in r).
+template <typename T>
+struct BsEngine {
+  mfs_mat A = nullptr;
+  int64_t n = 0;
+  T *x = nullptr, *r = nullptr, *r0 = nullptr, *p = nullptr, *v = nullptr, *sv = nullptr, *t = nullptr;
+  const T* val = nullptr;
+  const T* dinv = nullptr;
+  BsState* st = nullptr;
+  double *part = nullptr, *tot = nullptr;
+  unsigned g_spmv = 0, g_vec = 0;
+  int chunk = 8;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  int init(mfs_mat A_, bool precond) {
+    A = A_;
+    n = A->n();
+    const bool f64 = std::is_same<T, double>::value;
+    if (!f64 && !A->val32_valid) MFS_TRY(mat_refresh_derived(A));
+    val = f64 ? (const T*)A->d_val64 : (const T*)A->d_val32;
+    dinv = precond ? (f64 ? (const T*)A->d_dinv64 : (const T*)A->d_dinv32) : nullptr;
+    for (T** b : {&x, &r, &r0, &p, &v, &sv, &t}) MFS_TRY(dmalloc((void**)b, sizeof(T) * n));
+    MFS_TRY(dmalloc((void**)&st, sizeof(BsState)));
+    g_spmv = (unsigned)spmv_grid(A);
+    g_vec = grid_for(n, 1024);
+    MFS_TRY(dmalloc((void**)&part, sizeof(double) * 2 * std::max(g_spmv, g_vec)));
+    MFS_TRY(dmalloc((void**)&tot, sizeof(double) * 8));
+    hipStream_t s = A->ctx->stream;
+    MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk; k++) launch_iteration(s);
+    MFS_HIP(hipStreamEndCapture(s, &graph));
+    MFS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return MFS_OK;
+  }
+
+  void launch_iteration(hipStream_t s) {
+    const int64_t ns = A->sell.n_slices;
+    if (A->bs == 1)
+      k_bs_spmv<T, 1, false><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, p, dinv, r0, v,
+                                                    A->nb, ns, st, part);
+    else
+      k_bs_spmv<T, 3, false><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, p, dinv, r0, v,
+                                                    A->nb, ns, st, part);
+    k_finish_sums<<<1, 256, 0, s>>>(part, g_spmv, 1, tot);
+    k_bs_s<T><<<g_vec, 256, 0, s>>>(r, v, sv, n, st, tot);
+    // t = A s^ with partials s.t (w1 = s) and t.t
+    if (A->bs == 1)
+      k_bs_spmv<T, 1, true><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, sv, dinv, sv, t,
+                                                   A->nb, ns, st, part);
+    else
+      k_bs_spmv<T, 3, true><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, sv, dinv, sv, t,
+                                                   A->nb, ns, st, part);
+    k_finish_sums<<<1, 256, 0, s>>>(part, g_spmv, 2, tot + 1);
+    k_bs_update<T><<<g_vec, 256, 0, s>>>(x, p, sv, t, r, r0, dinv, n, st, tot, part);
+    k_finish_sums<<<1, 256, 0, s>>>(part, g_vec, 2, tot + 3);
+    k_bs_state<<<1, 1, 0, s>>>(st, tot);
+    k_bs_p<T><<<g_vec, 256, 0, s>>>(p, r, v, n, st);
+  }
+
+  // r holds the right-hand side; x returns the solution of A x = r.
+  int run(double rtol, int maxit, BsState* out) {
+    hipStream_t s = A->ctx->stream;
+    k_bs_start<T><<<g_vec, 256, 0, s>>>(r, r0, p, x, n, part);
+    k_finish_sums<<<1, 256, 0, s>>>(part, g_vec, 1, tot);
+    k_bs_init_state<<<1, 1, 0, s>>>(st, tot, rtol, maxit);
+    MFS_HIP(hipGetLastError());
+    BsState* hs = (BsState*)A->ctx->pinned;
+    MFS_HIP(hipMemcpyAsync(hs, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipStreamSynchronize(s));
+    while (!hs->done) {
+      MFS_HIP(hipGraphLaunch(exec, s));
+      MFS_HIP(hipMemcpyAsync(hs, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
+      MFS_HIP(hipStreamSynchronize(s));
+    }
+    *out = *hs;
+    return MFS_OK;
+  }
+
+  void destroy() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    for (T* b : {x, r, r0, p, v, sv, t}) dfree(b);
+    x = r = r0 = p = v = sv = t = nullptr;
+    dfree(st);
+    dfree(part);
+    dfree(tot);
+    st = nullptr;
+    part = tot = nullptr;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Solver object
 // ---------------------------------------------------------------------------
 struct mfs_solver_s {
@@ -557,6 +796,8 @@ struct mfs_solver_s {
   void* z = nullptr;
   double* dh = nullptr;  // device h / y (64 doubles)
   ReduceScratch redg;
+  BsEngine<double> bs64;
+  BsEngine<float> bs32;
 };
 
 extern "C" void mfs_solve_desc_default(mfs_solve_desc* d) {
@@ -576,6 +817,8 @@ extern "C" int mfs_solver_destroy(mfs_solver S) {
   mfs_mat A = S->A;
   if (S->has64) S->cg64.destroy();
   if (S->has32) S->cg32.destroy();
+  S->bs64.destroy();
+  S->bs32.destroy();
   dfree(S->x64);
   dfree(S->b64);
   dfree(S->r64);
@@ -595,7 +838,7 @@ extern "C" int mfs_solver_destroy(mfs_solver S) {
 extern "C" int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solver* out) {
   MFS_CHECK_ARG(A && desc && out, "mfs_solver_create: bad argument");
   MFS_CHECK_ARG(desc->method == MFS_PCG_JACOBI || desc->method == MFS_CG || desc->method == MFS_GMRES ||
-                    desc->method == MFS_MINRES,
+                    desc->method == MFS_MINRES || desc->method == MFS_BICGSTAB,
                 "mfs_solver_create: unknown method");
   MFS_CHECK_ARG(desc->precision == MFS_PREC_F64 || desc->precision == MFS_PREC_MIXED,
                 "mfs_solver_create: unknown precision");
@@ -621,7 +864,12 @@ extern "C" int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solv
   tryit(dmalloc((void**)&S->dout, 8 * 64));
   if (s == MFS_OK) tryit(reduce_scratch_alloc(S->red64, 2 * spmv_grid(A) + 64));
   const bool precond = desc->method != MFS_CG;
-  if (desc->method == MFS_MINRES) {
+  if (desc->method == MFS_BICGSTAB) {
+    if (desc->precision == MFS_PREC_F64)
+      tryit(S->bs64.init(A, true));
+    else
+      tryit(S->bs32.init(A, true));
+  } else if (desc->method == MFS_MINRES) {
     const size_t sv = desc->precision == MFS_PREC_F64 ? 8 : 4;
     if (desc->precision == MFS_PREC_MIXED && !A->val32_valid) tryit(mat_refresh_derived(A));
     for (void*& v : S->mr) tryit(dmalloc(&v, sv * n));
@@ -924,6 +1172,45 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
     MFS_TRY(run_f64(S->cg64));
   } else if (is_cg) {
     MFS_TRY(run_mixed(S->cg32));
+  } else if (d.method == MFS_BICGSTAB) {
+    // BiCGStab runs, each from the current FP64 residual: FP64 to tol, or
+    // FP32 to the inner tolerance with FP64 refinement around it
+    while (rel > d.tol && iters < d.maxit) {
+      const double rn = std::sqrt(rr);
+      BsState hs;
+      if (d.precision == MFS_PREC_F64) {
+        MFS_HIP(hipMemcpyAsync(S->bs64.r, S->r64, 8 * n, hipMemcpyDeviceToDevice, s));
+        MFS_TRY(S->bs64.run(0.5 * d.tol * std::sqrt(bb) / rn, d.maxit - iters, &hs));
+        k_add_f64<<<gv, 256, 0, s>>>(S->x64, S->bs64.x, n);
+      } else {
+        float* dummy = S->bs32.v;
+        k_demote_scaled<<<gv, 256, 0, s>>>(S->r64, 1.0 / rn, S->bs32.r, dummy, n);
+        const double itol = std::min(0.5, std::max(d.inner_tol, 0.5 * d.tol / rel));
+        MFS_TRY(S->bs32.run(itol, d.maxit - iters, &hs));
+        k_promote_add<<<gv, 256, 0, s>>>(S->x64, S->bs32.x, nullptr, rn, n);
+      }
+      MFS_HIP(hipGetLastError());
+      iters += hs.iter;
+      outer++;
+      const double prev = rel;
+      MFS_TRY(true_residual(S, &rr, &bb));
+      rel = std::sqrt(rr / bb);
+      // the FP64 run solved for the unscaled residual, the FP32 run for r / ||r||
+      rep->recursive_relres = d.precision == MFS_PREC_F64 ? std::sqrt(hs.rr / bb) : std::sqrt(hs.rr) * prev;
+      if (!std::isfinite(rel)) {
+        status = MFS_ERR_NONFINITE;
+        break;
+      }
+      if (hs.done == BS_BREAKDOWN && !(rel < 0.99 * prev)) {
+        status = MFS_ERR_NONFINITE;
+        break;
+      }
+      if (rel > 0.99 * prev && hs.iter <= 1) {
+        status = MFS_ERR_NOT_CONVERGED;
+        break;
+      }
+    }
+    if (status == MFS_OK && rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
   } else if (d.method == MFS_MINRES) {
     // MINRES runs, each from the current FP64 residual (refinement steps)
     while (rel > d.tol && iters < d.maxit) {

@@ -110,3 +110,29 @@ def msfem_global_E(p):
     local = (I % n) + n * ((J % n) + n * (K % n))
     idx = (cell[:, None] * 6 * n ** 3 + 6 * local[:, None] + np.arange(6)[None, :]).ravel()
     return p["E"][idx]
+
+
+def convection_diffusion(m, peclet, seed=0):
+    """Non-symmetric test system: upwind 5-point -Laplace + c.grad on an
+    m x m interior grid of the unit square (CSR, Dirichlet 0 built in),
+    c = (peclet, peclet/2), and a N(0,1) right-hand side from seed."""
+    h = 1.0 / (m + 1)
+    cx, cy = peclet, 0.5 * peclet
+    rows, cols, vals = [], [], []
+    for j in range(m):
+        for i in range(m):
+            r = i + m * j
+            entries = {r: 4.0 / h ** 2 + (abs(cx) + abs(cy)) / h}
+            for di, dj, coef in ((-1, 0, -1 / h ** 2 - max(cx, 0) / h), (1, 0, -1 / h ** 2 + min(cx, 0) / h),
+                                 (0, -1, -1 / h ** 2 - max(cy, 0) / h), (0, 1, -1 / h ** 2 + min(cy, 0) / h)):
+                ii, jj = i + di, j + dj
+                if 0 <= ii < m and 0 <= jj < m:
+                    entries[ii + m * jj] = coef
+            for c in sorted(entries):
+                rows.append(r)
+                cols.append(c)
+                vals.append(entries[c])
+    n = m * m
+    rowptr = np.searchsorted(np.array(rows), np.arange(n + 1)).astype(np.int32)
+    b = np.random.default_rng(seed).normal(size=n)
+    return rowptr, np.array(cols, np.int32), np.array(vals), b

@@ -73,6 +73,7 @@ _sig("orc_csr_mv", None, [C.c_int, ip, ip, dp, dp, dp])
 _sig("orc_pcg", C.c_int, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp])
 _sig("orc_gmres", C.c_int, [C.c_int, ip, ip, dp, dp, dp, C.c_int, C.c_double, C.c_int, C.c_int, dp])
 _sig("orc_dense_solve", C.c_int, [C.c_int, ip, ip, dp, dp, dp])
+_sig("orc_bicgstab", C.c_int, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp])
 _sig("orc_refpath_solve", C.c_int, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp])
 _sig("orc_bicg_diag", C.c_double, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp, ip])
 _sig("orc_msfem_cell", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, C.c_double,
@@ -231,6 +232,15 @@ def gmres(rowptr, col, val, b, m=30, x0=None, tol=1e-10, maxit=100000, precond=T
     it = lib.orc_gmres(n, _p(_i32(rowptr), C.c_int), _p(_i32(col), C.c_int), _p(_f64(val), C.c_double),
                        _p(_f64(b), C.c_double), _p(x, C.c_double), m, tol, maxit, int(precond), C.byref(rr))
     return x, it, rr.value
+
+
+def bicgstab(rowptr, col, val, b, x0=None, tol=1e-10, maxit=100000, precond=True):
+    n = len(rowptr) - 1
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    tr = C.c_double()
+    it = lib.orc_bicgstab(n, _p(_i32(rowptr), C.c_int), _p(_i32(col), C.c_int), _p(_f64(val), C.c_double),
+                          _p(_f64(b), C.c_double), _p(x, C.c_double), tol, maxit, int(precond), C.byref(tr))
+    return x, it, tr.value
 
 
 def dense_solve(rowptr, col, val, b):

@@ -705,6 +705,56 @@ EXPORT int orc_pcg(int n, const int* rp, const int* ci, const double* v, const d
   return k;
 }
 
+// BiCGStab with right Jacobi preconditioning when precond==1: the
+// reference's BiCG_Stab_Preconditioning (CSSD_Matrix.h:1058-1163) with its
+// inner M-solve replaced by M = diag(A): p^ = M^-1 p, v = A p^, alpha =
+// (r,r0)/(r0,v), s = r - alpha v, s^ = M^-1 s, t = A s^, omega = (t,s)/(t,t),
+// x += alpha p^ + omega s^, r = s - omega t, beta = (rho'/rho)(alpha/omega),
+// p = r + beta (p - omega v) (CSSD_Matrix.h:1094-1115).  Stops when the
+// recursive ||r|| <= tol ||b|| (the reference checks the true residual every
+// 10 steps, :1117-1126).  Returns iterations; true relative residual out.
+EXPORT int orc_bicgstab(int n, const int* rp, const int* ci, const double* v, const double* b, double* x, double tol,
+                        int maxit, int precond, double* true_relres) {
+  std::vector<double> r(n), r0(n), p(n), vv(n), sv(n), t(n), ph(n), sh(n), q(n), dinv(n, 1.0);
+  if (precond)
+    for (int i = 0; i < n; i++)
+      for (int k = rp[i]; k < rp[i + 1]; k++)
+        if (ci[k] == i && v[k] != 0) dinv[i] = 1.0 / v[k];
+  csr_mv(n, rp, ci, v, x, q.data(), 1);
+  for (int i = 0; i < n; i++) r[i] = b[i] - q[i];
+  r0 = r;
+  p = r;
+  const double bb = dotp(n, b, b) > 0 ? dotp(n, b, b) : 1.0;
+  double rho = dotp(n, r0.data(), r.data());
+  int k = 0;
+  double rr = dotp(n, r.data(), r.data());
+  while (k < maxit && rr > tol * tol * bb) {
+    for (int i = 0; i < n; i++) ph[i] = dinv[i] * p[i];
+    csr_mv(n, rp, ci, v, ph.data(), vv.data(), 1);
+    const double alpha = rho / dotp(n, r0.data(), vv.data());
+    for (int i = 0; i < n; i++) sv[i] = r[i] - alpha * vv[i];
+    for (int i = 0; i < n; i++) sh[i] = dinv[i] * sv[i];
+    csr_mv(n, rp, ci, v, sh.data(), t.data(), 1);
+    const double omega = dotp(n, t.data(), sv.data()) / dotp(n, t.data(), t.data());
+    for (int i = 0; i < n; i++) {
+      x[i] += alpha * ph[i] + omega * sh[i];
+      r[i] = sv[i] - omega * t[i];
+    }
+    const double rho_new = dotp(n, r0.data(), r.data());
+    rr = dotp(n, r.data(), r.data());
+    k++;
+    if (!std::isfinite(rr) || rho_new == 0.0 || omega == 0.0) break;
+    const double beta = (rho_new / rho) * (alpha / omega);
+    rho = rho_new;
+    for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * vv[i]);
+  }
+  csr_mv(n, rp, ci, v, x, q.data(), 1);
+  double tr = 0;
+  for (int i = 0; i < n; i++) tr += (b[i] - q[i]) * (b[i] - q[i]);
+  if (true_relres) *true_relres = std::sqrt(tr / bb);
+  return k;
+}
+
 // Restarted GMRES(m) with modified Gram-Schmidt and Givens rotations, right
 // Jacobi preconditioning when precond==1 (GMRES, CSSD_Matrix.h:1727-1989 with
 // M generalised from 5; the reference's row scaling is a preconditioner and is

@@ -91,6 +91,9 @@ int main(int argc, char** argv) {
   const double r3 = A.GMRES(10000, 1e-12);
   if (!(r3 <= 1e-12) || err(A.X) > 1e-9) return fail("GMRES");
   std::fill(A.X.begin(), A.X.end(), 0.0);
+  const double r5 = A.BiCG_Stab(10000, 1e-12);
+  if (!(r5 <= 1e-12) || err(A.X) > 1e-9) return fail("BiCG_Stab");
+  std::fill(A.X.begin(), A.X.end(), 0.0);
   std::vector<double> diag(A.Diag);
   const double r4 = A.BiCG_Diag(10000, 1e-10, diag);
   if (!(r4 < 1e-10) || err(A.X) > 1e-8) return fail("BiCG_Diag");

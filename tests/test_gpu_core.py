@@ -393,3 +393,42 @@ def test_minres_scalar_matches_pcg(ctx):
     x.copy_from(S.x0)
     rep = mfs.Solver(S.A, method=mfs.MINRES, tol=1e-10).run(S.b, x)
     assert rep.true_relres <= 1e-10 and rel(x.download(), xo) < 1e-8
+
+
+@pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
+def test_bicgstab_nonsymmetric_matches_oracle(ctx, precision):
+    """BiCGStab (right Jacobi) on an upwind convection-diffusion system:
+    solution against the oracle's BiCGStab and numpy's dense solve; in FP64
+    the iteration count follows the oracle's."""
+    rp, col, val, b = problems.convection_diffusion(40, 60.0, seed=3)
+    n = len(b)
+    A = mfs.Mat.csr(ctx, rp, col, val)
+    bv, x = mfs.Vec(ctx, n, mfs.F64, b), mfs.Vec(ctx, n, mfs.F64, np.zeros(n))
+    rep = mfs.Solver(A, method=mfs.BICGSTAB, precision=precision, tol=1e-10).run(bv, x)
+    assert rep.status == 0 and rep.true_relres <= 1e-10, rep
+    xo, it_o, tr = orc.bicgstab(rp, col, val, b, tol=0.5e-10)
+    assert tr <= 1e-10
+    xg = x.download()
+    assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
+    Ad = np.zeros((n, n))
+    for i in range(n):
+        Ad[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    xd = np.linalg.solve(Ad, b)
+    assert np.linalg.norm(xg - xd) <= 1e-8 * np.linalg.norm(xd)
+    if precision == mfs.PREC_F64:
+        assert abs(rep.iterations - it_o) <= 0.15 * it_o + 3, (rep.iterations, it_o)
+
+
+def test_bicgstab_reference_elastic_operator(ctx):
+    """The reference element convention's two-phase elastic system (symmetric
+    indefinite, clamped bottom, free sides): BiCGStab reaches dense LU's
+    solution."""
+    p = problems.elastic_cube(5, field="grf")
+    S = assemble(ctx, p)
+    O = oracle_elastic(p)
+    xd = orc.dense_solve(O.rowptr, O.col, O.val, O.rhs)
+    x = mfs.Vec(ctx, S.A.n)
+    x.copy_from(S.x0)
+    rep = mfs.Solver(S.A, method=mfs.BICGSTAB, tol=1e-10, maxit=50000).run(S.b, x)
+    assert rep.status == 0 and rep.true_relres <= 1e-10, rep
+    assert rel(x.download(), xd) < 1e-7

@@ -72,6 +72,7 @@ struct mfs_mat_s {
   int64_t* d_slice_base = nullptr;
   int32_t* d_slice_width = nullptr;
   int32_t* d_col = nullptr;     // sell_entries
+  int16_t* d_col16 = nullptr;   // sell_entries: column - row, when all fit (else null)
   int64_t* d_pos = nullptr;     // nnzb: CSR slot -> SELL entry
   int64_t* d_diag_pos = nullptr;// nb: SELL entry of the diagonal block
   double* d_val64 = nullptr;    // bs*bs*sell_entries
@@ -200,9 +201,12 @@ __device__ __forceinline__ bool grid_sum(T (&v)[NV], double* partials, unsigned*
 // gathers are then issued together: two latencies per U entries.  Entries
 // past the slice width re-read the last entry (a valid index) and use value
 // 0, so every load is unconditional.
-template <typename T, int BS, typename XL>
+// Column indices are int32 block columns, or (CI = int16_t) block-column
+// offsets from the row, which halve the index traffic when every offset of
+// the matrix fits in 16 bits.
+template <typename T, int BS, typename XL, typename CI>
 __device__ __forceinline__ void sell_row_x(const int64_t* __restrict__ slice_base,
-                                           const int32_t* __restrict__ slice_width, const int32_t* __restrict__ col,
+                                           const int32_t* __restrict__ slice_width, const CI* __restrict__ col,
                                            const T* __restrict__ val, int64_t slice, int lane, int64_t nb, XL xl,
                                            T (&acc)[BS]) {
   constexpr int C = 64;
@@ -213,7 +217,8 @@ __device__ __forceinline__ void sell_row_x(const int64_t* __restrict__ slice_bas
   (void)nb;
   const int64_t sb = slice_base[slice];
   const int w = slice_width[slice];
-  const int32_t* cp = col + sb + lane;
+  const CI* cp = col + sb + lane;
+  const int32_t rbase = sizeof(CI) == 2 ? (int32_t)(slice * C + lane) : 0;
   const T* vp = val + sb * B2 + lane;
   for (int j0 = 0; j0 < w; j0 += U) {
     int32_t c[U];
@@ -222,7 +227,7 @@ __device__ __forceinline__ void sell_row_x(const int64_t* __restrict__ slice_bas
     for (int u = 0; u < U; u++) {
       const int j = min(j0 + u, w - 1);
       const bool in = j0 + u < w;
-      c[u] = __builtin_nontemporal_load(cp + (int64_t)j * C);
+      c[u] = (int32_t)__builtin_nontemporal_load(cp + (int64_t)j * C) + rbase;
 #pragma unroll
       for (int e = 0; e < B2; e++) {
         const T t = __builtin_nontemporal_load(vp + ((int64_t)j * B2 + e) * C);
@@ -242,9 +247,9 @@ __device__ __forceinline__ void sell_row_x(const int64_t* __restrict__ slice_bas
   }
 }
 
-template <typename T, int BS>
+template <typename T, int BS, typename CI>
 __device__ __forceinline__ void sell_row(const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
-                                         const int32_t* __restrict__ col, const T* __restrict__ val,
+                                         const CI* __restrict__ col, const T* __restrict__ val,
                                          const T* __restrict__ x, int64_t slice, int lane, int64_t nb, T (&acc)[BS]) {
   sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb, [&](int64_t i) { return x[i]; }, acc);
 }

@@ -460,12 +460,23 @@ struct MrEngine {
     if (const char* e = std::getenv("MFS_FINISH_SUMS")) fin = e[0] == '1';  // force either way (tests)
     MFS_TRY(dmalloc((void**)&tot_rz, sizeof(double) * 2 * KR));
     MFS_TRY(dmalloc((void**)&tot_pq, sizeof(double) * KR));
+    // MFS_NO_GRAPH=1 launches the same kernels directly (kernel tracing)
+    const char* e = std::getenv("MFS_NO_GRAPH");
+    if (e && e[0] == '1') return MFS_OK;
     hipStream_t s = A->ctx->stream;
     MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < chunk; k++) launch_iteration(s, k & 1);
     MFS_HIP(hipStreamEndCapture(s, &graph));
     MFS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     return MFS_OK;
+  }
+
+  void launch_chunk(hipStream_t s) {
+    if (exec) {
+      (void)hipGraphLaunch(exec, s);
+      return;
+    }
+    for (int k = 0; k < chunk; k++) launch_iteration(s, k & 1);
   }
 
   void launch_iteration(hipStream_t s, int parity) {
@@ -517,7 +528,8 @@ struct MrEngine {
     MrState* hs = (MrState*)A->ctx->pinned;
     hs->alldone = 0;
     while (!hs->alldone) {
-      MFS_HIP(hipGraphLaunch(exec, s));
+      launch_chunk(s);
+      MFS_HIP(hipGetLastError());
       MFS_HIP(hipMemcpyAsync(hs, st, sizeof(MrState), hipMemcpyDeviceToHost, s));
       MFS_HIP(hipStreamSynchronize(s));
     }

@@ -2,6 +2,8 @@
 // kernels of the FEM solve path for gfx950.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -354,6 +356,27 @@ int mat_create_from_pattern(mfs_ctx ctx, int bs, Pattern&& P, mfs_mat* out) {
   MFS_HIP(hipMemcpyAsync(A->d_pos, A->sell.pos.data(), sizeof(int64_t) * A->nnzb, hipMemcpyHostToDevice, st));
   MFS_HIP(hipMemcpyAsync(A->d_diag_pos, diag_pos.data(), sizeof(int64_t) * A->nb, hipMemcpyHostToDevice, st));
   MFS_HIP(hipMemsetAsync(A->d_val64, 0, sizeof(double) * nval, st));
+  // 16-bit row-relative columns when every offset fits (structured meshes
+  // in natural order: offsets up to (n+1)^2 + n + 2 for n^3 cells)
+  if (!std::getenv("MFS_NO_COL16")) {
+    std::vector<int16_t> c16(A->sell_entries);
+    bool fits = true;
+    for (int64_t q = 0; q < A->sell.n_slices && fits; q++)
+      for (int32_t j = 0; j < A->sell.slice_width[q] && fits; j++)
+        for (int l = 0; l < A->sell.C; l++) {
+          const int64_t at = A->sell.slice_base[q] + (int64_t)j * A->sell.C + l;
+          const int64_t d = (int64_t)A->sell.col[at] - (q * A->sell.C + l);
+          if (d < -32768 || d > 32767) {
+            fits = false;
+            break;
+          }
+          c16[at] = (int16_t)d;
+        }
+    if (fits && dmalloc((void**)&A->d_col16, sizeof(int16_t) * A->sell_entries) == MFS_OK) {
+      MFS_HIP(hipMemcpyAsync(A->d_col16, c16.data(), sizeof(int16_t) * A->sell_entries, hipMemcpyHostToDevice, st));
+    }
+    MFS_HIP(hipStreamSynchronize(st));
+  }
   MFS_HIP(hipStreamSynchronize(st));
   // the host SELL column copy is not needed after upload
   std::vector<int32_t>().swap(A->sell.col);
@@ -446,6 +469,7 @@ void mat_release(mfs_mat A) {
   dfree(A->d_slice_base);
   dfree(A->d_slice_width);
   dfree(A->d_col);
+  dfree(A->d_col16);
   dfree(A->d_pos);
   dfree(A->d_diag_pos);
   dfree(A->d_val64);
@@ -516,10 +540,10 @@ extern "C" int mfs_mat_export_csr(mfs_mat A, int32_t* rowptr, int32_t* col, doub
 // MODE 1: y = A x and grid dot x.y (CG denominator p.Ap).
 // MODE 2: y = b - A x and grid ||y||^2, ||b||^2 (true residual).
 // ---------------------------------------------------------------------------
-template <typename T, int BS, int MODE>
+template <typename T, int BS, int MODE, typename CI>
 __global__ __launch_bounds__(256) void k_sell_spmv(const int64_t* __restrict__ slice_base,
                                                    const int32_t* __restrict__ slice_width,
-                                                   const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                   const CI* __restrict__ col, const T* __restrict__ val,
                                                    const T* __restrict__ x, T* __restrict__ y,
                                                    const T* __restrict__ b, int64_t nb, int64_t n_slices,
                                                    double* partials, unsigned* counter, double* out) {
@@ -529,7 +553,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(const int64_t* __restrict__ s
   T acc[BS];
   const int64_t r = slice * C + lane;
   if (slice < n_slices) {
-    sell_row<T, BS>(slice_base, slice_width, col, val, x, slice, lane, nb, acc);
+    sell_row<T, BS, CI>(slice_base, slice_width, col, val, x, slice, lane, nb, acc);
   } else {
 #pragma unroll
     for (int a = 0; a < BS; a++) acc[a] = 0;
@@ -604,6 +628,20 @@ __global__ __launch_bounds__(256) void k_finish_sums_multi(const double* __restr
 }
 
 namespace mfs {
+template <typename T, int BS, int MODE>
+static void sell_spmv_launch(mfs_mat A, const T* val, const void* x, void* y, const void* b, double* partials,
+                             unsigned* counter, double* out, unsigned grid, hipStream_t s) {
+  const int64_t ns = A->sell.n_slices;
+  if (A->d_col16)
+    k_sell_spmv<T, BS, MODE, int16_t><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val,
+                                                           (const T*)x, (T*)y, (const T*)b, A->nb, ns, partials,
+                                                           counter, out);
+  else
+    k_sell_spmv<T, BS, MODE, int32_t><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val,
+                                                           (const T*)x, (T*)y, (const T*)b, A->nb, ns, partials,
+                                                           counter, out);
+}
+
 template <int MODE>
 static int spmv_mode_launch(mfs_mat A, int prec, const void* x, void* y, const void* b, double* partials,
                             unsigned* counter, double* out, hipStream_t s) {
@@ -611,23 +649,15 @@ static int spmv_mode_launch(mfs_mat A, int prec, const void* x, void* y, const v
   const unsigned grid = (unsigned)((ns + 3) / 4);
   if (prec == MFS_F64) {
     if (A->bs == 1)
-      k_sell_spmv<double, 1, MODE><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, A->d_val64,
-                                                        (const double*)x, (double*)y, (const double*)b, A->nb, ns,
-                                                        partials, counter, out);
+      sell_spmv_launch<double, 1, MODE>(A, A->d_val64, x, y, b, partials, counter, out, grid, s);
     else
-      k_sell_spmv<double, 3, MODE><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, A->d_val64,
-                                                        (const double*)x, (double*)y, (const double*)b, A->nb, ns,
-                                                        partials, counter, out);
+      sell_spmv_launch<double, 3, MODE>(A, A->d_val64, x, y, b, partials, counter, out, grid, s);
   } else {
     if (!A->val32_valid) MFS_TRY(mat_refresh_derived(A));
     if (A->bs == 1)
-      k_sell_spmv<float, 1, MODE><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, A->d_val32,
-                                                       (const float*)x, (float*)y, (const float*)b, A->nb, ns,
-                                                       partials, counter, out);
+      sell_spmv_launch<float, 1, MODE>(A, A->d_val32, x, y, b, partials, counter, out, grid, s);
     else
-      k_sell_spmv<float, 3, MODE><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, A->d_val32,
-                                                       (const float*)x, (float*)y, (const float*)b, A->nb, ns,
-                                                       partials, counter, out);
+      sell_spmv_launch<float, 3, MODE>(A, A->d_val32, x, y, b, partials, counter, out, grid, s);
   }
   MFS_HIP(hipGetLastError());
   if (MODE != 0) k_finish_sums<<<1, 256, 0, s>>>(partials, grid, 2, out);

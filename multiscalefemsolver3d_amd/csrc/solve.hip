@@ -86,10 +86,10 @@ __device__ __forceinline__ void block_partials(T (&v)[NV], double* __restrict__ 
 // K1: from K2's partials (r.z, r.r): stop test, beta = rz / rz_prev;
 // p_new = z + beta p_old (own rows written, gathered rows formed on the
 // fly), q = A p_new, per-block partial of p_new.q.
-template <typename T, int BS>
+template <typename T, int BS, typename CI>
 __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__ slice_base,
                                                      const int32_t* __restrict__ slice_width,
-                                                     const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                     const CI* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ z, const T* __restrict__ p_old,
                                                      T* __restrict__ p_new, T* __restrict__ q, int64_t nb,
                                                      int64_t n_slices, CgState* st, const double* __restrict__ part_rz,
@@ -310,16 +310,34 @@ struct PcgEngine {
     return capture();
   }
 
-  void launch_iteration(hipStream_t s, int parity) {
+  // K1 with 16-bit row-relative columns when the matrix has them
+  void launch_k1(hipStream_t s, T* p_old, T* p_new) {
     const int64_t ns = A->sell.n_slices;
+    if (A->d_col16) {
+      if (A->bs == 1)
+        k_cg_dir_spmv<T, 1, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val, z,
+                                                            p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
+                                                            part_pq());
+      else
+        k_cg_dir_spmv<T, 3, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val, z,
+                                                            p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
+                                                            part_pq());
+    } else {
+      if (A->bs == 1)
+        k_cg_dir_spmv<T, 1, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z,
+                                                            p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
+                                                            part_pq());
+      else
+        k_cg_dir_spmv<T, 3, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z,
+                                                            p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
+                                                            part_pq());
+    }
+  }
+
+  void launch_iteration(hipStream_t s, int parity) {
     T* p_old = pbuf[parity];
     T* p_new = pbuf[parity ^ 1];
-    if (A->bs == 1)
-      k_cg_dir_spmv<T, 1><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z, p_old, p_new, q,
-                                                 A->nb, ns, st, rz_src(), rz_g(), part_pq());
-    else
-      k_cg_dir_spmv<T, 3><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z, p_old, p_new, q,
-                                                 A->nb, ns, st, rz_src(), rz_g(), part_pq());
+    launch_k1(s, p_old, p_new);
     finish_pq(s);
     k_cg_update<T><<<g_vec, 256, 0, s>>>(x, p_new, r, z, q, dinv, n, st, pq_src(), pq_g(), part_rz());
     finish_rz(s);
@@ -706,6 +724,9 @@ struct BsEngine {
     g_vec = grid_for(n, 1024);
     MFS_TRY(dmalloc((void**)&part, sizeof(double) * 2 * std::max(g_spmv, g_vec)));
     MFS_TRY(dmalloc((void**)&tot, sizeof(double) * 8));
+    // MFS_NO_GRAPH=1 launches the same kernels directly (kernel tracing)
+    const char* e = std::getenv("MFS_NO_GRAPH");
+    if (e && e[0] == '1') return MFS_OK;
     hipStream_t s = A->ctx->stream;
     MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < chunk; k++) launch_iteration(s);
@@ -749,7 +770,11 @@ struct BsEngine {
     MFS_HIP(hipMemcpyAsync(hs, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
     MFS_HIP(hipStreamSynchronize(s));
     while (!hs->done) {
-      MFS_HIP(hipGraphLaunch(exec, s));
+      if (exec)
+        MFS_HIP(hipGraphLaunch(exec, s));
+      else
+        for (int k = 0; k < chunk; k++) launch_iteration(s);
+      MFS_HIP(hipGetLastError());
       MFS_HIP(hipMemcpyAsync(hs, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
       MFS_HIP(hipStreamSynchronize(s));
     }
@@ -1342,18 +1367,10 @@ int time_cg_kernel(PcgEngine<T>& E, int kernel, int reps, double* avg_ms, double
   hipEvent_t e0, e1;
   MFS_HIP(hipEventCreate(&e0));
   MFS_HIP(hipEventCreate(&e1));
-  const int64_t ns = A->sell.n_slices;
   MFS_HIP(hipEventRecord(e0, s));
   for (int i = 0; i < reps; i++) {
     if (kernel == 0) {
-      if (A->bs == 1)
-        k_cg_dir_spmv<T, 1><<<E.g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, E.val, E.z,
-                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.rz_src(),
-                                                     E.rz_g(), E.part_pq());
-      else
-        k_cg_dir_spmv<T, 3><<<E.g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, E.val, E.z,
-                                                     E.pbuf[0], E.pbuf[1], E.q, A->nb, ns, E.st, E.rz_src(),
-                                                     E.rz_g(), E.part_pq());
+      E.launch_k1(s, E.pbuf[0], E.pbuf[1]);
     } else {
       k_cg_update<T><<<E.g_vec, 256, 0, s>>>(E.x, E.pbuf[1], E.r, E.z, E.q, E.dinv, n, E.st, E.pq_src(), E.pq_g(),
                                              E.part_rz());

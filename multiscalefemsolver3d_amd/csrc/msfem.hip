@@ -208,6 +208,9 @@ __device__ void tet_grads(const int (&ijk)[4][3], double hf, double (&g)[4][3], 
   V = fabs(det) / 6.0;
 }
 
+// Step index of axis a in the tet's axis order (xyz,xzy,yxz,yzx,zxy,zyx).
+__device__ constexpr int kStepOf[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {2, 0, 1}, {1, 2, 0}, {2, 1, 0}};
+
 // Elastic block (I=i, J=j) of a tet: textbook V(lam dI_a dJ_b + mu dI_b dJ_a +
 // mu delta dI.dJ); the reference convention is its transpose.
 __device__ __forceinline__ void elastic_block(const double* dI, const double* dJ, double vl, double vm, int conv,
@@ -927,83 +930,130 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
                                                   double nu, double h, int conv, const double* __restrict__ x64,
                                                   const double* __restrict__ ft, int form,
                                                   double* __restrict__ Kc) {
-  __shared__ double phi[12][NB];   // element values of the 24 bases
-  __shared__ double Ke[12][12];
-  __shared__ double KP[12][NB];
+  // GT tets per round: three barriers per round instead of per tet
+  constexpr int GT = 8;
+  __shared__ double phi[GT][12][NB];  // element values of the 24 bases
+  __shared__ double Ke[GT][12][12];
+  __shared__ double KP[GT][12][NB];
   const int cell = blockIdx.x, tid = threadIdx.x;
   const Cell C = cells[cell];
   const int n = C.n, m = n - 1;
   const double hf = h / n;
   const double lamf = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), muf = 1.0 / (2.0 * (1.0 + nu));
-  double acc[3] = {0, 0, 0};  // thread owns pairs tid, tid+256, tid+512 (< 576)
+  const int econv = form == MFS_BASIS_COMPONENTWISE ? (int)MFS_ELEM_TEXTBOOK : conv;
+  // register tiles: wave wv accumulates tets 2wv, 2wv+1 of every round; lane
+  // l owns the 3x3 tile (rows 3(l/8).., columns 3(l%8)..) of the 24x24 sum
+  const int wv = tid >> 6, ln = tid & 63, ti = ln >> 3, tj = ln & 7;
+  double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   const int ntet = 6 * n * n * n;
-  for (int t = 0; t < ntet; t++) {
-    int ijk[4][3];
-    tet_nodes(t, n, ijk);
-    // element basis values
-    for (int q = tid; q < 12 * NB; q += blockDim.x) {
-      const int p = q / NB, bq = q % NB;
-      const int vtx = p / 3, a = p % 3;
-      const int i = ijk[vtx][0], j = ijk[vtx][1], k = ijk[vtx][2];
-      const bool interior = i >= 1 && i <= m && j >= 1 && j <= m && k >= 1 && k <= m;
-      double val;
-      if (interior) {
-        val = x64[(C.int0 + interior_index(i, j, k, m)) * 3 * NB + a * NB + bq];
-      } else {
-        double g[3];
-        if (ft)
-          face_trace_at(ft + C.ft0, n, i, j, k, bq, g);
-        else
-          hat_trace_at(n, i, j, k, bq, g);
-        val = g[a];
+  for (int t0 = 0; t0 < ntet; t0 += GT) {
+    // element basis values of GT tets (zero past the last tet)
+    for (int q = tid; q < GT * 12 * NB; q += blockDim.x) {
+      const int g = q / (12 * NB), p = (q / NB) % 12, bq = q % NB;
+      const int t = t0 + g;
+      double val = 0.0;
+      if (t < ntet) {
+        // vertex v of the tet: the hex corner plus one along every axis
+        // stepped within the first v steps of the tet's axis order
+        const int vtx = p / 3, a = p % 3;
+        const int hex = t / 6, pm = t % 6;
+        const int i = hex % n + (vtx > kStepOf[pm][0]), j = (hex / n) % n + (vtx > kStepOf[pm][1]),
+                  k = hex / (n * n) + (vtx > kStepOf[pm][2]);
+        if (i >= 1 && i <= m && j >= 1 && j <= m && k >= 1 && k <= m) {
+          val = x64[(C.int0 + interior_index(i, j, k, m)) * 3 * NB + a * NB + bq];
+        } else {
+          double gv[3];
+          if (ft)
+            face_trace_at(ft + C.ft0, n, i, j, k, bq, gv);
+          else
+            hat_trace_at(n, i, j, k, bq, gv);
+          val = gv[a];
+        }
       }
-      phi[p][bq] = val;
+      phi[g][p][bq] = val;
     }
-    if (form == MFS_BASIS_COMPONENTWISE) {
-      __syncthreads();
-      double sum = 0.0;
-      const int p = tid / 8, c = tid % 8;  // 12 x 8 (element dof, coarse node)
-      if (tid < 96) sum = phi[p][3 * c] + phi[p][3 * c + 1] + phi[p][3 * c + 2];
-      __syncthreads();
-      if (tid < 96)
-        for (int a2 = 0; a2 < 3; a2++) phi[p][3 * c + a2] = (p % 3 == a2) ? sum : 0.0;
-    }
-    if (tid < 16) {
-      double g[4][3], V;
-      tet_grads(ijk, hf, g, V);
-      const int I = tid / 4, J = tid % 4;
-      const double Et = E[C.tet0 + t];
+    // element matrices: GT x 16 blocks
+    if (tid < GT * 16) {
+      const int g = tid / 16, I = (tid / 4) % 4, J = tid % 4;
+      const int t = t0 + g;
       double Kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      elastic_block(g[I], g[J], V * Et * lamf, V * Et * muf,
-                    form == MFS_BASIS_COMPONENTWISE ? (int)MFS_ELEM_TEXTBOOK : conv, Kb);
+      if (t < ntet) {
+        int ijk[4][3];
+        tet_nodes(t, n, ijk);
+        double gr[4][3], V;
+        tet_grads(ijk, hf, gr, V);
+        const double Et = E[C.tet0 + t];
+        elastic_block(gr[I], gr[J], V * Et * lamf, V * Et * muf, econv, Kb);
+      }
       for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) Ke[3 * I + a][3 * J + b] = Kb[a][b];
+        for (int b = 0; b < 3; b++) Ke[g][3 * I + a][3 * J + b] = Kb[a][b];
     }
     __syncthreads();
-    for (int q = tid; q < 12 * NB; q += blockDim.x) {
-      const int p = q / NB, bq = q % NB;
-      double s = 0;
-      for (int u = 0; u < 12; u++) s += Ke[p][u] * phi[u][bq];
-      KP[p][bq] = s;
+    if (form == MFS_BASIS_COMPONENTWISE) {
+      // Psi_{c,a2}[p] = (p's component == a2) * sum_d phi_{c,d}[p]
+      for (int q = tid; q < GT * 12 * 8; q += blockDim.x) {
+        const int g = q / 96, p = (q / 8) % 12, c = q % 8;
+        const double sum = phi[g][p][3 * c] + phi[g][p][3 * c + 1] + phi[g][p][3 * c + 2];
+        for (int a2 = 0; a2 < 3; a2++) phi[g][p][3 * c + a2] = (p % 3 == a2) ? sum : 0.0;
+      }
+      __syncthreads();
+    }
+    {
+      // KP = Ke phi: one 3x3 tile (rows 3pb.., columns 3cb..) of one tet per thread
+      static_assert(GT * 4 * 8 == 256, "one KP tile per thread");
+      const int g = tid >> 5, pb = (tid >> 3) & 3, cb = tid & 7;
+      double kp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+      for (int u = 0; u < 12; u++) {
+        double kr[3], pr[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          kr[r] = Ke[g][3 * pb + r][u];
+          pr[r] = phi[g][u][3 * cb + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) kp[r][c] += kr[r] * pr[c];
+      }
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) KP[g][3 * pb + r][3 * cb + c] = kp[r][c];
     }
     __syncthreads();
-    for (int w = 0; w < 3; w++) {
-      const int pr = tid + 256 * w;
-      if (pr >= NB * NB) break;
-      const int bi = pr / NB, bj = pr % NB;
-      double s = 0;
-      for (int p = 0; p < 12; p++) s += phi[p][bi] * KP[p][bj];
-      acc[w] += s;
+#pragma unroll
+    for (int gg = 0; gg < 2; gg++) {
+      const int g = 2 * wv + gg;
+#pragma unroll
+      for (int p = 0; p < 12; p++) {
+        double a[3], b[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          a[r] = phi[g][p][3 * ti + r];
+          b[r] = KP[g][p][3 * tj + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[r][c] += a[r] * b[c];
+      }
     }
     __syncthreads();
   }
+  // sum the four waves' tiles (phi is free now), then store
+  double* red = &phi[0][0][0];  // 4 x 576 doubles fit in GT*12*NB
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) red[wv * NB * NB + (3 * ti + r) * NB + 3 * tj + c] = acc[r][c];
+  __syncthreads();
   const bool bt = form == MFS_BASIS_COMPONENTWISE && conv == MFS_ELEM_REFERENCE;
-  for (int w = 0; w < 3; w++) {
-    const int pr = tid + 256 * w;
-    if (pr >= NB * NB) continue;
+  for (int pr = tid; pr < NB * NB; pr += blockDim.x) {
     const int bi = pr / NB, bj = pr % NB;
+    const double v = (red[pr] + red[NB * NB + pr]) + (red[2 * NB * NB + pr] + red[3 * NB * NB + pr]);
     const int out = bt ? (3 * (bi / 3) + bj % 3) * NB + 3 * (bj / 3) + bi % 3 : pr;
-    Kc[(int64_t)cell * NB * NB + out] = acc[w];
+    Kc[(int64_t)cell * NB * NB + out] = v;
   }
 }
 

@@ -258,10 +258,22 @@ int mfs_renumber(int64_t n_nodes, int64_t n_tets, const int32_t* tets, const int
  * diagonal.  Two calls: rowptr only (col NULL), then col. */
 int mfs_portrait(int64_t n_rows, int64_t n_tets, const int32_t* tets, int32_t* rowptr, int32_t* col);
 /* Write a Gmsh 2.2 ASCII mesh in the layout SimpleGrid::ReadFromMSH reads
- * (Library/Math.h:327-520): nodes, then tets (type 4) with physical tag =
- * material+1.  Boundary triangles (type 2) carry their tag. */
+ * (Library/Math.h:327-520): nodes, then boundary triangles (type 2) with
+ * their tag and parent tet, then tets (type 4) with the material in the
+ * physical-tag field (the reader's domain id). */
 int mfs_write_gmsh(const char* path, int64_t n_nodes, const double* xyz, int64_t n_tets, const int32_t* tets,
                    const int32_t* material, int64_t n_tris, const int32_t* tris, const int32_t* tri_tags);
+/* Read a Gmsh 2.2 ASCII mesh with SimpleGrid::ReadFromMSH's rules
+ * (Library/Math.h:327-520): 9-integer type-2 rows are boundary triangles
+ * (tag = 4th field; parent tet = 5th field in the "modified" layout, whose
+ * header line holds 4 numbers), 10-integer type-4 rows are tets (material =
+ * 4th field).  Ids become 0-based.  Two calls: with NULL arrays it returns
+ * the counts, then it fills the arrays (sizes must match).  Parent tets
+ * absent from the file come from the node-tet adjacency; the reference
+ * searches all tets and rewrites the file (:455-511), this reader never
+ * writes. */
+int mfs_read_gmsh(const char* path, int64_t* n_nodes, double* xyz, int64_t* n_tets, int32_t* tets,
+                  int32_t* material, int64_t* n_tris, int32_t* tris, int32_t* tri_tags, int32_t* tri_parent);
 
 /* ---- MsFEM batched local problems ----------------------------------------
  * Replaces Grid_forMech_Poly_Order1::CreateFunctionalSpaces'

@@ -205,6 +205,7 @@ _sig("mfs_msfem_coarse_system", C.c_int, [vp, c_dp, c_dp, C.c_int64, c_i32p, c_d
                                           C.POINTER(vp)])
 _sig("mfs_msfem_top_traction", C.c_int, [vp, c_dp, c_dp])
 _sig("mfs_msfem_downscale", C.c_int, [vp, c_dp, C.c_int, c_dp])
+_sig("mfs_read_gmsh", C.c_int, [C.c_char_p, c_i64p, c_dp, c_i64p, c_i32p, c_i32p, c_i64p, c_i32p, c_i32p, c_i32p])
 _sig("mfs_msfem_error", C.c_int, [vp, c_dp, c_dp, c_dp])
 _sig("mfs_batch_trace_status", C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)])
 
@@ -305,6 +306,24 @@ def write_gmsh(path, xyz, tets, material=None, tris=None, tri_tags=None):
                             ptr(i32(tets), C.c_int32),
                             ptr(i32(material), C.c_int32) if material is not None else None, len(tris),
                             ptr(tris, C.c_int32), ptr(tri_tags, C.c_int32)), "write_gmsh")
+
+
+def read_gmsh(path):
+    """(xyz, tets, material, tris, tri_tags, tri_parent) of a Gmsh 2.2 file,
+    0-based, read with SimpleGrid::ReadFromMSH's rules."""
+    nn, nt, nf = C.c_int64(), C.c_int64(), C.c_int64()
+    _chk(lib.mfs_read_gmsh(path.encode(), C.byref(nn), None, C.byref(nt), None, None, C.byref(nf), None, None,
+                           None), "read_gmsh")
+    xyz = np.empty((nn.value, 3))
+    tets = np.empty((nt.value, 4), np.int32)
+    mat = np.empty(nt.value, np.int32)
+    tris = np.empty((nf.value, 3), np.int32)
+    tags = np.empty(nf.value, np.int32)
+    par = np.empty(nf.value, np.int32)
+    _chk(lib.mfs_read_gmsh(path.encode(), C.byref(nn), ptr(xyz, C.c_double), C.byref(nt), ptr(tets, C.c_int32),
+                           ptr(mat, C.c_int32), C.byref(nf), ptr(tris, C.c_int32), ptr(tags, C.c_int32),
+                           ptr(par, C.c_int32)), "read_gmsh")
+    return xyz, tets, mat, tris, tags, par
 
 
 # ---------------------------------------------------------------------------

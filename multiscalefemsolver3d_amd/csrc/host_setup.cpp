@@ -452,3 +452,167 @@ extern "C" int mfs_write_gmsh(const char* path, int64_t nn, const double* xyz, i
   std::fclose(f);
   return MFS_OK;
 }
+
+// Gmsh 2.2 ASCII reader with SimpleGrid::ReadFromMSH's rules (Library/
+// Math.h:327-520): element rows are read as integer lists; 9 integers of
+// type 2 are a boundary triangle (physical tag in the 4th field, parent tet
+// in the 5th when the header line holds 4 numbers, the "modified" layout),
+// 10 integers of type 4 a tet (4th field = material), 8 integers of type 1
+// an edge (skipped).  Node ids are 1-based in the file.
+namespace {
+struct GmshMesh {
+  std::vector<double> xyz;
+  std::vector<int32_t> tets, mat, tris, tags, parent;
+};
+
+bool read_line(FILE* f, std::string& line) {
+  line.clear();
+  int c;
+  while ((c = std::fgetc(f)) != EOF) {
+    if (c == '\n') return true;
+    line.push_back((char)c);
+  }
+  return !line.empty();
+}
+
+void ints_of(const std::string& line, std::vector<long long>& out) {
+  out.clear();
+  const char* p = line.c_str();
+  while (*p) {
+    while (*p == ' ' || *p == '\t' || *p == '\r') p++;
+    if (!*p) break;
+    char* end = nullptr;
+    const long long v = std::strtoll(p, &end, 10);
+    if (end == p) break;
+    out.push_back(v);
+    p = end;
+  }
+}
+
+int parse_gmsh(const char* path, GmshMesh& M) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) {
+    set_error(std::string("mfs_read_gmsh: cannot open ") + path);
+    return MFS_ERR_ARG;
+  }
+  std::string line;
+  std::vector<long long> iv;
+  bool modified = false, ok = true;
+  int section = 0;  // 1 format, 2 nodes, 3 elements
+  int64_t expect = -1;
+  while (ok && read_line(f, line)) {
+    if (line.rfind("$MeshFormat", 0) == 0) {
+      section = 1;
+      continue;
+    }
+    if (line.rfind("$Nodes", 0) == 0) {
+      section = 2;
+      expect = -1;
+      continue;
+    }
+    if (line.rfind("$Elements", 0) == 0) {
+      section = 3;
+      expect = -1;
+      continue;
+    }
+    if (line.rfind("$End", 0) == 0) {
+      section = 0;
+      continue;
+    }
+    if (section == 1) {
+      ints_of(line, iv);
+      // "2 0 8 1" parses as 4 numbers in the modified layout, "2.2 0 8" as fewer
+      modified = iv.size() == 4;
+    } else if (section == 2) {
+      if (expect < 0) {
+        ints_of(line, iv);
+        if (iv.empty()) ok = false;
+        expect = iv.empty() ? 0 : iv[0];
+        M.xyz.reserve(3 * (size_t)expect);
+        continue;
+      }
+      long long id;
+      double x, y, z;
+      if (std::sscanf(line.c_str(), "%lld %lf %lf %lf", &id, &x, &y, &z) != 4 ||
+          id != (long long)M.xyz.size() / 3 + 1) {
+        ok = false;
+        break;
+      }
+      M.xyz.push_back(x);
+      M.xyz.push_back(y);
+      M.xyz.push_back(z);
+    } else if (section == 3) {
+      ints_of(line, iv);
+      if (expect < 0) {
+        expect = iv.empty() ? 0 : iv[0];
+        continue;
+      }
+      if (iv.size() == 9 && iv[1] == 2) {
+        for (int k = 6; k < 9; k++) M.tris.push_back((int32_t)(iv[k] - 1));
+        M.tags.push_back((int32_t)iv[3]);
+        M.parent.push_back(modified ? (int32_t)(iv[4] - 1) : -1);
+      } else if (iv.size() == 10 && iv[1] == 4) {
+        for (int k = 6; k < 10; k++) M.tets.push_back((int32_t)(iv[k] - 1));
+        M.mat.push_back((int32_t)iv[3]);
+      }
+    }
+  }
+  std::fclose(f);
+  const int64_t nn = (int64_t)M.xyz.size() / 3;
+  for (int32_t v : M.tets) ok = ok && v >= 0 && v < nn;
+  for (int32_t v : M.tris) ok = ok && v >= 0 && v < nn;
+  if (!ok) {
+    set_error(std::string("mfs_read_gmsh: malformed file ") + path);
+    return MFS_ERR_ARG;
+  }
+  // parent tets missing from the file: from the node-tet adjacency
+  const int64_t nt = (int64_t)M.mat.size();
+  bool need = false;
+  for (int32_t pt : M.parent) need = need || pt < 0 || pt >= nt;
+  if (need && nt > 0) {
+    std::vector<int64_t> nptr;
+    std::vector<int32_t> nlist;
+    node_tets(nn, nt, M.tets.data(), nptr, nlist);
+    for (size_t t = 0; t < M.parent.size(); t++) {
+      if (M.parent[t] >= 0 && M.parent[t] < nt) continue;
+      M.parent[t] = -1;
+      const int32_t a = M.tris[3 * t], b = M.tris[3 * t + 1], c = M.tris[3 * t + 2];
+      for (int64_t k = nptr[a]; k < nptr[a + 1] && M.parent[t] < 0; k++) {
+        const int32_t* e = M.tets.data() + 4 * (int64_t)nlist[k];
+        int hits = 0;
+        for (int v = 0; v < 4; v++) hits += (e[v] == b) + (e[v] == c);
+        if (hits == 2) M.parent[t] = nlist[k];
+      }
+    }
+  }
+  return MFS_OK;
+}
+}  // namespace
+
+extern "C" int mfs_read_gmsh(const char* path, int64_t* n_nodes, double* xyz, int64_t* n_tets, int32_t* tets,
+                             int32_t* material, int64_t* n_tris, int32_t* tris, int32_t* tri_tags,
+                             int32_t* tri_parent) {
+  if (!path || !n_nodes || !n_tets || !n_tris) {
+    set_error("mfs_read_gmsh: bad argument");
+    return MFS_ERR_ARG;
+  }
+  GmshMesh M;
+  const int st = parse_gmsh(path, M);
+  if (st != MFS_OK) return st;
+  const int64_t nn = (int64_t)M.xyz.size() / 3, nt = (int64_t)M.mat.size(), nf = (int64_t)M.tags.size();
+  const bool fill = xyz || tets || material || tris || tri_tags || tri_parent;
+  if (fill && (*n_nodes != nn || *n_tets != nt || *n_tris != nf)) {
+    set_error("mfs_read_gmsh: array sizes differ from the file's counts (call with NULL arrays first)");
+    return MFS_ERR_ARG;
+  }
+  *n_nodes = nn;
+  *n_tets = nt;
+  *n_tris = nf;
+  if (xyz) std::copy(M.xyz.begin(), M.xyz.end(), xyz);
+  if (tets) std::copy(M.tets.begin(), M.tets.end(), tets);
+  if (material) std::copy(M.mat.begin(), M.mat.end(), material);
+  if (tris) std::copy(M.tris.begin(), M.tris.end(), tris);
+  if (tri_tags) std::copy(M.tags.begin(), M.tags.end(), tri_tags);
+  if (tri_parent) std::copy(M.parent.begin(), M.parent.end(), tri_parent);
+  return MFS_OK;
+}

@@ -135,12 +135,20 @@ struct Tmpl {  // per distinct n: gather lists of interior rows x stencil
   int64_t ent0;  // offset into g_ent
 };
 
+// Task queues: one per XCD.  Workgroups are dispatched round-robin over the
+// 8 XCDs, so workgroup b pulls from queue b % 8 first; all three tasks of a
+// cell sit in the same queue, so the cell matrix they stream stays in one
+// XCD's L2.  An exhausted queue sends the workgroup on to the next one.
+constexpr int NQ = 8;
+constexpr int QSTRIDE = 32;  // counters 128 bytes apart
+
 struct SolveArgs {
   const Cell* cells;
   const int* task_cell;
   const int* task_group;
   int ntasks;
-  unsigned* queue;
+  int seg[NQ + 1];   // tasks [seg[q], seg[q+1]) belong to queue q
+  unsigned* queue;   // NQ counters, QSTRIDE apart
   const float* K32;
   const double* K64;
   const float* dinv32;   // [int0 + v][3]
@@ -612,8 +620,23 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
   T* R = (fp64 ? (T*)A.scr64 : (T*)A.scr32) + (int64_t)blockIdx.x * 3 * A.scr_stride;
   T* D = R + A.scr_stride;  // inner correction
   T* Q = D + A.scr_stride;  // A p
+  int tried = 0;  // queues found empty (thread 0)
+  const int home = blockIdx.x % NQ;
   for (;;) {
-    if (tid == 0) task_s = (int)__hip_atomic_fetch_add(A.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
+      int t = A.ntasks;
+      while (tried < NQ) {
+        const int q = (home + tried) % NQ;
+        const int k = (int)__hip_atomic_fetch_add(A.queue + q * QSTRIDE, 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        if (A.seg[q] + k < A.seg[q + 1]) {
+          t = A.seg[q] + k;
+          break;
+        }
+        tried++;
+      }
+      task_s = t;
+    }
     __syncthreads();
     // uniform across the workgroup: keep the task, its cell record and every
     // pointer derived from them in scalar registers
@@ -913,7 +936,7 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
 
 __global__ void k_batch_reset(unsigned* queue, int* flags, int nflags) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) *queue = 0u;
+  if (t < NQ) queue[t * QSTRIDE] = 0u;
   for (int i = t; i < nflags; i += gridDim.x * blockDim.x) flags[i] = 0;
 }
 
@@ -1436,6 +1459,7 @@ struct mfs_batch_s {
   int64_t total_int = 0, total_tets = 0;
   int max_n = 0, max_nint = 0;
   int ntasks = 0;
+  int seg[NQ + 1] = {};  // task ranges of the per-XCD queues
   bool assembled = false;
   int tuned_wg[2] = {0, 0}, tuned_per_cu[2] = {0, 0};  // per precision (0 mixed, 1 FP64)
   int tuned_var[2] = {-1, -1};                        // kernel variant (rows per thread)
@@ -1591,12 +1615,18 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   std::vector<int> order(d->n_cells);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->cells[a].n > B->cells[b].n; });
+  // cell of rank i goes to queue i % NQ (every queue sees the same size mix);
+  // within a queue largest cells first, the 3 tasks of a cell together
   std::vector<int> tc, tg;
-  for (int c : order)
-    for (int g = 0; g < NB / RG; g++) {
-      tc.push_back(c);
-      tg.push_back(g);
-    }
+  for (int q = 0; q < NQ; q++) {
+    B->seg[q] = (int)tc.size();
+    for (size_t i = q; i < order.size(); i += NQ)
+      for (int g = 0; g < NB / RG; g++) {
+        tc.push_back(order[i]);
+        tg.push_back(g);
+      }
+  }
+  B->seg[NQ] = (int)tc.size();
   B->ntasks = (int)tc.size();
   int st = MFS_OK;
   auto alloc = [&](void** p, size_t bytes) {
@@ -1619,7 +1649,7 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   alloc((void**)&B->d_flags, sizeof(int) * NB * d->n_cells);
   alloc((void**)&B->d_task_cell, sizeof(int) * B->ntasks);
   alloc((void**)&B->d_task_group, sizeof(int) * B->ntasks);
-  alloc((void**)&B->d_queue, 64);
+  alloc((void**)&B->d_queue, sizeof(unsigned) * NQ * QSTRIDE);
   if (B->trace == MFS_TRACE_REFERENCE) {
     alloc((void**)&B->d_et, sizeof(double) * B->et_total);
     alloc((void**)&B->d_ft, sizeof(double) * B->ft_total);
@@ -1787,6 +1817,11 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   A.task_cell = task_cell ? task_cell : B->d_task_cell;
   A.task_group = task_group ? task_group : B->d_task_group;
   A.ntasks = ntasks >= 0 ? ntasks : B->ntasks;
+  // a custom task list is one segment in queue 0; MFS_ONE_QUEUE=1 keeps the
+  // cell-size order in a single queue (A/B runs)
+  static const bool one_queue = std::getenv("MFS_ONE_QUEUE") && std::getenv("MFS_ONE_QUEUE")[0] == '1';
+  for (int q = 0; q <= NQ; q++)
+    A.seg[q] = ntasks >= 0 ? (q == 0 ? 0 : ntasks) : (one_queue ? (q == 0 ? 0 : B->ntasks) : B->seg[q]);
   A.queue = B->d_queue;
   A.K32 = B->d_K32;
   A.K64 = B->d_K64;

@@ -132,6 +132,51 @@ def test_gmsh_writer_layout(tmp_path):
     assert all(r[1] == 4 and r[3] == 0 for r in tet_rows)
 
 
+def test_gmsh_reader_round_trip_and_parent_search(tmp_path):
+    """The reader returns what the writer wrote (coordinates to the written
+    16 decimals, everything else exactly, parents included); a file in the
+    plain layout (header "2.2 0 8", no parent field) gets its parents from
+    the adjacency, checked here by brute force."""
+    x, t = mfs.mesh_box(3)
+    mat = (np.arange(len(t)) % 3).astype(np.int32)
+    tris = np.concatenate([mfs.box_face_tris(3, 3, 3, f) for f in (0, 5)])
+    tags = np.repeat(np.array([1, 6], np.int32), len(tris) // 2)
+    path = str(tmp_path / "rt.msh")
+    mfs.write_gmsh(path, x, t, mat, tris, tags)
+    x2, t2, m2, tr2, tg2, par = mfs.read_gmsh(path)
+    assert np.abs(x2 - x).max() < 1e-15
+    assert np.array_equal(t2, t) and np.array_equal(m2, mat)
+    assert np.array_equal(tr2, tris) and np.array_equal(tg2, tags)
+    for f, p_ in zip(tris, par):
+        assert set(f.tolist()) <= set(t[p_].tolist())
+    # plain layout: rewrite the header and drop the parent (5th field -> 0)
+    lines = open(path).read().split("\n")
+    lines[1] = "2.2 0 8"
+    out = []
+    for l in lines:
+        f = l.split()
+        if len(f) == 9 and f[1] == "2":
+            f[4] = "0"
+            l = "\t".join(f)
+        out.append(l)
+    plain = str(tmp_path / "plain.msh")
+    open(plain, "w").write("\n".join(out))
+    _, _, _, tr3, _, par3 = mfs.read_gmsh(plain)
+    assert np.array_equal(tr3, tris)
+    for f, p_ in zip(tris, par3):
+        owners = [k for k in range(len(t)) if set(f.tolist()) <= set(t[k].tolist())]
+        assert p_ in owners
+
+
+def test_gmsh_reader_rejects_bad_input(tmp_path):
+    bad = tmp_path / "bad.msh"
+    bad.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n2\n1 0 0 0\n5 1 1 1\n$EndNodes\n")
+    with pytest.raises(Exception):
+        mfs.read_gmsh(str(bad))
+    with pytest.raises(Exception):
+        mfs.read_gmsh(str(tmp_path / "missing.msh"))
+
+
 def test_error_reporting_without_device():
     code = mfs.lib.mfs_mesh_box_counts(0, 1, 1, None, None)
     assert code == -1

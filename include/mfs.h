@@ -263,6 +263,30 @@ int mfs_portrait(int64_t n_rows, int64_t n_tets, const int32_t* tets, int32_t* r
  * physical-tag field (the reader's domain id). */
 int mfs_write_gmsh(const char* path, int64_t n_nodes, const double* xyz, int64_t n_tets, const int32_t* tets,
                    const int32_t* material, int64_t n_tris, const int32_t* tris, const int32_t* tri_tags);
+/* Property file of the Mechanical driver, param_for_solver_%d.txt as
+ * MechanicalMain.cpp:859-1201 reads it: log flag, threads, mesh format (10:
+ * ReadFromMSH, 9: ReadFromMSH_v2208), mesh path, critical residual, the
+ * Dirichlet rows (tag digit; per component a value, or '*' when free), the
+ * Neumann rows (tag, value, direction; direction 0 0 0 makes the value a
+ * normal pressure), the materials (E, nu per domain) and the result
+ * directory.  Tags are single digits in this format. */
+typedef struct {
+  int tag;
+  int fixed[3];     /* 1: component prescribed, 0: free                     */
+  double value[3];
+} mfs_dirichlet_spec;
+typedef struct {
+  int tag;
+  double value;
+  double direction[3];
+} mfs_neumann_spec;
+typedef struct {
+  double E, nu;
+} mfs_material_spec;
+int mfs_write_solver_params(const char* path, int log_flag, int threads, int mesh_format, const char* mesh_path,
+                            double critical_residual, int n_dirichlet, const mfs_dirichlet_spec* dirichlet,
+                            int n_neumann, const mfs_neumann_spec* neumann, int n_materials,
+                            const mfs_material_spec* materials, const char* result_dir);
 /* Read a Gmsh 2.2 ASCII mesh with SimpleGrid::ReadFromMSH's rules
  * (Library/Math.h:327-520): 9-integer type-2 rows are boundary triangles
  * (tag = 4th field; parent tet = 5th field in the "modified" layout, whose

@@ -205,6 +205,8 @@ _sig("mfs_msfem_coarse_system", C.c_int, [vp, c_dp, c_dp, C.c_int64, c_i32p, c_d
                                           C.POINTER(vp)])
 _sig("mfs_msfem_top_traction", C.c_int, [vp, c_dp, c_dp])
 _sig("mfs_msfem_downscale", C.c_int, [vp, c_dp, C.c_int, c_dp])
+_sig("mfs_write_solver_params", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_double, C.c_int,
+                                          vp, C.c_int, vp, C.c_int, vp, C.c_char_p])
 _sig("mfs_read_gmsh", C.c_int, [C.c_char_p, c_i64p, c_dp, c_i64p, c_i32p, c_i32p, c_i64p, c_i32p, c_i32p, c_i32p])
 _sig("mfs_msfem_error", C.c_int, [vp, c_dp, c_dp, c_dp])
 _sig("mfs_batch_trace_status", C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)])
@@ -306,6 +308,43 @@ def write_gmsh(path, xyz, tets, material=None, tris=None, tri_tags=None):
                             ptr(i32(tets), C.c_int32),
                             ptr(i32(material), C.c_int32) if material is not None else None, len(tris),
                             ptr(tris, C.c_int32), ptr(tri_tags, C.c_int32)), "write_gmsh")
+
+
+class DirichletSpec(C.Structure):
+    _fields_ = [("tag", C.c_int), ("fixed", C.c_int * 3), ("value", C.c_double * 3)]
+
+
+class NeumannSpec(C.Structure):
+    _fields_ = [("tag", C.c_int), ("value", C.c_double), ("direction", C.c_double * 3)]
+
+
+class MaterialSpec(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double)]
+
+
+def write_solver_params(path, mesh_path, result_dir, dirichlet=(), neumann=(), materials=((1e9, 0.3),),
+                        threads=16, critical_residual=1e-8, mesh_format=10, log_flag=0):
+    """Write the Mechanical driver's property file.  dirichlet: (tag,
+    (vx|None, vy|None, vz|None)); neumann: (tag, value, (dx, dy, dz));
+    materials: (E, nu) per domain."""
+    D = (DirichletSpec * max(1, len(dirichlet)))()
+    for i, (tag, vals) in enumerate(dirichlet):
+        D[i].tag = tag
+        for a, v in enumerate(vals):
+            D[i].fixed[a] = 0 if v is None else 1
+            D[i].value[a] = 0.0 if v is None else float(v)
+    N = (NeumannSpec * max(1, len(neumann)))()
+    for i, (tag, val, d) in enumerate(neumann):
+        N[i].tag, N[i].value = tag, float(val)
+        for a in range(3):
+            N[i].direction[a] = float(d[a])
+    M = (MaterialSpec * len(materials))()
+    for i, (E, nu) in enumerate(materials):
+        M[i].E, M[i].nu = float(E), float(nu)
+    _chk(lib.mfs_write_solver_params(path.encode(), log_flag, threads, mesh_format, mesh_path.encode(),
+                                     critical_residual, len(dirichlet), C.cast(D, vp), len(neumann),
+                                     C.cast(N, vp), len(materials), C.cast(M, vp), result_dir.encode()),
+         "mfs_write_solver_params")
 
 
 def read_gmsh(path):

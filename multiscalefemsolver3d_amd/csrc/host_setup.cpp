@@ -616,3 +616,64 @@ extern "C" int mfs_read_gmsh(const char* path, int64_t* n_nodes, double* xyz, in
   if (tri_parent) std::copy(M.parent.begin(), M.parent.end(), tri_parent);
   return MFS_OK;
 }
+
+// Property file of the Mechanical driver (param_for_solver_%d.txt, read by
+// MechanicalMain.cpp:859-1201): one item per line, no comments (the first
+// line is read with fscanf and a comment there would be taken as the next
+// line).  Dirichlet rows: the tag as one character, then per component a
+// value followed by a space or '*' for a free component; the reader only
+// takes a value when a space follows it (:963-1000).  Neumann rows: tag,
+// value, direction (0 0 0: the value is a normal pressure, :1040-1080).
+extern "C" int mfs_write_solver_params(const char* path, int log_flag, int threads, int mesh_format,
+                                       const char* mesh_path, double critical_residual, int n_dirichlet,
+                                       const mfs_dirichlet_spec* dirichlet, int n_neumann,
+                                       const mfs_neumann_spec* neumann, int n_materials,
+                                       const mfs_material_spec* materials, const char* result_dir) {
+  if (!path || !mesh_path || !result_dir || threads < 1 || (mesh_format != 9 && mesh_format != 10) ||
+      n_dirichlet < 0 || n_neumann < 0 || n_materials < 1 || (n_dirichlet > 0 && !dirichlet) ||
+      (n_neumann > 0 && !neumann) || !materials) {
+    set_error("mfs_write_solver_params: bad argument");
+    return MFS_ERR_ARG;
+  }
+  for (int i = 0; i < n_dirichlet; i++)
+    if (dirichlet[i].tag < 0 || dirichlet[i].tag > 9) {
+      set_error("mfs_write_solver_params: boundary tags are one digit (0..9) in this format");
+      return MFS_ERR_ARG;
+    }
+  for (int i = 0; i < n_neumann; i++)
+    if (neumann[i].tag < 0 || neumann[i].tag > 9) {
+      set_error("mfs_write_solver_params: boundary tags are one digit (0..9) in this format");
+      return MFS_ERR_ARG;
+    }
+  for (const char* q : {mesh_path, result_dir})
+    if (std::strchr(q, '#') || std::strchr(q, '\n')) {
+      set_error("mfs_write_solver_params: paths may not contain '#' or a newline");
+      return MFS_ERR_ARG;
+    }
+  FILE* f = std::fopen(path, "w");
+  if (!f) {
+    set_error(std::string("mfs_write_solver_params: cannot open ") + path);
+    return MFS_ERR_ARG;
+  }
+  std::fprintf(f, "%d\n%d\n%d\n%s\n%.17g\n%d\n", log_flag ? 1 : 0, threads, mesh_format, mesh_path,
+               critical_residual, n_dirichlet);
+  for (int i = 0; i < n_dirichlet; i++) {
+    std::fprintf(f, "%d", dirichlet[i].tag);
+    for (int a = 0; a < 3; a++) {
+      if (dirichlet[i].fixed[a])
+        std::fprintf(f, " %.17g ", dirichlet[i].value[a]);
+      else
+        std::fprintf(f, " *");
+    }
+    std::fprintf(f, "\n");
+  }
+  std::fprintf(f, "%d\n", n_neumann);
+  for (int i = 0; i < n_neumann; i++)
+    std::fprintf(f, "%d %.17g %.17g %.17g %.17g\n", neumann[i].tag, neumann[i].value, neumann[i].direction[0],
+                 neumann[i].direction[1], neumann[i].direction[2]);
+  std::fprintf(f, "%d\n", n_materials);
+  for (int i = 0; i < n_materials; i++) std::fprintf(f, "%.17g %.17g\n", materials[i].E, materials[i].nu);
+  std::fprintf(f, "%s\n", result_dir);
+  std::fclose(f);
+  return MFS_OK;
+}

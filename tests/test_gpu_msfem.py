@@ -349,3 +349,30 @@ def test_msfem_vs_fem_error_metric(ctx, trace, form):
     assert np.all(np.isfinite(err))
     if form == mfs.BASIS_PER_DIRECTION:
         assert err[2] < 0.2, err
+
+
+@pytest.mark.slow
+def test_reference_traces_16cubed_full_batch(ctx):
+    """The 16^3-cell batch (98,304 volume problems, 294,912 face problems)
+    with reference traces: every face solve within the trace tolerance,
+    every volume problem converged, spot cells of two sizes against the
+    oracle's hierarchy, and symmetric coarse matrices with translations in
+    their null space."""
+    p = problems.msfem_cells(16)
+    B = make_ref_batch(ctx, p)
+    worst, bad, ms = B.trace_status()
+    assert bad == 0 and worst < 1e-11
+    rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-8)
+    assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-8
+    for n in np.unique(p["cell_n"])[:2]:
+        c = int(np.nonzero(p["cell_n"] == n)[0][0])
+        basis_o, _, _ = orc.msfem_cell_reftrace(int(n), p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"],
+                                                tol=1e-12)
+        err = np.linalg.norm(B.basis(c) - basis_o) / np.linalg.norm(basis_o)
+        assert err < 1e-6, (c, n, err)
+    Kc = B.coarse_matrices()
+    assert np.abs(Kc - np.transpose(Kc, (0, 2, 1))).max() < 1e-6 * np.abs(Kc).max()
+    for d in range(3):
+        t = np.zeros(24)
+        t[d::3] = 1
+        assert np.abs(Kc @ t).max() < 1e-5 * np.abs(Kc).max()

@@ -177,6 +177,65 @@ def test_gmsh_reader_rejects_bad_input(tmp_path):
         mfs.read_gmsh(str(tmp_path / "missing.msh"))
 
 
+def parse_params_like_the_reference(text):
+    """MechanicalMain.cpp:859-1201 restated: first line by %d; then lines up
+    to '\n' or '#', empty ones skipped; the tag is the line's first
+    character; a Dirichlet value counts only when a space follows it, '*'
+    frees a component; Neumann rows are value + direction."""
+    lines = text.split("\n")
+    first = int(lines[0].split()[0])
+    rest = [l.split("#")[0] for l in lines[1:]]
+    rest = [l for l in rest if l.strip()]
+    it = iter(rest)
+    out = {"log": first, "threads": int(next(it).split()[0]), "format": int(next(it).split()[0]),
+           "mesh": next(it), "residual": float(next(it).split()[0])}
+    nd = int(next(it).split()[0])
+    out["dirichlet"] = []
+    for _ in range(nd):
+        line = next(it)
+        tag = int(line[0])
+        vals, tok, jj = [None, None, None], "", 0
+        for ch in line[1:]:
+            if ch == "*":
+                jj += 1
+            elif ch == " ":
+                if tok:
+                    vals[jj] = float(tok)
+                    jj += 1
+                    tok = ""
+            elif ch != "\t":
+                tok += ch
+        out["dirichlet"].append((tag, tuple(vals)))
+    nn = int(next(it).split()[0])
+    out["neumann"] = []
+    for _ in range(nn):
+        line = next(it)
+        f = [float(v) for v in line[1:].split()]
+        out["neumann"].append((int(line[0]), f[0], tuple(f[1:4])))
+    nm = int(next(it).split()[0])
+    out["materials"] = [tuple(float(v) for v in next(it).split()[:2]) for _ in range(nm)]
+    out["result"] = next(it)
+    return out
+
+
+def test_solver_params_writer_reads_back_with_the_reference_rules(tmp_path):
+    path = str(tmp_path / "param_for_solver_0.txt")
+    dirichlet = [(5, (0.0, 0.0, 0.0)), (3, (None, 1.25e-3, None))]
+    neumann = [(6, 1e6, (0.0, 0.0, -1.0)), (2, 2.5e5, (0.0, 0.0, 0.0))]
+    materials = [(1e9, 0.3), (2e11, 0.25)]
+    mfs.write_solver_params(path, "/data/mesh/cube.msh", "/data/out", dirichlet, neumann, materials, threads=16,
+                            critical_residual=1e-8)
+    got = parse_params_like_the_reference(open(path).read())
+    assert got["log"] == 0 and got["threads"] == 16 and got["format"] == 10
+    assert got["mesh"] == "/data/mesh/cube.msh" and got["result"] == "/data/out"
+    assert got["residual"] == 1e-8
+    assert got["dirichlet"] == dirichlet
+    assert got["neumann"] == neumann
+    assert got["materials"] == materials
+    with pytest.raises(Exception):
+        mfs.write_solver_params(path, "m.msh", "out", [(12, (0, 0, 0))])
+
+
 def test_error_reporting_without_device():
     code = mfs.lib.mfs_mesh_box_counts(0, 1, 1, None, None)
     assert code == -1

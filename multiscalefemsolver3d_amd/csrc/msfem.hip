@@ -956,8 +956,10 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
   // GT tets per round: three barriers per round instead of per tet
   constexpr int GT = 8;
   __shared__ double phi[GT][12][NB];  // element values of the 24 bases
-  __shared__ double Ke[GT][12][12];
+  __shared__ double KeS[6][12][12];   // unit-modulus element matrix of each of the 6 tet shapes
   __shared__ double KP[GT][12][NB];
+  __shared__ double Eg[GT];
+  __shared__ int Pg[GT];
   const int cell = blockIdx.x, tid = threadIdx.x;
   const Cell C = cells[cell];
   const int n = C.n, m = n - 1;
@@ -969,6 +971,18 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
   const int wv = tid >> 6, ln = tid & 63, ti = ln >> 3, tj = ln & 7;
   double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   const int ntet = 6 * n * n * n;
+  // every tet of the cell is a translate of one of 6 shapes: Ke = E_t KeS[shape]
+  if (tid < 6 * 16) {
+    const int p = tid / 16, I = (tid / 4) % 4, J = tid % 4;
+    int ijk[4][3];
+    tet_nodes(p, n, ijk);  // hex 0, shape p
+    double gr[4][3], V;
+    tet_grads(ijk, hf, gr, V);
+    double Kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    elastic_block(gr[I], gr[J], V * lamf, V * muf, econv, Kb);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) KeS[p][3 * I + a][3 * J + b] = Kb[a][b];
+  }
   for (int t0 = 0; t0 < ntet; t0 += GT) {
     // element basis values of GT tets (zero past the last tet)
     for (int q = tid; q < GT * 12 * NB; q += blockDim.x) {
@@ -995,21 +1009,11 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
       }
       phi[g][p][bq] = val;
     }
-    // element matrices: GT x 16 blocks
-    if (tid < GT * 16) {
-      const int g = tid / 16, I = (tid / 4) % 4, J = tid % 4;
-      const int t = t0 + g;
-      double Kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      if (t < ntet) {
-        int ijk[4][3];
-        tet_nodes(t, n, ijk);
-        double gr[4][3], V;
-        tet_grads(ijk, hf, gr, V);
-        const double Et = E[C.tet0 + t];
-        elastic_block(gr[I], gr[J], V * Et * lamf, V * Et * muf, econv, Kb);
-      }
-      for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) Ke[g][3 * I + a][3 * J + b] = Kb[a][b];
+    // moduli and shapes of the round's tets (modulus 0 past the last tet)
+    if (tid < GT) {
+      const int t = t0 + tid;
+      Eg[tid] = t < ntet ? E[C.tet0 + t] : 0.0;
+      Pg[tid] = t < ntet ? t % 6 : 0;
     }
     __syncthreads();
     if (form == MFS_BASIS_COMPONENTWISE) {
@@ -1025,13 +1029,15 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
       // KP = Ke phi: one 3x3 tile (rows 3pb.., columns 3cb..) of one tet per thread
       static_assert(GT * 4 * 8 == 256, "one KP tile per thread");
       const int g = tid >> 5, pb = (tid >> 3) & 3, cb = tid & 7;
+      const int shp = Pg[g];
+      const double et = Eg[g];
       double kp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
       for (int u = 0; u < 12; u++) {
         double kr[3], pr[3];
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-          kr[r] = Ke[g][3 * pb + r][u];
+          kr[r] = KeS[shp][3 * pb + r][u];
           pr[r] = phi[g][u][3 * cb + r];
         }
 #pragma unroll
@@ -1042,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_galerkin(const Cell* __restrict__ cells
 #pragma unroll
       for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) KP[g][3 * pb + r][3 * cb + c] = kp[r][c];
+        for (int c = 0; c < 3; c++) KP[g][3 * pb + r][3 * cb + c] = et * kp[r][c];
     }
     __syncthreads();
 #pragma unroll

@@ -376,3 +376,19 @@ def test_reference_traces_16cubed_full_batch(ctx):
         t = np.zeros(24)
         t[d::3] = 1
         assert np.abs(Kc @ t).max() < 1e-5 * np.abs(Kc).max()
+
+
+@pytest.mark.parametrize("n", [2, 12])
+def test_reference_traces_extreme_subdivisions(ctx, n):
+    """Smallest (one interior face node) and largest (148 KB of LDS per
+    face workgroup) cells of the reference-trace mode against the oracle."""
+    p = problems.msfem_cells(1, n_fixed=n, seed_E=5)
+    B = make_ref_batch(ctx, p)
+    worst, bad, _ = B.trace_status()
+    assert bad == 0 and worst < 1e-11
+    rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-10)
+    assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-10
+    basis_o, Kc_o, _ = orc.msfem_cell_reftrace(n, p["origin"][0], p["h"], problems.cell_E(p, 0), p["nu"], tol=1e-13)
+    assert np.linalg.norm(B.basis(0) - basis_o) / np.linalg.norm(basis_o) < 1e-8
+    Kc = B.coarse_matrices()[0]
+    assert np.abs(Kc - Kc_o).max() < 1e-7 * np.abs(Kc_o).max()

@@ -1120,8 +1120,9 @@ __global__ void k_edge_profiles(const Cell* __restrict__ cells, int ncells, cons
   const Cell C = cells[cell];
   const int n = C.n;
   const int dir = e / 4, b1 = e & 1, b2 = (e >> 1) & 1;
-  const int o1 = dir == 0 ? 1 : 0, o2 = dir == 2 ? 1 : 2;  // other axes, increasing
-  // first face: axis o1, side b1; the edge's fixed coordinate on o2 is b2*n
+  const int o1 = dir == 0 ? 1 : 0;  // lower of the two other axes
+  // first face: axis o1, side b1; the edge's fixed coordinate on the higher
+  // other axis is b2*n
   int A1, A2;
   face_axes(o1, A1, A2);
   double* out = et + C.et0 + (int64_t)e * (n + 1) * 2;
@@ -1160,7 +1161,6 @@ __global__ void k_edge_profiles(const Cell* __restrict__ cells, int ncells, cons
 // recomputed per node) checks every solve against the trace tolerance.
 // ---------------------------------------------------------------------------
 constexpr int FS = 7;       // face stencil points
-constexpr int F_CENTER = 3;
 constexpr int FRHS = 12;    // face bases: 4 corners x 3 drives
 constexpr int F_THREADS = 256;
 constexpr int kMaxFaceBw = 3 * MAXN + 2;  // face_bw(MAXN - 1)
@@ -2379,11 +2379,10 @@ extern "C" int mfs_msfem_top_traction(mfs_batch B, const double* t, double* load
 // Fine displacement of one cell from coarse nodal values (3 per coarse
 // node): u = sum_{q,d} u_c[node(q)][d] phi_{q,d}, the downscaling the
 // Mechanical driver applies (MechanicalMain.cpp:2176-2181).
-extern "C" int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u_fine) {
-  MFS_CHECK_ARG(B && u_coarse && u_fine && cell >= 0 && cell < B->ncells, "mfs_msfem_downscale: bad argument");
-  int nx, ny, nz;
-  std::vector<int> gi;
-  MFS_TRY(coarse_grid(B, &nx, &ny, &nz, gi));
+// u_fine of one cell from the coarse nodal values, the cell grid positions
+// gi known (coarse_grid).
+static int downscale_cell(mfs_batch B, const double* u_coarse, int cell, int nx, int ny,
+                          const std::vector<int>& gi, double* u_fine) {
   const int sx = nx + 1, sy = ny + 1;
   const int n = B->cells[cell].n;
   const int64_t N = 3ll * (n + 1) * (n + 1) * (n + 1);
@@ -2403,6 +2402,14 @@ extern "C" int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell
   return MFS_OK;
 }
 
+extern "C" int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u_fine) {
+  MFS_CHECK_ARG(B && u_coarse && u_fine && cell >= 0 && cell < B->ncells, "mfs_msfem_downscale: bad argument");
+  int nx, ny, nz;
+  std::vector<int> gi;
+  MFS_TRY(coarse_grid(B, &nx, &ny, &nz, gi));
+  return downscale_cell(B, u_coarse, cell, nx, ny, gi, u_fine);
+}
+
 // MsFEM-vs-FEM relative error per component (MechanicalMain.cpp:2112-2221):
 // one-point rule per fine tet (centroid, weight V), the MsFEM field from the
 // downscaled cell solution and the FEM field from the global fine grid.
@@ -2419,7 +2426,7 @@ extern "C" int mfs_msfem_error(mfs_batch B, const double* u_coarse, const double
   double num[3] = {0, 0, 0}, den[3] = {0, 0, 0};
   std::vector<double> uf(3ull * s * s * s);
   for (int c = 0; c < B->ncells; c++) {
-    MFS_TRY(mfs_msfem_downscale(B, u_coarse, c, uf.data()));
+    MFS_TRY(downscale_cell(B, u_coarse, c, nx, ny, gi, uf.data()));
     for (int t = 0; t < 6 * n * n * n; t++) {
       const int hex = t / 6, p = t % 6;
       int cc[3] = {hex % n, (hex / n) % n, hex / (n * n)};

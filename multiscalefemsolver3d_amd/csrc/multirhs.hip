@@ -110,10 +110,10 @@ __device__ __forceinline__ void vstore(T* __restrict__ p, const T (&o)[KR]) {
 // sum over the row's blocks of val(a, b) * X_j(BS*col + b), X supplied by
 // xl(i, out[KR]).  Indices and values of U entries are loaded before their
 // operands are gathered (see sell_row_x).
-template <typename T, int BS, int KR, typename XL>
+template <typename T, int BS, int KR, typename XL, typename CI>
 __device__ __forceinline__ void sell_row_multi(const int64_t* __restrict__ slice_base,
                                                const int32_t* __restrict__ slice_width,
-                                               const int32_t* __restrict__ col, const T* __restrict__ val,
+                                               const CI* __restrict__ col, const T* __restrict__ val,
                                                int64_t slice, int lane, XL xl, T (&acc)[BS][KR]) {
   constexpr int C = 64;
   constexpr int U = 2;
@@ -124,7 +124,8 @@ __device__ __forceinline__ void sell_row_multi(const int64_t* __restrict__ slice
     for (int j = 0; j < KR; j++) acc[a][j] = 0;
   const int64_t sb = slice_base[slice];
   const int w = slice_width[slice];
-  const int32_t* cp = col + sb + lane;
+  const CI* cp = col + sb + lane;
+  const int32_t rbase = sizeof(CI) == 2 ? (int32_t)(slice * C + lane) : 0;  // row-relative 16-bit columns
   const T* vp = val + sb * B2 + lane;
   for (int j0 = 0; j0 < w; j0 += U) {
     int32_t c[U];
@@ -133,7 +134,7 @@ __device__ __forceinline__ void sell_row_multi(const int64_t* __restrict__ slice
     for (int u = 0; u < U; u++) {
       const int jj = min(j0 + u, w - 1);
       const bool in = j0 + u < w;
-      c[u] = __builtin_nontemporal_load(cp + (int64_t)jj * C);
+      c[u] = (int32_t)__builtin_nontemporal_load(cp + (int64_t)jj * C) + rbase;
 #pragma unroll
       for (int e = 0; e < B2; e++) {
         const T t = __builtin_nontemporal_load(vp + ((int64_t)jj * B2 + e) * C);
@@ -258,10 +259,10 @@ __global__ void k_mr_state(MrState* st, const double* tol2, int k, int maxit) {
 // K1: stop tests and beta per system from K2's partials; P_new = Z + beta
 // P_old (own rows written, gathered rows formed on the fly); Q = A P_new;
 // partials p.q per system.
-template <typename T, int BS, int KR>
+template <typename T, int BS, int KR, typename CI>
 __global__ __launch_bounds__(256) void k_mr_dir_spmm(const int64_t* __restrict__ slice_base,
                                                      const int32_t* __restrict__ slice_width,
-                                                     const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                     const CI* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ Z, const T* __restrict__ P_old,
                                                      T* __restrict__ P_new, T* __restrict__ Q, int64_t nb,
                                                      int64_t n_slices, MrState* st, const double* __restrict__ part_rz,
@@ -485,12 +486,21 @@ struct MrEngine {
     T* p_new = P[parity ^ 1];
     const double* rz_in = fin ? tot_rz : part_rz;
     const unsigned rz_g = fin ? 1u : g_vec;
-    if (A->bs == 1)
-      k_mr_dir_spmm<T, 1, KR><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, Z, p_old,
-                                                     p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq);
+    // 16-bit row-relative columns when the matrix has them
+    if (A->d_col16 && A->bs == 1)
+      k_mr_dir_spmm<T, 1, KR, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val,
+                                                              Z, p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g,
+                                                              part_pq);
+    else if (A->d_col16)
+      k_mr_dir_spmm<T, 3, KR, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val,
+                                                              Z, p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g,
+                                                              part_pq);
+    else if (A->bs == 1)
+      k_mr_dir_spmm<T, 1, KR, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, Z,
+                                                              p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq);
     else
-      k_mr_dir_spmm<T, 3, KR><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, Z, p_old,
-                                                     p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq);
+      k_mr_dir_spmm<T, 3, KR, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, Z,
+                                                              p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq);
     if (fin) k_finish_sums_multi<<<KR, 256, 0, s>>>(part_pq, g_spmv, tot_pq);
     k_mr_update<T, KR><<<g_vec, 256, 0, s>>>(X, p_new, R, Z, Q, dinv, n, st, fin ? tot_pq : part_pq,
                                              fin ? 1u : g_spmv, part_rz);

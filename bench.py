@@ -190,6 +190,7 @@ def main():
                          "serial dots/sweeps) on the same system: " + sample}
         log(f"[bench] cpu baseline {value:.0f} ms ({sample})")
 
+    nonsym = bench_bicgstab(mfs, problems, ctx, cpu=not args.no_cpu)
     msfem = None if args.no_msfem else bench_msfem(mfs, problems, ctx, cpu=not args.no_cpu)
     learned_res = None if args.no_learned else bench_learned(mfs, ctx, args.learned_test_cells)
     large = None if args.no_large else bench_large(mfs, problems, ctx, args.large_n, not args.large_spmv_only)
@@ -238,6 +239,7 @@ def main():
         "k_cg_update_f32": {"avg_us": round(k2_ms * 1e3, 3), "bytes_per_launch": k2_bytes,
                             "gbs": round(k2_gbs, 1)},
         "multi_rhs_6": multi6,
+        "bicgstab_nonsymmetric": nonsym,
         "spmv_f32_gbs": round(spmv_gbs, 1),
         "spmv_f32_us": round(spmv_ms * 1e3, 3),
         "cpu_baseline": cpu,
@@ -247,6 +249,41 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_bicgstab(mfs, problems, ctx, cpu=True):
+    """Non-symmetric solve (the BiCG_Stab surface): upwind convection-
+    diffusion on a 512^2 grid (262,144 unknowns, Peclet 200) to 1e-8 with
+    right-Jacobi BiCGStab, FP64 and mixed, against GMRES(30) mixed; the
+    oracle's BiCGStab on the host threads as the CPU reference (a bounded
+    sample of iterations, projected)."""
+    import numpy as np
+    rp, col, val, b = problems.convection_diffusion(512, 200.0, seed=9)
+    n = len(b)
+    A = mfs.Mat.csr(ctx, rp, col, val)
+    bv = mfs.Vec(ctx, n, mfs.F64, b)
+    res = {"unknowns": n, "nnz": int(len(val)), "tol": 1e-8}
+    for name, method, prec in (("bicgstab_fp64", mfs.BICGSTAB, mfs.PREC_F64),
+                               ("bicgstab_mixed", mfs.BICGSTAB, mfs.PREC_MIXED),
+                               ("gmres30_mixed", mfs.GMRES, mfs.PREC_MIXED)):
+        S = mfs.Solver(A, method=method, precision=prec, tol=1e-8, maxit=20000)
+        x = mfs.Vec(ctx, n, mfs.F64, np.zeros(n))
+        S.run(bv, x, check=False)  # warm (graph capture, first launches)
+        x.upload(np.zeros(n))
+        r = S.run(bv, x, check=False)
+        res[name] = {"ms": round(r.seconds * 1e3, 2), "iterations": r.iterations, "outer": r.outer_steps,
+                     "true_relres": r.true_relres, "status": r.status}
+    if cpu:
+        import oracle as orc
+        t0 = time.perf_counter()
+        _, it_c, _ = orc.bicgstab(rp, col, val, b, tol=1e-8, maxit=200)
+        sec = time.perf_counter() - t0
+        it_g = res["bicgstab_fp64"]["iterations"]
+        res["cpu_oracle_bicgstab"] = {"ms_projected": round(sec / max(it_c, 1) * it_g * 1e3, 1),
+                                      "sample": f"{it_c} iterations timed ({sec:.2f}s, 1 thread), projected to "
+                                                f"{it_g} (the FP64 GPU count)"}
+    log(f"[bench] non-symmetric (BiCGStab): {res}")
+    return res
 
 
 def bench_msfem(mfs, problems, ctx, cpu=True):

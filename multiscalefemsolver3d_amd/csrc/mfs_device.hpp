@@ -247,6 +247,59 @@ __device__ __forceinline__ void sell_row_x(const int64_t* __restrict__ slice_bas
   }
 }
 
+// Row products with two operands in one pass over the matrix: a1 = A_row·x1,
+// a2 = A_row·x2 (same batching as sell_row_x).  Nothing here depends on a
+// scalar computed by the caller, so a kernel can issue these loads before it
+// has finished reducing the previous kernel's partial sums.
+template <typename T, int BS, typename CI>
+__device__ __forceinline__ void sell_row_pair(const int64_t* __restrict__ slice_base,
+                                              const int32_t* __restrict__ slice_width, const CI* __restrict__ col,
+                                              const T* __restrict__ val, const T* __restrict__ x1,
+                                              const T* __restrict__ x2, int64_t slice, int lane, T (&a1)[BS],
+                                              T (&a2)[BS]) {
+  constexpr int C = 64;
+  constexpr int U = BS == 1 ? 8 : 4;
+  constexpr int B2 = BS * BS;
+#pragma unroll
+  for (int a = 0; a < BS; a++) a1[a] = a2[a] = 0;
+  const int64_t sb = slice_base[slice];
+  const int w = slice_width[slice];
+  const CI* cp = col + sb + lane;
+  const int32_t rbase = sizeof(CI) == 2 ? (int32_t)(slice * C + lane) : 0;
+  const T* vp = val + sb * B2 + lane;
+  for (int j0 = 0; j0 < w; j0 += U) {
+    int32_t c[U];
+    T v[U][B2];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = min(j0 + u, w - 1);
+      const bool in = j0 + u < w;
+      c[u] = (int32_t)__builtin_nontemporal_load(cp + (int64_t)j * C) + rbase;
+#pragma unroll
+      for (int e = 0; e < B2; e++) {
+        const T t = __builtin_nontemporal_load(vp + ((int64_t)j * B2 + e) * C);
+        v[u][e] = in ? t : (T)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      T y1[BS], y2[BS];
+#pragma unroll
+      for (int b = 0; b < BS; b++) {
+        y1[b] = x1[(int64_t)BS * c[u] + b];
+        y2[b] = x2[(int64_t)BS * c[u] + b];
+      }
+#pragma unroll
+      for (int a = 0; a < BS; a++)
+#pragma unroll
+        for (int b = 0; b < BS; b++) {
+          a1[a] += v[u][BS * a + b] * y1[b];
+          a2[a] += v[u][BS * a + b] * y2[b];
+        }
+    }
+  }
+}
+
 template <typename T, int BS, typename CI>
 __device__ __forceinline__ void sell_row(const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
                                          const CI* __restrict__ col, const T* __restrict__ val,

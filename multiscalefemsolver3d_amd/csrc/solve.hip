@@ -67,6 +67,42 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, un
   __syncthreads();
 }
 
+// The same sum split in two so that a kernel can overlap the partials' load
+// latency with its own loads: partials_issue loads the first P partials of
+// each value per thread (G <= 256*P needs nothing more), partials_finish adds
+// any further ones and reduces over the block.  The order is the same in
+// every block, so every block still obtains bit-identical totals.
+template <int NV, int P>
+__device__ __forceinline__ void partials_issue(const double* __restrict__ part, unsigned G, double (&pre)[NV][P]) {
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int u = 0; u < P; u++) {
+      const unsigned b = threadIdx.x + u * 256;
+      pre[k][u] = b < G ? part[(size_t)k * G + b] : 0.0;
+    }
+}
+
+template <int NV, int P>
+__device__ __forceinline__ void partials_finish(const double* __restrict__ part, unsigned G,
+                                                const double (&pre)[NV][P], double (&tot)[NV]) {
+  __shared__ double red[NV][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = pre[k][0];
+#pragma unroll
+    for (int u = 1; u < P; u++) s += pre[k][u];
+    for (unsigned b = threadIdx.x + P * 256; b < G; b += 256) s += part[(size_t)k * G + b];
+    s = wave_sum(s);
+    if (lane == 0) red[k][wave] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) tot[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+  __syncthreads();
+}
+
 // Block partial of NV values into part[k*gridDim + blockIdx].
 template <typename T, int NV>
 __device__ __forceinline__ void block_partials(T (&v)[NV], double* __restrict__ part) {
@@ -84,8 +120,12 @@ __device__ __forceinline__ void block_partials(T (&v)[NV], double* __restrict__ 
 }
 
 // K1: from K2's partials (r.z, r.r): stop test, beta = rz / rz_prev;
-// p_new = z + beta p_old (own rows written, gathered rows formed on the
-// fly), q = A p_new, per-block partial of p_new.q.
+// p_new = z + beta p_old (own rows written), q = A p_new, per-block partial
+// of p_new.q.  Scalar rows compute q = A z + beta A p_old (both products in
+// one pass over the matrix): those loads do not depend on beta, so they are
+// issued while the partials of the previous kernel are still in flight.
+// 3x3 blocks form p_new on the fly for the gathered columns once beta is
+// known (two operand gathers per block column would cost 30 more registers).
 template <typename T, int BS, typename CI>
 __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__ slice_base,
                                                      const int32_t* __restrict__ slice_width,
@@ -94,9 +134,25 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
                                                      T* __restrict__ p_new, T* __restrict__ q, int64_t nb,
                                                      int64_t n_slices, CgState* st, const double* __restrict__ part_rz,
                                                      unsigned g_rz, double* __restrict__ part_pq) {
-  if (st->done) return;
+  const int was_done = st->done;
+  double pre[2][2];
+  partials_issue<2, 2>(part_rz, g_rz, pre);
+  if (was_done) return;
+  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t r = slice * 64 + lane;
+  T az[BS], ap[BS], zo[BS], po[BS];
+#pragma unroll
+  for (int a = 0; a < BS; a++) az[a] = ap[a] = zo[a] = po[a] = 0;
+  if constexpr (BS == 1) {
+    if (slice < n_slices) sell_row_pair<T, BS>(slice_base, slice_width, col, val, z, p_old, slice, lane, az, ap);
+    if (r < nb) {
+      zo[0] = z[r];
+      po[0] = p_old[r];
+    }
+  }
   double t[2];
-  sum_partials<2>(part_rz, g_rz, t);
+  partials_finish<2, 2>(part_rz, g_rz, pre, t);
   const double rz = t[0], rr = t[1];
   const int iter = st->iter2;
   int done = CG_RUN;
@@ -107,7 +163,8 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
   else if (iter >= st->maxit)
     done = CG_MAXIT;
   const T beta = (iter > 0 || st->warm) ? (T)(rz / st->rz2) : (T)0;
-  __syncthreads();
+  // every thread of this block has read the stop flag (partials_finish
+  // ends with a barrier) before block 0 rewrites it
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rz1 = rz;
     st->rr = rr;
@@ -115,44 +172,64 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
     st->done = done;
   }
   if (done != CG_RUN) return;
-  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
-  const int lane = threadIdx.x & 63;
-  const int64_t r = slice * 64 + lane;
-  T acc[BS];
-  if (slice < n_slices) {
-    // operand x_j = z_j + beta p_j (the new search direction), formed on the fly
-    sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb,
-                      [&](int64_t i) { return z[i] + beta * p_old[i]; }, acc);
-  } else {
+  if constexpr (BS != 1) {
+    // az = A p_new with the operand z_j + beta p_j formed on the fly
+    if (slice < n_slices)
+      sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb,
+                        [&](int64_t i) { return z[i] + beta * p_old[i]; }, az);
+    if (r < nb) {
 #pragma unroll
-    for (int a = 0; a < BS; a++) acc[a] = 0;
+      for (int a = 0; a < BS; a++) {
+        zo[a] = z[BS * r + a];
+        po[a] = p_old[BS * r + a];
+      }
+    }
   }
   T red[1] = {0};
   if (r < nb) {
 #pragma unroll
     for (int a = 0; a < BS; a++) {
       const int64_t i = BS * r + a;
-      const T pn = z[i] + beta * p_old[i];
+      const T pn = zo[a] + beta * po[a];
+      const T qa = az[a] + beta * ap[a];
       p_new[i] = pn;
-      q[i] = acc[a];
-      red[0] += acc[a] * pn;
+      q[i] = qa;
+      red[0] += qa * pn;
     }
   }
   block_partials<T, 1>(red, part_pq);
 }
 
 // K2: alpha = rz / sum(part_pq); x += alpha p; r -= alpha q; z = dinv r;
-// per-block partials of r.z and r.r.
+// per-block partials of r.z and r.r.  The first pass's vector loads are
+// issued before the partials of K1 are reduced.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* __restrict__ p, T* __restrict__ r,
                                                    T* __restrict__ z, const T* __restrict__ q,
                                                    const T* __restrict__ dinv, int64_t n, CgState* st,
                                                    const double* __restrict__ part_pq, unsigned g_pq,
                                                    double* __restrict__ part_rz) {
-  if (st->done) return;
-  double pq[1];
-  sum_partials<1>(part_pq, g_pq, pq);
+  const int was_done = st->done;
   const double rz = st->rz1;
+  double pre[1][5];
+  partials_issue<1, 5>(part_pq, g_pq, pre);
+  if (was_done) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // first pass: four elements per thread, loaded together
+  T xv[4], pv[4], rv[4], qv[4], dv[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int64_t i = i0 + k * stride;
+    const bool in = i < n;
+    xv[k] = in ? x[i] : (T)0;
+    pv[k] = in ? p[i] : (T)0;
+    rv[k] = in ? r[i] : (T)0;
+    qv[k] = in ? q[i] : (T)0;
+    dv[k] = dinv && in ? dinv[i] : (T)1;
+  }
+  double pq[1];
+  partials_finish<1, 5>(part_pq, g_pq, pre, pq);
   const bool ok = pq[0] != 0.0 && isfinite(pq[0]);
   const T alpha = ok ? (T)(rz / pq[0]) : (T)0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -161,12 +238,23 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* _
     st->iter2 = st->iter1 + 1;
   }
   T red[2] = {0, 0};
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // four elements per pass so that their loads are in flight together
-  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int64_t i = i0 + k * stride;
+    if (i >= n) continue;
+    x[i] = xv[k] + alpha * pv[k];
+    const T ri = rv[k] - alpha * qv[k];
+    r[i] = ri;
+    const T zi = dv[k] * ri;
+    z[i] = zi;
+    red[0] += ri * zi;
+    red[1] += ri * ri;
+  }
+  // further passes when the grid holds fewer than n/4 threads
+  for (int64_t j0 = i0 + 4 * stride; j0 < n; j0 += 4 * stride) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int64_t i = i0 + k * stride;
+      const int64_t i = j0 + k * stride;
       if (i >= n) continue;
       x[i] += alpha * p[i];
       const T ri = r[i] - alpha * q[i];

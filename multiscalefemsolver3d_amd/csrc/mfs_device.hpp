@@ -316,7 +316,8 @@ int64_t spmv_grid(mfs_mat A);
 }  // namespace mfs
 
 // Fence-free two-stage reductions: a kernel writes one partial per block and
-// value (part[k*G + b]); k_finish_sums, one block, adds them in block order.
+// value (part[k*G + b]); k_finish_sums, one block per value, adds them in
+// block order.
 template <typename T, int NV>
 __device__ __forceinline__ void write_block_partials(T (&v)[NV], double* __restrict__ part) {
   __shared__ double red_bp[NV][16];
@@ -334,5 +335,4 @@ __device__ __forceinline__ void write_block_partials(T (&v)[NV], double* __restr
   }
 }
 
-__global__ void k_finish_sums(const double* __restrict__ part, unsigned G, int nv, double* __restrict__ out);
-__global__ void k_finish_sums_multi(const double* __restrict__ part, unsigned G, double* __restrict__ out);
+__global__ void k_finish_sums(const double* __restrict__ part, unsigned G, double* __restrict__ out);

@@ -435,7 +435,7 @@ struct MrEngine {
   MrState* st = nullptr;
   double *part_rz = nullptr, *part_pq = nullptr, *dtol = nullptr;
   unsigned g_spmv = 0, g_vec = 0;
-  // large systems: the partials are added once by k_finish_sums_multi
+  // large systems: the partials are added once by k_finish_sums
   // (tot_rz: 2*KR values, tot_pq: KR) instead of by every consumer block
   bool fin = false;
   double *tot_rz = nullptr, *tot_pq = nullptr;
@@ -501,14 +501,14 @@ struct MrEngine {
     else
       k_mr_dir_spmm<T, 3, KR, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, Z,
                                                               p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq);
-    if (fin) k_finish_sums_multi<<<KR, 256, 0, s>>>(part_pq, g_spmv, tot_pq);
+    if (fin) k_finish_sums<<<KR, 256, 0, s>>>(part_pq, g_spmv, tot_pq);
     k_mr_update<T, KR><<<g_vec, 256, 0, s>>>(X, p_new, R, Z, Q, dinv, n, st, fin ? tot_pq : part_pq,
                                              fin ? 1u : g_spmv, part_rz);
     finish_rz(s);
   }
 
   void finish_rz(hipStream_t s) {
-    if (fin) k_finish_sums_multi<<<2 * KR, 256, 0, s>>>(part_rz, g_vec, tot_rz);
+    if (fin) k_finish_sums<<<2 * KR, 256, 0, s>>>(part_rz, g_vec, tot_rz);
   }
 
   // continuation data of the last run, per system

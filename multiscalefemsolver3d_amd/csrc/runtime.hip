@@ -580,35 +580,11 @@ __global__ __launch_bounds__(256) void k_sell_spmv(const int64_t* __restrict__ s
   if constexpr (MODE != 0) write_block_partials<T, 2>(red, partials);
 }
 
-// One block: out[k] = sum over b of part[k*G + b], k < nv (block order).
-__global__ void k_finish_sums(const double* __restrict__ part, unsigned G, int nv, double* __restrict__ out) {
-  __shared__ double red[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = 0; k < nv; k++) {
-    // eight independent loads in flight per thread
-    double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned b0 = threadIdx.x; b0 < G; b0 += 8 * blockDim.x) {
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const unsigned b = b0 + u * blockDim.x;
-        if (b < G) s8[u] += part[(size_t)k * G + b];
-      }
-    }
-    double s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
-    s = wave_sum(s);
-    if (lane == 0) red[wave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) out[k] = (red[0] + red[1]) + (red[2] + red[3]);
-    __syncthreads();
-  }
-}
-
 // Block k adds up value k's G partials part[k*G + b] in a fixed order and
-// writes out[k] (grid = number of values).  Used between a producer with a
-// large grid and a consumer that would otherwise have every block re-add
-// all partials.
-__global__ __launch_bounds__(256) void k_finish_sums_multi(const double* __restrict__ part, unsigned G,
-                                                           double* __restrict__ out) {
+// writes out[k] (grid = number of values, so the values are summed in
+// parallel).
+__global__ __launch_bounds__(256) void k_finish_sums(const double* __restrict__ part, unsigned G,
+                                                     double* __restrict__ out) {
   __shared__ double red[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int k = blockIdx.x;
@@ -660,7 +636,7 @@ static int spmv_mode_launch(mfs_mat A, int prec, const void* x, void* y, const v
       sell_spmv_launch<float, 3, MODE>(A, A->d_val32, x, y, b, partials, counter, out, grid, s);
   }
   MFS_HIP(hipGetLastError());
-  if (MODE != 0) k_finish_sums<<<1, 256, 0, s>>>(partials, grid, 2, out);
+  if (MODE != 0) k_finish_sums<<<2, 256, 0, s>>>(partials, grid, out);
   return MFS_OK;
 }
 
@@ -736,7 +712,7 @@ extern "C" int mfs_dot(mfs_vec x, mfs_vec y, double* result) {
     k_dot<double><<<g, 256, 0, s>>>((const double*)x->d, (const double*)y->d, x->n, rs.partials, rs.counter, out);
   else
     k_dot<float><<<g, 256, 0, s>>>((const float*)x->d, (const float*)y->d, x->n, rs.partials, rs.counter, out);
-  k_finish_sums<<<1, 256, 0, s>>>(rs.partials, g, 1, out);
+  k_finish_sums<<<1, 256, 0, s>>>(rs.partials, g, out);
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, s));
   MFS_HIP(hipStreamSynchronize(s));

@@ -368,10 +368,10 @@ struct PcgEngine {
   const double* pq_src() { return fin ? tot : part_pq(); }
   unsigned pq_g() { return fin ? 1u : g_spmv; }
   void finish_rz(hipStream_t s) {
-    if (fin) k_finish_sums_multi<<<2, 256, 0, s>>>(part_rz(), g_vec, tot + 1);
+    if (fin) k_finish_sums<<<2, 256, 0, s>>>(part_rz(), g_vec, tot + 1);
   }
   void finish_pq(hipStream_t s) {
-    if (fin) k_finish_sums_multi<<<1, 256, 0, s>>>(part_pq(), g_spmv, tot);
+    if (fin) k_finish_sums<<<1, 256, 0, s>>>(part_pq(), g_spmv, tot);
   }
 
   int init(mfs_mat A_, bool precond, int chunk_) {
@@ -647,139 +647,224 @@ __global__ void k_minres_start(const T* __restrict__ r, T* __restrict__ v, T* __
 //   s^ = M^-1 s, t = A s^, omega = (t.s) / (t.t), x += alpha p^ + omega s^,
 //   r = s - omega t, rho' = r0.r, beta = (rho'/rho)(alpha/omega),
 //   p = r + beta (p - omega v).
-// Scalars stay on the device: each reduction ends in k_finish_sums (block
-// order, deterministic), a one-thread state kernel updates rho / beta and
-// the stop flag, and every kernel returns at once when the flag is set, so a
-// chunk of iterations is one hipGraph launch.
+// Three kernels per iteration, no separate reduction or scalar kernels:
+//   K_a: sums K_c's partials (r0.r, r.r) -> stop test, beta; gathers
+//        p^ = r^ + beta w^ on the fly (r^ = M^-1 r, w^ = p^_old - omega v^
+//        stored by K_c); v = A p^, v^ = M^-1 v; partial r0.v.
+//   K_b: sums K_a's partials -> alpha; gathers s^ = r^ - alpha v^ on the
+//        fly; t = A s^, s = r - alpha v; partials t.s, t.t.
+//   K_c: sums K_b's partials -> omega; x += alpha p^ + omega s^,
+//        r = s - omega t, r^ = M^-1 r, w^ = p^ - omega v^; partials r0.r, r.r.
+// Every block of a consumer adds the producer's partials in the same order
+// (identical scalars in all blocks); scalars another kernel needs are
+// written by block 0 into fields that kernel does not write.  Scalar rows
+// form the gathered operands as two products (A r^ + beta A w^, A r^ -
+// alpha A v^), so their loads are issued before the partials are reduced.
 // ---------------------------------------------------------------------------
 struct BsState {
-  double rho, tol2, rr, alpha, omega, beta;
-  int done, iter, maxit;
+  double rho_a, rho_b;  // rho of this iteration (K_a), of the previous one (K_b)
+  double alpha, omega, beta, tol2, rr;
+  int done, iter, iter_c, maxit;
 };
 enum { BS_RUN = 0, BS_CONVERGED = 1, BS_BREAKDOWN = 2, BS_MAXIT = 3 };
 
-// y = A (dinv o u); per-block partials of w1.y (and y.w2 when W2)
-template <typename T, int BS, bool W2>
-__global__ __launch_bounds__(256) void k_bs_spmv(const int64_t* __restrict__ slice_base,
-                                                 const int32_t* __restrict__ slice_width,
-                                                 const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                 const T* __restrict__ u, const T* __restrict__ dinv,
-                                                 const T* __restrict__ w1, T* __restrict__ y, int64_t nb,
-                                                 int64_t n_slices, const BsState* st, double* __restrict__ part) {
-  if (st->done) return;
+// y = A (x1 + c x2) for one slice row; scalar rows as two products (a1, a2)
+// combined by the caller, 3x3 blocks with the operand formed on the fly.
+template <typename T, int BS, typename CI>
+__device__ __forceinline__ void bs_rows_pair(const int64_t* __restrict__ slice_base,
+                                             const int32_t* __restrict__ slice_width, const CI* __restrict__ col,
+                                             const T* __restrict__ val, const T* __restrict__ x1,
+                                             const T* __restrict__ x2, int64_t slice, int64_t n_slices, int lane,
+                                             T (&a1)[BS], T (&a2)[BS]) {
+#pragma unroll
+  for (int a = 0; a < BS; a++) a1[a] = a2[a] = 0;
+  if (slice < n_slices) sell_row_pair<T, BS>(slice_base, slice_width, col, val, x1, x2, slice, lane, a1, a2);
+}
+
+// K_a
+template <typename T, int BS, typename CI>
+__global__ __launch_bounds__(256) void k_bs_dir_spmv(const int64_t* __restrict__ slice_base,
+                                                     const int32_t* __restrict__ slice_width,
+                                                     const CI* __restrict__ col, const T* __restrict__ val,
+                                                     const T* __restrict__ rh, const T* __restrict__ wh,
+                                                     const T* __restrict__ r0, const T* __restrict__ dinv,
+                                                     T* __restrict__ v, T* __restrict__ vh, int64_t nb,
+                                                     int64_t n_slices, BsState* st, const double* __restrict__ part_c,
+                                                     unsigned g_c, double* __restrict__ part_a) {
+  const int was_done = st->done;
+  double pre[2][4];
+  partials_issue<2, 4>(part_c, g_c, pre);
+  if (was_done) return;
   const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
-  const int64_t r = slice * 64 + lane;
-  T acc[BS];
-  if (slice < n_slices) {
-    sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb,
-                      [&](int64_t i) { return dinv ? dinv[i] * u[i] : u[i]; }, acc);
-  } else {
-#pragma unroll
-    for (int a = 0; a < BS; a++) acc[a] = 0;
+  const int64_t row = slice * 64 + lane;
+  T a1[BS], a2[BS];
+  if constexpr (BS == 1) bs_rows_pair<T, BS>(slice_base, slice_width, col, val, rh, wh, slice, n_slices, lane, a1, a2);
+  double t[2];
+  partials_finish<2, 4>(part_c, g_c, pre, t);
+  const double rho = t[0], rr = t[1];
+  const int it = st->iter_c;
+  int done = BS_RUN;
+  double beta = 0.0;
+  if (it > 0) {
+    const double alpha = st->alpha, omega = st->omega;
+    if (!isfinite(alpha) || !isfinite(omega) || !isfinite(rho) || !isfinite(rr))
+      done = BS_BREAKDOWN;
+    else if (rr <= st->tol2)
+      done = BS_CONVERGED;
+    else if (it >= st->maxit)
+      done = BS_MAXIT;
+    else if (rho == 0.0 || omega == 0.0)
+      done = BS_BREAKDOWN;
+    beta = (rho / st->rho_b) * (alpha / omega);
   }
-  T red[2] = {0, 0};
-  if (r < nb) {
+  // every thread of this block has read the stop flag (partials_finish
+  // ends with a barrier) before block 0 rewrites it
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rho_a = rho;
+    st->rr = rr;
+    st->iter = it;
+    st->beta = beta;
+    st->done = done;
+  }
+  if (done != BS_RUN) return;
+  const T b = (T)beta;
+  if constexpr (BS != 1) {
+    a2[0] = a2[1] = a2[2] = 0;
+    if (slice < n_slices)
+      sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb,
+                        [&](int64_t i) { return rh[i] + b * wh[i]; }, a1);
+    else
+      a1[0] = a1[1] = a1[2] = 0;
+  }
+  T red[1] = {0};
+  if (row < nb) {
 #pragma unroll
     for (int a = 0; a < BS; a++) {
-      const int64_t i = BS * r + a;
-      y[i] = acc[a];
-      red[0] += w1[i] * acc[a];
-      if (W2) red[1] += acc[a] * acc[a];
+      const int64_t i = BS * row + a;
+      const T vi = a1[a] + b * a2[a];
+      v[i] = vi;
+      vh[i] = dinv ? dinv[i] * vi : vi;
+      red[0] += r0[i] * vi;
     }
   }
-  if (W2)
-    block_partials<T, 2>(red, part);
-  else {
-    T one[1] = {red[0]};
-    block_partials<T, 1>(one, part);
+  block_partials<T, 1>(red, part_a);
+}
+
+// K_b
+template <typename T, int BS, typename CI>
+__global__ __launch_bounds__(256) void k_bs_s_spmv(const int64_t* __restrict__ slice_base,
+                                                   const int32_t* __restrict__ slice_width,
+                                                   const CI* __restrict__ col, const T* __restrict__ val,
+                                                   const T* __restrict__ rh, const T* __restrict__ vh,
+                                                   const T* __restrict__ r, const T* __restrict__ v,
+                                                   T* __restrict__ s, T* __restrict__ t, int64_t nb,
+                                                   int64_t n_slices, BsState* st, const double* __restrict__ part_a,
+                                                   unsigned g_a, double* __restrict__ part_b) {
+  const int was_done = st->done;
+  const double rho = st->rho_a;
+  double pre[1][4];
+  partials_issue<1, 4>(part_a, g_a, pre);
+  if (was_done) return;
+  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = slice * 64 + lane;
+  T a1[BS], a2[BS];
+  if constexpr (BS == 1) bs_rows_pair<T, BS>(slice_base, slice_width, col, val, rh, vh, slice, n_slices, lane, a1, a2);
+  double r0v[1];
+  partials_finish<1, 4>(part_a, g_a, pre, r0v);
+  const double alpha = rho / r0v[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alpha;
+    st->rho_b = rho;
   }
+  const T al = (T)alpha;
+  if constexpr (BS != 1) {
+    a2[0] = a2[1] = a2[2] = 0;
+    if (slice < n_slices)
+      sell_row_x<T, BS>(slice_base, slice_width, col, val, slice, lane, nb,
+                        [&](int64_t i) { return rh[i] - al * vh[i]; }, a1);
+    else
+      a1[0] = a1[1] = a1[2] = 0;
+  }
+  T red[2] = {0, 0};
+  if (row < nb) {
+#pragma unroll
+    for (int a = 0; a < BS; a++) {
+      const int64_t i = BS * row + a;
+      const T ti = a1[a] - al * a2[a];
+      const T si = r[i] - al * v[i];
+      s[i] = si;
+      t[i] = ti;
+      red[0] += ti * si;
+      red[1] += ti * ti;
+    }
+  }
+  block_partials<T, 2>(red, part_b);
 }
 
-// s = r - alpha v, alpha = rho / (r0.v)
+// K_c
 template <typename T>
-__global__ __launch_bounds__(256) void k_bs_s(const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ s,
-                                              int64_t n, const BsState* st, const double* __restrict__ tot) {
-  if (st->done) return;
-  const T alpha = (T)(st->rho / tot[0]);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    s[i] = r[i] - alpha * v[i];
-}
-
-// omega = (t.s)/(t.t); x += alpha dinv p + omega dinv s; r = s - omega t;
-// per-block partials of r0.r and r.r
-template <typename T>
-__global__ __launch_bounds__(256) void k_bs_update(T* __restrict__ x, const T* __restrict__ p, const T* __restrict__ s,
-                                                   const T* __restrict__ t, T* __restrict__ r,
+__global__ __launch_bounds__(256) void k_bs_update(T* __restrict__ x, T* __restrict__ r, T* __restrict__ rh,
+                                                   T* __restrict__ wh, const T* __restrict__ vh,
+                                                   const T* __restrict__ s, const T* __restrict__ t,
                                                    const T* __restrict__ r0, const T* __restrict__ dinv, int64_t n,
-                                                   const BsState* st, const double* __restrict__ tot,
-                                                   double* __restrict__ part) {
-  if (st->done) return;
-  const T alpha = (T)(st->rho / tot[0]), omega = (T)(tot[1] / tot[2]);
+                                                   BsState* st, const double* __restrict__ part_b, unsigned g_b,
+                                                   double* __restrict__ part_c) {
+  const int was_done = st->done;
+  const T alpha = (T)st->alpha, beta = (T)st->beta;
+  double pre[2][4];
+  partials_issue<2, 4>(part_b, g_b, pre);
+  if (was_done) return;
+  double ts[2];
+  partials_finish<2, 4>(part_b, g_b, pre, ts);
+  const double omega = ts[0] / ts[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->omega = omega;
+    st->iter_c = st->iter + 1;
+  }
+  const T om = (T)omega;
   T red[2] = {0, 0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const T di = dinv ? dinv[i] : (T)1;
-    x[i] += di * (alpha * p[i] + omega * s[i]);
-    const T ri = s[i] - omega * t[i];
+    const T ph = rh[i] + beta * wh[i];
+    const T si = s[i];
+    x[i] += alpha * ph + om * (di * si);
+    const T ri = si - om * t[i];
     r[i] = ri;
+    rh[i] = di * ri;
+    wh[i] = ph - om * vh[i];
     red[0] += r0[i] * ri;
+    red[1] += ri * ri;
+  }
+  block_partials<T, 2>(red, part_c);
+}
+
+// r0 = r (the right-hand side), r^ = M^-1 r, w^ = 0, x = 0; partials of
+// r0.r and r.r (both r.r) in K_a's input layout
+template <typename T>
+__global__ __launch_bounds__(256) void k_bs_start(const T* __restrict__ r, T* __restrict__ r0, T* __restrict__ rh,
+                                                  T* __restrict__ wh, T* __restrict__ x, const T* __restrict__ dinv,
+                                                  int64_t n, double* __restrict__ part) {
+  T red[2] = {0, 0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = r[i];
+    r0[i] = ri;
+    rh[i] = dinv ? dinv[i] * ri : ri;
+    wh[i] = 0;
+    x[i] = 0;
+    red[0] += ri * ri;
     red[1] += ri * ri;
   }
   block_partials<T, 2>(red, part);
 }
 
-// tot: [0] r0.v, [1] t.s, [2] t.t, [3] r0.r, [4] r.r
-__global__ void k_bs_state(BsState* st, const double* __restrict__ tot) {
-  if (st->done) return;
-  const double alpha = st->rho / tot[0], omega = tot[1] / tot[2], rho = tot[3], rr = tot[4];
-  st->iter++;
-  st->rr = rr;
-  st->alpha = alpha;
-  st->omega = omega;
-  if (!isfinite(alpha) || !isfinite(omega) || !isfinite(rho) || !isfinite(rr))
-    st->done = BS_BREAKDOWN;
-  else if (rr <= st->tol2)
-    st->done = BS_CONVERGED;
-  else if (st->iter >= st->maxit)
-    st->done = BS_MAXIT;
-  else if (rho == 0.0 || omega == 0.0)
-    st->done = BS_BREAKDOWN;
-  if (st->done) return;
-  st->beta = (rho / st->rho) * (alpha / omega);
-  st->rho = rho;
-}
-
-// p = r + beta (p - omega v)
-template <typename T>
-__global__ __launch_bounds__(256) void k_bs_p(T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ v,
-                                              int64_t n, const BsState* st) {
-  if (st->done) return;
-  const T beta = (T)st->beta, omega = (T)st->omega;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = r[i] + beta * (p[i] - omega * v[i]);
-}
-
-// r0 = p = r (the right-hand side; x starts at 0); partial of r.r
-template <typename T>
-__global__ __launch_bounds__(256) void k_bs_start(const T* __restrict__ r, T* __restrict__ r0, T* __restrict__ p,
-                                                  T* __restrict__ x, int64_t n, double* __restrict__ part) {
-  T red[1] = {0};
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const T ri = r[i];
-    r0[i] = ri;
-    p[i] = ri;
-    x[i] = 0;
-    red[0] += ri * ri;
-  }
-  block_partials<T, 1>(red, part);
-}
-
 __global__ void k_bs_init_state(BsState* st, const double* __restrict__ tot, double rtol, int maxit) {
-  st->rho = tot[0];
+  st->rho_a = st->rho_b = tot[0];
   st->tol2 = rtol * rtol * tot[0];
   st->rr = tot[0];
   st->alpha = st->omega = st->beta = 0.0;
-  st->iter = 0;
+  st->iter = st->iter_c = 0;
   st->maxit = maxit;
   st->done = tot[0] > 0.0 ? (isfinite(tot[0]) ? BS_RUN : BS_BREAKDOWN) : BS_CONVERGED;
 }
@@ -789,15 +874,25 @@ template <typename T>
 struct BsEngine {
   mfs_mat A = nullptr;
   int64_t n = 0;
-  T *x = nullptr, *r = nullptr, *r0 = nullptr, *p = nullptr, *v = nullptr, *sv = nullptr, *t = nullptr;
+  T *x = nullptr, *r = nullptr, *r0 = nullptr, *rh = nullptr, *wh = nullptr, *v = nullptr, *vh = nullptr,
+    *sv = nullptr, *t = nullptr;
   const T* val = nullptr;
   const T* dinv = nullptr;
   BsState* st = nullptr;
-  double *part = nullptr, *tot = nullptr;
+  // partials of K_a (1 value), K_b (2), K_c / start (2); tot: reduced
+  // values when the partials are first added once by k_finish_sums
+  double *part_a = nullptr, *part_b = nullptr, *part_c = nullptr, *tot = nullptr;
   unsigned g_spmv = 0, g_vec = 0;
+  bool fin = false;
   int chunk = 8;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+
+  const double* src_a() { return fin ? tot : part_a; }
+  const double* src_b() { return fin ? tot + 1 : part_b; }
+  const double* src_c() { return fin ? tot + 3 : part_c; }
+  unsigned ga() { return fin ? 1u : g_spmv; }
+  unsigned gc() { return fin ? 1u : g_vec; }
 
   int init(mfs_mat A_, bool precond) {
     A = A_;
@@ -806,12 +901,19 @@ struct BsEngine {
     if (!f64 && !A->val32_valid) MFS_TRY(mat_refresh_derived(A));
     val = f64 ? (const T*)A->d_val64 : (const T*)A->d_val32;
     dinv = precond ? (f64 ? (const T*)A->d_dinv64 : (const T*)A->d_dinv32) : nullptr;
-    for (T** b : {&x, &r, &r0, &p, &v, &sv, &t}) MFS_TRY(dmalloc((void**)b, sizeof(T) * n));
+    for (T** b : {&x, &r, &r0, &rh, &wh, &v, &vh, &sv, &t}) MFS_TRY(dmalloc((void**)b, sizeof(T) * n));
     MFS_TRY(dmalloc((void**)&st, sizeof(BsState)));
     g_spmv = (unsigned)spmv_grid(A);
-    g_vec = grid_for(n, 1024);
-    MFS_TRY(dmalloc((void**)&part, sizeof(double) * 2 * std::max(g_spmv, g_vec)));
+    const int64_t cus = std::max(A->ctx->cu_count, 64);
+    g_vec = (unsigned)std::min<int64_t>(4 * cus, std::max<int64_t>(cus, (n + 1023) / 1024));
+    MFS_TRY(dmalloc((void**)&part_a, sizeof(double) * g_spmv));
+    MFS_TRY(dmalloc((void**)&part_b, sizeof(double) * 2 * g_spmv));
+    MFS_TRY(dmalloc((void**)&part_c, sizeof(double) * 2 * g_vec));
     MFS_TRY(dmalloc((void**)&tot, sizeof(double) * 8));
+    // every block of a consumer adds all partials of its producer; for
+    // large grids a one-block-per-value kernel adds them once instead
+    fin = (uint64_t)g_spmv * std::max(g_spmv, g_vec) > 2000000ull;
+    if (const char* e = std::getenv("MFS_FINISH_SUMS")) fin = e[0] == '1';  // force either way (tests)
     // MFS_NO_GRAPH=1 launches the same kernels directly (kernel tracing)
     const char* e = std::getenv("MFS_NO_GRAPH");
     if (e && e[0] == '1') return MFS_OK;
@@ -823,36 +925,47 @@ struct BsEngine {
     return MFS_OK;
   }
 
-  void launch_iteration(hipStream_t s) {
+  template <int BS, typename CI>
+  void launch_spmvs(hipStream_t s, const CI* col, int which) {
     const int64_t ns = A->sell.n_slices;
-    if (A->bs == 1)
-      k_bs_spmv<T, 1, false><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, p, dinv, r0, v,
-                                                    A->nb, ns, st, part);
+    if (which == 0)
+      k_bs_dir_spmv<T, BS, CI><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, col, val, rh, wh, r0, dinv,
+                                                      v, vh, A->nb, ns, st, src_c(), gc(), part_a);
     else
-      k_bs_spmv<T, 3, false><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, p, dinv, r0, v,
-                                                    A->nb, ns, st, part);
-    k_finish_sums<<<1, 256, 0, s>>>(part, g_spmv, 1, tot);
-    k_bs_s<T><<<g_vec, 256, 0, s>>>(r, v, sv, n, st, tot);
-    // t = A s^ with partials s.t (w1 = s) and t.t
-    if (A->bs == 1)
-      k_bs_spmv<T, 1, true><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, sv, dinv, sv, t,
-                                                   A->nb, ns, st, part);
-    else
-      k_bs_spmv<T, 3, true><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, sv, dinv, sv, t,
-                                                   A->nb, ns, st, part);
-    k_finish_sums<<<1, 256, 0, s>>>(part, g_spmv, 2, tot + 1);
-    k_bs_update<T><<<g_vec, 256, 0, s>>>(x, p, sv, t, r, r0, dinv, n, st, tot, part);
-    k_finish_sums<<<1, 256, 0, s>>>(part, g_vec, 2, tot + 3);
-    k_bs_state<<<1, 1, 0, s>>>(st, tot);
-    k_bs_p<T><<<g_vec, 256, 0, s>>>(p, r, v, n, st);
+      k_bs_s_spmv<T, BS, CI><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, col, val, rh, vh, r, v, sv,
+                                                    t, A->nb, ns, st, src_a(), ga(), part_b);
+  }
+  void spmv(hipStream_t s, int which) {
+    if (A->d_col16) {
+      if (A->bs == 1)
+        launch_spmvs<1>(s, A->d_col16, which);
+      else
+        launch_spmvs<3>(s, A->d_col16, which);
+    } else {
+      if (A->bs == 1)
+        launch_spmvs<1>(s, A->d_col, which);
+      else
+        launch_spmvs<3>(s, A->d_col, which);
+    }
+  }
+
+  void launch_iteration(hipStream_t s) {
+    spmv(s, 0);
+    if (fin) k_finish_sums<<<1, 256, 0, s>>>(part_a, g_spmv, tot);
+    spmv(s, 1);
+    if (fin) k_finish_sums<<<2, 256, 0, s>>>(part_b, g_spmv, tot + 1);
+    k_bs_update<T><<<g_vec, 256, 0, s>>>(x, r, rh, wh, vh, sv, t, r0, dinv, n, st, src_b(), fin ? 1u : g_spmv,
+                                         part_c);
+    if (fin) k_finish_sums<<<2, 256, 0, s>>>(part_c, g_vec, tot + 3);
   }
 
   // r holds the right-hand side; x returns the solution of A x = r.
   int run(double rtol, int maxit, BsState* out) {
     hipStream_t s = A->ctx->stream;
-    k_bs_start<T><<<g_vec, 256, 0, s>>>(r, r0, p, x, n, part);
-    k_finish_sums<<<1, 256, 0, s>>>(part, g_vec, 1, tot);
-    k_bs_init_state<<<1, 1, 0, s>>>(st, tot, rtol, maxit);
+    k_bs_start<T><<<g_vec, 256, 0, s>>>(r, r0, rh, wh, x, dinv, n, part_c);
+    k_finish_sums<<<1, 256, 0, s>>>(part_c, g_vec, tot + 6);
+    k_bs_init_state<<<1, 1, 0, s>>>(st, tot + 6, rtol, maxit);
+    if (fin) k_finish_sums<<<2, 256, 0, s>>>(part_c, g_vec, tot + 3);
     MFS_HIP(hipGetLastError());
     BsState* hs = (BsState*)A->ctx->pinned;
     MFS_HIP(hipMemcpyAsync(hs, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
@@ -875,13 +988,12 @@ struct BsEngine {
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
-    for (T* b : {x, r, r0, p, v, sv, t}) dfree(b);
-    x = r = r0 = p = v = sv = t = nullptr;
+    for (T* b : {x, r, r0, rh, wh, v, vh, sv, t}) dfree(b);
+    x = r = r0 = rh = wh = v = vh = sv = t = nullptr;
     dfree(st);
-    dfree(part);
-    dfree(tot);
+    for (double* b : {part_a, part_b, part_c, tot}) dfree(b);
     st = nullptr;
-    part = tot = nullptr;
+    part_a = part_b = part_c = tot = nullptr;
   }
 };
 
@@ -1054,9 +1166,9 @@ static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double
     // CGS2: two passes of h = V^T w, w -= V h
     for (int pass = 0; pass < 2; pass++) {
       k_multidot<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->redg.partials);
-      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, j + 1, S->dh + pass * 32);
+      k_finish_sums<<<j + 1, 256, 0, s>>>(S->redg.partials, gv, S->dh + pass * 32);
       k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->dh + pass * 32, S->redg.partials);
-      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh + 62);
+      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 62);
     }
     MFS_HIP(hipGetLastError());
     MFS_HIP(hipMemcpyAsync(hp, S->dh, 64 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1131,7 +1243,7 @@ static int minres_run(mfs_solver S, const T* dinv, double rtol, int maxsteps, in
   T* zj = zbuf[0];
   T* znext = rhs;
   k_minres_start<T><<<gv, 256, 0, s>>>(rhs, v[1], zj, dinv, x, v[0], w[1], w[0], n, S->redg.partials);
-  k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh);
+  k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh);
   MFS_HIP(hipMemcpyAsync(hp, S->dh, 8, hipMemcpyDeviceToHost, s));
   MFS_HIP(hipStreamSynchronize(s));
   double g = std::sqrt(std::max(hp[0], 0.0)), g_prev = 0.0;
@@ -1147,10 +1259,10 @@ static int minres_run(mfs_solver S, const T* dinv, double rtol, int maxsteps, in
     MFS_TRY(spmv_launch(A, prec, zj, Az, s));
     // delta = <A z_j, z_j>
     k_multidot<T><<<gv, 256, 0, s>>>(zj, Az, n, 1, S->redg.partials);
-    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh + 1);
+    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 1);
     k_minres_lanczos<T><<<gv, 256, 0, s>>>(Az, v[1], v[0], v[2], znext, dinv, S->dh + 1, g, g_prev, n,
                                            S->redg.partials);
-    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, 1, S->dh + 2);
+    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 2);
     MFS_HIP(hipMemcpyAsync(hp, S->dh + 1, 16, hipMemcpyDeviceToHost, s));
     MFS_HIP(hipStreamSynchronize(s));
     const double delta = hp[0];

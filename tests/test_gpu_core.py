@@ -419,6 +419,33 @@ def test_bicgstab_nonsymmetric_matches_oracle(ctx, precision):
         assert abs(rep.iterations - it_o) <= 0.15 * it_o + 3, (rep.iterations, it_o)
 
 
+@pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
+def test_bicgstab_reduction_and_index_paths_agree(ctx, monkeypatch, precision):
+    """The BiCGStab kernels sum their producer's partials in every block, or
+    read totals that a separate kernel added once (MFS_FINISH_SUMS), and read
+    16-bit or 32-bit column indices (MFS_NO_COL16 at matrix creation): all
+    four paths reach the tolerance with the same iterations and solution."""
+    rp, col, val, b = problems.convection_diffusion(48, 40.0, seed=5)
+    n = len(b)
+    out = {}
+    for fin in ("0", "1"):
+        for col16 in ("0", "1"):
+            monkeypatch.setenv("MFS_FINISH_SUMS", fin)
+            if col16 == "1":
+                monkeypatch.setenv("MFS_NO_COL16", "1")
+            else:
+                monkeypatch.delenv("MFS_NO_COL16", raising=False)
+            A = mfs.Mat.csr(ctx, rp, col, val)
+            bv, x = mfs.Vec(ctx, n, mfs.F64, b), mfs.Vec(ctx, n, mfs.F64, np.zeros(n))
+            rep = mfs.Solver(A, method=mfs.BICGSTAB, precision=precision, tol=1e-10).run(bv, x)
+            assert rep.status == 0 and rep.true_relres <= 1e-10, (fin, col16, rep)
+            out[fin, col16] = (rep.iterations, x.download())
+    it0, x0 = out["0", "0"]
+    for key, (it, xk) in out.items():
+        assert abs(it - it0) <= 0.05 * it0 + 2, (key, it, it0)
+        assert np.linalg.norm(xk - x0) <= 1e-8 * np.linalg.norm(x0), key
+
+
 def test_bicgstab_reference_elastic_operator(ctx):
     """The reference element convention's two-phase elastic system (symmetric
     indefinite, clamped bottom, free sides): BiCGStab reaches dense LU's

@@ -251,9 +251,11 @@ __device__ __forceinline__ void sell_row_x(const int64_t* __restrict__ slice_bas
 // a2 = A_row·x2 (same batching as sell_row_x).  Nothing here depends on a
 // scalar computed by the caller, so a kernel can issue these loads before it
 // has finished reducing the previous kernel's partial sums.
+//
+// The caller passes the slice's start and width (sb, w; w = 0 for a slice
+// past the end), so that it can load them as early as it likes.
 template <typename T, int BS, typename CI>
-__device__ __forceinline__ void sell_row_pair(const int64_t* __restrict__ slice_base,
-                                              const int32_t* __restrict__ slice_width, const CI* __restrict__ col,
+__device__ __forceinline__ void sell_row_pair(int64_t sb, int w, const CI* __restrict__ col,
                                               const T* __restrict__ val, const T* __restrict__ x1,
                                               const T* __restrict__ x2, int64_t slice, int lane, T (&a1)[BS],
                                               T (&a2)[BS]) {
@@ -262,8 +264,6 @@ __device__ __forceinline__ void sell_row_pair(const int64_t* __restrict__ slice_
   constexpr int B2 = BS * BS;
 #pragma unroll
   for (int a = 0; a < BS; a++) a1[a] = a2[a] = 0;
-  const int64_t sb = slice_base[slice];
-  const int w = slice_width[slice];
   const CI* cp = col + sb + lane;
   const int32_t rbase = sizeof(CI) == 2 ? (int32_t)(slice * C + lane) : 0;
   const T* vp = val + sb * B2 + lane;

@@ -135,17 +135,22 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
                                                      int64_t n_slices, CgState* st, const double* __restrict__ part_rz,
                                                      unsigned g_rz, double* __restrict__ part_pq) {
   const int was_done = st->done;
-  double pre[2][2];
-  partials_issue<2, 2>(part_rz, g_rz, pre);
-  if (was_done) return;
   const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t r = slice * 64 + lane;
+  // the slice's start and width are loaded with the stop flag and the
+  // partials, ahead of the branch on the flag
+  const bool in_range = slice < n_slices;
+  const int64_t sb = in_range ? slice_base[slice] : 0;
+  const int w = in_range ? slice_width[slice] : 0;
+  double pre[2][2];
+  partials_issue<2, 2>(part_rz, g_rz, pre);
+  if (was_done) return;
   T az[BS], ap[BS], zo[BS], po[BS];
 #pragma unroll
   for (int a = 0; a < BS; a++) az[a] = ap[a] = zo[a] = po[a] = 0;
   if constexpr (BS == 1) {
-    if (slice < n_slices) sell_row_pair<T, BS>(slice_base, slice_width, col, val, z, p_old, slice, lane, az, ap);
+    sell_row_pair<T, BS>(sb, w, col, val, z, p_old, slice, lane, az, ap);
     if (r < nb) {
       zo[0] = z[r];
       po[0] = p_old[r];
@@ -676,9 +681,9 @@ __device__ __forceinline__ void bs_rows_pair(const int64_t* __restrict__ slice_b
                                              const T* __restrict__ val, const T* __restrict__ x1,
                                              const T* __restrict__ x2, int64_t slice, int64_t n_slices, int lane,
                                              T (&a1)[BS], T (&a2)[BS]) {
-#pragma unroll
-  for (int a = 0; a < BS; a++) a1[a] = a2[a] = 0;
-  if (slice < n_slices) sell_row_pair<T, BS>(slice_base, slice_width, col, val, x1, x2, slice, lane, a1, a2);
+  const bool in_range = slice < n_slices;
+  sell_row_pair<T, BS>(in_range ? slice_base[slice] : 0, in_range ? slice_width[slice] : 0, col, val, x1, x2, slice,
+                       lane, a1, a2);
 }
 
 // K_a

@@ -126,6 +126,17 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of the
+// MI355X (workgroup b runs on XCD b % 8), and each XCD has its own L2.
+// xcd_block() renumbers them so that XCD x takes one contiguous range of
+// the logical block order: in a SELL kernel, neighbouring slices, which
+// gather the same operand lines, then share one L2.  A bijection of
+// [0, gridDim.x) for any grid size.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned G = gridDim.x, b = blockIdx.x, x = b & 7u, per = G >> 3, rem = G & 7u;
+  return x * per + min(x, rem) + (b >> 3);
+}
+
 // Block-level sum of NV values; result valid in thread 0.  Fixed order.
 template <typename T, int NV, int BLOCK>
 __device__ __forceinline__ void block_sum(T (&v)[NV], T* lds /* NV * BLOCK/64 */) {

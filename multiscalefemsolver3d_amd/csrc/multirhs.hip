@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_mr_residual(const int64_t* __restrict__
                                                      const T* __restrict__ X, const T* __restrict__ Bv,
                                                      T* __restrict__ R, int64_t nb, int64_t n_slices,
                                                      double* __restrict__ part) {
-  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T acc[BS][KR];
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_mr_dir_spmm(const int64_t* __restrict__
     st->alldone = all;
   }
   if (all) return;
-  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T acc[BS][KR];

@@ -548,7 +548,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(const int64_t* __restrict__ s
                                                    const T* __restrict__ b, int64_t nb, int64_t n_slices,
                                                    double* partials, unsigned* counter, double* out) {
   constexpr int C = 64;
-  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / C;
+  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / C;
   const int lane = threadIdx.x & (C - 1);
   T acc[BS];
   const int64_t r = slice * C + lane;

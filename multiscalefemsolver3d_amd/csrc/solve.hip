@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
                                                      int64_t n_slices, CgState* st, const double* __restrict__ part_rz,
                                                      unsigned g_rz, double* __restrict__ part_pq) {
   const int was_done = st->done;
-  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t r = slice * 64 + lane;
   // the slice's start and width are loaded with the stop flag and the
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(256) void k_bs_dir_spmv(const int64_t* __restrict__
   double pre[2][4];
   partials_issue<2, 4>(part_c, g_c, pre);
   if (was_done) return;
-  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T a1[BS], a2[BS];
@@ -771,7 +771,7 @@ __global__ __launch_bounds__(256) void k_bs_s_spmv(const int64_t* __restrict__ s
   double pre[1][4];
   partials_issue<1, 4>(part_a, g_a, pre);
   if (was_done) return;
-  const int64_t slice = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T a1[BS], a2[BS];

@@ -80,6 +80,7 @@ struct mfs_mat_s {
   double* d_dinv64 = nullptr;   // bs*nb inverse diagonal (Jacobi)
   float* d_dinv32 = nullptr;
   bool val32_valid = false;
+  int xcd_map = 1;  // SELL kernels: contiguous slice ranges per XCD (xcd_block)
   int64_t n() const { return bs * nb; }
   int64_t nnz() const { return (int64_t)bs * bs * nnzb; }
 };
@@ -128,11 +129,12 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs of the
 // MI355X (workgroup b runs on XCD b % 8), and each XCD has its own L2.
-// xcd_block() renumbers them so that XCD x takes one contiguous range of
+// xcd_block(1) renumbers them so that XCD x takes one contiguous range of
 // the logical block order: in a SELL kernel, neighbouring slices, which
 // gather the same operand lines, then share one L2.  A bijection of
-// [0, gridDim.x) for any grid size.
-__device__ __forceinline__ unsigned xcd_block() {
+// [0, gridDim.x) for any grid size.  xcd_block(0) keeps the dispatch order.
+__device__ __forceinline__ unsigned xcd_block(int on) {
+  if (!on) return blockIdx.x;
   const unsigned G = gridDim.x, b = blockIdx.x, x = b & 7u, per = G >> 3, rem = G & 7u;
   return x * per + min(x, rem) + (b >> 3);
 }

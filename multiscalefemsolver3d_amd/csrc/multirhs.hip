@@ -163,8 +163,8 @@ __global__ __launch_bounds__(256) void k_mr_residual(const int64_t* __restrict__
                                                      const int32_t* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ X, const T* __restrict__ Bv,
                                                      T* __restrict__ R, int64_t nb, int64_t n_slices,
-                                                     double* __restrict__ part) {
-  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
+                                                     double* __restrict__ part, int xcd) {
+  const int64_t slice = (int64_t)xcd_block(xcd) * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T acc[BS][KR];
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_mr_dir_spmm(const int64_t* __restrict__
                                                      const T* __restrict__ Z, const T* __restrict__ P_old,
                                                      T* __restrict__ P_new, T* __restrict__ Q, int64_t nb,
                                                      int64_t n_slices, MrState* st, const double* __restrict__ part_rz,
-                                                     unsigned g_rz, double* __restrict__ part_pq) {
+                                                     unsigned g_rz, double* __restrict__ part_pq, int xcd) {
   if (st->alldone) return;
   double t[2 * KR];
   mr_sum_partials<2 * KR>(part_rz, g_rz, t);
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_mr_dir_spmm(const int64_t* __restrict__
     st->alldone = all;
   }
   if (all) return;
-  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block(xcd) * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T acc[BS][KR];
@@ -490,17 +490,19 @@ struct MrEngine {
     if (A->d_col16 && A->bs == 1)
       k_mr_dir_spmm<T, 1, KR, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val,
                                                               Z, p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g,
-                                                              part_pq);
+                                                              part_pq, A->xcd_map);
     else if (A->d_col16)
       k_mr_dir_spmm<T, 3, KR, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val,
                                                               Z, p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g,
-                                                              part_pq);
+                                                              part_pq, A->xcd_map);
     else if (A->bs == 1)
       k_mr_dir_spmm<T, 1, KR, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, Z,
-                                                              p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq);
+                                                              p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq,
+                                                              A->xcd_map);
     else
       k_mr_dir_spmm<T, 3, KR, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, Z,
-                                                              p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq);
+                                                              p_old, p_new, Q, A->nb, ns, st, rz_in, rz_g, part_pq,
+                                                              A->xcd_map);
     if (fin) k_finish_sums<<<KR, 256, 0, s>>>(part_pq, g_spmv, tot_pq);
     k_mr_update<T, KR><<<g_vec, 256, 0, s>>>(X, p_new, R, Z, Q, dinv, n, st, fin ? tot_pq : part_pq,
                                              fin ? 1u : g_spmv, part_rz);
@@ -577,10 +579,10 @@ int mr_true_residual(mfs_mat A, const double* X64, const double* B64, double* R6
   const unsigned g = (unsigned)spmv_grid(A);
   if (A->bs == 1)
     k_mr_residual<double, 1, KR><<<g, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, A->d_val64, X64, B64,
-                                                   R64, A->nb, ns, part);
+                                                   R64, A->nb, ns, part, A->xcd_map);
   else
     k_mr_residual<double, 3, KR><<<g, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, A->d_val64, X64, B64,
-                                                   R64, A->nb, ns, part);
+                                                   R64, A->nb, ns, part, A->xcd_map);
   MFS_HIP(hipGetLastError());
   std::vector<double> hpart((size_t)2 * KR * g);
   MFS_HIP(hipMemcpyAsync(hpart.data(), part, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, s));

@@ -356,6 +356,9 @@ int mat_create_from_pattern(mfs_ctx ctx, int bs, Pattern&& P, mfs_mat* out) {
   MFS_HIP(hipMemcpyAsync(A->d_pos, A->sell.pos.data(), sizeof(int64_t) * A->nnzb, hipMemcpyHostToDevice, st));
   MFS_HIP(hipMemcpyAsync(A->d_diag_pos, diag_pos.data(), sizeof(int64_t) * A->nb, hipMemcpyHostToDevice, st));
   MFS_HIP(hipMemsetAsync(A->d_val64, 0, sizeof(double) * nval, st));
+  // contiguous slice ranges per XCD in the SELL kernels (xcd_block);
+  // MFS_XCD_MAP=0 keeps the dispatch order (A/B runs)
+  if (const char* e = std::getenv("MFS_XCD_MAP")) A->xcd_map = e[0] == '1';
   // 16-bit row-relative columns when every offset fits (structured meshes
   // in natural order: offsets up to (n+1)^2 + n + 2 for n^3 cells)
   if (!std::getenv("MFS_NO_COL16")) {
@@ -546,9 +549,9 @@ __global__ __launch_bounds__(256) void k_sell_spmv(const int64_t* __restrict__ s
                                                    const CI* __restrict__ col, const T* __restrict__ val,
                                                    const T* __restrict__ x, T* __restrict__ y,
                                                    const T* __restrict__ b, int64_t nb, int64_t n_slices,
-                                                   double* partials, unsigned* counter, double* out) {
+                                                   double* partials, unsigned* counter, double* out, int xcd) {
   constexpr int C = 64;
-  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / C;
+  const int64_t slice = (int64_t)xcd_block(xcd) * 4 + threadIdx.x / C;
   const int lane = threadIdx.x & (C - 1);
   T acc[BS];
   const int64_t r = slice * C + lane;
@@ -611,11 +614,11 @@ static void sell_spmv_launch(mfs_mat A, const T* val, const void* x, void* y, co
   if (A->d_col16)
     k_sell_spmv<T, BS, MODE, int16_t><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val,
                                                            (const T*)x, (T*)y, (const T*)b, A->nb, ns, partials,
-                                                           counter, out);
+                                                           counter, out, A->xcd_map);
   else
     k_sell_spmv<T, BS, MODE, int32_t><<<grid, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val,
                                                            (const T*)x, (T*)y, (const T*)b, A->nb, ns, partials,
-                                                           counter, out);
+                                                           counter, out, A->xcd_map);
 }
 
 template <int MODE>

@@ -133,9 +133,9 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
                                                      const T* __restrict__ z, const T* __restrict__ p_old,
                                                      T* __restrict__ p_new, T* __restrict__ q, int64_t nb,
                                                      int64_t n_slices, CgState* st, const double* __restrict__ part_rz,
-                                                     unsigned g_rz, double* __restrict__ part_pq) {
+                                                     unsigned g_rz, double* __restrict__ part_pq, int xcd) {
   const int was_done = st->done;
-  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block(xcd) * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t r = slice * 64 + lane;
   // the slice's start and width are loaded with the stop flag and the
@@ -410,20 +410,20 @@ struct PcgEngine {
       if (A->bs == 1)
         k_cg_dir_spmv<T, 1, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val, z,
                                                             p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
-                                                            part_pq());
+                                                            part_pq(), A->xcd_map);
       else
         k_cg_dir_spmv<T, 3, int16_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col16, val, z,
                                                             p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
-                                                            part_pq());
+                                                            part_pq(), A->xcd_map);
     } else {
       if (A->bs == 1)
         k_cg_dir_spmv<T, 1, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z,
                                                             p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
-                                                            part_pq());
+                                                            part_pq(), A->xcd_map);
       else
         k_cg_dir_spmv<T, 3, int32_t><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, A->d_col, val, z,
                                                             p_old, p_new, q, A->nb, ns, st, rz_src(), rz_g(),
-                                                            part_pq());
+                                                            part_pq(), A->xcd_map);
     }
   }
 
@@ -695,12 +695,12 @@ __global__ __launch_bounds__(256) void k_bs_dir_spmv(const int64_t* __restrict__
                                                      const T* __restrict__ r0, const T* __restrict__ dinv,
                                                      T* __restrict__ v, T* __restrict__ vh, int64_t nb,
                                                      int64_t n_slices, BsState* st, const double* __restrict__ part_c,
-                                                     unsigned g_c, double* __restrict__ part_a) {
+                                                     unsigned g_c, double* __restrict__ part_a, int xcd) {
   const int was_done = st->done;
   double pre[2][4];
   partials_issue<2, 4>(part_c, g_c, pre);
   if (was_done) return;
-  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block(xcd) * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T a1[BS], a2[BS];
@@ -765,13 +765,13 @@ __global__ __launch_bounds__(256) void k_bs_s_spmv(const int64_t* __restrict__ s
                                                    const T* __restrict__ r, const T* __restrict__ v,
                                                    T* __restrict__ s, T* __restrict__ t, int64_t nb,
                                                    int64_t n_slices, BsState* st, const double* __restrict__ part_a,
-                                                   unsigned g_a, double* __restrict__ part_b) {
+                                                   unsigned g_a, double* __restrict__ part_b, int xcd) {
   const int was_done = st->done;
   const double rho = st->rho_a;
   double pre[1][4];
   partials_issue<1, 4>(part_a, g_a, pre);
   if (was_done) return;
-  const int64_t slice = (int64_t)xcd_block() * 4 + threadIdx.x / 64;
+  const int64_t slice = (int64_t)xcd_block(xcd) * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * 64 + lane;
   T a1[BS], a2[BS];
@@ -935,10 +935,10 @@ struct BsEngine {
     const int64_t ns = A->sell.n_slices;
     if (which == 0)
       k_bs_dir_spmv<T, BS, CI><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, col, val, rh, wh, r0, dinv,
-                                                      v, vh, A->nb, ns, st, src_c(), gc(), part_a);
+                                                      v, vh, A->nb, ns, st, src_c(), gc(), part_a, A->xcd_map);
     else
       k_bs_s_spmv<T, BS, CI><<<g_spmv, 256, 0, s>>>(A->d_slice_base, A->d_slice_width, col, val, rh, vh, r, v, sv,
-                                                    t, A->nb, ns, st, src_a(), ga(), part_b);
+                                                    t, A->nb, ns, st, src_a(), ga(), part_b, A->xcd_map);
   }
   void spmv(hipStream_t s, int which) {
     if (A->d_col16) {

@@ -8,8 +8,8 @@
 // while the others continue.
 //
 // Layout: the vectors of the KR systems are interleaved, element (i, j) at
-// v[i*KR + j], so the operand gather of one column is one 16- or 32-byte
-// load.  MIXED precision runs the same FP64 outer refinement as the
+// v[i*KR + j], so the operand gather of one column is one contiguous 16-,
+// 24- or 32-byte read (KR = 4, 6 or 8 FP32 systems).  MIXED precision runs the same FP64 outer refinement as the
 // single-system solver (true residual, unit scaling, x += scale d per
 // system) around the FP32 iteration.
 #include <hip/hip_runtime.h>
@@ -81,27 +81,46 @@ __device__ __forceinline__ void mr_block_partials(double (&v)[NQ], double* __res
         (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
 }
 
-// The KR values of one element (16-byte aligned: KR*sizeof(T) is a multiple
-// of 16 for the instantiated KR) move as 16-byte vectors.
+// The KR values of one element move as 16-byte vectors when KR*sizeof(T)
+// is a multiple of 16 (elements then 16-byte aligned), else as 8-byte ones
+// (6 FP32 systems: 24-byte elements, 8-byte aligned).
+template <typename T, int KR>
+struct VecWidth {
+  static_assert((KR * sizeof(T)) % 8 == 0, "interleaved element must be whole 8-byte words");
+  static constexpr int B = (KR * sizeof(T)) % 16 == 0 ? 16 : 8;  // bytes per load
+  static constexpr int NV = KR * sizeof(T) / B;                   // loads per element
+};
+template <int B>
+struct VecWord;
+template <>
+struct VecWord<16> {
+  using type = uint4;
+};
+template <>
+struct VecWord<8> {
+  using type = uint2;
+};
+
 template <typename T, int KR>
 __device__ __forceinline__ void vload(const T* __restrict__ p, T (&o)[KR]) {
-  static_assert((KR * sizeof(T)) % 16 == 0, "interleaved element must be whole 16-byte vectors");
-  constexpr int NV = KR * sizeof(T) / 16;
-  const uint4* q = reinterpret_cast<const uint4*>(p);
+  constexpr int B = VecWidth<T, KR>::B, NV = VecWidth<T, KR>::NV;
+  using W = typename VecWord<B>::type;
+  const W* q = reinterpret_cast<const W*>(p);
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    const uint4 u = q[v];
-    __builtin_memcpy(&o[v * (16 / sizeof(T))], &u, 16);
+    const W u = q[v];
+    __builtin_memcpy(&o[v * (B / sizeof(T))], &u, B);
   }
 }
 template <typename T, int KR>
 __device__ __forceinline__ void vstore(T* __restrict__ p, const T (&o)[KR]) {
-  constexpr int NV = KR * sizeof(T) / 16;
-  uint4* q = reinterpret_cast<uint4*>(p);
+  constexpr int B = VecWidth<T, KR>::B, NV = VecWidth<T, KR>::NV;
+  using W = typename VecWord<B>::type;
+  W* q = reinterpret_cast<W*>(p);
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    uint4 u;
-    __builtin_memcpy(&u, &o[v * (16 / sizeof(T))], 16);
+    W u;
+    __builtin_memcpy(&u, &o[v * (B / sizeof(T))], B);
     q[v] = u;
   }
 }
@@ -763,5 +782,6 @@ extern "C" int mfs_solve_multi(mfs_mat A, const mfs_solve_desc* desc, int k, mfs
   MFS_HIP(hipSetDevice(A->ctx->device));
   std::memset(reports, 0, sizeof(mfs_solve_report) * k);
   if (k <= 4) return solve_multi<4>(A, *desc, k, b, x, reports);
+  if (k <= 6) return solve_multi<6>(A, *desc, k, b, x, reports);
   return solve_multi<8>(A, *desc, k, b, x, reports);
 }

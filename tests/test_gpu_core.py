@@ -333,11 +333,12 @@ def test_laplace64_vs_oracle_solution(ctx):
 
 @pytest.mark.parametrize("kind", ["scalar", "elastic"])
 @pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
-@pytest.mark.parametrize("k", [1, 3, 6])
+@pytest.mark.parametrize("k", [1, 3, 5, 6, 8])
 def test_multi_rhs_matches_single_solves(ctx, kind, precision, k):
     """k systems sharing one matrix, solved as one iteration stream (one
     SpMM per iteration), give the solutions of k separate solves; the padding
-    systems of the interleaved layout (k = 1, 3 -> 4, 6 -> 8) stay inert."""
+    systems of the interleaved layout (k = 1, 3 -> 4, 5 -> 6) stay inert, and
+    the 6-system layout (24-byte FP32 elements, 8-byte loads) matches too."""
     if kind == "scalar":
         p = problems.laplace_inclusions(10, count=30, rmin=0.05, rmax=0.15)
         S = assemble(ctx, p)

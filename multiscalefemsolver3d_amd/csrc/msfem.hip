@@ -1961,9 +1961,11 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
   float ms = 0;
   MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
   // Problems the FP32 inner iteration could not solve (the reference's
-  // indefinite local operators, see k_batch_solve) are solved again from the
-  // zero guess in FP64, as the reference falls back to another solver
-  // (FEM/FEM.h:3726-3737).
+  // indefinite local operators, see k_batch_solve) are solved again in FP64,
+  // as the reference falls back to another solver (FEM/FEM.h:3726-3737),
+  // starting from where the FP32 run stopped (a start whose residual exceeds
+  // the zero guess's is dropped by the kernel's guess check and counted in
+  // n_fallback).
   rep->n_precision_fallback = 0;
   if (d->precision == MFS_PREC_MIXED) {
     const int nq = B->ncells * NB;
@@ -1989,7 +1991,7 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
       MFS_HIP(hipMemcpyAsync(dtg, tg.data(), sizeof(int) * tg.size(), hipMemcpyHostToDevice, s));
       mfs_batch_solve_desc d64 = *d;
       d64.precision = MFS_PREC_F64;
-      d64.use_guess = 0;
+      d64.use_guess = 1;
       d64.variant = 0;
       int wg64, pc64, var64;
       MFS_TRY(pick_shape(B, &d64, &wg64, &pc64, &var64));

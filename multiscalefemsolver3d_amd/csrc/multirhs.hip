@@ -40,23 +40,30 @@ struct MrState {
 
 enum { MR_RUN = 0, MR_CONV = 1, MR_BREAK = 2, MR_MAXIT = 3 };
 
-// Sum of NV*KR per-block partials part[q*G + b] (q < NV*KR) in block order,
-// done by every block of the consuming kernel (identical totals).
-template <int NQ>
+// Sum of NQ values' per-block partials part[q*G + b] in block order, done by
+// every block of the consuming kernel (identical totals).  The first P
+// partials of every value are loaded together (NQ*P loads in flight per
+// thread; G <= 256*P needs no more), so the NQ sums cost one memory latency
+// instead of one each.
+template <int NQ, int P>
 __device__ __forceinline__ void mr_sum_partials(const double* __restrict__ part, unsigned G, double (&tot)[NQ]) {
   __shared__ double red[NQ][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double pre[NQ][P];
+#pragma unroll
+  for (int q = 0; q < NQ; q++)
+#pragma unroll
+    for (int u = 0; u < P; u++) {
+      const unsigned b = threadIdx.x + u * 256;
+      pre[q][u] = b < G ? part[(size_t)q * G + b] : 0.0;
+    }
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    double s4[4] = {0, 0, 0, 0};
-    for (unsigned b0 = threadIdx.x; b0 < G; b0 += 4 * 256) {
+    double s = pre[q][0];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const unsigned b = b0 + u * 256;
-        if (b < G) s4[u] += part[(size_t)q * G + b];
-      }
-    }
-    const double s = wave_sum((s4[0] + s4[1]) + (s4[2] + s4[3]));
+    for (int u = 1; u < P; u++) s += pre[q][u];
+    for (unsigned b = threadIdx.x + P * 256; b < G; b += 256) s += part[(size_t)q * G + b];
+    s = wave_sum(s);
     if (lane == 0) red[q][wave] = s;
   }
   __syncthreads();
@@ -288,7 +295,7 @@ __global__ __launch_bounds__(256) void k_mr_dir_spmm(const int64_t* __restrict__
                                                      unsigned g_rz, double* __restrict__ part_pq, int xcd) {
   if (st->alldone) return;
   double t[2 * KR];
-  mr_sum_partials<2 * KR>(part_rz, g_rz, t);
+  mr_sum_partials<2 * KR, 4>(part_rz, g_rz, t);
   const int it = st->iter2;
   T beta[KR];
   int code[KR];
@@ -371,7 +378,7 @@ __global__ __launch_bounds__(256) void k_mr_update(T* __restrict__ X, const T* _
                                                    double* __restrict__ part_rz) {
   if (st->alldone) return;
   double pq[KR];
-  mr_sum_partials<KR>(part_pq, g_pq, pq);
+  mr_sum_partials<KR, 5>(part_pq, g_pq, pq);
   T alpha[KR];
 #pragma unroll
   for (int j = 0; j < KR; j++) {

@@ -205,6 +205,29 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
   block_partials<T, 1>(red, part_pq);
 }
 
+// Element range [e0, e1) of this block of K2.  With the XCD slice order
+// (xcd = 1) the blocks running on XCD x split the rows that K1's blocks on
+// XCD x own (g1 = K1's grid, 256 block rows of bs elements per K1 block), so
+// the z values K2 writes are the ones the same XCD gathers next; otherwise
+// equal contiguous chunks.  Chunks are multiples of 256 elements.  (Passing
+// host-computed ranges instead of these few divisions measured slower.)
+__device__ __forceinline__ void update_range(int64_t n, int xcd, unsigned g1, int bs, int64_t& e0, int64_t& e1) {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  if (!xcd || G < 8) {
+    const int64_t chunk = ((n + G - 1) / G + 255) / 256 * 256;
+    e0 = min(n, (int64_t)b * chunk);
+    e1 = min(n, e0 + chunk);
+    return;
+  }
+  const unsigned x = b & 7u, k = b >> 3, cnt = (G - x + 7) / 8;
+  const unsigned per = g1 >> 3, rem = g1 & 7u;
+  const int64_t s1 = (int64_t)x * per + min(x, rem), c1 = per + (x < rem ? 1 : 0);
+  const int64_t r0 = min(n, s1 * 256 * bs), r1 = min(n, (s1 + c1) * 256 * bs);
+  const int64_t chunk = ((r1 - r0 + cnt - 1) / cnt + 255) / 256 * 256;
+  e0 = min(r1, r0 + (int64_t)k * chunk);
+  e1 = min(r1, e0 + chunk);
+}
+
 // K2: alpha = rz / sum(part_pq); x += alpha p; r -= alpha q; z = dinv r;
 // per-block partials of r.z and r.r.  The first pass's vector loads are
 // issued before the partials of K1 are reduced.
@@ -213,20 +236,21 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* _
                                                    T* __restrict__ z, const T* __restrict__ q,
                                                    const T* __restrict__ dinv, int64_t n, CgState* st,
                                                    const double* __restrict__ part_pq, unsigned g_pq,
-                                                   double* __restrict__ part_rz) {
+                                                   double* __restrict__ part_rz, int xcd, unsigned g1, int bs) {
   const int was_done = st->done;
   const double rz = st->rz1;
   double pre[1][5];
   partials_issue<1, 5>(part_pq, g_pq, pre);
   if (was_done) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t e0, e1;
+  update_range(n, xcd, g1, bs, e0, e1);
+  const int64_t i0 = e0 + threadIdx.x;
   // first pass: four elements per thread, loaded together
   T xv[4], pv[4], rv[4], qv[4], dv[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const int64_t i = i0 + k * stride;
-    const bool in = i < n;
+    const int64_t i = i0 + k * 256;
+    const bool in = i < e1;
     xv[k] = in ? x[i] : (T)0;
     pv[k] = in ? p[i] : (T)0;
     rv[k] = in ? r[i] : (T)0;
@@ -245,8 +269,8 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* _
   T red[2] = {0, 0};
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const int64_t i = i0 + k * stride;
-    if (i >= n) continue;
+    const int64_t i = i0 + k * 256;
+    if (i >= e1) continue;
     x[i] = xv[k] + alpha * pv[k];
     const T ri = rv[k] - alpha * qv[k];
     r[i] = ri;
@@ -255,12 +279,12 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* _
     red[0] += ri * zi;
     red[1] += ri * ri;
   }
-  // further passes when the grid holds fewer than n/4 threads
-  for (int64_t j0 = i0 + 4 * stride; j0 < n; j0 += 4 * stride) {
+  // further passes when a block's chunk exceeds 1,024 elements
+  for (int64_t j0 = i0 + 1024; j0 < e1; j0 += 1024) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int64_t i = j0 + k * stride;
-      if (i >= n) continue;
+      const int64_t i = j0 + k * 256;
+      if (i >= e1) continue;
       x[i] += alpha * p[i];
       const T ri = r[i] - alpha * q[i];
       r[i] = ri;
@@ -432,7 +456,8 @@ struct PcgEngine {
     T* p_new = pbuf[parity ^ 1];
     launch_k1(s, p_old, p_new);
     finish_pq(s);
-    k_cg_update<T><<<g_vec, 256, 0, s>>>(x, p_new, r, z, q, dinv, n, st, pq_src(), pq_g(), part_rz());
+    k_cg_update<T><<<g_vec, 256, 0, s>>>(x, p_new, r, z, q, dinv, n, st, pq_src(), pq_g(), part_rz(), A->xcd_map,
+                                         g_spmv, A->bs);
     finish_rz(s);
   }
 
@@ -1339,19 +1364,38 @@ extern "C" int mfs_solver_run(mfs_solver S, mfs_vec b, mfs_vec x, mfs_solve_repo
   const unsigned gv = grid_for(n, 1024);
   const bool is_cg = d.method == MFS_PCG_JACOBI || d.method == MFS_CG;
   // plain FP64 (P)CG on b - A x0: r = r64 already; x correction starts at x0
+  // FP64 PCG to tol.  When the recursion reports convergence but the true
+  // residual is still above tol (the rounding floor of long runs on
+  // ill-conditioned or indefinite operators, where the recursive and true
+  // residuals part by a few per cent), the iteration continues from the
+  // true residual with the kept search direction (residual replacement, as
+  // in the mixed path), for at most 8 steps.
   auto run_f64 = [&](auto& E) -> int {
     MFS_HIP(hipMemcpyAsync(E.r, S->r64, 8 * n, hipMemcpyDeviceToDevice, s));
     MFS_HIP(hipMemcpyAsync(E.x, S->x64, 8 * n, hipMemcpyDeviceToDevice, s));
-    CgState hs;
-    MFS_TRY(E.run(d.tol, bb, d.maxit, &hs));
-    iters = hs.iter1;
-    outer = 1;
-    MFS_HIP(hipMemcpyAsync(S->x64, E.x, 8 * n, hipMemcpyDeviceToDevice, s));
-    rep->recursive_relres = std::sqrt(hs.rr / bb);
-    MFS_TRY(true_residual(S, &rr, &bb));
-    rel = std::sqrt(rr / bb);
-    if (hs.done == CG_BREAKDOWN) status = MFS_ERR_NONFINITE;
-    else if (rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
+    E.can_keep = false;
+    double keep_ratio = 0.0;
+    for (int step = 0; step < 8; step++) {
+      CgState hs;
+      MFS_TRY(E.run(d.tol, bb, d.maxit - iters, &hs, keep_ratio));
+      iters += hs.iter1;
+      outer++;
+      MFS_HIP(hipMemcpyAsync(S->x64, E.x, 8 * n, hipMemcpyDeviceToDevice, s));
+      const double predicted = std::sqrt(hs.rr / bb);
+      rep->recursive_relres = predicted;
+      MFS_TRY(true_residual(S, &rr, &bb));
+      rel = std::sqrt(rr / bb);
+      if (hs.done == CG_BREAKDOWN || !std::isfinite(rel)) {
+        status = MFS_ERR_NONFINITE;
+        break;
+      }
+      if (rel <= d.tol || hs.done != CG_CONVERGED || iters >= d.maxit) break;
+      // continue from the true residual (same units); restart instead when
+      // the recursion has drifted far from it
+      MFS_HIP(hipMemcpyAsync(E.r, S->r64, 8 * n, hipMemcpyDeviceToDevice, s));
+      keep_ratio = (E.can_keep && rel <= 2.0 * predicted) ? 1.0 : 0.0;
+    }
+    if (status == MFS_OK && rel > d.tol) status = MFS_ERR_NOT_CONVERGED;
     return MFS_OK;
   };
   // FP64 outer refinement around FP32 (P)CG.  When the FP32 recursion
@@ -1578,7 +1622,7 @@ int time_cg_kernel(PcgEngine<T>& E, int kernel, int reps, double* avg_ms, double
       E.launch_k1(s, E.pbuf[0], E.pbuf[1]);
     } else {
       k_cg_update<T><<<E.g_vec, 256, 0, s>>>(E.x, E.pbuf[1], E.r, E.z, E.q, E.dinv, n, E.st, E.pq_src(), E.pq_g(),
-                                             E.part_rz());
+                                             E.part_rz(), E.A->xcd_map, E.g_spmv, E.A->bs);
     }
   }
   MFS_HIP(hipEventRecord(e1, s));

@@ -139,6 +139,30 @@ __device__ __forceinline__ unsigned xcd_block(int on) {
   return x * per + min(x, rem) + (b >> 3);
 }
 
+// Element range [e0, e1) of this block of a vector-update kernel that
+// feeds a SELL kernel of grid g1 (256 block rows of bs elements per block).
+// With the XCD slice order (xcd = 1) the update blocks running on XCD x
+// split the rows that the SELL kernel's blocks on XCD x own, so the values
+// written are gathered next from the same L2; otherwise equal contiguous
+// chunks.  Chunks are multiples of gran elements.
+__device__ __forceinline__ void update_range(int64_t n, int xcd, unsigned g1, int bs, int gran, int64_t& e0,
+                                             int64_t& e1) {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  if (!xcd || G < 8) {
+    const int64_t chunk = ((n + G - 1) / G + gran - 1) / gran * gran;
+    e0 = min(n, (int64_t)b * chunk);
+    e1 = min(n, e0 + chunk);
+    return;
+  }
+  const unsigned x = b & 7u, k = b >> 3, cnt = (G - x + 7) / 8;
+  const unsigned per = g1 >> 3, rem = g1 & 7u;
+  const int64_t s1 = (int64_t)x * per + min(x, rem), c1 = per + (x < rem ? 1 : 0);
+  const int64_t r0 = min(n, s1 * 256 * bs), r1 = min(n, (s1 + c1) * 256 * bs);
+  const int64_t chunk = ((r1 - r0 + cnt - 1) / cnt + gran - 1) / gran * gran;
+  e0 = min(r1, r0 + (int64_t)k * chunk);
+  e1 = min(r1, e0 + chunk);
+}
+
 // Block-level sum of NV values; result valid in thread 0.  Fixed order.
 template <typename T, int NV, int BLOCK>
 __device__ __forceinline__ void block_sum(T (&v)[NV], T* lds /* NV * BLOCK/64 */) {

@@ -375,7 +375,7 @@ __global__ __launch_bounds__(256) void k_mr_update(T* __restrict__ X, const T* _
                                                    T* __restrict__ Z, const T* __restrict__ Q,
                                                    const T* __restrict__ dinv, int64_t n, MrState* st,
                                                    const double* __restrict__ part_pq, unsigned g_pq,
-                                                   double* __restrict__ part_rz) {
+                                                   double* __restrict__ part_rz, int xcd, unsigned g1, int bs) {
   if (st->alldone) return;
   double pq[KR];
   mr_sum_partials<KR, 5>(part_pq, g_pq, pq);
@@ -395,7 +395,10 @@ __global__ __launch_bounds__(256) void k_mr_update(T* __restrict__ X, const T* _
   double red[2 * KR];
 #pragma unroll
   for (int q = 0; q < 2 * KR; q++) red[q] = 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  // the rows that this XCD's blocks of k_mr_dir_spmm gather next
+  int64_t e0, e1;
+  update_range(n, xcd, g1, bs, 64, e0, e1);
+  for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
     const T d = dinv ? dinv[i] : (T)1;
     T xx[KR], pp[KR], rr[KR], qq[KR], zz[KR];
     vload<T, KR>(X + i * KR, xx);
@@ -531,7 +534,7 @@ struct MrEngine {
                                                               A->xcd_map);
     if (fin) k_finish_sums<<<KR, 256, 0, s>>>(part_pq, g_spmv, tot_pq);
     k_mr_update<T, KR><<<g_vec, 256, 0, s>>>(X, p_new, R, Z, Q, dinv, n, st, fin ? tot_pq : part_pq,
-                                             fin ? 1u : g_spmv, part_rz);
+                                             fin ? 1u : g_spmv, part_rz, A->xcd_map, g_spmv, A->bs);
     finish_rz(s);
   }
 

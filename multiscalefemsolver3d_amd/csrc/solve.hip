@@ -205,29 +205,6 @@ __global__ __launch_bounds__(256) void k_cg_dir_spmv(const int64_t* __restrict__
   block_partials<T, 1>(red, part_pq);
 }
 
-// Element range [e0, e1) of this block of K2.  With the XCD slice order
-// (xcd = 1) the blocks running on XCD x split the rows that K1's blocks on
-// XCD x own (g1 = K1's grid, 256 block rows of bs elements per K1 block), so
-// the z values K2 writes are the ones the same XCD gathers next; otherwise
-// equal contiguous chunks.  Chunks are multiples of 256 elements.  (Passing
-// host-computed ranges instead of these few divisions measured slower.)
-__device__ __forceinline__ void update_range(int64_t n, int xcd, unsigned g1, int bs, int64_t& e0, int64_t& e1) {
-  const unsigned G = gridDim.x, b = blockIdx.x;
-  if (!xcd || G < 8) {
-    const int64_t chunk = ((n + G - 1) / G + 255) / 256 * 256;
-    e0 = min(n, (int64_t)b * chunk);
-    e1 = min(n, e0 + chunk);
-    return;
-  }
-  const unsigned x = b & 7u, k = b >> 3, cnt = (G - x + 7) / 8;
-  const unsigned per = g1 >> 3, rem = g1 & 7u;
-  const int64_t s1 = (int64_t)x * per + min(x, rem), c1 = per + (x < rem ? 1 : 0);
-  const int64_t r0 = min(n, s1 * 256 * bs), r1 = min(n, (s1 + c1) * 256 * bs);
-  const int64_t chunk = ((r1 - r0 + cnt - 1) / cnt + 255) / 256 * 256;
-  e0 = min(r1, r0 + (int64_t)k * chunk);
-  e1 = min(r1, e0 + chunk);
-}
-
 // K2: alpha = rz / sum(part_pq); x += alpha p; r -= alpha q; z = dinv r;
 // per-block partials of r.z and r.r.  The first pass's vector loads are
 // issued before the partials of K1 are reduced.
@@ -243,7 +220,7 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* _
   partials_issue<1, 5>(part_pq, g_pq, pre);
   if (was_done) return;
   int64_t e0, e1;
-  update_range(n, xcd, g1, bs, e0, e1);
+  update_range(n, xcd, g1, bs, 256, e0, e1);
   const int64_t i0 = e0 + threadIdx.x;
   // first pass: four elements per thread, loaded together
   T xv[4], pv[4], rv[4], qv[4], dv[4];
@@ -840,7 +817,7 @@ __global__ __launch_bounds__(256) void k_bs_update(T* __restrict__ x, T* __restr
                                                    const T* __restrict__ s, const T* __restrict__ t,
                                                    const T* __restrict__ r0, const T* __restrict__ dinv, int64_t n,
                                                    BsState* st, const double* __restrict__ part_b, unsigned g_b,
-                                                   double* __restrict__ part_c) {
+                                                   double* __restrict__ part_c, int xcd, unsigned g1, int bs) {
   const int was_done = st->done;
   const T alpha = (T)st->alpha, beta = (T)st->beta;
   double pre[2][4];
@@ -855,7 +832,10 @@ __global__ __launch_bounds__(256) void k_bs_update(T* __restrict__ x, T* __restr
   }
   const T om = (T)omega;
   T red[2] = {0, 0};
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  // the rows that this XCD's blocks of k_bs_dir_spmv gather next
+  int64_t e0, e1;
+  update_range(n, xcd, g1, bs, 64, e0, e1);
+  for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
     const T di = dinv ? dinv[i] : (T)1;
     const T ph = rh[i] + beta * wh[i];
     const T si = s[i];
@@ -985,7 +965,7 @@ struct BsEngine {
     spmv(s, 1);
     if (fin) k_finish_sums<<<2, 256, 0, s>>>(part_b, g_spmv, tot + 1);
     k_bs_update<T><<<g_vec, 256, 0, s>>>(x, r, rh, wh, vh, sv, t, r0, dinv, n, st, src_b(), fin ? 1u : g_spmv,
-                                         part_c);
+                                         part_c, A->xcd_map, g_spmv, A->bs);
     if (fin) k_finish_sums<<<2, 256, 0, s>>>(part_c, g_vec, tot + 3);
   }
 

@@ -361,6 +361,7 @@ struct PcgEngine {
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // end of the chunks in the two status slots
 
   // Large systems: one extra kernel per reduction adds the producer's
   // partials once (tot), instead of every consumer block re-adding them all
@@ -474,19 +475,47 @@ struct PcgEngine {
     }
     finish_rz(s);
     MFS_HIP(hipGetLastError());
+    // Chunks of iterations; each ends with a copy of the state into one of
+    // two pinned slots.  While the residual is far above the tolerance one
+    // more chunk is queued before the host waits for the previous one, so
+    // the GPU does not idle during the host's check; near convergence the
+    // host checks every chunk (a chunk queued after convergence returns at
+    // once in every kernel and changes nothing).
     CgState* hs = (CgState*)A->ctx->pinned;
-    hs->done = CG_RUN;
-    while (hs->done == CG_RUN) {
+    for (hipEvent_t* e : {&ev[0], &ev[1]})
+      if (!*e) MFS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    auto submit = [&](int slot) -> int {
       if (use_graph) {
         MFS_HIP(hipGraphLaunch(exec, s));
       } else {
         for (int k = 0; k < chunk; k++) launch_iteration(s, k & 1);
         MFS_HIP(hipGetLastError());
       }
-      MFS_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-      MFS_HIP(hipStreamSynchronize(s));
+      MFS_HIP(hipMemcpyAsync(hs + slot, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+      MFS_HIP(hipEventRecord(ev[slot], s));
+      return MFS_OK;
+    };
+    int head = 0, inflight = 1;
+    bool ahead = false;
+    MFS_TRY(submit(0));
+    for (;;) {
+      if (ahead && inflight == 1) {
+        MFS_TRY(submit(head ^ 1));
+        inflight = 2;
+      }
+      MFS_HIP(hipEventSynchronize(ev[head]));
+      inflight--;
+      if (hs[head].done != CG_RUN) break;
+      ahead = hs[head].rr > 100.0 * hs[head].tol2;
+      head ^= 1;
+      if (inflight == 0) {
+        MFS_TRY(submit(head));
+        inflight = 1;
+      }
     }
-    *host_state = *hs;
+    *host_state = hs[head];
+    hs = host_state;
+    if (inflight) MFS_HIP(hipStreamSynchronize(s));
     host_state->iter2 = hs->iter1;  // iterations completed when K1 stopped
     // the newest direction is in pbuf[iter & 1]; rz2 holds r.z of the
     // iteration before the last (the denominator of the next beta)
@@ -499,6 +528,9 @@ struct PcgEngine {
   void destroy() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    ev[0] = ev[1] = nullptr;
     for (T* b : {x, r, z, q, pbuf[0], pbuf[1]}) dfree(b);
     dfree(st);
     dfree(tot);
@@ -897,6 +929,7 @@ struct BsEngine {
   int chunk = 8;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // end of the chunks in the two status slots
 
   const double* src_a() { return fin ? tot : part_a; }
   const double* src_b() { return fin ? tot + 1 : part_b; }
@@ -980,16 +1013,44 @@ struct BsEngine {
     BsState* hs = (BsState*)A->ctx->pinned;
     MFS_HIP(hipMemcpyAsync(hs, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
     MFS_HIP(hipStreamSynchronize(s));
-    while (!hs->done) {
+    if (hs->done) {
+      *out = *hs;
+      return MFS_OK;
+    }
+    // chunks of iterations, one queued ahead while the residual is far above
+    // the tolerance (as in the PCG engine)
+    for (hipEvent_t* e : {&ev[0], &ev[1]})
+      if (!*e) MFS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    auto submit = [&](int slot) -> int {
       if (exec)
         MFS_HIP(hipGraphLaunch(exec, s));
       else
         for (int k = 0; k < chunk; k++) launch_iteration(s);
       MFS_HIP(hipGetLastError());
-      MFS_HIP(hipMemcpyAsync(hs, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
-      MFS_HIP(hipStreamSynchronize(s));
+      MFS_HIP(hipMemcpyAsync(hs + slot, st, sizeof(BsState), hipMemcpyDeviceToHost, s));
+      MFS_HIP(hipEventRecord(ev[slot], s));
+      return MFS_OK;
+    };
+    int head = 0, inflight = 1;
+    bool ahead = false;
+    MFS_TRY(submit(0));
+    for (;;) {
+      if (ahead && inflight == 1) {
+        MFS_TRY(submit(head ^ 1));
+        inflight = 2;
+      }
+      MFS_HIP(hipEventSynchronize(ev[head]));
+      inflight--;
+      if (hs[head].done) break;
+      ahead = hs[head].rr > 100.0 * hs[head].tol2;
+      head ^= 1;
+      if (inflight == 0) {
+        MFS_TRY(submit(head));
+        inflight = 1;
+      }
     }
-    *out = *hs;
+    *out = hs[head];
+    if (inflight) MFS_HIP(hipStreamSynchronize(s));
     return MFS_OK;
   }
 
@@ -998,6 +1059,9 @@ struct BsEngine {
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
+    for (hipEvent_t& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    ev[0] = ev[1] = nullptr;
     for (T* b : {x, r, r0, rh, wh, v, vh, sv, t}) dfree(b);
     x = r = r0 = rh = wh = v = vh = sv = t = nullptr;
     dfree(st);

@@ -471,6 +471,7 @@ struct MrEngine {
   int chunk = 16;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // end of the chunks in the two status slots
 
   int init(mfs_mat A_, bool precond) {
     A = A_;
@@ -566,15 +567,41 @@ struct MrEngine {
     k_mr_state<<<1, 64, 0, s>>>(st, dtol, k, maxit);
     MFS_HIP(hipGetLastError());
     MFS_HIP(hipStreamSynchronize(s));
+    // chunks of iterations, one queued ahead while some running system's
+    // residual is far above its tolerance (as in the single-system engine)
     MrState* hs = (MrState*)A->ctx->pinned;
-    hs->alldone = 0;
-    while (!hs->alldone) {
+    for (hipEvent_t* e : {&ev[0], &ev[1]})
+      if (!*e) MFS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    auto submit = [&](int slot) -> int {
       launch_chunk(s);
       MFS_HIP(hipGetLastError());
-      MFS_HIP(hipMemcpyAsync(hs, st, sizeof(MrState), hipMemcpyDeviceToHost, s));
-      MFS_HIP(hipStreamSynchronize(s));
+      MFS_HIP(hipMemcpyAsync(hs + slot, st, sizeof(MrState), hipMemcpyDeviceToHost, s));
+      MFS_HIP(hipEventRecord(ev[slot], s));
+      return MFS_OK;
+    };
+    int head = 0, inflight = 1;
+    bool ahead = false;
+    MFS_TRY(submit(0));
+    for (;;) {
+      if (ahead && inflight == 1) {
+        MFS_TRY(submit(head ^ 1));
+        inflight = 2;
+      }
+      MFS_HIP(hipEventSynchronize(ev[head]));
+      inflight--;
+      if (hs[head].alldone) break;
+      ahead = false;
+      for (int j = 0; j < MR_MAX; j++)
+        ahead |= hs[head].done[j] == MR_RUN && hs[head].rr[j] > 100.0 * hs[head].tol2[j];
+      head ^= 1;
+      if (inflight == 0) {
+        MFS_TRY(submit(head));
+        inflight = 1;
+      }
     }
-    *out = *hs;
+    *out = hs[head];
+    hs = out;
+    if (inflight) MFS_HIP(hipStreamSynchronize(s));
     // every system's newest direction is in P[iter & 1] (stopped systems
     // copied theirs through the later iterations)
     last_iter = hs->iter1;
@@ -587,6 +614,9 @@ struct MrEngine {
   }
 
   void destroy() {
+    for (hipEvent_t& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    ev[0] = ev[1] = nullptr;
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (T* b : {X, R, Z, Q, P[0], P[1]}) dfree(b);

@@ -590,7 +590,7 @@ __global__ __launch_bounds__(256) void k_orth_update(const T* __restrict__ V, T*
 
 template <typename T>
 __global__ void k_scale_into(const T* __restrict__ w, T* __restrict__ v, const double* __restrict__ nrm, int64_t n) {
-  const T inv = (T)(1.0 / sqrt(nrm[0]));
+  const T inv = nrm[0] > 0.0 ? (T)(1.0 / sqrt(nrm[0])) : (T)0;  // a zero vector stays zero
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     v[i] = w[i] * inv;
 }
@@ -1189,7 +1189,8 @@ extern "C" int mfs_solver_create(mfs_mat A, const mfs_solve_desc* desc, mfs_solv
     tryit(dmalloc(&S->V, sv * n * (desc->restart + 1)));
     tryit(dmalloc(&S->w, sv * n));
     tryit(dmalloc(&S->z, sv * n));
-    tryit(dmalloc((void**)&S->dh, 8 * 64));
+    // 64 doubles of h / y, then the rotation state of k_gmres_rot (m <= 30)
+    tryit(dmalloc((void**)&S->dh, 8 * (64 + 31 * 30 + 3 * 30 + 1 + 3 * 30)));
     if (s == MFS_OK) tryit(reduce_scratch_alloc(S->redg, 32 * 1024 + 64));
   }
   if (s != MFS_OK) {
@@ -1212,6 +1213,55 @@ static int true_residual(mfs_solver S, double* rr, double* bb) {
   return MFS_OK;
 }
 
+// GMRES Arnoldi step j on the device (one thread): column j of H from the
+// two CGS2 passes (dh[i] + dh[32 + i], i <= j) and ||w||^2 (dh[62]), the
+// previous Givens rotations applied, the new rotation, g updated and
+// est[j] = |g[j+1]|.  gm: H[(m+1) m] (row i, column j at i m + j), cs[m],
+// sn[m], g[m+1], est[m], hn[m], bad[m] (1: zero or nonfinite pivot; later
+// steps then change nothing).
+__global__ void k_gmres_rot(const double* __restrict__ dh, double* __restrict__ gm, int j, int m) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* H = gm;
+  double* cs = H + (m + 1) * m;
+  double* sn = cs + m;
+  double* g = sn + m;
+  double* est = g + m + 1;
+  double* hnv = est + m;
+  double* bad = hnv + m;
+  if (j > 0 && bad[j - 1] != 0.0) {
+    bad[j] = 1.0;
+    return;
+  }
+  const double hn = sqrt(fmax(dh[62], 0.0));
+  for (int i = 0; i <= j; i++) H[i * m + j] = dh[i] + dh[32 + i];
+  H[(j + 1) * m + j] = hn;
+  for (int i = 0; i < j; i++) {
+    const double a = H[i * m + j], c = H[(i + 1) * m + j];
+    H[i * m + j] = cs[i] * a + sn[i] * c;
+    H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+  }
+  const double a = H[j * m + j], c = H[(j + 1) * m + j];
+  const double d = sqrt(a * a + c * c);
+  hnv[j] = hn;
+  if (!(d > 0) || !isfinite(d)) {
+    bad[j] = 1.0;
+    return;
+  }
+  bad[j] = 0.0;
+  cs[j] = a / d;
+  sn[j] = c / d;
+  H[j * m + j] = d;
+  H[(j + 1) * m + j] = 0;
+  g[j + 1] = -sn[j] * g[j];
+  g[j] = cs[j] * g[j];
+  est[j] = fabs(g[j + 1]);
+}
+
+__global__ void k_gmres_rot_init(double* __restrict__ gm, int m, double beta) {
+  double* g = gm + (m + 1) * m + 2 * m;
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) g[i] = i == 0 ? beta : 0.0;
+}
+
 template <typename T>
 static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double beta, double tol_abs, int m,
                        int maxsteps, int* steps_done, double* est_abs, T* u_out) {
@@ -1225,53 +1275,61 @@ static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double
   T* z = (T*)S->z;
   const unsigned gv = grid_for(n, 1024);
   const int prec = std::is_same<T, double>::value ? MFS_F64 : MFS_F32;
-  std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
-  g[0] = beta;
-  // v0 = rhs / beta
+  // The Arnoldi steps and their Givens rotations run on the device; the
+  // host reads the residual estimates every kSync steps (not every step) and
+  // uses the first step whose estimate reaches tol_abs, so the steps run
+  // past it are ignored and the result is that of a per-step check.
+  constexpr int kSync = 6;
+  double* gm = S->dh + 64;
   double* hp = (double*)A->ctx->pinned;
   hp[0] = beta * beta;
   MFS_HIP(hipMemcpyAsync(S->dh + 40, hp, 8, hipMemcpyHostToDevice, s));
   k_scale_into<T><<<gv, 256, 0, s>>>(rhs, V, S->dh + 40, n);
-  int j = 0;
+  k_gmres_rot_init<<<1, 64, 0, s>>>(gm, m, beta);
+  const size_t off_est = (size_t)(m + 1) * m + 3 * m + 1;  // est, hn, bad follow
+  int j = 0, used = -1;
   double est = beta;
-  for (; j < m && *steps_done < maxsteps; j++) {
-    k_dscale<T><<<gv, 256, 0, s>>>(V + (int64_t)j * n, dinv, z, n);
-    MFS_TRY(spmv_launch(A, prec, z, w, s));
-    // CGS2: two passes of h = V^T w, w -= V h
-    for (int pass = 0; pass < 2; pass++) {
-      k_multidot<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->redg.partials);
-      k_finish_sums<<<j + 1, 256, 0, s>>>(S->redg.partials, gv, S->dh + pass * 32);
-      k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->dh + pass * 32, S->redg.partials);
-      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 62);
+  while (j < m && *steps_done + j < maxsteps && used < 0) {
+    const int j0 = j;
+    const int nb = std::min(kSync, std::min(m - j, maxsteps - *steps_done - j));
+    if (nb <= 0) break;
+    for (int t = 0; t < nb; t++, j++) {
+      k_dscale<T><<<gv, 256, 0, s>>>(V + (int64_t)j * n, dinv, z, n);
+      MFS_TRY(spmv_launch(A, prec, z, w, s));
+      // CGS2: two passes of h = V^T w, w -= V h
+      for (int pass = 0; pass < 2; pass++) {
+        k_multidot<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->redg.partials);
+        k_finish_sums<<<j + 1, 256, 0, s>>>(S->redg.partials, gv, S->dh + pass * 32);
+        k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->dh + pass * 32, S->redg.partials);
+        k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 62);
+      }
+      k_scale_into<T><<<gv, 256, 0, s>>>(w, V + (int64_t)(j + 1) * n, S->dh + 62, n);
+      k_gmres_rot<<<1, 64, 0, s>>>(S->dh, gm, j, m);
     }
     MFS_HIP(hipGetLastError());
-    MFS_HIP(hipMemcpyAsync(hp, S->dh, 64 * sizeof(double), hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipMemcpyAsync(hp, gm + off_est, 3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
     MFS_HIP(hipStreamSynchronize(s));
-    const double hn = std::sqrt(std::max(hp[62], 0.0));
-    for (int i = 0; i <= j; i++) H[i * m + j] = hp[i] + hp[32 + i];
-    H[(j + 1) * m + j] = hn;
-    if (hn > 0) k_scale_into<T><<<gv, 256, 0, s>>>(w, V + (int64_t)(j + 1) * n, S->dh + 62, n);
-    for (int i = 0; i < j; i++) {
-      double a = H[i * m + j], c = H[(i + 1) * m + j];
-      H[i * m + j] = cs[i] * a + sn[i] * c;
-      H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
-    }
-    double a = H[j * m + j], c = H[(j + 1) * m + j];
-    double d = std::sqrt(a * a + c * c);
-    if (!(d > 0) || !std::isfinite(d)) return MFS_ERR_NONFINITE;
-    cs[j] = a / d;
-    sn[j] = c / d;
-    H[j * m + j] = d;
-    H[(j + 1) * m + j] = 0;
-    g[j + 1] = -sn[j] * g[j];
-    g[j] = cs[j] * g[j];
-    (*steps_done)++;
-    est = std::fabs(g[j + 1]);
-    if (est <= tol_abs || hn == 0) {
-      j++;
-      break;
+    const double* est_h = hp;
+    const double* hn_h = hp + m;
+    const double* bad_h = hp + 2 * m;
+    for (int t = j0; t < j; t++) {
+      if (bad_h[t] != 0.0) return MFS_ERR_NONFINITE;
+      est = est_h[t];
+      if (est <= tol_abs || hn_h[t] == 0) {
+        used = t + 1;
+        break;
+      }
     }
   }
+  if (used < 0) used = j;
+  *steps_done += used;
+  j = used;
+  // H (upper triangle of the first j columns) and g for the back substitution
+  std::vector<double> H((size_t)(m + 1) * m), g(m + 1);
+  MFS_HIP(hipMemcpyAsync(hp, gm, ((size_t)(m + 1) * m + 3 * m + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+  MFS_HIP(hipStreamSynchronize(s));
+  std::copy(hp, hp + (size_t)(m + 1) * m, H.begin());
+  std::copy(hp + (size_t)(m + 1) * m + 2 * m, hp + (size_t)(m + 1) * m + 3 * m + 1, g.begin());
   std::vector<double> y(j);
   for (int i = j - 1; i >= 0; i--) {
     double sacc = g[i];

@@ -544,30 +544,23 @@ struct PcgEngine {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_multidot(const T* __restrict__ V, const T* __restrict__ w, int64_t n,
-                                                  int kcount, double* __restrict__ partials) {
-  // partials[k*G + b] = sum over this block's rows of v_k . w, 8 bases per
-  // pass so w is read ceil(kcount/8) times
-  __shared__ double red[8][4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k0 = 0; k0 < kcount; k0 += 8) {
-    T s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-      const T wi = w[i];
+                                                  double* __restrict__ partials) {
+  // block (x, k): partials[k*G + x] = this block's rows of v_k . w (G =
+  // gridDim.x); every block reduces one value, and the blocks of all k run
+  // at once (w is re-read from the cache by each k)
+  const int k = blockIdx.y;
+  const T* __restrict__ v = V + (int64_t)k * n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  T s[4] = {0, 0, 0, 0};
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
 #pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (k0 + k < kcount) s[k] += V[(int64_t)(k0 + k) * n + i] * wi;
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * stride;
+      if (i < n) s[u] += v[i] * w[i];
     }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const double t = wave_sum((double)s[k]);
-      if (lane == 0) red[k][wave] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < 8 && k0 + threadIdx.x < kcount)
-      partials[(size_t)(k0 + threadIdx.x) * gridDim.x + blockIdx.x] =
-          (red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]);
-    __syncthreads();
   }
+  T t[1] = {(s[0] + s[1]) + (s[2] + s[3])};
+  write_block_partials<T, 1>(t, partials + (size_t)k * gridDim.x);
 }
 
 // w -= sum_k h_k v_k (h in device memory, double); per-block partial ||w||^2.
@@ -1094,6 +1087,8 @@ struct mfs_solver_s {
   void* w = nullptr;
   void* z = nullptr;
   double* dh = nullptr;  // device h / y (64 doubles)
+  // GMRES: one captured graph per Arnoldi step index j (FP64 and FP32 inner)
+  hipGraphExec_t gstep[2][30] = {};
   ReduceScratch redg;
   BsEngine<double> bs64;
   BsEngine<float> bs32;
@@ -1127,6 +1122,9 @@ extern "C" int mfs_solver_destroy(mfs_solver S) {
   dfree(S->w);
   dfree(S->z);
   dfree(S->dh);
+  for (auto& row : S->gstep)
+    for (hipGraphExec_t& g : row)
+      if (g) (void)hipGraphExecDestroy(g);
   reduce_scratch_free(S->red64);
   reduce_scratch_free(S->redg);
   delete S;
@@ -1281,6 +1279,28 @@ static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double
   // past it are ignored and the result is that of a per-step check.
   constexpr int kSync = 6;
   double* gm = S->dh + 64;
+  // one Arnoldi step with its rotation; replayed from a graph per j (the
+  // launches of a step are otherwise the host-side bottleneck at ~12 per
+  // step), directly with MFS_NO_GRAPH=1 (kernel tracing)
+  const char* ng = std::getenv("MFS_NO_GRAPH");
+  const bool use_graph = !(ng && ng[0] == '1');
+  // the FP32 copy of the values is refreshed outside any capture
+  if (prec == MFS_F32 && !A->val32_valid) MFS_TRY(mat_refresh_derived(A));
+  const unsigned gmd = std::max(1u, gv / 4);  // multi-dot blocks per vector (4 rows per thread)
+  auto arnoldi_step = [&](int jj) -> int {
+    k_dscale<T><<<gv, 256, 0, s>>>(V + (int64_t)jj * n, dinv, z, n);
+    MFS_TRY(spmv_launch(A, prec, z, w, s));
+    // CGS2: two passes of h = V^T w, w -= V h
+    for (int pass = 0; pass < 2; pass++) {
+      k_multidot<T><<<dim3(gmd, jj + 1), 256, 0, s>>>(V, w, n, S->redg.partials);
+      k_finish_sums<<<jj + 1, 256, 0, s>>>(S->redg.partials, gmd, S->dh + pass * 32);
+      k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, jj + 1, S->dh + pass * 32, S->redg.partials);
+      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 62);
+    }
+    k_scale_into<T><<<gv, 256, 0, s>>>(w, V + (int64_t)(jj + 1) * n, S->dh + 62, n);
+    k_gmres_rot<<<1, 64, 0, s>>>(S->dh, gm, jj, m);
+    return MFS_OK;
+  };
   double* hp = (double*)A->ctx->pinned;
   hp[0] = beta * beta;
   MFS_HIP(hipMemcpyAsync(S->dh + 40, hp, 8, hipMemcpyHostToDevice, s));
@@ -1294,17 +1314,21 @@ static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double
     const int nb = std::min(kSync, std::min(m - j, maxsteps - *steps_done - j));
     if (nb <= 0) break;
     for (int t = 0; t < nb; t++, j++) {
-      k_dscale<T><<<gv, 256, 0, s>>>(V + (int64_t)j * n, dinv, z, n);
-      MFS_TRY(spmv_launch(A, prec, z, w, s));
-      // CGS2: two passes of h = V^T w, w -= V h
-      for (int pass = 0; pass < 2; pass++) {
-        k_multidot<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->redg.partials);
-        k_finish_sums<<<j + 1, 256, 0, s>>>(S->redg.partials, gv, S->dh + pass * 32);
-        k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, j + 1, S->dh + pass * 32, S->redg.partials);
-        k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 62);
+      if (use_graph) {
+        hipGraphExec_t& ge = S->gstep[std::is_same<T, double>::value ? 0 : 1][j];
+        if (!ge) {
+          hipGraph_t g;
+          MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          const int st = arnoldi_step(j);
+          MFS_HIP(hipStreamEndCapture(s, &g));
+          MFS_TRY(st);
+          MFS_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+          (void)hipGraphDestroy(g);
+        }
+        MFS_HIP(hipGraphLaunch(ge, s));
+      } else {
+        MFS_TRY(arnoldi_step(j));
       }
-      k_scale_into<T><<<gv, 256, 0, s>>>(w, V + (int64_t)(j + 1) * n, S->dh + 62, n);
-      k_gmres_rot<<<1, 64, 0, s>>>(S->dh, gm, j, m);
     }
     MFS_HIP(hipGetLastError());
     MFS_HIP(hipMemcpyAsync(hp, gm + off_est, 3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1390,7 +1414,7 @@ static int minres_run(mfs_solver S, const T* dinv, double rtol, int maxsteps, in
   for (; j < maxsteps; j++) {
     MFS_TRY(spmv_launch(A, prec, zj, Az, s));
     // delta = <A z_j, z_j>
-    k_multidot<T><<<gv, 256, 0, s>>>(zj, Az, n, 1, S->redg.partials);
+    k_multidot<T><<<dim3(gv, 1), 256, 0, s>>>(zj, Az, n, S->redg.partials);
     k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 1);
     k_minres_lanczos<T><<<gv, 256, 0, s>>>(Az, v[1], v[0], v[2], znext, dinv, S->dh + 1, g, g_prev, n,
                                            S->redg.partials);

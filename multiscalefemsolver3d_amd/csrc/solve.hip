@@ -1217,8 +1217,11 @@ static int true_residual(mfs_solver S, double* rr, double* bb) {
 // est[j] = |g[j+1]|.  gm: H[(m+1) m] (row i, column j at i m + j), cs[m],
 // sn[m], g[m+1], est[m], hn[m], bad[m] (1: zero or nonfinite pivot; later
 // steps then change nothing).
-__global__ void k_gmres_rot(const double* __restrict__ dh, double* __restrict__ gm, int j, int m) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(64) void k_gmres_rot(const double* __restrict__ dh, double* __restrict__ gm, int j,
+                                                  int m) {
+  // the column and the previous rotations are staged in LDS by the wave, so
+  // the serial rotation chain of thread 0 reads no global memory
+  __shared__ double col[32], c_s[32], s_s[32];
   double* H = gm;
   double* cs = H + (m + 1) * m;
   double* sn = cs + m;
@@ -1226,33 +1229,43 @@ __global__ void k_gmres_rot(const double* __restrict__ dh, double* __restrict__ 
   double* est = g + m + 1;
   double* hnv = est + m;
   double* bad = hnv + m;
+  const int tid = threadIdx.x;
   if (j > 0 && bad[j - 1] != 0.0) {
-    bad[j] = 1.0;
+    if (tid == 0) bad[j] = 1.0;
     return;
   }
-  const double hn = sqrt(fmax(dh[62], 0.0));
-  for (int i = 0; i <= j; i++) H[i * m + j] = dh[i] + dh[32 + i];
-  H[(j + 1) * m + j] = hn;
-  for (int i = 0; i < j; i++) {
-    const double a = H[i * m + j], c = H[(i + 1) * m + j];
-    H[i * m + j] = cs[i] * a + sn[i] * c;
-    H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+  if (tid <= j) col[tid] = dh[tid] + dh[32 + tid];
+  if (tid < j) {
+    c_s[tid] = cs[tid];
+    s_s[tid] = sn[tid];
   }
-  const double a = H[j * m + j], c = H[(j + 1) * m + j];
+  __syncthreads();
+  if (tid != 0) return;
+  const double hn = sqrt(fmax(dh[62], 0.0));
+  double below = hn;  // entry j + 1 of the column
+  for (int i = 0; i < j; i++) {
+    const double a = col[i], c = col[i + 1];
+    col[i] = c_s[i] * a + s_s[i] * c;
+    col[i + 1] = -s_s[i] * a + c_s[i] * c;
+  }
+  const double a = col[j], c = below;
   const double d = sqrt(a * a + c * c);
   hnv[j] = hn;
+  for (int i = 0; i < j; i++) H[i * m + j] = col[i];
   if (!(d > 0) || !isfinite(d)) {
     bad[j] = 1.0;
     return;
   }
   bad[j] = 0.0;
-  cs[j] = a / d;
-  sn[j] = c / d;
+  const double cj = a / d, sj = c / d;
+  cs[j] = cj;
+  sn[j] = sj;
   H[j * m + j] = d;
   H[(j + 1) * m + j] = 0;
-  g[j + 1] = -sn[j] * g[j];
-  g[j] = cs[j] * g[j];
-  est[j] = fabs(g[j + 1]);
+  const double gj = g[j];
+  g[j + 1] = -sj * gj;
+  g[j] = cj * gj;
+  est[j] = fabs(sj * gj);
 }
 
 __global__ void k_gmres_rot_init(double* __restrict__ gm, int m, double beta) {

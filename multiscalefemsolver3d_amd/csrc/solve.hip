@@ -1308,7 +1308,8 @@ static int gmres_cycle(mfs_solver S, const T* val, const T* dinv, T* rhs, double
       k_multidot<T><<<dim3(gmd, jj + 1), 256, 0, s>>>(V, w, n, S->redg.partials);
       k_finish_sums<<<jj + 1, 256, 0, s>>>(S->redg.partials, gmd, S->dh + pass * 32);
       k_orth_update<T><<<gv, 256, 0, s>>>(V, w, n, jj + 1, S->dh + pass * 32, S->redg.partials);
-      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 62);
+      // ||w||^2 is needed after the second pass only
+      if (pass == 1) k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 62);
     }
     k_scale_into<T><<<gv, 256, 0, s>>>(w, V + (int64_t)(jj + 1) * n, S->dh + 62, n);
     k_gmres_rot<<<1, 64, 0, s>>>(S->dh, gm, jj, m);

@@ -163,68 +163,6 @@ __device__ __forceinline__ void update_range(int64_t n, int xcd, unsigned g1, in
   e1 = min(r1, e0 + chunk);
 }
 
-// Block-level sum of NV values; result valid in thread 0.  Fixed order.
-template <typename T, int NV, int BLOCK>
-__device__ __forceinline__ void block_sum(T (&v)[NV], T* lds /* NV * BLOCK/64 */) {
-  const int lane = threadIdx.x & (MFS_WAVE - 1), wave = threadIdx.x / MFS_WAVE;
-  constexpr int NW = BLOCK / MFS_WAVE;
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    T s = wave_sum(v[k]);
-    if (lane == 0) lds[k * NW + wave] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-      T s = 0;
-      for (int w = 0; w < NW; w++) s += lds[k * NW + w];
-      v[k] = s;
-    }
-  }
-}
-
-// Grid reduction: every block contributes NV values; the last block to
-// arrive sums all partials in block order (deterministic) and returns true
-// with the totals in `tot` (valid in all threads of that block).  The counter
-// is reset by the last block, so the kernel can be replayed (HIP graphs).
-template <typename T, int NV, int BLOCK>
-__device__ __forceinline__ bool grid_sum(T (&v)[NV], double* partials, unsigned* counter, double (&tot)[NV]) {
-  __shared__ T lds[NV * (BLOCK / MFS_WAVE)];
-  __shared__ double red[NV][BLOCK / MFS_WAVE];
-  __shared__ bool last;
-  block_sum<T, NV, BLOCK>(v, lds);
-  const unsigned G = gridDim.x;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) partials[(size_t)k * G + blockIdx.x] = (double)v[k];
-    __threadfence();
-    unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (prev == G - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  const int lane = threadIdx.x & (MFS_WAVE - 1), wave = threadIdx.x / MFS_WAVE;
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < G; b += BLOCK)
-      s += __hip_atomic_load(&partials[(size_t)k * G + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = wave_sum(s);
-    if (lane == 0) red[k][wave] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (int w = 0; w < BLOCK / MFS_WAVE; w++) s += red[k][w];
-    tot[k] = s;
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
-}
-
 // SELL-64 row product: thread `lane` of slice `slice` accumulates its block
 // row of A X into acc[BS], where X(i) is supplied by the functor xl (x[i],
 // or z[i] + beta p[i] for the fused CG kernel).  Values of entry j,

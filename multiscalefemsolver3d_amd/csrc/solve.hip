@@ -37,41 +37,14 @@ inline unsigned grid_for(int64_t n, int64_t cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
-// Sum of the G per-block partials part[k*G + b] (k < NV) in block order, done
-// redundantly by every block of the consuming kernel: the kernel boundary
-// makes the producer's partials visible, so no fences or atomics are needed,
-// and every block obtains bit-identical totals.
-template <int NV>
-__device__ __forceinline__ void sum_partials(const double* __restrict__ part, unsigned G, double (&tot)[NV]) {
-  __shared__ double red[NV][4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    // eight independent loads in flight per thread (a plain strided loop
-    // would wait for each load before issuing the next)
-    double s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (unsigned b0 = threadIdx.x; b0 < G; b0 += 8 * 256) {
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const unsigned b = b0 + u * 256;
-        if (b < G) s8[u] += part[(size_t)k * G + b];
-      }
-    }
-    double s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
-    s = wave_sum(s);
-    if (lane == 0) red[k][wave] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++) tot[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
-  __syncthreads();
-}
-
-// The same sum split in two so that a kernel can overlap the partials' load
+// Sum of the G per-block partials part[k*G + b] (k < NV) of the previous
+// kernel, done redundantly by every block of the consuming kernel: the kernel
+// boundary makes the producer's partials visible, so no fences or atomics are
+// needed.  Split in two so that a kernel can overlap the partials' load
 // latency with its own loads: partials_issue loads the first P partials of
 // each value per thread (G <= 256*P needs nothing more), partials_finish adds
 // any further ones and reduces over the block.  The order is the same in
-// every block, so every block still obtains bit-identical totals.
+// every block, so every block obtains bit-identical totals.
 template <int NV, int P>
 __device__ __forceinline__ void partials_issue(const double* __restrict__ part, unsigned G, double (&pre)[NV][P]) {
 #pragma unroll

@@ -821,6 +821,25 @@ __global__ __launch_bounds__(256) void k_bs_update(T* __restrict__ x, T* __restr
   double pre[2][4];
   partials_issue<2, 4>(part_b, g_b, pre);
   if (was_done) return;
+  // the rows that this XCD's blocks of k_bs_dir_spmv gather next; the loads
+  // of the first 4 elements per thread are issued before omega is known
+  int64_t e0, e1;
+  update_range(n, xcd, g1, bs, 64, e0, e1);
+  const int64_t i0 = e0 + threadIdx.x;
+  T xv[4], rv[4], wv[4], vv[4], sv[4], tv[4], r0v[4], dv[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int64_t i = i0 + u * 256;
+    const bool in = i < e1;
+    xv[u] = in ? x[i] : (T)0;
+    rv[u] = in ? rh[i] : (T)0;
+    wv[u] = in ? wh[i] : (T)0;
+    vv[u] = in ? vh[i] : (T)0;
+    sv[u] = in ? s[i] : (T)0;
+    tv[u] = in ? t[i] : (T)0;
+    r0v[u] = in ? r0[i] : (T)0;
+    dv[u] = dinv && in ? dinv[i] : (T)1;
+  }
   double ts[2];
   partials_finish<2, 4>(part_b, g_b, pre, ts);
   const double omega = ts[0] / ts[1];
@@ -830,10 +849,21 @@ __global__ __launch_bounds__(256) void k_bs_update(T* __restrict__ x, T* __restr
   }
   const T om = (T)omega;
   T red[2] = {0, 0};
-  // the rows that this XCD's blocks of k_bs_dir_spmv gather next
-  int64_t e0, e1;
-  update_range(n, xcd, g1, bs, 64, e0, e1);
-  for (int64_t i = e0 + threadIdx.x; i < e1; i += blockDim.x) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int64_t i = i0 + u * 256;
+    if (i >= e1) continue;
+    const T ph = rv[u] + beta * wv[u];
+    x[i] = xv[u] + (alpha * ph + om * (dv[u] * sv[u]));
+    const T ri = sv[u] - om * tv[u];
+    r[i] = ri;
+    rh[i] = dv[u] * ri;
+    wh[i] = ph - om * vv[u];
+    red[0] += r0v[u] * ri;
+    red[1] += ri * ri;
+  }
+  // further elements when a block's range exceeds 1,024
+  for (int64_t i = i0 + 1024; i < e1; i += blockDim.x) {
     const T di = dinv ? dinv[i] : (T)1;
     const T ph = rh[i] + beta * wh[i];
     const T si = s[i];

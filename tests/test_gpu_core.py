@@ -460,3 +460,49 @@ def test_bicgstab_reference_elastic_operator(ctx):
     rep = mfs.Solver(S.A, method=mfs.BICGSTAB, tol=1e-10, maxit=50000).run(S.b, x)
     assert rep.status == 0 and rep.true_relres <= 1e-10, rep
     assert rel(x.download(), xd) < 1e-7
+
+
+@pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
+def test_slice_order_paths_agree(ctx, monkeypatch, precision):
+    """The SELL kernels take one contiguous range of slices per XCD (and the
+    update kernels the matching rows) or keep the dispatch order
+    (MFS_XCD_MAP=0/1, read at matrix creation): PCG and BiCGStab reach the
+    tolerance in both orders with the same iterations (up to the rounding of
+    the differently grouped partial sums) and solutions."""
+    p = problems.laplace_inclusions(16)
+    rp, col, val, b = problems.convection_diffusion(48, 40.0, seed=8)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MFS_XCD_MAP", mode)
+        S = assemble(ctx, p)
+        x = mfs.Vec(ctx, S.A.n)
+        x.copy_from(S.x0)
+        rep = mfs.Solver(S.A, precision=precision, tol=1e-10).run(S.b, x)
+        assert rep.status == 0 and rep.true_relres <= 1e-10, (mode, rep)
+        A = mfs.Mat.csr(ctx, rp, col, val)
+        bv, xb = mfs.Vec(ctx, len(b), mfs.F64, b), mfs.Vec(ctx, len(b), mfs.F64, np.zeros(len(b)))
+        rb = mfs.Solver(A, method=mfs.BICGSTAB, precision=precision, tol=1e-10).run(bv, xb)
+        assert rb.status == 0 and rb.true_relres <= 1e-10, (mode, rb)
+        out[mode] = (rep.iterations, x.download(), rb.iterations, xb.download())
+    (i0, x0, j0, y0), (i1, x1, j1, y1) = out["0"], out["1"]
+    assert abs(i0 - i1) <= 0.02 * i0 + 2 and abs(j0 - j1) <= 0.05 * j0 + 3, (i0, i1, j0, j1)
+    assert rel(x1, x0) < 1e-8 and rel(y1, y0) < 1e-8
+
+
+@pytest.mark.parametrize("precision", [mfs.PREC_F64, mfs.PREC_MIXED])
+def test_gmres_graph_and_direct_launches_agree(ctx, monkeypatch, precision):
+    """GMRES replays each Arnoldi step from a graph captured per step index,
+    or launches the same kernels directly (MFS_NO_GRAPH=1): identical steps
+    and solutions."""
+    rp, col, val, b = problems.convection_diffusion(40, 60.0, seed=4)
+    n = len(b)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MFS_NO_GRAPH", mode)
+        A = mfs.Mat.csr(ctx, rp, col, val)
+        bv, x = mfs.Vec(ctx, n, mfs.F64, b), mfs.Vec(ctx, n, mfs.F64, np.zeros(n))
+        rep = mfs.Solver(A, method=mfs.GMRES, precision=precision, tol=1e-9).run(bv, x)
+        assert rep.status == 0 and rep.true_relres <= 1e-9, (mode, rep)
+        out[mode] = (rep.iterations, x.download())
+    assert out["0"][0] == out["1"][0]
+    assert rel(out["0"][1], out["1"][1]) < 1e-12

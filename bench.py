@@ -27,9 +27,9 @@ METRIC = "time-to-1e-8 solve & SpMV HBM GB/s: mixed+batched+learned vs FP64 GPU 
 HBM_PEAK_GBS = 8000.0
 # Bytes per launch of k_cg_dir_spmv<float,1> (16-bit column offsets) on this
 # workload between the L2 caches and the memory fabric, from rocprofv3 --pmc
-# FETCH_SIZE (13,307.2 KiB, x2.000 from a 512 MiB copy) and WRITE_SIZE
+# FETCH_SIZE (13,300.2 KiB, x2.000 from a 512 MiB copy) and WRITE_SIZE
 # (2,179.2 KiB, x1.000), medians over 7,752 / 7,731 full dispatches
-TRAFFIC_K1 = 29484646
+TRAFFIC_K1 = 29470310
 
 
 def log(*a):

@@ -506,3 +506,21 @@ def test_gmres_graph_and_direct_launches_agree(ctx, monkeypatch, precision):
         out[mode] = (rep.iterations, x.download())
     assert out["0"][0] == out["1"][0]
     assert rel(out["0"][1], out["1"][1]) < 1e-12
+
+
+@pytest.mark.parametrize("maxit", [1, 7, 13])
+def test_gmres_and_bicgstab_stop_at_maxit(ctx, maxit):
+    """An iteration cap that falls inside GMRES's 6-step check interval or
+    BiCGStab's 8-iteration graph chunk is honoured exactly: the report gives
+    the capped count and NOT_CONVERGED, and the residual is finite (GMRES's
+    cannot grow)."""
+    rp, col, val, b = problems.convection_diffusion(64, 100.0, seed=6)
+    n = len(b)
+    A = mfs.Mat.csr(ctx, rp, col, val)
+    bv = mfs.Vec(ctx, n, mfs.F64, b)
+    for method in (mfs.GMRES, mfs.BICGSTAB):
+        x = mfs.Vec(ctx, n, mfs.F64, np.zeros(n))
+        rep = mfs.Solver(A, method=method, precision=mfs.PREC_F64, tol=1e-12, maxit=maxit).run(bv, x, check=False)
+        assert rep.iterations == maxit, (method, maxit, rep)
+        assert rep.status == -3, (method, rep)  # MFS_ERR_NOT_CONVERGED
+        assert 0 < rep.true_relres < (1.0 if method == mfs.GMRES else 10.0), (method, rep)

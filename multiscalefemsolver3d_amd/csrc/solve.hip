@@ -589,12 +589,53 @@ __global__ void k_combine(const T* __restrict__ V, const double* __restrict__ y,
 // ---------------------------------------------------------------------------
 // v_new = q - (delta/g) v - (g/g_prev) v_old; z_new = dinv v_new;
 // per-block partial of z_new . v_new.   delta from device memory.
+// MINRES recurrence scalars on the device (doubles): the Lanczos norms, the
+// Givens rotation of the last two steps, eta, the update coefficients of
+// the current step, and the stop record (-1 while running, else the index of
+// the last step whose update applies; kMsBad = 1 when that step's rotation
+// failed and its update must not apply either).
+enum {
+  kMsG, kMsGPrev, kMsC, kMsCPrev, kMsSPrev, kMsS, kMsEta, kMsEta0, kMsA1, kMsA2, kMsA3, kMsCEta, kMsInvG, kMsRtol,
+  kMsStop, kMsBad, kMsCount
+};
+
+// one thread: rotation of step `step` from delta (dh[1]) and ||v_{j+1}||_M^2 (dh[2])
+__global__ void k_minres_rot(const double* __restrict__ dh, double* __restrict__ ms, int step) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || ms[kMsStop] >= 0.0) return;
+  const double delta = dh[1], g_next = sqrt(fmax(dh[2], 0.0));
+  const double g = ms[kMsG], c = ms[kMsC], c_prev = ms[kMsCPrev], s_prev = ms[kMsSPrev], s_cur = ms[kMsS];
+  const double a0 = c * delta - c_prev * s_cur * g;
+  const double a1 = sqrt(a0 * a0 + g_next * g_next);
+  if (!isfinite(delta) || !isfinite(g_next) || !(a1 > 0)) {
+    ms[kMsBad] = 1.0;
+    ms[kMsStop] = step;
+    return;
+  }
+  const double c_next = a0 / a1, s_next = g_next / a1, eta = ms[kMsEta];
+  ms[kMsA1] = a1;
+  ms[kMsA2] = s_cur * delta + c_prev * c * g;
+  ms[kMsA3] = s_prev * g;
+  ms[kMsCEta] = c_next * eta;
+  ms[kMsInvG] = g_next > 0 ? 1.0 / g_next : 0.0;
+  const double eta_next = -s_next * eta;
+  ms[kMsEta] = eta_next;
+  ms[kMsCPrev] = c;
+  ms[kMsC] = c_next;
+  ms[kMsSPrev] = s_cur;
+  ms[kMsS] = s_next;
+  ms[kMsGPrev] = g;
+  ms[kMsG] = g_next;
+  if (fabs(eta_next) <= ms[kMsRtol] * ms[kMsEta0] || g_next == 0) ms[kMsStop] = step;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_minres_lanczos(const T* __restrict__ q, const T* __restrict__ v,
                                                         const T* __restrict__ v_old, T* __restrict__ v_new,
                                                         T* __restrict__ z_new, const T* __restrict__ dinv,
-                                                        const double* __restrict__ delta, double g, double g_prev,
-                                                        int64_t n, double* __restrict__ partials) {
+                                                        const double* __restrict__ delta,
+                                                        const double* __restrict__ ms, int64_t n,
+                                                        double* __restrict__ partials) {
+  const double g = ms[kMsG], g_prev = ms[kMsGPrev];
   const T c1 = (T)(delta[0] / g), c2 = g_prev > 0 ? (T)(g / g_prev) : (T)0;
   T red[1] = {0};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -610,9 +651,13 @@ __global__ __launch_bounds__(256) void k_minres_lanczos(const T* __restrict__ q,
 template <typename T>
 __global__ __launch_bounds__(256) void k_minres_update(const T* __restrict__ z, const T* __restrict__ w,
                                                        const T* __restrict__ w_old, T* __restrict__ w_new,
-                                                       T* __restrict__ x, T* __restrict__ z_next, double a1, double a2,
-                                                       double a3, double ceta, double inv_g_next, int64_t n) {
-  const T ia1 = (T)(1.0 / a1), ta2 = (T)a2, ta3 = (T)a3, tc = (T)ceta, tg = (T)inv_g_next;
+                                                       T* __restrict__ x, T* __restrict__ z_next,
+                                                       const double* __restrict__ ms, int step, int64_t n) {
+  // steps after the stopping one, and a step whose rotation failed, change nothing
+  const double stop = ms[kMsStop];
+  if (stop >= 0.0 && ((double)step > stop || ms[kMsBad] != 0.0)) return;
+  const T ia1 = (T)(1.0 / ms[kMsA1]), ta2 = (T)ms[kMsA2], ta3 = (T)ms[kMsA3], tc = (T)ms[kMsCEta],
+          tg = (T)ms[kMsInvG];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const T wn = (z[i] - ta3 * w_old[i] - ta2 * w[i]) * ia1;
     w_new[i] = wn;
@@ -1419,55 +1464,56 @@ static int minres_run(mfs_solver S, const T* dinv, double rtol, int maxsteps, in
   k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh);
   MFS_HIP(hipMemcpyAsync(hp, S->dh, 8, hipMemcpyDeviceToHost, s));
   MFS_HIP(hipStreamSynchronize(s));
-  double g = std::sqrt(std::max(hp[0], 0.0)), g_prev = 0.0;
+  const double g = std::sqrt(std::max(hp[0], 0.0));
   if (!(g > 0) || !std::isfinite(g)) {
     *est_rel = std::isfinite(g) ? 0.0 : 1.0;
     return std::isfinite(g) ? MFS_OK : MFS_ERR_NONFINITE;
   }
-  const double eta0 = g;
-  double eta = g, c_prev = 1, c = 1, s_prev = 0, s_cur = 0;
+  // recurrence state on the device; the host checks it every kSync steps
+  // (steps launched past the stopping one leave x unchanged)
+  constexpr int kSync = 8;
+  double* ms = S->dh + 8;
+  {
+    double init[kMsCount] = {};
+    init[kMsG] = g;
+    init[kMsC] = init[kMsCPrev] = 1.0;
+    init[kMsEta] = init[kMsEta0] = g;
+    init[kMsRtol] = rtol;
+    init[kMsStop] = -1.0;
+    for (int q = 0; q < kMsCount; q++) hp[q] = init[q];
+    MFS_HIP(hipMemcpyAsync(ms, hp, sizeof(double) * kMsCount, hipMemcpyHostToDevice, s));
+  }
   k_scale_vec<T><<<gv, 256, 0, s>>>(zj, 1.0 / g, n);
   int j = 0;
-  for (; j < maxsteps; j++) {
-    MFS_TRY(spmv_launch(A, prec, zj, Az, s));
-    // delta = <A z_j, z_j>
-    k_multidot<T><<<dim3(gv, 1), 256, 0, s>>>(zj, Az, n, S->redg.partials);
-    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 1);
-    k_minres_lanczos<T><<<gv, 256, 0, s>>>(Az, v[1], v[0], v[2], znext, dinv, S->dh + 1, g, g_prev, n,
-                                           S->redg.partials);
-    k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 2);
-    MFS_HIP(hipMemcpyAsync(hp, S->dh + 1, 16, hipMemcpyDeviceToHost, s));
-    MFS_HIP(hipStreamSynchronize(s));
-    const double delta = hp[0];
-    const double g_next = std::sqrt(std::max(hp[1], 0.0));
-    if (!std::isfinite(delta) || !std::isfinite(g_next)) return MFS_ERR_NONFINITE;
-    const double a0 = c * delta - c_prev * s_cur * g;
-    const double a1 = std::sqrt(a0 * a0 + g_next * g_next);
-    const double a2 = s_cur * delta + c_prev * c * g;
-    const double a3 = s_prev * g;
-    if (!(a1 > 0)) return MFS_ERR_NONFINITE;
-    const double c_next = a0 / a1, s_next = g_next / a1;
-    k_minres_update<T><<<gv, 256, 0, s>>>(zj, w[1], w[0], w[2], x, znext, a1, a2, a3, c_next * eta,
-                                          g_next > 0 ? 1.0 / g_next : 0.0, n);
-    MFS_HIP(hipGetLastError());
-    eta = -s_next * eta;
-    // shift the recurrences and rotate buffers
-    c_prev = c;
-    c = c_next;
-    s_prev = s_cur;
-    s_cur = s_next;
-    g_prev = g;
-    g = g_next;
-    std::swap(v[0], v[1]);
-    std::swap(v[1], v[2]);  // v_{j-1} <- v_j, v_j <- v_{j+1}
-    std::swap(w[0], w[1]);
-    std::swap(w[1], w[2]);
-    std::swap(zj, znext);
-    if (std::fabs(eta) <= rtol * eta0 || g_next == 0) {
-      j++;
-      break;
+  double stop = -1.0, bad = 0.0;
+  while (j < maxsteps && stop < 0.0) {
+    const int nb = std::min(kSync, maxsteps - j);
+    for (int t = 0; t < nb; t++, j++) {
+      MFS_TRY(spmv_launch(A, prec, zj, Az, s));
+      // delta = <A z_j, z_j>
+      k_multidot<T><<<dim3(gv, 1), 256, 0, s>>>(zj, Az, n, S->redg.partials);
+      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 1);
+      k_minres_lanczos<T><<<gv, 256, 0, s>>>(Az, v[1], v[0], v[2], znext, dinv, S->dh + 1, ms, n,
+                                             S->redg.partials);
+      k_finish_sums<<<1, 256, 0, s>>>(S->redg.partials, gv, S->dh + 2);
+      k_minres_rot<<<1, 64, 0, s>>>(S->dh, ms, j);
+      k_minres_update<T><<<gv, 256, 0, s>>>(zj, w[1], w[0], w[2], x, znext, ms, j, n);
+      // rotate the buffers: v_{j-1} <- v_j, v_j <- v_{j+1} (the same for w), z
+      std::swap(v[0], v[1]);
+      std::swap(v[1], v[2]);
+      std::swap(w[0], w[1]);
+      std::swap(w[1], w[2]);
+      std::swap(zj, znext);
     }
+    MFS_HIP(hipGetLastError());
+    MFS_HIP(hipMemcpyAsync(hp, ms, sizeof(double) * kMsCount, hipMemcpyDeviceToHost, s));
+    MFS_HIP(hipStreamSynchronize(s));
+    stop = hp[kMsStop];
+    bad = hp[kMsBad];
   }
+  if (bad != 0.0) return MFS_ERR_NONFINITE;
+  if (stop >= 0.0) j = (int)stop + 1;
+  const double eta = hp[kMsEta], eta0 = hp[kMsEta0];
   // the correction must end in mr[1]: x never moves
   *steps += j;
   *est_rel = std::fabs(eta) / eta0;

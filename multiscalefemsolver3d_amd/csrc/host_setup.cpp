@@ -5,7 +5,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
+#include <exception>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -490,7 +493,9 @@ void ints_of(const std::string& line, std::vector<long long>& out) {
 }
 
 int parse_gmsh(const char* path, GmshMesh& M) {
-  FILE* f = std::fopen(path, "r");
+  // closed on every exit, exceptions included
+  std::unique_ptr<FILE, int (*)(FILE*)> file(std::fopen(path, "r"), &std::fclose);
+  FILE* f = file.get();
   if (!f) {
     set_error(std::string("mfs_read_gmsh: cannot open ") + path);
     return MFS_ERR_ARG;
@@ -526,9 +531,15 @@ int parse_gmsh(const char* path, GmshMesh& M) {
     } else if (section == 2) {
       if (expect < 0) {
         ints_of(line, iv);
-        if (iv.empty()) ok = false;
-        expect = iv.empty() ? 0 : iv[0];
-        M.xyz.reserve(3 * (size_t)expect);
+        // the count only sizes a reservation: a negative or implausible one
+        // (more nodes than a 2^31-entry index space holds) marks the file
+        // malformed instead of reaching the allocator
+        if (iv.empty() || iv[0] < 0 || iv[0] > (long long)INT32_MAX) {
+          ok = false;
+          break;
+        }
+        expect = iv[0];
+        M.xyz.reserve(3 * (size_t)std::min<int64_t>(expect, 1 << 24));
         continue;
       }
       long long id;
@@ -557,7 +568,6 @@ int parse_gmsh(const char* path, GmshMesh& M) {
       }
     }
   }
-  std::fclose(f);
   const int64_t nn = (int64_t)M.xyz.size() / 3;
   for (int32_t v : M.tets) ok = ok && v >= 0 && v < nn;
   for (int32_t v : M.tris) ok = ok && v >= 0 && v < nn;
@@ -597,7 +607,13 @@ extern "C" int mfs_read_gmsh(const char* path, int64_t* n_nodes, double* xyz, in
     return MFS_ERR_ARG;
   }
   GmshMesh M;
-  const int st = parse_gmsh(path, M);
+  int st;
+  try {
+    st = parse_gmsh(path, M);
+  } catch (const std::exception& e) {  // allocation failures of a huge file
+    set_error(std::string("mfs_read_gmsh: ") + e.what());
+    return MFS_ERR_OOM;
+  }
   if (st != MFS_OK) return st;
   const int64_t nn = (int64_t)M.xyz.size() / 3, nt = (int64_t)M.mat.size(), nf = (int64_t)M.tags.size();
   const bool fill = xyz || tets || material || tris || tri_tags || tri_parent;

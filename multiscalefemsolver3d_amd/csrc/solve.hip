@@ -584,11 +584,11 @@ __global__ void k_combine(const T* __restrict__ V, const double* __restrict__ y,
 // ---------------------------------------------------------------------------
 // Preconditioned MINRES for symmetric (possibly indefinite) A with SPD M =
 // diag(A) (Elman, Silvester & Wathen, "Finite Elements and Fast Iterative
-// Solvers", Algorithm 2.4).  Scalar recurrences on the host, one 16-byte
-// read-back per iteration.
+// Solvers", Algorithm 2.4).  The scalar recurrence (Lanczos norms, Givens
+// rotations, eta) stays on the device: a one-thread kernel per step updates
+// it, and the host reads the stop record every 8 steps; updates of steps
+// launched past the stopping one leave x unchanged.
 // ---------------------------------------------------------------------------
-// v_new = q - (delta/g) v - (g/g_prev) v_old; z_new = dinv v_new;
-// per-block partial of z_new . v_new.   delta from device memory.
 // MINRES recurrence scalars on the device (doubles): the Lanczos norms, the
 // Givens rotation of the last two steps, eta, the update coefficients of
 // the current step, and the stop record (-1 while running, else the index of
@@ -628,6 +628,8 @@ __global__ void k_minres_rot(const double* __restrict__ dh, double* __restrict__
   if (fabs(eta_next) <= ms[kMsRtol] * ms[kMsEta0] || g_next == 0) ms[kMsStop] = step;
 }
 
+// v_new = q - (delta/g) v - (g/g_prev) v_old; z_new = dinv v_new;
+// per-block partial of z_new . v_new.   delta from device memory.
 template <typename T>
 __global__ __launch_bounds__(256) void k_minres_lanczos(const T* __restrict__ q, const T* __restrict__ v,
                                                         const T* __restrict__ v_old, T* __restrict__ v_new,

@@ -524,3 +524,64 @@ def test_gmres_and_bicgstab_stop_at_maxit(ctx, maxit):
         assert rep.iterations == maxit, (method, maxit, rep)
         assert rep.status == -3, (method, rep)  # MFS_ERR_NOT_CONVERGED
         assert 0 < rep.true_relres < (1.0 if method == mfs.GMRES else 10.0), (method, rep)
+
+
+def minres_esw(A, b, x0, dinv, steps):
+    """Jacobi-preconditioned MINRES, Elman, Silvester & Wathen, "Finite
+    Elements and Fast Iterative Solvers", Algorithm 2.4, in numpy (FP64):
+    the iterate after `steps` Lanczos steps from x0."""
+    x = x0.copy()
+    v_old = np.zeros_like(b)
+    v = b - A @ x
+    z = dinv * v
+    g_old, g = 1.0, np.sqrt(z @ v)
+    eta = g
+    s_old = s = 0.0
+    c_old = c = 1.0
+    w_old = np.zeros_like(b)
+    w = np.zeros_like(b)
+    for _ in range(steps):
+        z = z / g
+        q = A @ z
+        delta = q @ z
+        v_new = q - (delta / g) * v - (g / g_old) * v_old
+        z_new = dinv * v_new
+        g_new = np.sqrt(z_new @ v_new)
+        a0 = c * delta - c_old * s * g
+        a1 = np.hypot(a0, g_new)
+        a2 = s * delta + c_old * c * g
+        a3 = s_old * g
+        c_new, s_new = a0 / a1, g_new / a1
+        w_new = (z - a3 * w_old - a2 * w) / a1
+        x = x + c_new * eta * w_new
+        eta = -s_new * eta
+        v_old, v, z = v, v_new, z_new
+        g_old, g = g, g_new
+        w_old, w = w, w_new
+        c_old, c, s_old, s = c, c_new, s, s_new
+    return x
+
+
+@pytest.mark.parametrize("maxit", [1, 7, 13])
+def test_minres_stops_at_maxit_with_the_textbook_iterate(ctx, maxit):
+    """An iteration cap inside MINRES's 8-step check interval is honoured
+    exactly (capped count, NOT_CONVERGED, finite residual), and the FP64
+    iterate after `maxit` steps is the one Algorithm 2.4 of Elman, Silvester
+    & Wathen gives on the same indefinite system (reference-convention
+    elastic operator, two-phase E, free sides): pins the device recurrence
+    step by step."""
+    import scipy.sparse as sp
+    p = problems.elastic_cube(4, field="grf")
+    S = assemble(ctx, p)
+    O = oracle_elastic(p)
+    A = sp.csr_matrix((O.val, O.col, O.rowptr), shape=(O.n, O.n))
+    x = mfs.Vec(ctx, S.A.n)
+    x.copy_from(S.x0)
+    rep = mfs.Solver(S.A, method=mfs.MINRES, precision=mfs.PREC_F64, tol=1e-14, maxit=maxit).run(S.b, x, check=False)
+    assert rep.iterations == maxit, rep
+    assert rep.status == -3, rep  # MFS_ERR_NOT_CONVERGED
+    assert np.isfinite(rep.true_relres) and rep.true_relres < 1.0, rep
+    xe = minres_esw(A, O.rhs, O.x0, 1.0 / A.diagonal(), maxit)
+    assert rel(x.download(), xe) < 1e-9, rel(x.download(), xe)
+    r = O.rhs - A @ xe
+    assert abs(np.linalg.norm(r) / np.linalg.norm(O.rhs) - rep.true_relres) < 1e-9

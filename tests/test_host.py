@@ -175,6 +175,12 @@ def test_gmsh_reader_rejects_bad_input(tmp_path):
         mfs.read_gmsh(str(bad))
     with pytest.raises(Exception):
         mfs.read_gmsh(str(tmp_path / "missing.msh"))
+    # negative and absurd $Nodes counts are malformed files, not allocations
+    for count in ("-5", "99999999999999", "x"):
+        huge = tmp_path / f"count{count}.msh"
+        huge.write_text(f"$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n{count}\n1 0 0 0\n$EndNodes\n")
+        with pytest.raises(mfs.MfsError, match="malformed"):
+            mfs.read_gmsh(str(huge))
 
 
 def parse_params_like_the_reference(text):

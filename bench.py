@@ -1,16 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X FEM solve path.
 
-Workload (default): electrostatics on a 64^3 structured tet mesh (1,572,864
-tets, 274,625 nodes), permittivity 1 with 200 seeded spherical inclusions of
-radius U(0.02,0.06) and permittivity 100, potential 0 at z=0 and 1 at z=1.
-A "step" is one complete solve to a true FP64 relative residual of 1e-8 with
-the mixed-precision Jacobi-PCG (FP32 inner iterations inside FP64 iterative
-refinement), starting from the Dirichlet-lifted zero guess, with the matrix
-and right-hand side resident in HBM.  The plain FP64 GPU Jacobi-PCG on the
-same system is timed beside it (the comparison point), and the reference's CPU solver
-path (CSSD rows, OpenMP SpMV, serial SSOR sweeps and dot products, restated
-in the oracle) is timed on the host cores.
+Headline workload: the learned-guess MsFEM elasticity solve on a 32^3 coarse
+grid (32,768 coarse cells, each a fine 8^3 x 6-tet sub-grid with lognormal
+Young's modulus, 24 local basis problems per cell = 786,432 local problems).
+A "step" is the per-batch hot path with the cell matrices resident in HBM:
+features of the moduli, inference of the 4-layer width-256 MLP (BF16 MFMA,
+FP32 accumulation) for all cells, guess assembly, and the mixed-precision
+batched persistent PCG (FP32 inner iterations, FP64 refinement) from that
+guess until every problem reaches a true FP64 relative residual of 1e-8.
+The network is trained once beforehand on 20,000 cells the batched solver
+solves itself (reported, not part of a step).  The comparison point is the
+plain FP64 batched PCG from the zero guess on the same cells.  The CPU
+baseline is the oracle's FP64 restatement of the reference's per-cell path
+(CSSD assembly, 24 PCG solves per cell), sampled and extrapolated.
+
+Sub-objects: 64^3 Laplace with inclusions (mixed vs FP64 Jacobi-PCG, the
+fused PCG kernel's roofline), six right-hand sides, non-symmetric BiCGStab,
+MsFEM on 16^3 cells with n ~ U{6..10} (Q1 and reference traces) and the
+128^3 elastic cube (SpMV at HBM scale, assembly, solvers).
 
 Prints one JSON line.
 """
@@ -25,15 +33,33 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-1e-8 solve & SpMV HBM GB/s: mixed+batched+learned vs FP64 GPU CG"
 HBM_PEAK_GBS = 8000.0
-# Bytes per launch of k_cg_dir_spmv<float,1> (16-bit column offsets) on this
-# workload between the L2 caches and the memory fabric, from rocprofv3 --pmc
-# FETCH_SIZE (13,300.2 KiB, x2.000 from a 512 MiB copy) and WRITE_SIZE
-# (2,179.2 KiB, x1.000), medians over 7,752 / 7,731 full dispatches
+# Bytes per launch of k_cg_dir_spmv<float,1> (16-bit column offsets) on the
+# 64^3 Laplace system between the L2 caches and the memory fabric, from
+# rocprofv3 --pmc FETCH_SIZE (13,300.2 KiB, x2.000 from a 512 MiB copy) and
+# WRITE_SIZE (2,179.2 KiB, x1.000), medians over 7,752 / 7,731 dispatches
 TRAFFIC_K1 = 29470310
+# HBM bytes of one k_batch_solve<float> launch on the headline workload
+# (FETCH_SIZE x2.000 + WRITE_SIZE x1.000 from rocprofv3 --pmc, the copy
+# calibration above); None until profiled
+TRAFFIC_BATCH = None
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def batch_bytes(iters, n, sv=4):
+    """Algorithmic bytes of one batched-solve launch on cells of n^3 hexes:
+    per cell-iteration one stencil SpMV (15 x 9 matrix values per interior
+    node, operand and result vectors) with the matrix read once for all 24
+    right-hand sides, plus the 11 vector passes of a PCG iteration per
+    right-hand side; a cell iterates as long as its slowest problem."""
+    import numpy as np
+    nint = (n - 1) ** 3
+    N = 3 * nint
+    it = np.asarray(iters, dtype=np.int64).reshape(-1, 24)
+    spmv = 135 * nint * sv + 2 * N * sv
+    return int(it.max(axis=1).sum() * spmv + it.sum() * 11 * N * sv)
 
 
 def main():
@@ -41,13 +67,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=64, help="mesh hexes per edge")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline work")
-    ap.add_argument("--no-msfem", action="store_true", help="skip the MsFEM batch measurement")
+    ap.add_argument("--cells", type=int, default=32768, help="headline: coarse cells (32^3)")
+    ap.add_argument("--train-cells", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--n", type=int, default=64, help="Laplace leg: mesh hexes per edge")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work per leg")
+    ap.add_argument("--no-laplace", action="store_true", help="skip the 64^3 Laplace measurement")
+    ap.add_argument("--no-msfem", action="store_true", help="skip the 16^3-cell MsFEM measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the 128^3 elastic measurement")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess MsFEM measurement")
-    ap.add_argument("--learned-test-cells", type=int, default=32768)
+    ap.add_argument("--no-bicgstab", action="store_true", help="skip the non-symmetric BiCGStab measurement")
+    ap.add_argument("--headline-only", action="store_true", help="only the headline workload")
     ap.add_argument("--large-n", type=int, default=128)
     ap.add_argument("--large-spmv-only", action="store_true", help="128^3: assembly and SpMV only, no solves")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -57,25 +87,214 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
+    if args.headline_only:
+        args.no_laplace = args.no_msfem = args.no_large = args.no_bicgstab = True
 
     import multiscalefemsolver3d_amd as mfs
     from multiscalefemsolver3d_amd import problems
 
+    ctx = mfs.Context(0)
+    head = bench_headline(mfs, ctx, args)
+    lap = None if args.no_laplace else bench_laplace(mfs, problems, ctx, args)
+    nonsym = None if args.no_bicgstab else bench_bicgstab(mfs, problems, ctx, cpu=not args.no_cpu)
+    msfem = None if args.no_msfem else bench_msfem(mfs, problems, ctx, cpu=not args.no_cpu)
+    large = None if args.no_large else bench_large(mfs, problems, ctx, args.large_n, not args.large_spmv_only)
+    if args.copy_calibration:
+        # one plain copy of 512 MiB: FETCH_SIZE and WRITE_SIZE of this kernel
+        # should each read 524,288 KiB; their ratio to that scales the counters
+        big = 64 * 1024 * 1024
+        src = mfs.Vec(ctx, big, mfs.F64)
+        dst = mfs.Vec(ctx, big, mfs.F64)
+        src.fill(1.0)
+        ctx.sync()
+        dst.copy_from(src)
+        ctx.sync()
+        log(f"[bench] copy calibration: {big * 8} bytes copied")
+        del src, dst
+
+    out = {
+        "metric": METRIC,
+        "value": head["ms_per_step"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer (bf16 mfma inference)",
+        "data": "synthetic (seeded lognormal moduli; the network is trained on cells the solver generates)",
+        "config": head["config"],
+        "fp64_gpu_cg_ms": head["fp64_batched_zero_ms"],
+        "speedup_vs_fp64_gpu_cg": round(head["fp64_batched_zero_ms"] / head["ms_per_step"], 3),
+        "headline": head["detail"],
+        "roofline": head["roofline"],
+        "cpu_baseline": head["cpu_baseline"],
+        "laplace64_inclusions": lap,
+        "bicgstab_nonsymmetric": nonsym,
+        "msfem_16cubed_cells": msfem,
+        "elastic_large": large,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_headline(mfs, ctx, args):
+    """Learned-guess MsFEM solve of 32^3 cells (n = 8): timed steps of
+    inference + guess + mixed batched solve; FP64 zero-guess batched PCG and
+    mixed zero-guess beside it; roofline of the persistent batch kernel."""
+    import numpy as np
+    from multiscalefemsolver3d_amd import learned
+    n, ncells = 8, args.cells
+    t0 = time.time()
+    model = learned.train(ctx, n=n, n_train=args.train_cells, hidden=256, layers=4, epochs=args.epochs, batch=256,
+                          lr=1e-3, tol=1e-8, log=log, tune=True)
+    train_wall = time.time() - t0
+    cn, org, h, E = learned.cells(ncells, n, 5)
+    B = mfs.Batch(ctx, cn, org, h, E)
+    t0 = time.perf_counter()
+    B.assemble()
+    ctx.sync()
+    asm_s = time.perf_counter() - t0
+    L = learned.LearnedSolver(ctx, model, B, ncells, tol=1e-8)
+    for _ in range(args.warmup):
+        L.solve()
+    ctx.sync()
+    t_start = time.perf_counter()
+    reps = [L.solve() for _ in range(args.steps)]
+    ctx.sync()
+    elapsed = time.perf_counter() - t_start
+    ms = 1000.0 * elapsed / args.steps
+    it_l, rr_l, fl_l = B.status()
+    ok = all(r.n_not_converged == 0 and r.max_true_relres <= 1e-8 for r in reps)
+    # inference alone (features + MLP + guess assembly)
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        L.infer()
+    ctx.sync()
+    infer_ms = 1000.0 * (time.perf_counter() - t1) / 3
+    kern_ms = 1000.0 * float(np.mean([r.kernel_seconds for r in reps]))
+    solve_ms = 1000.0 * float(np.mean([r.seconds for r in reps]))
+    nbytes = batch_bytes(it_l, n)
+    gbs = nbytes / (kern_ms * 1e-3) / 1e9
+    log(f"[bench] headline: {ms:.1f} ms per step (solve {solve_ms:.1f} ms device, persistent kernel "
+        f"{kern_ms:.1f} ms = {gbs:.0f} GB/s algorithmic, inference {infer_ms:.1f} ms), mean iterations "
+        f"{it_l.mean():.1f}, all converged {ok}")
+    # comparison points on the same cells: plain FP64 batched PCG and mixed
+    # batched PCG, both from the zero guess (wall clock like the steps)
+    def timed(**kw):
+        best = None
+        for _ in range(3):
+            ctx.sync()
+            t = time.perf_counter()
+            r = B.solve(tol=1e-8, **kw)
+            ctx.sync()
+            w = time.perf_counter() - t
+            if best is None or w < best[0]:
+                best = (w, r)
+        return best
+    w64, r64 = timed(precision=mfs.PREC_F64, **model.shape[mfs.PREC_F64])
+    it64, _, _ = B.status()
+    w0, r0 = timed(precision=mfs.PREC_MIXED, **model.shape[mfs.PREC_MIXED])
+    it0, _, _ = B.status()
+    ms64 = 1000.0 * w64
+    log(f"[bench] FP64 batched zero guess {ms64:.1f} ms (mean it {it64.mean():.1f}); mixed zero guess "
+        f"{1000 * w0:.1f} ms (mean it {it0.mean():.1f}); learned {ms:.1f} ms -> {ms64 / ms:.2f}x")
+    detail = {
+        "problems": 24 * ncells, "cell_n": n, "interior_dofs_per_cell": 3 * (n - 1) ** 3,
+        "all_converged_to_1e-8": ok, "max_true_relres": max(r.max_true_relres for r in reps),
+        "mean_iterations": {"learned_mixed": round(float(it_l.mean()), 2), "zero_mixed": round(float(it0.mean()), 2),
+                            "zero_fp64": round(float(it64.mean()), 2)},
+        "max_iterations": {"learned_mixed": int(it_l.max()), "zero_mixed": int(it0.max()),
+                           "zero_fp64": int(it64.max())},
+        "iterations_saved_learned_vs_zero_mixed": round(1 - float(it_l.mean()) / float(it0.mean()), 4),
+        "step_breakdown_ms": {"inference_features_guess": round(infer_ms, 2), "batched_solve_device": round(solve_ms, 2),
+                              "persistent_kernel": round(kern_ms, 2)},
+        "fp32_to_fp64_fallbacks": reps[-1].n_precision_fallback, "rejected_guesses": reps[-1].n_fallback,
+        "mixed_zero_guess_ms": round(1000 * w0, 2),
+        "speedup_vs_mixed_zero_guess": round(1000 * w0 / ms, 3),
+        "training": dict(model.info, wall_s=round(train_wall, 1)),
+        "cell_assembly_s": round(asm_s, 3),
+        "timing": "wall clock of K steps with the context synchronised on both sides",
+    }
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BATCH,
+            "kernel": "k_batch_solve<float,R> (persistent batched PCG, FP32 inner / FP64 refinement)",
+            "bytes_per_launch": nbytes, "avg_ms": round(kern_ms, 3),
+            "note": "algorithmic bytes: per cell-iteration the 15-point stencil matrix once for all 24 "
+                    "right-hand sides plus 11 vector passes per right-hand side and iteration"}
+    # the GEMM of the network alone at a large size (MFMA utilisation)
+    gms, gtf = mfs.gemm_bf16_time(ctx, 8192, 8192, 8192, reps=5)
+    detail["gemm_bf16_kernel_8192cubed"] = {"ms": round(gms, 3), "tflops": round(gtf, 1),
+                                            "frac_dense_bf16_peak": round(gtf / 2500.0, 4)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = msfem_cpu_baseline_uniform(learned, n, ncells, args.cpu_budget)
+    L.close()
+    B.close()
+    config = {"workload": f"msfem_learned_{round(ncells ** (1 / 3))}^3_cells_n{n}", "coarse_cells": ncells,
+              "fine_hexes_per_cell": n ** 3, "local_problems": 24 * ncells, "tol": 1e-8,
+              "network": "MLP 3072-256-256-256-24696, BF16 MFMA / FP32 accumulation",
+              "training_pairs": args.train_cells,
+              "solver": "batched persistent Jacobi-PCG, FP32 inner + FP64 refinement, learned initial guess",
+              "parallelism": "single GPU"}
+    return {"ms_per_step": round(ms, 3), "fp64_batched_zero_ms": round(ms64, 3), "detail": detail,
+            "roofline": roof, "cpu_baseline": cpu, "config": config}
+
+
+def msfem_cpu_baseline_uniform(learned, n, ncells, budget):
+    """The oracle's FP64 restatement of the reference's per-cell path on
+    test cells of the headline batch (CSSD assembly; per basis the full
+    cell system with eliminated boundary rows in component-major order,
+    solved by the SSOR-PCG continuation with BCG_Stab2 fallback of
+    FEM.h:3699-3770 down to 1e-8; the Galerkin product), one cell per host
+    thread as the reference's OpenMP loop over cells runs them; as many
+    cells as fit the time budget, extrapolated to the whole batch."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle as orc
+    threads = min(os.cpu_count() or 1, 16)
+    cn, org, h, E = learned.cells(threads, n, 5)
+    nt = 6 * n ** 3
+
+    def one(c):
+        t0 = time.perf_counter()
+        orc.msfem_cell_refpath(n, tuple(org[c]), h, E[c * nt:(c + 1) * nt], 0.3, min_residual=1e-8)
+        return time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    rounds = 0
+    per = []
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        while rounds == 0 or (time.perf_counter() - t0 < budget and rounds < 8):
+            per += list(ex.map(one, range(threads)))
+            rounds += 1
+    wall = time.perf_counter() - t0
+    cells_done = rounds * threads
+    est_ms = wall / cells_done * ncells * 1e3
+    return {"value": round(est_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"{cells_done} cells (n={n}; 24 local problems each on the reference's path: SSOR-PCG "
+                      f"continuation with BCG_Stab2 fallback to 1e-8, oracle restatement) on {threads} threads in "
+                      f"{wall:.1f}s, mean {np.mean(per):.3f}s per cell under that load; extrapolated to {ncells} "
+                      f"cells"}
+
+
+def bench_laplace(mfs, problems, ctx, args):
+    """64^3 Laplace with 200 inclusions: mixed vs FP64 Jacobi-PCG
+    to 1e-8, the fused PCG kernel's roofline, plain SpMV, six right-hand
+    sides, the reference CPU solver path."""
+    import numpy as np
     t0 = time.time()
     p = problems.laplace_inclusions(args.n)
-    log(f"[bench] mesh+field {time.time() - t0:.1f}s: nodes {len(p['xyz'])} tets {len(p['tets'])}")
-    ctx = mfs.Context(0)
-    t0 = time.time()
+    ctx.sync()
     S = mfs.Assembly(ctx, mfs.PROBLEM_SCALAR, p["xyz"], p["tets"], fixed_nodes=p["fixed_nodes"],
                      fixed_values=p["fixed_values"])
     setup_s = time.time() - t0
     S.run(p["sigma"])
     asm_ms = S.time(10)
     A = S.A
-    log(f"[bench] host setup {setup_s:.2f}s, GPU assembly {asm_ms:.3f} ms, N={A.n} nnz={A.nnz}")
-
     mixed = mfs.Solver(A, precision=mfs.PREC_MIXED, tol=1e-8, inner_tol=1e-4)
     fp64 = mfs.Solver(A, precision=mfs.PREC_F64, tol=1e-8)
     x = mfs.Vec(ctx, A.n)
@@ -84,55 +303,65 @@ def main():
         x.copy_from(S.x0)
         return solver.run(S.b, x)
 
-    for _ in range(args.warmup):
+    for _ in range(2):
         run(mixed)
         run(fp64)
+    reps = 10
     ctx.sync()
-    t_start = time.perf_counter()
-    reps = []
-    for _ in range(args.steps):
-        reps.append(run(mixed))
+    t = time.perf_counter()
+    for _ in range(reps):
+        r_m = run(mixed)
     ctx.sync()
-    elapsed = time.perf_counter() - t_start
-    ms_mixed = 1000.0 * elapsed / args.steps
+    ms_mixed = 1000.0 * (time.perf_counter() - t) / reps
     x_mixed = x.download()
-    reps64 = []
     ctx.sync()
-    t64 = time.perf_counter()
-    for _ in range(max(3, args.steps // 2)):
-        reps64.append(run(fp64))
+    t = time.perf_counter()
+    for _ in range(5):
+        r_d = run(fp64)
     ctx.sync()
-    ms_fp64 = 1000.0 * (time.perf_counter() - t64) / len(reps64)
+    ms_fp64 = 1000.0 * (time.perf_counter() - t) / 5
     x_fp64 = x.download()
-    r_m, r_d = reps[-1], reps64[-1]
     agree = float(np.linalg.norm(x_mixed - x_fp64) / np.linalg.norm(x_fp64))
-    log(f"[bench] mixed {ms_mixed:.2f} ms ({r_m.iterations} inner it, {r_m.outer_steps} outer, "
-        f"relres {r_m.true_relres:.2e}); fp64 {ms_fp64:.2f} ms ({r_d.iterations} it, relres {r_d.true_relres:.2e}); "
-        f"|x_mixed-x_fp64|/|x_fp64| = {agree:.2e}")
-
-    # Dominant kernel of the mixed solve: the FP32 fused direction + SpMV
-    # kernel of the PCG iteration (k_cg_dir_spmv<float,1>), timed with HIP
-    # events on the library stream; the update kernel beside it.
+    log(f"[bench] laplace: mixed {ms_mixed:.2f} ms ({r_m.iterations} inner it, {r_m.outer_steps} outer), "
+        f"fp64 {ms_fp64:.2f} ms ({r_d.iterations} it); |x_mixed-x_fp64|/|x_fp64| = {agree:.2e}")
     k1_ms, k1_bytes = mixed.kernel_time(0, 200)
     k2_ms, k2_bytes = mixed.kernel_time(1, 200)
     k1_gbs = k1_bytes / (k1_ms * 1e-3) / 1e9
     k2_gbs = k2_bytes / (k2_ms * 1e-3) / 1e9
-    k1_name = "k_cg_dir_spmv<float,1> (FP32 direction update fused with the SELL-64 SpMV)"
-    log(f"[bench] k_cg_dir_spmv<float,1> {k1_ms * 1e3:.2f} us = {k1_gbs:.0f} GB/s; k_cg_update<float> "
-        f"{k2_ms * 1e3:.2f} us = {k2_gbs:.0f} GB/s")
-    # plain SELL SpMV in both precisions
     X32, Y32 = mfs.Vec(ctx, A.n, mfs.F32, x_mixed), mfs.Vec(ctx, A.n, mfs.F32)
     spmv_ms = A.spmv_time(X32, Y32, 200, mfs.F32)
-    spmv_bytes = A.spmv_bytes(mfs.F32)
+    spmv_gbs = A.spmv_bytes(mfs.F32) / (spmv_ms * 1e-3) / 1e9
     X64, Y64 = mfs.Vec(ctx, A.n, mfs.F64, x_mixed), mfs.Vec(ctx, A.n, mfs.F64)
     spmv64_ms = A.spmv_time(X64, Y64, 200, mfs.F64)
-    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     spmv64_gbs = A.spmv_bytes(mfs.F64) / (spmv64_ms * 1e-3) / 1e9
-    log(f"[bench] SpMV f32 {spmv_ms * 1e3:.1f} us = {spmv_gbs:.0f} GB/s; f64 {spmv64_ms * 1e3:.1f} us = "
-        f"{spmv64_gbs:.0f} GB/s (algorithmic bytes; working set is cache-resident)")
+    multi6 = bench_multi6(mfs, ctx, S, A, mixed)
+    cpu = None
+    if not args.no_cpu:
+        cpu = laplace_cpu_baseline(A, S, r_d.iterations, args.cpu_budget)
+    out = {"nodes": len(p["xyz"]), "tets": len(p["tets"]), "dofs": A.n, "nnz": A.nnz, "tol": 1e-8,
+           "host_setup_s": round(setup_s, 2), "assembly_ms": round(asm_ms, 4),
+           "mixed_ms": round(ms_mixed, 3), "fp64_ms": round(ms_fp64, 3),
+           "speedup_mixed_vs_fp64": round(ms_fp64 / ms_mixed, 3),
+           "iterations": {"mixed_inner": r_m.iterations, "mixed_outer": r_m.outer_steps, "fp64": r_d.iterations},
+           "true_relres": {"mixed": r_m.true_relres, "fp64": r_d.true_relres},
+           "mixed_vs_fp64_solution_rel_diff": agree,
+           "k_cg_dir_spmv_f32": {"avg_us": round(k1_ms * 1e3, 3), "bytes_per_launch": k1_bytes,
+                                 "gbs": round(k1_gbs, 1), "frac_hbm": round(k1_gbs / HBM_PEAK_GBS, 4),
+                                 "traffic": TRAFFIC_K1,
+                                 "note": "working set (35 MB) is resident in the 256 MB Infinity Cache: "
+                                         "effective bandwidth"},
+           "k_cg_update_f32": {"avg_us": round(k2_ms * 1e3, 3), "bytes_per_launch": k2_bytes,
+                               "gbs": round(k2_gbs, 1)},
+           "spmv_f32_us": round(spmv_ms * 1e3, 3), "spmv_f32_gbs": round(spmv_gbs, 1),
+           "spmv_f64_gbs": round(spmv64_gbs, 1), "multi_rhs_6": multi6, "cpu_baseline": cpu}
+    S.close()
+    return out
 
-    # six right-hand sides with this matrix (the homogenisation driver's load
-    # cases): one iteration stream with one SpMM per iteration vs six solves
+
+def bench_multi6(mfs, ctx, S, A, mixed):
+    """Six right-hand sides with one matrix (the homogenisation driver's load
+    cases): one iteration stream with one SpMM per iteration vs six solves."""
+    import numpy as np
     rng = np.random.default_rng(6)
     b0 = S.b.download()
     bs6 = [mfs.Vec(ctx, A.n, mfs.F64, b0 * (1.0 + 0.2 * j) + rng.normal(size=A.n) * np.abs(b0).max() * 0.05)
@@ -158,97 +387,43 @@ def main():
         mixed.run(bs6[j], xs6[j])
     ctx.sync()
     seq_ms = 1000.0 * (time.perf_counter() - t6)
-    multi6 = {"systems": 6, "joint_ms": round(joint_ms, 2), "sequential_ms": round(seq_ms, 2),
-              "speedup": round(seq_ms / joint_ms, 3), "max_true_relres": max(r.true_relres for r in reps6),
-              "iterations": [r.iterations for r in reps6], "precision": "f32 inner / f64 outer"}
     log(f"[bench] 6 right-hand sides: joint {joint_ms:.1f} ms vs sequential {seq_ms:.1f} ms")
+    return {"systems": 6, "joint_ms": round(joint_ms, 2), "sequential_ms": round(seq_ms, 2),
+            "speedup": round(seq_ms / joint_ms, 3), "max_true_relres": max(r.true_relres for r in reps6),
+            "iterations": [r.iterations for r in reps6], "precision": "f32 inner / f64 outer"}
 
-    cpu = None
-    if not args.no_cpu:
-        import oracle as orc
-        rp, col, val = A.export_csr()
-        b = S.b.download()
-        x0 = S.x0.download()
-        threads = min(os.cpu_count() or 1, 16)
-        # probe the per-iteration cost, then run as many iterations as fit the budget
-        _, it_p, _, sec_p = orc.refpath_solve(rp, col, val, b, x0, tol=1e-8, maxit=6, mode=0, threads=threads)
-        per_it = sec_p / max(it_p, 1)
-        budget_it = int(max(10, args.cpu_budget / max(per_it, 1e-6)))
-        xc, it_c, rr_c, sec_c = orc.refpath_solve(rp, col, val, b, x0, tol=1e-8, maxit=budget_it, mode=0,
-                                                  threads=threads)
-        done = rr_c <= 1e-8
-        if done:
-            value = sec_c * 1000.0
-            sample = f"full SSOR-PCG solve to 1e-8 ({it_c} iterations, true relres {rr_c:.2e})"
-        else:
-            # project with the FP64 GPU PCG iteration count (labelled projection)
-            value = sec_c / it_c * r_d.iterations * 1000.0
-            sample = (f"{it_c} SSOR-PCG iterations timed ({sec_c:.1f}s), projected to {r_d.iterations} "
-                      f"iterations (the FP64 GPU PCG count)")
-        cpu = {"value": round(value, 2), "unit": "ms", "cores": threads, "kind": "port",
-               "sample": "reference CPU solver path (MSG_PreconditioningSSOR, CSSD rows, OpenMP dynamic SpMV, "
-                         "serial dots/sweeps) on the same system: " + sample}
-        log(f"[bench] cpu baseline {value:.0f} ms ({sample})")
 
-    nonsym = bench_bicgstab(mfs, problems, ctx, cpu=not args.no_cpu)
-    msfem = None if args.no_msfem else bench_msfem(mfs, problems, ctx, cpu=not args.no_cpu)
-    learned_res = None if args.no_learned else bench_learned(mfs, ctx, args.learned_test_cells)
-    large = None if args.no_large else bench_large(mfs, problems, ctx, args.large_n, not args.large_spmv_only)
-    if args.copy_calibration:
-        # one plain copy of 512 MiB: FETCH_SIZE and WRITE_SIZE of this kernel
-        # should each read 524,288 KiB; their ratio to that scales the counters
-        big = 64 * 1024 * 1024
-        src = mfs.Vec(ctx, big, mfs.F64)
-        dst = mfs.Vec(ctx, big, mfs.F64)
-        src.fill(1.0)
-        ctx.sync()
-        dst.copy_from(src)
-        ctx.sync()
-        log(f"[bench] copy calibration: {big * 8} bytes copied")
-        del src, dst
-
-    out = {
-        "metric": METRIC,
-        "value": round(ms_mixed, 3),
-        "unit": "ms",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_mixed, 3),
-        "higher_is_better": False,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": "f32 inner / f64 outer",
-        "data": "synthetic (seeded structured mesh and inclusion field)",
-        "config": {"workload": f"laplace{args.n}^3_inclusions200", "nodes": len(p["xyz"]), "tets": len(p["tets"]),
-                   "dofs": A.n, "nnz": A.nnz, "tol": 1e-8, "solver": "Jacobi-PCG, FP32 inner + FP64 refinement",
-                   "parallelism": "single GPU"},
-        "fp64_gpu_cg_ms": round(ms_fp64, 3),
-        "speedup_vs_fp64_gpu_cg": round(ms_fp64 / ms_mixed, 3),
-        "iterations": {"mixed_inner": r_m.iterations, "mixed_outer": r_m.outer_steps, "fp64": r_d.iterations},
-        "true_relres": {"mixed": r_m.true_relres, "fp64": r_d.true_relres},
-        "mixed_vs_fp64_solution_rel_diff": agree,
-        "assembly_ms": round(asm_ms, 4),
-        "spmv_f64_gbs": round(spmv64_gbs, 1),
-        "roofline": {"bound": "hbm", "achieved": round(k1_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(k1_gbs / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_K1,
-                     "kernel": k1_name,
-                     "bytes_per_launch": k1_bytes, "avg_us": round(k1_ms * 1e3, 3),
-                     "note": "working set (35 MB) is resident in the 256 MB Infinity Cache; achieved is "
-                             "algorithmic bytes / launch time"},
-        "k_cg_update_f32": {"avg_us": round(k2_ms * 1e3, 3), "bytes_per_launch": k2_bytes,
-                            "gbs": round(k2_gbs, 1)},
-        "multi_rhs_6": multi6,
-        "bicgstab_nonsymmetric": nonsym,
-        "spmv_f32_gbs": round(spmv_gbs, 1),
-        "spmv_f32_us": round(spmv_ms * 1e3, 3),
-        "cpu_baseline": cpu,
-        "msfem_16cubed_cells": msfem,
-        "learned_32cubed_cells": learned_res,
-        "elastic_large": large,
-    }
-    print(json.dumps(out), flush=True)
-    ctx.close()
+def laplace_cpu_baseline(A, S, gpu_it, budget):
+    """The reference's CPU solver path restated in the oracle
+    (MSG_PreconditioningSSOR on CSSD rows, OpenMP dynamic SpMV, serial dots
+    and sweeps) on the same system: to 1e-8 if it fits the budget, else a
+    timed prefix projected with the SSOR-PCG's own convergence rate."""
+    import numpy as np
+    import oracle as orc
+    rp, col, val = A.export_csr()
+    b = S.b.download()
+    x0 = S.x0.download()
+    threads = min(os.cpu_count() or 1, 16)
+    _, it_p, _, sec_p = orc.refpath_solve(rp, col, val, b, x0, tol=1e-8, maxit=6, mode=0, threads=threads)
+    per_it = sec_p / max(it_p, 1)
+    budget_it = int(max(10, budget / max(per_it, 1e-6)))
+    xc, it_c, rr_c, sec_c = orc.refpath_solve(rp, col, val, b, x0, tol=1e-8, maxit=budget_it, mode=0,
+                                              threads=threads)
+    if rr_c <= 1e-8:
+        value = sec_c * 1000.0
+        sample = f"full SSOR-PCG solve to 1e-8 ({it_c} iterations, true relres {rr_c:.2e})"
+    else:
+        # the SSOR-PCG's own rate over the timed iterations: relres falls
+        # like rr_c ** (k / it_c); iterations to 1e-8 at that rate
+        rate_it = it_c * np.log(1e-8) / np.log(rr_c) if 0 < rr_c < 1 else float("nan")
+        value = sec_c / it_c * rate_it * 1000.0
+        sample = (f"{it_c} SSOR-PCG iterations timed ({sec_c:.1f}s, true relres {rr_c:.2e}), projected to "
+                  f"{rate_it:.0f} iterations to 1e-8 with their own average convergence rate (the FP64 GPU "
+                  f"Jacobi-PCG needs {gpu_it})")
+    log(f"[bench] laplace cpu baseline {value:.0f} ms ({sample})")
+    return {"value": round(value, 2), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": "reference CPU solver path (MSG_PreconditioningSSOR, CSSD rows, OpenMP dynamic SpMV, "
+                      "serial dots/sweeps) on the same system: " + sample}
 
 
 def bench_bicgstab(mfs, problems, ctx, cpu=True):
@@ -389,8 +564,9 @@ def bench_msfem_reference_trace(mfs, ctx, p):
 
 def msfem_cpu_baseline(p, counts, per_n=8):
     """The reference's CPU path for the local problems, restated in the
-    oracle: per cell, CSSD assembly of the cell matrix, 24 Jacobi-PCG solves
-    to 1e-8 and the Galerkin product.  Cells run concurrently on the host
+    oracle: per cell, CSSD assembly of the cell matrix, 24 solves with the
+    SSOR-PCG continuation and BCG_Stab2 fallback of FEM.h:3699-3770 to 1e-8
+    and the Galerkin product.  Cells run concurrently on the host
     threads (one cell per thread, as an OpenMP loop over cells would); a
     sample of `per_n` cells per subdivision is timed and the batch total is
     extrapolated with the cell-size histogram."""
@@ -406,7 +582,7 @@ def msfem_cpu_baseline(p, counts, per_n=8):
     def one(c):
         n = int(p["cell_n"][c])
         t0 = time.perf_counter()
-        orc.msfem_cell(n, tuple(p["origin"][c]), p["h"], problems.cell_E(p, c), p["nu"], tol=1e-8)
+        orc.msfem_cell_refpath(n, tuple(p["origin"][c]), p["h"], problems.cell_E(p, c), p["nu"], min_residual=1e-8)
         return n, time.perf_counter() - t0
 
     t0 = time.perf_counter()
@@ -420,28 +596,6 @@ def msfem_cpu_baseline(p, counts, per_n=8):
             "sample": f"{len(sample)} cells ({per_n} per n) solved concurrently on {threads} threads in "
                       f"{wall:.1f}s; per-cell seconds under that load {dict((f'n={n}', round(t, 3)) for n, t in per.items())}"
                       f"; batch total = sum(count_n * t_n) / {threads}"}
-
-
-def bench_learned(mfs, ctx, n_test):
-    """Learned initial guess for the MsFEM local problems at fine
-    n = 8: 20,000 training cells solved by the batched solver, a 4-layer MLP
-    of width 256 trained on the GPU (BF16 MFMA, FP32 accumulation), then the
-    32,768 cells of a 32^3 coarse grid solved to 1e-8 from the zero, the
-    trilinear-hat and the learned guess."""
-    from multiscalefemsolver3d_amd import learned
-    try:
-        out = learned.run(ctx, n=8, n_train=20000, n_test=n_test, hidden=256, layers=4, epochs=20, batch=256,
-                          lr=1e-3, tol=1e-8, log=log, tune=True)
-    except Exception as e:  # reported, not fatal for the primary measurement
-        log(f"[bench] learned: failed: {e}")
-        return {"error": str(e)}
-    # the MLP's GEMM kernel alone on a large product (MFMA utilisation of the
-    # kernel, separate from the small-batch launch overheads of training)
-    ms, tf = mfs.gemm_bf16_time(ctx, 8192, 8192, 8192, reps=5)
-    out["gemm_bf16_kernel_8192cubed"] = {"ms": round(ms, 3), "tflops": round(tf, 1),
-                                         "frac_dense_bf16_peak": round(tf / 2500.0, 4)}
-    log(f"[bench] k_gemm_bf16 8192^3: {ms:.2f} ms = {tf:.0f} TFLOP/s")
-    return out
 
 
 def bench_large(mfs, problems, ctx, n, solves=True):

@@ -339,7 +339,11 @@ typedef struct {
   int n_not_converged;
   int n_fallback;       /* guesses rejected (worse than zero guess)           */
   int n_precision_fallback; /* problems the FP32 inner solve could not finish,
-                               re-solved in FP64                                */
+                               re-solved in FP64 (a second persistent launch
+                               queued behind the first, reading the list of
+                               failed tasks the first one writes)              */
+  double kernel_seconds; /* device time of the first persistent launch alone
+                            (seconds: both launches)                          */
 } mfs_batch_report;
 
 int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* desc, mfs_batch* out);

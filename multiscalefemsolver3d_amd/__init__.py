@@ -83,7 +83,7 @@ class BatchSolveDesc(C.Structure):
 class BatchReport(C.Structure):
     _fields_ = [("seconds", C.c_double), ("max_iterations", C.c_int), ("mean_iterations", C.c_double),
                 ("max_true_relres", C.c_double), ("n_not_converged", C.c_int), ("n_fallback", C.c_int),
-                ("n_precision_fallback", C.c_int)]
+                ("n_precision_fallback", C.c_int), ("kernel_seconds", C.c_double)]
 
 
 class MlpDesc(C.Structure):
@@ -646,6 +646,7 @@ class BatchResult:
     n_fallback: int
     n_precision_fallback: int
     status: int
+    kernel_seconds: float = 0.0
 
 
 def batch_solve_desc(precision=PREC_MIXED, tol=1e-8, maxit=20000, inner_tol=1e-4, use_graph=True, wg_size=0,
@@ -687,7 +688,8 @@ class Batch:
         if check:
             _chk(code, "mfs_batch_solve")
         return BatchResult(rep.seconds, rep.max_iterations, rep.mean_iterations, rep.max_true_relres,
-                           rep.n_not_converged, rep.n_fallback, rep.n_precision_fallback, code)
+                           rep.n_not_converged, rep.n_fallback, rep.n_precision_fallback, code,
+                           rep.kernel_seconds)
 
     def autotune(self, **kw):
         d = batch_solve_desc(**kw)

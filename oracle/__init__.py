@@ -78,6 +78,9 @@ _sig("orc_refpath_solve", C.c_int, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c
 _sig("orc_bicg_diag", C.c_double, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp, ip])
 _sig("orc_msfem_cell", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, C.c_double,
                                  C.c_double, dp, dp, ip])
+_sig("orc_msfem_cell_refpath", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, C.c_double,
+                                         C.c_double, dp, dp, ip])
+_sig("orc_bcg_stab2_csr", C.c_int, [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp])
 _sig("orc_elem_seg", C.c_double, [dp, C.c_double, C.c_double, C.c_int, dp])
 _sig("orc_elem_tri", C.c_double, [dp, C.c_double, C.c_double, C.c_int, dp])
 _sig("orc_edge_1d", C.c_int, [C.c_int, dp, dp, C.c_double, C.c_int, dp, dp, dp])
@@ -280,6 +283,28 @@ def msfem_cell(n, origin, h, E, nu, tol=1e-12):
     it = C.c_int()
     lib.orc_msfem_cell(n, *origin, h, _p(_f64(E), C.c_double), nu, tol, _p(basis, C.c_double), _p(Kc, C.c_double),
                        C.byref(it))
+    return basis, Kc, it.value
+
+
+def bcg_stab2(rowptr, col, val, b, x0=None, E=1e-10, maxiter=100000):
+    """The reference's unpreconditioned BCG_Stab2 (x, iterations, true relres)."""
+    x = np.zeros(len(b)) if x0 is None else _f64(x0).copy()
+    rr = C.c_double()
+    it = lib.orc_bcg_stab2_csr(len(b), _p(_i32(rowptr), C.c_int), _p(_i32(col), C.c_int), _p(_f64(val), C.c_double),
+                               _p(_f64(b), C.c_double), _p(x, C.c_double), E, maxiter, C.byref(rr))
+    return x, it, rr.value
+
+
+def msfem_cell_refpath(n, origin, h, E, nu, min_residual=1e-8):
+    """Bases of one cell solved on the reference's CPU path (full system with
+    eliminated boundary rows, component-major DOFs, SSOR-PCG continuation
+    with BCG_Stab2 fallback, FEM.h:3699-3770); total iterations."""
+    N = 3 * (n + 1) ** 3
+    basis = np.empty((24, N))
+    Kc = np.empty((24, 24))
+    it = C.c_int()
+    lib.orc_msfem_cell_refpath(n, *origin, h, _p(_f64(E), C.c_double), nu, min_residual, _p(basis, C.c_double),
+                               _p(Kc, C.c_double), C.byref(it))
     return basis, Kc, it.value
 
 

@@ -1095,6 +1095,192 @@ EXPORT int orc_msfem_cell(int n, double x0, double y0, double z0, double h, cons
   return N;
 }
 
+// BCG_Stab2 (CSSD_Matrix.h:634-729) restated: unpreconditioned BiCGStab with
+// the shadow residual r~ = r0; stops when the recursive ||r||/||F|| < E,
+// when |beta| <= 1e-16 (the reference's "NO SOLUTION") or after maxiter
+// iterations.  Returns iterations; true relative residual out.
+static int orc_bcg_stab2(const Cssd& A, const std::vector<double>& F, std::vector<double>& X, double E, int maxiter,
+                         double* true_relres) {
+  const int N = A.n();
+  std::vector<double> r(N), rt(N), p(N), s(N), q(N), h(N);
+  A.mult(X, q, 1);
+  for (int i = 0; i < N; i++) {
+    r[i] = F[i] - q[i];
+    rt[i] = r[i];
+    p[i] = r[i];
+  }
+  const double f_norma = std::sqrt(dotp(N, F.data(), F.data()));
+  int i;
+  for (i = 0; i < maxiter; i++) {
+    const double a1 = dotp(N, r.data(), rt.data());
+    A.mult(p, q, 1);
+    const double alfa = a1 / dotp(N, q.data(), rt.data());
+    for (int j = 0; j < N; j++) s[j] = r[j] - alfa * q[j];
+    A.mult(s, h, 1);
+    const double w = dotp(N, s.data(), h.data()) / dotp(N, h.data(), h.data());
+    for (int j = 0; j < N; j++) {
+      X[j] += alfa * p[j] + w * s[j];
+      r[j] = s[j] - w * h[j];
+    }
+    const double beta = dotp(N, r.data(), rt.data()) / a1 * alfa / w;
+    for (int j = 0; j < N; j++) p[j] = r[j] + beta * (p[j] - w * q[j]);
+    const double a_norma = std::sqrt(dotp(N, r.data(), r.data())) / f_norma;
+    if (a_norma < E) break;
+    if (std::fabs(beta) <= 1e-16) break;
+  }
+  A.mult(X, q, 1);
+  double t = 0;
+  for (int j = 0; j < N; j++) t += (F[j] - q[j]) * (F[j] - q[j]);
+  if (true_relres) *true_relres = std::sqrt(t) / f_norma;
+  return i;
+}
+
+EXPORT int orc_bcg_stab2_csr(int n, const int* rp, const int* ci, const double* v, const double* b, double* x,
+                             double E, int maxiter, double* true_relres) {
+  Cssd A = cssd_from_csr(n, rp, ci, v);
+  std::vector<double> F(b, b + n), X(x, x + n);
+  const int it = orc_bcg_stab2(A, F, X, E, maxiter, true_relres);
+  std::memcpy(x, X.data(), sizeof(double) * n);
+  return it;
+}
+
+// The reference's volume-problem solve (FEM.h:3699-3770 on SLAE_in_old_DOF):
+// needed = max(10^-(ii+1), min_residual); SSOR-PCG (MSG_PreconditioningSSOR,
+// w = 0.75) from the current X with max(N/10, 1000) iterations; on success
+// ii++ and the step counter restarts, otherwise BCG_Stab2 with the same
+// budget; the loop ends when the residual exceeds 100x the best, reaches
+// min_residual, or after 5 steps without success.  The best X is kept,
+// exactly as the reference does: when the residual reaches min_residual the
+// loop breaks before that X is recorded as the best, so the X returned is
+// the best one of the earlier steps.  Returns the total iterations.
+static int orc_reference_continuation(int N, const int* rp, const int* ci, const double* v, const double* F, double* X,
+                                      double min_residual) {
+  const int max_size = std::max(N / 10, 1000);
+  std::vector<double> best(X, X + N);
+  double best_res = 1e25, cur = 1;
+  int ii = 1, total = 0;
+  Cssd A;
+  bool have_cssd = false;
+  for (int i = 0; i <= 4; i++) {
+    double needed = std::pow(10., -1 * (ii + 1));
+    needed = needed < min_residual ? min_residual : needed;
+    double rr = 1, sec = 0;
+    total += orc_refpath_solve(N, rp, ci, v, F, X, needed, max_size, 0, 1, &rr, &sec);
+    cur = std::fabs(rr);
+    if (cur < needed) {
+      i = 0;
+      ii++;
+    } else {
+      if (!have_cssd) {
+        A = cssd_from_csr(N, rp, ci, v);
+        have_cssd = true;
+      }
+      std::vector<double> Fv(F, F + N), Xv(X, X + N);
+      total += orc_bcg_stab2(A, Fv, Xv, needed, max_size, &rr);
+      std::memcpy(X, Xv.data(), sizeof(double) * N);
+      cur = std::fabs(rr);
+    }
+    if (cur > best_res * 100) break;
+    if (cur <= min_residual) {
+      best_res = cur;
+      break;
+    }
+    if (cur < best_res) {
+      best_res = cur;
+      std::memcpy(best.data(), X, sizeof(double) * N);
+    }
+  }
+  std::memcpy(X, best.data(), sizeof(double) * N);
+  return total;
+}
+
+// MsFEM local problems of one cell solved on the reference's CPU path, for
+// the CPU baseline: the cell system of every basis with all fine DOFs, the
+// boundary ones eliminated (identity rows, FEM.h:3382-3473), renumbered
+// component-major (DOF node + a * nodes, FEM.h:3474-3500), then
+// orc_reference_continuation; the Galerkin product as in orc_msfem_cell.
+// Returns N; *iters_out = total iterations over the 24 problems.
+EXPORT int orc_msfem_cell_refpath(int n, double x0, double y0, double z0, double h, const double* E, double nu,
+                                  double min_residual, double* basis, double* Kc, int* iters_out) {
+  const int s = n + 1, nn = s * s * s;
+  const i64 nt = 6 * (i64)n * n * n;
+  std::vector<double> xyz(3 * (size_t)nn);
+  std::vector<int> tets(4 * nt);
+  orc_mesh_box(n, n, n, x0, y0, z0, h / n, h / n, h / n, xyz.data(), tets.data());
+  void* raw = orc_elastic_system(nn, nt, xyz.data(), tets.data(), nullptr, E, nu, 0, nullptr, nullptr, 0, nullptr,
+                                 nullptr, 0, 0);
+  OrcSystem* S = (OrcSystem*)raw;
+  std::vector<int> rp, ci;
+  std::vector<double> vv;
+  orc_to_csr(S->A, rp, ci, vv);
+  const int N = 3 * nn;
+  std::vector<char> bnd(nn, 0);
+  for (int k = 0; k < s; k++)
+    for (int j = 0; j < s; j++)
+      for (int i = 0; i < s; i++)
+        if (i == 0 || j == 0 || k == 0 || i == n || j == n || k == n) bnd[i + s * (j + s * k)] = 1;
+  // component-major permutation: new = node + a * nn
+  std::vector<int> perm(N);
+  for (int d = 0; d < N; d++) perm[d] = d / 3 + (d % 3) * nn;
+  int total = 0;
+  for (int c = 0; c < 8; c++) {
+    const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+    for (int d = 0; d < 3; d++) {
+      double* phi = basis + (size_t)(3 * c + d) * N;
+      std::vector<double> g(N, 0.0);
+      for (int nd = 0; nd < nn; nd++) {
+        if (!bnd[nd]) continue;
+        const double X = (xyz[3 * nd] - x0) / h, Y = (xyz[3 * nd + 1] - y0) / h, Z = (xyz[3 * nd + 2] - z0) / h;
+        g[3 * nd + d] = (cx ? X : 1 - X) * (cy ? Y : 1 - Y) * (cz ? Z : 1 - Z);
+      }
+      // eliminated system in component-major order: boundary rows identity
+      // with F = g, interior rows keep their interior columns, F -= A_IB g
+      std::vector<std::vector<std::pair<int, double>>> rows(N);
+      std::vector<double> F(N, 0.0), Xs(N, 0.0);
+      for (int dd = 0; dd < N; dd++) {
+        const int r = perm[dd];
+        if (bnd[dd / 3]) {
+          rows[r].push_back({r, 1.0});
+          F[r] = g[dd];
+          Xs[r] = g[dd];
+          continue;
+        }
+        double f = 0;
+        for (int k = rp[dd]; k < rp[dd + 1]; k++) {
+          const int cc = ci[k];
+          if (bnd[cc / 3]) {
+            f -= vv[k] * g[cc];
+            rows[r].push_back({perm[cc], 0.0});  // eliminated column (kept in the portrait)
+          } else {
+            rows[r].push_back({perm[cc], vv[k]});
+          }
+        }
+        F[r] = f;
+      }
+      std::vector<int> prp(N + 1, 0), pci;
+      std::vector<double> pv;
+      for (int r = 0; r < N; r++) {
+        std::sort(rows[r].begin(), rows[r].end());
+        for (auto& e : rows[r]) {
+          pci.push_back(e.first);
+          pv.push_back(e.second);
+        }
+        prp[r + 1] = (int)pci.size();
+      }
+      total += orc_reference_continuation(N, prp.data(), pci.data(), pv.data(), F.data(), Xs.data(), min_residual);
+      for (int dd = 0; dd < N; dd++) phi[dd] = Xs[perm[dd]];
+    }
+  }
+  std::vector<double> KP((size_t)24 * N);
+  for (int b2 = 0; b2 < 24; b2++)
+    csr_mv(N, rp.data(), ci.data(), vv.data(), basis + (size_t)b2 * N, KP.data() + (size_t)b2 * N, 1);
+  for (int a = 0; a < 24; a++)
+    for (int b2 = 0; b2 < 24; b2++) Kc[a * 24 + b2] = dotp(N, basis + (size_t)a * N, KP.data() + (size_t)b2 * N);
+  orc_sys_free(raw);
+  if (iters_out) *iters_out = total;
+  return N;
+}
+
 // ---------------------------------------------------------------------------
 // MsFEM reference-trace mode: the edge -> face -> volume hierarchy of
 // Grid_forMech_Poly_Order1::CreateFunctionalSpaces (MsFEM_Grid.h:5026-5506).

@@ -294,7 +294,7 @@ def test_laplace64_full_size_mixed_vs_fp64(ctx):
 
 @pytest.mark.parametrize("method,precision,tol,bound", [(mfs.PCG_JACOBI, mfs.PREC_F64, 1e-12, 1e-8),
                                                          (mfs.MINRES, mfs.PREC_F64, 1e-12, 1e-8),
-                                                         (mfs.MINRES, mfs.PREC_MIXED, 1e-10, 1e-6)])
+                                                         (mfs.MINRES, mfs.PREC_MIXED, 1e-12, 1e-8)])
 def test_elastic20_full_size_vs_oracle(ctx, method, precision, tol, bound):
     """20^3 elastic cube (27,783 DOFs; clamped bottom, pressure on top,
     reference element convention): the GPU solution matches the oracle's FP64

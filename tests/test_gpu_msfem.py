@@ -119,18 +119,23 @@ def test_guess_exact_and_bad_guess_fallback(ctx, small):
 
 @pytest.mark.slow
 def test_msfem_16cubed_cells_full_batch(ctx):
-    """16^3 coarse cells, n ~ U{6..10}, lognormal E: 98,304 local problems."""
+    """16^3 coarse cells, n ~ U{6..10}, lognormal E: 98,304 local problems.
+    At the configuration's tolerance (1e-8) every problem converges; solved
+    to 1e-11, one cell of every subdivision n = 6..10 matches the oracle's
+    FP64 bases within 1e-8 (relative L2)."""
     p = problems.msfem_cells(16)
     B = make_batch(ctx, p)
     rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-8)
     print("16^3-cell batch, mixed:", rep)
     assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-8
-    # spot-check one cell of each size against the oracle
-    for n in np.unique(p["cell_n"])[:2]:
+    rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-11)
+    assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-11
+    assert sorted(np.unique(p["cell_n"]).tolist()) == [6, 7, 8, 9, 10]
+    for n in range(6, 11):
         c = int(np.nonzero(p["cell_n"] == n)[0][0])
-        basis_o, _, _ = orc.msfem_cell(int(n), p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"], tol=1e-12)
+        basis_o, _, _ = orc.msfem_cell(int(n), p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"], tol=1e-13)
         err = np.linalg.norm(B.basis(c) - basis_o) / np.linalg.norm(basis_o)
-        assert err < 1e-6, (c, n, err)
+        assert err < 1e-8, (c, n, err)
 
 
 def test_coarse_system_assembly_and_affine_patch(ctx):
@@ -364,12 +369,14 @@ def test_reference_traces_16cubed_full_batch(ctx):
     assert bad == 0 and worst < 1e-11
     rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-8)
     assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-8
-    for n in np.unique(p["cell_n"])[:2]:
+    rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-11)
+    assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-11
+    for n in range(6, 11):
         c = int(np.nonzero(p["cell_n"] == n)[0][0])
         basis_o, _, _ = orc.msfem_cell_reftrace(int(n), p["origin"][c], p["h"], problems.cell_E(p, c), p["nu"],
-                                                tol=1e-12)
+                                                tol=1e-13)
         err = np.linalg.norm(B.basis(c) - basis_o) / np.linalg.norm(basis_o)
-        assert err < 1e-6, (c, n, err)
+        assert err < 1e-8, (c, n, err)
     Kc = B.coarse_matrices()
     assert np.abs(Kc - np.transpose(Kc, (0, 2, 1))).max() < 1e-6 * np.abs(Kc).max()
     for d in range(3):

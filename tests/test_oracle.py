@@ -578,3 +578,37 @@ def test_mlp_oracle_gradient_central_differences():
         bm[l][0] -= h
         num = (orc.mlp_loss_grad(Ws, bp, X, T, False)[0] - orc.mlp_loss_grad(Ws, bm, X, T, False)[0]) / (2 * h)
         assert gb[l][0] == pytest.approx(num, rel=1e-5, abs=1e-9)
+
+
+def test_bcg_stab2_restatement_matches_dense_solve():
+    """BCG_Stab2 (unpreconditioned BiCGStab, CSSD_Matrix.h:634-729) reaches
+    the dense solution of a non-symmetric system and of a diagonal one with
+    distinct entries (with A = c I its first step has s = 0 and omega = 0/0,
+    as in the reference, so that case is not used)."""
+    rp, col, val, b = problems.convection_diffusion(10, 20.0)
+    A = np.zeros((len(rp) - 1,) * 2)
+    for i in range(len(rp) - 1):
+        A[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    xd = np.linalg.solve(A, b)
+    x, it, tr = orc.bcg_stab2(rp, col, val, b, E=1e-12)
+    assert tr < 1e-11 and np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd), (it, tr)
+    n = 30
+    rp1, col1 = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    d = np.linspace(1.0, 9.0, n)
+    x, it, tr = orc.bcg_stab2(rp1, col1, d, b[:n], E=1e-13)
+    assert 1 <= it < n and tr < 1e-12 and np.abs(x - b[:n] / d).max() < 1e-11
+
+
+def test_msfem_cell_on_the_reference_path_agrees_with_the_jacobi_path():
+    """The CPU-baseline restatement of the reference's volume solves (full
+    system, component-major DOFs, SSOR-PCG continuation with BCG_Stab2
+    fallback) gives the same bases as the Jacobi-PCG cell solve to the level
+    its stopping rule leaves (the best iterate of the step before the one
+    that reaches the minimum residual), and the same coarse matrix."""
+    n = 4
+    E = orc.field_lognormal(6 * n ** 3, 1, 3, 0, np.log(1e9), 1.0)
+    b0, K0, _ = orc.msfem_cell(n, (0, 0, 0), 0.25, E, NU, tol=1e-13)
+    b1, K1, it = orc.msfem_cell_refpath(n, (0, 0, 0), 0.25, E, NU, min_residual=1e-10)
+    assert it > 24
+    assert np.linalg.norm(b1 - b0) <= 1e-6 * np.linalg.norm(b0)
+    assert np.abs(K1 - K0).max() <= 1e-5 * np.abs(K0).max()

@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 128^3 elastic measurement")
     ap.add_argument("--no-bicgstab", action="store_true", help="skip the non-symmetric BiCGStab measurement")
     ap.add_argument("--headline-only", action="store_true", help="only the headline workload")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="only the network's BF16 MFMA GEMM (8192^3) and one inference pass of the headline "
+                         "network over 32,768 random inputs (counter runs)")
     ap.add_argument("--large-n", type=int, default=128)
     ap.add_argument("--large-spmv-only", action="store_true", help="128^3: assembly and SpMV only, no solves")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -94,6 +97,10 @@ def main():
     from multiscalefemsolver3d_amd import problems
 
     ctx = mfs.Context(0)
+    if args.gemm_only:
+        print(json.dumps(bench_gemm(mfs, ctx)), flush=True)
+        ctx.close()
+        return
     head = bench_headline(mfs, ctx, args)
     lap = None if args.no_laplace else bench_laplace(mfs, problems, ctx, args)
     nonsym = None if args.no_bicgstab else bench_bicgstab(mfs, problems, ctx, cpu=not args.no_cpu)
@@ -242,6 +249,28 @@ def bench_headline(mfs, ctx, args):
               "parallelism": "single GPU"}
     return {"ms_per_step": round(ms, 3), "fp64_batched_zero_ms": round(ms64, 3), "detail": detail,
             "roofline": roof, "cpu_baseline": cpu, "config": config}
+
+
+def bench_gemm(mfs, ctx):
+    """The MLP's BF16 MFMA GEMM kernel alone at 8192^3, then one inference
+    pass of the headline network (3072-256-256-256-24696) over 32,768
+    random inputs in chunks of 8,192."""
+    import numpy as np
+    ms, tf = mfs.gemm_bf16_time(ctx, 8192, 8192, 8192, reps=5)
+    net = mfs.MLP(ctx, [3072, 256, 256, 256, 24696], seed=11, lr=1e-3)
+    ns = 32768
+    X = mfs.Vec(ctx, ns * 3072, mfs.F32, np.random.default_rng(0).normal(size=ns * 3072).astype(np.float32))
+    Y = mfs.Vec(ctx, ns * 24696, mfs.F32)
+    net.forward_vec(ns, X, Y, 8192)
+    ctx.sync()
+    t0 = time.perf_counter()
+    net.forward_vec(ns, X, Y, 8192)
+    ctx.sync()
+    inf_ms = (time.perf_counter() - t0) * 1e3
+    flops = 2.0 * ns * (3072 * 256 + 256 * 256 * 2 + 256 * 24696)
+    return {"gemm_8192cubed_ms": round(ms, 3), "gemm_tflops": round(tf, 1),
+            "frac_dense_bf16_peak": round(tf / 2500.0, 4), "inference_ms": round(inf_ms, 3),
+            "inference_tflops": round(flops / (inf_ms * 1e-3) / 1e12, 1)}
 
 
 def msfem_cpu_baseline_uniform(learned, n, ncells, budget):

@@ -7,6 +7,7 @@ the import fails, there is no Python fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 from dataclasses import dataclass
@@ -21,6 +22,19 @@ if not os.path.exists(LIB_PATH):
         f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' "
         "or make -C multiscalefemsolver3d_amd)")
 lib = C.CDLL(LIB_PATH)
+
+# Objects still alive when the interpreter exits are not destroyed through
+# the library: the process is ending and the device runtime may already be
+# tearing down, so finalizers only release device memory before that point.
+_EXITING = False
+
+
+def _at_exit():
+    global _EXITING
+    _EXITING = True
+
+
+atexit.register(_at_exit)
 
 MFS_OK = 0
 ERRORS = {-1: "ARG", -2: "OOM", -3: "NOT_CONVERGED", -4: "NONFINITE", -5: "HIP", -6: "UNSUPPORTED",
@@ -389,6 +403,8 @@ class Context:
             self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -432,6 +448,8 @@ class Vec:
         self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -489,6 +507,8 @@ class Mat:
         self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -553,6 +573,8 @@ class Solver:
         self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -619,6 +641,8 @@ class Assembly:
         self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -787,6 +811,8 @@ class Batch:
         self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:
@@ -871,6 +897,8 @@ class MLP:
         self.h = None
 
     def __del__(self):
+        if _EXITING:
+            return
         try:
             self.close()
         except Exception:

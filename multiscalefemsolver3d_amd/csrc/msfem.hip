@@ -1835,10 +1835,11 @@ static const void* batch_kernel(int precision, int var) {
   return var == 0 ? (const void*)k_batch_solve<float, kRptF32[0]> : (const void*)k_batch_solve<float, kRptF32[1]>;
 }
 
-// pass: BL_MAIN = the whole batch (the FP32 launch also lists its failed
-// tasks for the FP64 pass), BL_FP64_FALLBACK = the FP64 pass over that list
-// (task count read on the device, so both launches can be queued together).
-enum { BL_MAIN = 0, BL_FP64_FALLBACK = 1 };
+// pass: BL_MAIN = the whole batch, BL_MAIN_LIST = the whole batch in FP32,
+// listing its failed tasks for the FP64 pass, BL_FP64_FALLBACK = the FP64
+// pass over that list (task count read on the device, so both launches can
+// be queued together).
+enum { BL_MAIN = 0, BL_MAIN_LIST = 1, BL_FP64_FALLBACK = 2 };
 static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu, int var, int pass) {
   hipStream_t s = B->ctx->stream;
   const int grid = B->ctx->cu_count * per_cu;
@@ -1875,7 +1876,7 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   A.iters = B->d_iters;
   A.relres = B->d_relres;
   A.flags = B->d_flags;
-  const bool list_failures = !fb && d->precision == MFS_PREC_MIXED;
+  const bool list_failures = pass == BL_MAIN_LIST && d->precision == MFS_PREC_MIXED;
   A.fb_count = list_failures ? B->d_fb : nullptr;
   A.fb_cell = list_failures ? B->d_fb + 1 : nullptr;
   A.fb_group = list_failures ? B->d_fb + 1 + B->ntasks : nullptr;
@@ -1980,7 +1981,8 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
   // the zero guess's is dropped by the kernel's guess check).  The FP32
   // launch lists those tasks on the device and the FP64 pass, queued right
   // behind it, reads the list: no host round trip between the two.
-  const bool mixed = d->precision == MFS_PREC_MIXED;
+  // the FP64 pass needs its search directions in LDS (cells with n <= 10)
+  const bool mixed = d->precision == MFS_PREC_MIXED && lds_bytes(B, MFS_PREC_F64) <= 160 * 1024 - 8192;
   mfs_batch_solve_desc d64 = *d;
   d64.precision = MFS_PREC_F64;
   d64.use_guess = 1;
@@ -2004,8 +2006,8 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
     hipGraphExec_t ge[2] = {nullptr, nullptr};
     for (int k = 0; k < (mixed ? 2 : 1) && st == MFS_OK; k++) {
       MFS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      st = k == 0 ? batch_launch(B, d, wg, pc, var, BL_MAIN) : batch_launch(B, &d64, wg64, pc64, var64,
-                                                                            BL_FP64_FALLBACK);
+      st = k == 0 ? batch_launch(B, d, wg, pc, var, mixed ? BL_MAIN_LIST : BL_MAIN)
+                  : batch_launch(B, &d64, wg64, pc64, var64, BL_FP64_FALLBACK);
       hipGraph_t gg = nullptr;
       const hipError_t ce = hipStreamEndCapture(s, &gg);
       g[k] = gg;
@@ -2032,7 +2034,7 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
     }
   } else {
     MFS_HIP(hipEventRecord(ev[0], s));
-    st = batch_launch(B, d, wg, pc, var, BL_MAIN);
+    st = batch_launch(B, d, wg, pc, var, mixed ? BL_MAIN_LIST : BL_MAIN);
     MFS_HIP(hipEventRecord(ev[1], s));
     if (st == MFS_OK && mixed) st = batch_launch(B, &d64, wg64, pc64, var64, BL_FP64_FALLBACK);
     MFS_HIP(hipEventRecord(ev[2], s));

@@ -38,7 +38,8 @@ def test_bases_match_oracle(ctx, small, precision):
 
 
 @pytest.mark.parametrize("precision,variant,wg", [
-    (p, v, w) for p in (mfs.PREC_MIXED, mfs.PREC_F64) for v, w in ((1, 64), (1, 256), (2, 64), (2, 192))])
+    (p, v, w) for p in (mfs.PREC_MIXED, mfs.PREC_F64)
+    for v, w in ((1, 64), (1, 256), (2, 64), (2, 192))])
 def test_kernel_variants_and_shapes_match_oracle(ctx, small, precision, variant, wg):
     """Every kernel variant (rows carried per thread) at small and large
     workgroups, where rows per thread exceed or fall short of the variant's

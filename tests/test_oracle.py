@@ -8,6 +8,7 @@ chosen so a plausible slip (dropped term, wrong sign or index, transposed
 operand) fails it.
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -612,3 +613,36 @@ def test_msfem_cell_on_the_reference_path_agrees_with_the_jacobi_path():
     assert it > 24
     assert np.linalg.norm(b1 - b0) <= 1e-6 * np.linalg.norm(b0)
     assert np.abs(K1 - K0).max() <= 1e-5 * np.abs(K0).max()
+
+
+def test_reference_convention_two_phase_cube_defeats_bicg_diag_but_not_minres():
+    """The 8^3 two-phase cube (E in {1e9, 2e11}, clamped bottom, free sides,
+    reference element convention): the reference's own Mechanical solve,
+    BiCG_Diag with diag = 1 in its continuation loop (needed = 10^-(ii+1)
+    absolute, N/10 iterations per call, FEM.h:2399-2485), ends with a true
+    relative residual above 1, while Jacobi MINRES (Elman-Silvester-Wathen
+    Alg. 2.4, numpy) reaches 1e-10 within 2,000 steps.  The operator has 42
+    negative eigenvalues and a Jacobi-scaled smallest |eigenvalue| of about
+    2e-4 (textbook element: none negative)."""
+    import importlib.util
+    import scipy.sparse as sp
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("gpu_core", os.path.join(here, "test_gpu_core.py"))
+    gc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gc)
+    p = problems.elastic_cube(8, field="grf")
+    O = orc.elastic_system(p["xyz"], p["tets"], p["E"], p["nu"], p["fixed_nodes"], p["fixed_values"], p["tris"],
+                           p["traction"], p["material"], convention=0)
+    A = sp.csr_matrix((O.val, O.col, O.rowptr), shape=(O.n, O.n))
+    b = O.rhs
+    x, ii = O.x0.copy(), 1
+    for _ in range(5):
+        needed = 10.0 ** (-(ii + 1))
+        x, it, res = orc.bicg_diag(O.rowptr, O.col, O.val, b, np.ones(O.n), x0=x, E=needed, maxiter=O.n // 10)
+        if res < needed:
+            ii += 1
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) > 1.0
+    xm = gc.minres_esw(A, b, O.x0, 1.0 / A.diagonal(), 2000)
+    assert np.linalg.norm(b - A @ xm) / np.linalg.norm(b) < 1e-10
+    ev = np.linalg.eigvalsh((sp.diags(1 / np.sqrt(A.diagonal())) @ A @ sp.diags(1 / np.sqrt(A.diagonal()))).toarray())
+    assert (ev < 0).sum() == 42 and np.abs(ev).min() < 5e-4

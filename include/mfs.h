@@ -190,6 +190,20 @@ int mfs_solve_multi(mfs_mat A, const mfs_solve_desc* desc, int k, mfs_vec* b, mf
                     mfs_solve_report* reports);
 /* One-shot convenience: create + run + destroy. */
 int mfs_solve(mfs_mat A, const mfs_solve_desc* desc, mfs_vec b, mfs_vec x, mfs_solve_report* report);
+/* mfs_solve without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code around
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The work vectors, iteration state and reduction partials live in a fixed
+ * 64 MiB scratch array of the library that each device holds until the
+ * process ends; kernels are launched directly and the host waits on the
+ * context stream after each iteration chunk (the same kernels and order as
+ * mfs_solve, hence the same iterates).  Methods MFS_PCG_JACOBI and MFS_CG,
+ * precisions MFS_PREC_F64 and MFS_PREC_MIXED.  Systems whose layout does
+ * not fit the scratch (about 0.9 M unknowns in FP64, 1.3 M mixed) return
+ * MFS_ERR_ARG with the size needed.  Every call on one device shares that
+ * device's scratch: calls that overlap on one device (two host threads,
+ * two contexts) race and must be serialised by the caller. */
+int mfs_solve_noalloc(mfs_mat A, const mfs_solve_desc* desc, mfs_vec b, mfs_vec x, mfs_solve_report* report);
 
 /* ---- assembly ---------------------------------------------------------- */
 typedef struct {

@@ -48,6 +48,10 @@ struct mfs_ctx_s {
   // small pinned host buffer for status read-backs
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // device address of this device's copy of the library's fixed scratch
+  // array (mfs_solve_noalloc), resolved once per device at context creation
+  void* noalloc_scratch = nullptr;
+  size_t noalloc_scratch_bytes = 0;
 };
 
 struct mfs_vec_s {

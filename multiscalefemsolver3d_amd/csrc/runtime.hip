@@ -54,6 +54,42 @@ extern "C" const char* mfs_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char* mfs_version(void) { return "mfs 0.1 (gfx950)"; }
 
 // ---------------------------------------------------------------------------
+// Fixed scratch of mfs_solve_noalloc: one file-scope device array, held by
+// every device the library runs on until the process ends.  Host code never
+// uses the array's name as an address: mfs_ctx_create resolves this device's
+// copy with hipGetSymbolAddress once per device and the context carries that
+// pointer.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device `device` must be current
+int resolve_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("mfs_ctx_create: device index beyond the scratch table");
+    return MFS_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return MFS_ERR_HIP;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return MFS_OK;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
 extern "C" int mfs_ctx_create(int device, mfs_ctx* out) {
@@ -90,6 +126,14 @@ extern "C" int mfs_ctx_create(int device, mfs_ctx* out) {
     set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
     return MFS_ERR_OOM;
   }
+  const int rs = resolve_scratch(device, &c->noalloc_scratch);
+  if (rs != MFS_OK) {
+    (void)hipHostFree(c->pinned);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rs;
+  }
+  c->noalloc_scratch_bytes = kNoAllocScratchBytes;
   *out = c;
   return MFS_OK;
 }

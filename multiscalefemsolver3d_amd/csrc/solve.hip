@@ -319,6 +319,25 @@ __global__ void k_promote_add(double* __restrict__ x64, const float* __restrict_
 // ---------------------------------------------------------------------------
 // PCG engine in precision T on one matrix.
 // ---------------------------------------------------------------------------
+// Bump allocator over a caller-provided device buffer (mfs_solve_noalloc):
+// 256-byte aligned slices; with base == nullptr it only measures.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+  bool ok = true;
+  template <typename T>
+  T* take(int64_t count) {
+    const size_t start = (off + 255) & ~size_t(255);
+    const size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(count, 1);
+    if (start + bytes > cap) {
+      ok = false;
+      return nullptr;
+    }
+    off = start + bytes;
+    return base ? reinterpret_cast<T*>(base + start) : nullptr;
+  }
+};
+
 template <typename T>
 struct PcgEngine {
   mfs_mat A = nullptr;
@@ -332,6 +351,9 @@ struct PcgEngine {
   unsigned g_spmv = 0, g_vec = 0;
   int chunk = 16;
   bool use_graph = true;
+  // external buffers (mfs_solve_noalloc): no graph, no events; the host
+  // waits for each chunk with a stream synchronisation
+  bool sync_only = false;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};  // end of the chunks in the two status slots
@@ -354,7 +376,7 @@ struct PcgEngine {
     if (fin) k_finish_sums<<<1, 256, 0, s>>>(part_pq(), g_spmv, tot);
   }
 
-  int init(mfs_mat A_, bool precond, int chunk_) {
+  int setup(mfs_mat A_, bool precond, int chunk_) {
     A = A_;
     n = A->n();
     chunk = std::max(2, chunk_ + (chunk_ & 1));  // even: the ping-pong parity repeats
@@ -362,20 +384,46 @@ struct PcgEngine {
     if (!f64 && !A->val32_valid) MFS_TRY(mat_refresh_derived(A));
     val = f64 ? (const T*)A->d_val64 : (const T*)A->d_val32;
     dinv = precond ? (f64 ? (const T*)A->d_dinv64 : (const T*)A->d_dinv32) : nullptr;
-    for (T** b : {&x, &r, &z, &q, &pbuf[0], &pbuf[1]}) MFS_TRY(dmalloc((void**)b, sizeof(T) * n));
-    MFS_TRY(dmalloc((void**)&st, sizeof(CgState)));
-    MFS_HIP(hipMemset(st, 0, sizeof(CgState)));
     g_spmv = (unsigned)spmv_grid(A);
     // update-kernel grid: about 4 elements per thread (its loop keeps 4 in
     // flight) but at least one block per CU; fewer blocks also means fewer
     // partial sums for every block of the next kernel to add up
     const int64_t cus = std::max(A->ctx->cu_count, 64);
     g_vec = (unsigned)std::min<int64_t>(4 * cus, std::max<int64_t>(cus, (n + 1023) / 1024));
-    MFS_TRY(reduce_scratch_alloc(red, (int64_t)g_spmv + 2 * (int64_t)g_vec + 64));
     fin = (uint64_t)g_spmv * g_vec > 2000000ull;
     if (const char* e = std::getenv("MFS_FINISH_SUMS")) fin = e[0] == '1';  // force either way (tests)
+    return MFS_OK;
+  }
+  int64_t partial_capacity() const { return (int64_t)g_spmv + 2 * (int64_t)g_vec + 64; }
+
+  int init(mfs_mat A_, bool precond, int chunk_) {
+    MFS_TRY(setup(A_, precond, chunk_));
+    for (T** b : {&x, &r, &z, &q, &pbuf[0], &pbuf[1]}) MFS_TRY(dmalloc((void**)b, sizeof(T) * n));
+    MFS_TRY(dmalloc((void**)&st, sizeof(CgState)));
+    MFS_HIP(hipMemset(st, 0, sizeof(CgState)));
+    MFS_TRY(reduce_scratch_alloc(red, partial_capacity()));
     MFS_TRY(dmalloc((void**)&tot, sizeof(double) * 4));
     return capture();
+  }
+
+  // The same engine on buffers carved from `ar` (no allocation, no graph);
+  // with ar.base == nullptr only the sizes are counted.  Never destroy()ed.
+  int init_ext(mfs_mat A_, bool precond, int chunk_, Arena& ar) {
+    MFS_TRY(setup(A_, precond, chunk_));
+    for (T** b : {&x, &r, &z, &q, &pbuf[0], &pbuf[1]}) *b = ar.take<T>(n);
+    st = ar.take<CgState>(1);
+    red.capacity = partial_capacity();
+    red.partials = ar.take<double>(red.capacity);
+    red.counter = ar.take<unsigned>(16);
+    tot = ar.take<double>(4);
+    use_graph = false;
+    sync_only = true;
+    if (ar.base && ar.ok) {
+      // the scratch keeps the previous call's contents: clear the state
+      MFS_HIP(hipMemsetAsync(st, 0, sizeof(CgState), A->ctx->stream));
+      MFS_HIP(hipMemsetAsync(red.counter, 0, 16 * sizeof(unsigned), A->ctx->stream));
+    }
+    return MFS_OK;
   }
 
   // K1 with 16-bit row-relative columns when the matrix has them
@@ -455,6 +503,23 @@ struct PcgEngine {
     // host checks every chunk (a chunk queued after convergence returns at
     // once in every kernel and changes nothing).
     CgState* hs = (CgState*)A->ctx->pinned;
+    if (sync_only) {
+      // one chunk at a time, the host waits on the stream
+      for (;;) {
+        for (int k = 0; k < chunk; k++) launch_iteration(s, k & 1);
+        MFS_HIP(hipGetLastError());
+        MFS_HIP(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        MFS_HIP(hipStreamSynchronize(s));
+        if (hs->done != CG_RUN) break;
+      }
+      *host_state = *hs;
+      host_state->iter2 = host_state->iter1;
+      last_iter = host_state->iter1;
+      last_rz_prev = host_state->rz2;
+      can_keep = host_state->done == CG_CONVERGED && host_state->iter1 > 0 && std::isfinite(host_state->rz2) &&
+                 host_state->rz2 > 0.0;
+      return MFS_OK;
+    }
     for (hipEvent_t* e : {&ev[0], &ev[1]})
       if (!*e) MFS_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     auto submit = [&](int slot) -> int {
@@ -1856,4 +1921,76 @@ extern "C" int mfs_solve(mfs_mat A, const mfs_solve_desc* desc, mfs_vec b, mfs_v
   int st = mfs_solver_run(S, b, x, report);
   mfs_solver_destroy(S);
   return st;
+}
+
+// ---------------------------------------------------------------------------
+// mfs_solve without device allocation: the solver's work vectors, state and
+// reduction partials are carved from the context's fixed scratch (a device
+// array of the library, 64 MiB per device), kernels are launched directly
+// (no graphs) and the host waits for each iteration chunk on the stream.
+// Same kernels and iteration order as mfs_solve, so the same iterates.
+// ---------------------------------------------------------------------------
+static int noalloc_layout(mfs_solver_s& S, Arena& ar) {
+  const int64_t n = S.n;
+  S.x64 = ar.take<double>(n);
+  S.b64 = ar.take<double>(n);
+  S.r64 = ar.take<double>(n);
+  S.dout = ar.take<double>(64);
+  S.red64.capacity = 2 * spmv_grid(S.A) + 64;
+  S.red64.partials = ar.take<double>(S.red64.capacity);
+  S.red64.counter = ar.take<unsigned>(16);
+  const bool precond = S.desc.method != MFS_CG;
+  const int chunk = S.desc.check_every > 0 ? S.desc.check_every : 16;
+  if (S.desc.precision == MFS_PREC_F64) {
+    S.has64 = true;
+    MFS_TRY(S.cg64.init_ext(S.A, precond, chunk, ar));
+  } else {
+    S.has32 = true;
+    MFS_TRY(S.cg32.init_ext(S.A, precond, chunk, ar));
+  }
+  return MFS_OK;
+}
+
+extern "C" int mfs_solve_noalloc(mfs_mat A, const mfs_solve_desc* desc, mfs_vec b, mfs_vec x,
+                                 mfs_solve_report* report) {
+  MFS_CHECK_ARG(A && desc && b && x && report, "mfs_solve_noalloc: bad argument");
+  MFS_CHECK_ARG(desc->method == MFS_PCG_JACOBI || desc->method == MFS_CG,
+                "mfs_solve_noalloc: method must be MFS_PCG_JACOBI or MFS_CG");
+  MFS_CHECK_ARG(desc->precision == MFS_PREC_F64 || desc->precision == MFS_PREC_MIXED,
+                "mfs_solve_noalloc: unknown precision");
+  MFS_CHECK_ARG(desc->tol > 0 && desc->maxit > 0, "mfs_solve_noalloc: tol and maxit must be positive");
+  mfs_ctx ctx = A->ctx;
+  MFS_CHECK_ARG(ctx->noalloc_scratch && ctx->noalloc_scratch_bytes > 0, "mfs_solve_noalloc: no scratch on this device");
+  MFS_HIP(hipSetDevice(ctx->device));
+  // size check first (measuring pass, nothing touched)
+  {
+    mfs_solver_s M;
+    M.A = A;
+    M.desc = *desc;
+    M.n = A->n();
+    Arena meas;
+    meas.cap = ~size_t(0) >> 1;
+    MFS_TRY(noalloc_layout(M, meas));
+    if (meas.off > ctx->noalloc_scratch_bytes) {
+      set_error("mfs_solve_noalloc: this system needs " + std::to_string(meas.off) +
+                " bytes of scratch, the fixed scratch holds " + std::to_string(ctx->noalloc_scratch_bytes) +
+                " (use mfs_solve for larger systems)");
+      return MFS_ERR_ARG;
+    }
+  }
+  mfs_solver_s S;
+  S.A = A;
+  S.desc = *desc;
+  S.n = A->n();
+  Arena ar;
+  ar.base = (char*)ctx->noalloc_scratch;
+  ar.cap = ctx->noalloc_scratch_bytes;
+  MFS_TRY(noalloc_layout(S, ar));
+  if (!ar.ok) {
+    set_error("mfs_solve_noalloc: scratch layout overflow");
+    return MFS_ERR_ARG;
+  }
+  MFS_HIP(hipMemsetAsync(S.red64.counter, 0, 16 * sizeof(unsigned), ctx->stream));
+  // S owns nothing: it is not passed to mfs_solver_destroy
+  return mfs_solver_run(&S, b, x, report);
 }

@@ -38,10 +38,11 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE (13,300.2 KiB, x2.000 from a 512 MiB copy) and
 # WRITE_SIZE (2,179.2 KiB, x1.000), medians over 7,752 / 7,731 dispatches
 TRAFFIC_K1 = 29470310
-# HBM bytes of one k_batch_solve<float> launch on the headline workload
-# (FETCH_SIZE x2.000 + WRITE_SIZE x1.000 from rocprofv3 --pmc, the copy
-# calibration above); None until profiled
-TRAFFIC_BATCH = None
+# HBM bytes of one k_batch_solve<float,2> launch on the headline workload
+# (learned guess, launch shape 192 / variant 1): rocprofv3 --pmc FETCH_SIZE
+# 796,173,679.1 KiB x2.000 + WRITE_SIZE 612,285,657.1 KiB x1.000 (the copy
+# calibration above)
+TRAFFIC_BATCH = 2257544207616
 
 
 def log(*a):
@@ -70,6 +71,8 @@ def main():
     ap.add_argument("--cells", type=int, default=32768, help="headline: coarse cells (32^3)")
     ap.add_argument("--train-cells", type=int, default=20000)
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--shape", default="", help="headline batch-solve launch shape WG,VARIANT for both precisions "
+                                               "instead of the autotuner (counter runs)")
     ap.add_argument("--n", type=int, default=64, help="Laplace leg: mesh hexes per edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work per leg")
@@ -78,6 +81,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 128^3 elastic measurement")
     ap.add_argument("--no-bicgstab", action="store_true", help="skip the non-symmetric BiCGStab measurement")
     ap.add_argument("--headline-only", action="store_true", help="only the headline workload")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the 128^3 elastic leg (assembly, SpMV, solvers; --large-spmv-only skips the solvers)")
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the network's BF16 MFMA GEMM (8192^3) and one inference pass of the headline "
                          "network over 32,768 random inputs (counter runs)")
@@ -97,8 +102,10 @@ def main():
     from multiscalefemsolver3d_amd import problems
 
     ctx = mfs.Context(0)
-    if args.gemm_only:
-        print(json.dumps(bench_gemm(mfs, ctx)), flush=True)
+    if args.gemm_only or args.large_only:
+        res = bench_gemm(mfs, ctx) if args.gemm_only else bench_large(mfs, problems, ctx, args.large_n,
+                                                                        not args.large_spmv_only)
+        print(json.dumps(res), flush=True)
         ctx.close()
         return
     head = bench_headline(mfs, ctx, args)
@@ -155,8 +162,12 @@ def bench_headline(mfs, ctx, args):
     from multiscalefemsolver3d_amd import learned
     n, ncells = 8, args.cells
     t0 = time.time()
+    shape = None
+    if args.shape:
+        wg, var = (int(v) for v in args.shape.split(","))
+        shape = {mfs.PREC_MIXED: {"wg_size": wg, "variant": var}, mfs.PREC_F64: {"wg_size": wg, "variant": var}}
     model = learned.train(ctx, n=n, n_train=args.train_cells, hidden=256, layers=4, epochs=args.epochs, batch=256,
-                          lr=1e-3, tol=1e-8, log=log, tune=True)
+                          lr=1e-3, tol=1e-8, log=log, tune=shape is None, shape=shape)
     train_wall = time.time() - t0
     cn, org, h, E = learned.cells(ncells, n, 5)
     B = mfs.Batch(ctx, cn, org, h, E)

@@ -2,11 +2,14 @@
 //
 //  1. k_tet_geometry: one thread per tet computes the P1 gradients and the
 //     volume-scaled material factors (V*sigma, or V*lambda and V*mu).
-//  2. k_gather_*: one thread per pattern slot sums that slot's element
+//  2. k_gather: one thread per SELL-64 entry sums that entry's element
 //     contributions in ascending tet order (the order in which the
 //     reference's serial SummPartOfMatrix visits elements, FEM.h:2146-2155)
-//     and writes the entry straight into the SELL-64 value array.  No
-//     atomics, so results are deterministic.
+//     and writes the entry's values.  The contribution lists are stored
+//     interleaved per group of 64 entries (entry k of the 64 lists side by
+//     side), so a wave reads one list position of its 64 entries with one
+//     coalesced load and writes each value of its 64 blocks with one
+//     coalesced store.  No atomics, so results are deterministic.
 //  3. k_dirichlet_*: elimination of fixed DOFs (FEM.h:505-619): fixed rows
 //     become identity rows, fixed columns move to the right-hand side.
 #include <hip/hip_runtime.h>
@@ -35,8 +38,8 @@ struct mfs_asm_s {
   int32_t* d_tets = nullptr;    // 4 per tet, DOF node ids
   double* d_coef = nullptr;     // per tet
   double* d_tet = nullptr;      // 16 doubles per tet: g[4][3], f0, f1, V, pad
-  int64_t* d_gptr = nullptr;    // nnzb+1
-  uint32_t* d_gent = nullptr;   // 16 per tet
+  int64_t* d_goff = nullptr;    // per group of 64 SELL entries: start of its interleaved lists (+1)
+  uint32_t* d_gil = nullptr;    // interleaved contribution lists (kNoContribution pads)
   int32_t* d_browptr = nullptr; // nb+1
   uint8_t* d_fixed = nullptr;   // per scalar DOF
   double* d_g = nullptr;        // per scalar DOF (fixed values)
@@ -104,68 +107,66 @@ __global__ __launch_bounds__(256) void k_tet_geometry(const double* __restrict__
   }
 }
 
-__global__ __launch_bounds__(256) void k_gather_scalar(const int64_t* __restrict__ gptr, const uint32_t* __restrict__ gent,
-                                                       const double* __restrict__ tet, const int64_t* __restrict__ pos,
-                                                       int64_t nnzb, double* __restrict__ sell) {
-  constexpr int U = 4;  // entries whose loads are in flight together
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nnzb; s += (int64_t)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    const int64_t k0 = gptr[s], k1 = gptr[s + 1];
-    for (int64_t kb = k0; kb < k1; kb += U) {
-      // indices first, then the records they point at, then the sums (the
-      // order of the additions stays ascending); entries past the list end
-      // re-read the last one and are skipped
-      uint32_t c[U];
+constexpr uint32_t kNoContribution = 0xffffffffu;
+
+// Element block (I=i, J=j) of tet record o (BS = 1: sigma V gI.gJ; BS = 3:
+//   textbook  K[a][b] = V(lam dI_a dJ_b + mu dI_b dJ_a + mu delta_ab dI.dJ)
+//   reference K[a][b] = textbook[b][a] (FEM_Grid.h:2159-2191 puts B_J^T on
+//   the left of D)), added to acc.
+template <int BS, int CONV>
+__device__ __forceinline__ void add_block(const double (&g)[8], double (&acc)[BS * BS]) {
+  const double* dI = g;
+  const double* dJ = g + 3;
+  const double dd = dI[0] * dJ[0] + dI[1] * dJ[1] + dI[2] * dJ[2];
+  if constexpr (BS == 1) {
+    acc[0] += g[6] * dd;
+  } else {
+    const double vl = g[6], vm = g[7];
 #pragma unroll
-      for (int u = 0; u < U; u++) c[u] = gent[min(kb + u, k1 - 1)];
-      double g[U][7];
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const double* o = tet + 16 * (int64_t)(c[u] >> 4);
-        const int i = (c[u] >> 2) & 3, j = c[u] & 3;
-        g[u][0] = o[3 * i];
-        g[u][1] = o[3 * i + 1];
-        g[u][2] = o[3 * i + 2];
-        g[u][3] = o[3 * j];
-        g[u][4] = o[3 * j + 1];
-        g[u][5] = o[3 * j + 2];
-        g[u][6] = o[12];
+      for (int b = 0; b < 3; b++) {
+        double v;
+        if (CONV == MFS_ELEM_TEXTBOOK)
+          v = vl * dI[a] * dJ[b] + vm * dI[b] * dJ[a];
+        else
+          v = vl * dI[b] * dJ[a] + vm * dI[a] * dJ[b];
+        if (a == b) v += vm * dd;
+        acc[3 * a + b] += v;
       }
-#pragma unroll
-      for (int u = 0; u < U; u++)
-        if (kb + u < k1) acc += g[u][6] * (g[u][0] * g[u][3] + g[u][1] * g[u][4] + g[u][2] * g[u][5]);
-    }
-    const int64_t p = pos[s];
-    sell[p] = acc;  // bs = 1: entry index == value index
   }
 }
 
-// Elastic block (I=i, J=j) of tet e:
-//   textbook  K[a][b] = V(lam dI_a dJ_b + mu dI_b dJ_a + mu delta_ab dI.dJ)
-//   reference K[a][b] = textbook[b][a] (FEM_Grid.h:2159-2191 puts B_J^T on
-//   the left of D).
-template <int CONV>
-__global__ __launch_bounds__(256) void k_gather_elastic(const int64_t* __restrict__ gptr,
-                                                        const uint32_t* __restrict__ gent,
-                                                        const double* __restrict__ tet,
-                                                        const int64_t* __restrict__ pos, int64_t nnzb,
-                                                        double* __restrict__ sell) {
+// One thread per SELL entry p (group p / 64, lane p % 64): the entry's
+// contributions are gil[goff[g] + k * 64 + lane], k ascending = tet order.
+template <int BS, int CONV>
+__global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ goff, const uint32_t* __restrict__ gil,
+                                                const double* __restrict__ tet, int64_t n_entries,
+                                                double* __restrict__ sell) {
   constexpr int C = 64;
-  constexpr int U = 4;  // entries whose loads are in flight together
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nnzb; s += (int64_t)gridDim.x * blockDim.x) {
-    double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    const int64_t k0 = gptr[s], k1 = gptr[s + 1];
-    for (int64_t kb = k0; kb < k1; kb += U) {
-      // indices first, then the records, then the sums in ascending entry
-      // order; entries past the list end re-read the last one and are skipped
+  constexpr int U = 4;  // list positions whose loads are in flight together
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n_entries;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t grp = p / C;
+    const int lane = (int)(p % C);
+    const int64_t k0 = goff[grp];
+    const int K = (int)((goff[grp + 1] - k0) / C);
+    double acc[BS * BS];
+#pragma unroll
+    for (int e = 0; e < BS * BS; e++) acc[e] = 0.0;
+    for (int kb = 0; kb < K; kb += U) {
+      // list entries first, then the records they point at, then the sums in
+      // ascending order; positions past this lane's list read tet 0 and are
+      // skipped
       uint32_t c[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) c[u] = gent[min(kb + u, k1 - 1)];
+      for (int u = 0; u < U; u++) c[u] = kb + u < K ? gil[k0 + (int64_t)(kb + u) * C + lane] : kNoContribution;
       double g[U][8];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const double* o = tet + 16 * (int64_t)(c[u] >> 4);
-        const int i = (c[u] >> 2) & 3, j = c[u] & 3;
+        const uint32_t cu = c[u] == kNoContribution ? 0u : c[u];
+        const double* o = tet + 16 * (int64_t)(cu >> 4);
+        const int i = (cu >> 2) & 3, j = cu & 3;
         g[u][0] = o[3 * i];
         g[u][1] = o[3 * i + 1];
         g[u][2] = o[3 * i + 2];
@@ -176,32 +177,16 @@ __global__ __launch_bounds__(256) void k_gather_elastic(const int64_t* __restric
         g[u][7] = o[13];
       }
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (kb + u >= k1) continue;
-        const double* dI = g[u];
-        const double* dJ = g[u] + 3;
-        const double vl = g[u][6], vm = g[u][7];
-        const double dd = dI[0] * dJ[0] + dI[1] * dJ[1] + dI[2] * dJ[2];
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-          for (int b = 0; b < 3; b++) {
-            double v;
-            if (CONV == MFS_ELEM_TEXTBOOK)
-              v = vl * dI[a] * dJ[b] + vm * dI[b] * dJ[a];
-            else
-              v = vl * dI[b] * dJ[a] + vm * dI[a] * dJ[b];
-            if (a == b) v += vm * dd;
-            K[a][b] += v;
-          }
-      }
+      for (int u = 0; u < U; u++)
+        if (c[u] != kNoContribution) add_block<BS, CONV>(g[u], acc);
     }
-    const int64_t p = pos[s];
-    const int64_t base = (p / C) * C * 9 + (p % C);
+    if constexpr (BS == 1) {
+      sell[p] = acc[0];
+    } else {
+      const int64_t base = grp * C * 9 + lane;
 #pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-      for (int b = 0; b < 3; b++) sell[base + (int64_t)(3 * a + b) * C] = K[a][b];
+      for (int e = 0; e < 9; e++) sell[base + (int64_t)e * C] = acc[e];
+    }
   }
 }
 
@@ -288,8 +273,8 @@ extern "C" int mfs_assemble_destroy(mfs_asm S) {
   dfree(S->d_tets);
   dfree(S->d_coef);
   dfree(S->d_tet);
-  dfree(S->d_gptr);
-  dfree(S->d_gent);
+  dfree(S->d_goff);
+  dfree(S->d_gil);
   dfree(S->d_browptr);
   dfree(S->d_fixed);
   dfree(S->d_g);
@@ -370,6 +355,45 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
       }
     }
   int st = mat_create_from_pattern(ctx, bs, std::move(P), &S->A);
+  // contribution lists interleaved per group of 64 SELL entries (k-th
+  // contribution of the group's 64 entries side by side, padded to the
+  // group's longest list)
+  std::vector<int64_t> goff;
+  std::vector<uint32_t> gil;
+  if (st == MFS_OK) {
+    const SellLayout& L = S->A->sell;
+    const Pattern& Pt = S->A->pat;
+    const int C = L.C;
+    const int64_t ngroups = L.slice_base[L.n_slices] / C;
+    goff.assign(ngroups + 1, 0);
+    auto slot_of = [&](int64_t q, int j, int lane) -> int64_t {
+      const int64_t r = q * C + lane;
+      if (r >= Pt.n || j >= Pt.rowptr[r + 1] - Pt.rowptr[r]) return -1;
+      return Pt.rowptr[r] + j;
+    };
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < L.n_slices; q++)
+      for (int j = 0; j < L.slice_width[q]; j++) {
+        int64_t K = 0;
+        for (int lane = 0; lane < C; lane++) {
+          const int64_t sl = slot_of(q, j, lane);
+          if (sl >= 0) K = std::max<int64_t>(K, G.ptr[sl + 1] - G.ptr[sl]);
+        }
+        goff[L.slice_base[q] / C + j + 1] = K * C;
+      }
+    for (int64_t g = 0; g < ngroups; g++) goff[g + 1] += goff[g];
+    gil.assign(goff[ngroups], kNoContribution);
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < L.n_slices; q++)
+      for (int j = 0; j < L.slice_width[q]; j++) {
+        const int64_t base = goff[L.slice_base[q] / C + j];
+        for (int lane = 0; lane < C; lane++) {
+          const int64_t sl = slot_of(q, j, lane);
+          if (sl < 0) continue;
+          for (int64_t k = G.ptr[sl]; k < G.ptr[sl + 1]; k++) gil[base + (k - G.ptr[sl]) * C + lane] = G.entry[k];
+        }
+      }
+  }
   if (st == MFS_OK) st = mfs_vec_create(ctx, ndof, MFS_F64, &S->b);
   if (st == MFS_OK) st = mfs_vec_create(ctx, ndof, MFS_F64, &S->x0);
   auto alloc = [&](void** p, size_t bytes) {
@@ -379,8 +403,8 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   alloc((void**)&S->d_tets, 4 * 4 * nt);
   alloc((void**)&S->d_coef, 8 * nt);
   alloc((void**)&S->d_tet, 8 * 16 * nt);
-  alloc((void**)&S->d_gptr, 8 * G.ptr.size());
-  alloc((void**)&S->d_gent, 4 * G.entry.size());
+  alloc((void**)&S->d_goff, 8 * goff.size());
+  alloc((void**)&S->d_gil, 4 * std::max<size_t>(1, gil.size()));
   alloc((void**)&S->d_browptr, 4 * browptr.size());
   alloc((void**)&S->d_fixed, ndof);
   alloc((void**)&S->d_g, 8 * ndof);
@@ -392,8 +416,8 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   hipStream_t s = ctx->stream;
   MFS_HIP(hipMemcpyAsync(S->d_xyz, rxyz.data(), 8 * 3 * nn, hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(S->d_tets, rt.data(), 4 * 4 * nt, hipMemcpyHostToDevice, s));
-  MFS_HIP(hipMemcpyAsync(S->d_gptr, G.ptr.data(), 8 * G.ptr.size(), hipMemcpyHostToDevice, s));
-  MFS_HIP(hipMemcpyAsync(S->d_gent, G.entry.data(), 4 * G.entry.size(), hipMemcpyHostToDevice, s));
+  MFS_HIP(hipMemcpyAsync(S->d_goff, goff.data(), 8 * goff.size(), hipMemcpyHostToDevice, s));
+  if (!gil.empty()) MFS_HIP(hipMemcpyAsync(S->d_gil, gil.data(), 4 * gil.size(), hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(S->d_browptr, browptr.data(), 4 * browptr.size(), hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(S->d_fixed, fixed.data(), ndof, hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(S->d_g, g.data(), 8 * ndof, hipMemcpyHostToDevice, s));
@@ -407,14 +431,13 @@ static int assemble_kernels(mfs_asm S, double nu) {
   hipStream_t s = S->ctx->stream;
   mfs_mat A = S->A;
   k_tet_geometry<<<grid_cap(S->nt), 256, 0, s>>>(S->d_xyz, S->d_tets, S->d_coef, S->nt, S->problem, nu, S->d_tet);
+  const int64_t ne = A->sell_entries;
   if (S->bs == 1)
-    k_gather_scalar<<<grid_cap(A->nnzb), 256, 0, s>>>(S->d_gptr, S->d_gent, S->d_tet, A->d_pos, A->nnzb, A->d_val64);
+    k_gather<1, MFS_ELEM_REFERENCE><<<grid_cap(ne), 256, 0, s>>>(S->d_goff, S->d_gil, S->d_tet, ne, A->d_val64);
   else if (S->convention == MFS_ELEM_TEXTBOOK)
-    k_gather_elastic<MFS_ELEM_TEXTBOOK><<<grid_cap(A->nnzb), 256, 0, s>>>(S->d_gptr, S->d_gent, S->d_tet, A->d_pos,
-                                                                          A->nnzb, A->d_val64);
+    k_gather<3, MFS_ELEM_TEXTBOOK><<<grid_cap(ne), 256, 0, s>>>(S->d_goff, S->d_gil, S->d_tet, ne, A->d_val64);
   else
-    k_gather_elastic<MFS_ELEM_REFERENCE><<<grid_cap(A->nnzb), 256, 0, s>>>(S->d_gptr, S->d_gent, S->d_tet, A->d_pos,
-                                                                           A->nnzb, A->d_val64);
+    k_gather<3, MFS_ELEM_REFERENCE><<<grid_cap(ne), 256, 0, s>>>(S->d_goff, S->d_gil, S->d_tet, ne, A->d_val64);
   if (S->bs == 1)
     k_dirichlet<1><<<grid_cap(A->nb), 256, 0, s>>>(S->d_browptr, A->d_col, A->d_slice_base, A->d_val64, S->d_fixed,
                                                    S->d_g, S->d_fbase, (double*)S->b->d, (double*)S->x0->d, A->nb);

@@ -20,6 +20,7 @@ using namespace mfs;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
 
@@ -174,6 +175,122 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int Kc, 
       for (int e = 0; e < 4; e++) {
         const int row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + e;
         const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (row >= M || col >= N) continue;
+        float v = acc[i][j][e];
+        if (ws) {
+          ws[((int64_t)blockIdx.z * M + row) * N + col] = v;
+          continue;
+        }
+        if (EPI != EPI_NONE) v += bias[col];
+        if (EPI == EPI_BIAS_RELU) v = v > 0.f ? v : 0.f;
+        C[(int64_t)row * ldc + col] = v;
+        if (Cbf) Cbf[(int64_t)row * ldc + col] = f2bf(v);
+      }
+}
+
+// Large GEMMs with both operands contiguous in k (the forward pass X W^T and
+// the benchmark): 256x256 output tile per 512-thread workgroup, each of the
+// 8 waves a 128x64 block as 4x2 v_mfma_f32_32x32x16_bf16 tiles, K staged
+// through LDS in steps of 32 with two buffers.  A wave reads (128 + 64) x 32
+// operand values from LDS per 128 x 64 x 32 of MFMA work, a third more
+// arithmetic per LDS byte than the 64x64 quadrants of k_gemm_bf16, and two
+// waves per SIMD overlap one's LDS reads with the other's MFMAs.  Operand
+// rows beyond M / N and k beyond the split's end are read as zeros.
+// Split-K and epilogue as k_gemm_bf16.
+constexpr int kBigThreads = 512;
+__device__ __forceinline__ void tile256_load(uint4 (&v)[2], const uint16_t* __restrict__ src, int64_t s_row, int row0,
+                                             int k0, int R, int K) {
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int c = threadIdx.x + q * kBigThreads;
+    const int row = c >> 2, kk = (c & 3) * 8;
+    const int gr = row0 + row, gk = k0 + kk;
+    const uint16_t* ptr = src + (int64_t)gr * s_row + gk;
+    if (gr < R && gk + 8 <= K && ((uintptr_t)ptr & 15) == 0) {
+      v[q] = *reinterpret_cast<const uint4*>(ptr);
+    } else {
+      uint16_t e8[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) e8[e] = (gr < R && gk + e < K) ? ptr[e] : (uint16_t)0;
+      v[q] = pack8(e8);
+    }
+  }
+}
+__device__ __forceinline__ void tile256_store(uint16_t (*dst)[40], const uint4 (&v)[2]) {
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int c = threadIdx.x + q * kBigThreads;
+    *reinterpret_cast<uint4*>(&dst[c >> 2][(c & 3) * 8]) = v[q];
+  }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(kBigThreads) void k_gemm_bf16_big(int M, int N, int K, int Kc,
+                                                               const uint16_t* __restrict__ A, int64_t sam,
+                                                               const uint16_t* __restrict__ B, int64_t sbn,
+                                                               float* __restrict__ C, int64_t ldc,
+                                                               const float* __restrict__ bias,
+                                                               uint16_t* __restrict__ Cbf, float* __restrict__ ws) {
+  __shared__ __align__(16) uint16_t As[2][256][40];
+  __shared__ __align__(16) uint16_t Bs[2][256][40];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 2, wn = w & 3;  // 2 x 4 waves of 128 x 64
+  const int m0 = blockIdx.y * 256, n0 = blockIdx.x * 256;
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
+  const int kb = blockIdx.z * Kc, ke = min(K, kb + Kc);
+  uint4 ra[2], rb[2];
+  if (kb < ke) {
+    tile256_load(ra, A, sam, m0, kb, M, ke);
+    tile256_load(rb, B, sbn, n0, kb, N, ke);
+    tile256_store(As[0], ra);
+    tile256_store(Bs[0], rb);
+  }
+  __syncthreads();
+  int buf = 0;
+  const int fr = lane & 31, fk = 8 * (lane >> 5);
+  for (int k0 = kb; k0 < ke; k0 += 32) {
+    const bool more = k0 + 32 < ke;
+    if (more) {
+      tile256_load(ra, A, sam, m0, k0 + 32, M, ke);
+      tile256_load(rb, B, sbn, n0, k0 + 32, N, ke);
+    }
+#pragma unroll
+    for (int ks = 0; ks < 32; ks += 16) {
+      bf16x8 af[4], bfr[2];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        af[i] = *reinterpret_cast<const bf16x8*>(&As[buf][wm * 128 + i * 32 + fr][ks + fk]);
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][wn * 64 + j * 32 + fr][ks + fk]);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      tile256_store(As[buf ^ 1], ra);
+      tile256_store(Bs[buf ^ 1], rb);
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  // 32x32 accumulator layout: value e of a lane is row 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int row = m0 + wm * 128 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
         if (row >= M || col >= N) continue;
         float v = acc[i][j][e];
         if (ws) {
@@ -433,10 +550,15 @@ extern "C" int mfs_mlp_set_params(mfs_mlp M, int l, const float* W, const float*
   return MFS_OK;
 }
 
+// Both operands contiguous in k and at least one 256x256 tile: the
+// 256x256 kernel.
+static bool gemm_big(int M, int N, int64_t sak, int64_t sbk) { return sak == 1 && sbk == 1 && M >= 256 && N >= 256; }
+
 // Split-K factor: enough workgroups to cover the device when the output has
-// few 128x128 tiles and K is long; every split keeps at least 256 of K.
-static int gemm_splits(int M, int N, int K, int* Kc) {
-  const int64_t tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+// few output tiles (tile x tile) and K is long; every split keeps at least
+// 256 of K.
+static int gemm_splits(int M, int N, int K, int* Kc, int tile = 128) {
+  const int64_t tiles = (int64_t)((M + tile - 1) / tile) * ((N + tile - 1) / tile);
   int sp = (int)std::min<int64_t>(std::max<int64_t>(1, 512 / tiles), std::max(1, K / 256));
   sp = std::min(sp, 64);
   int kc = (K + sp - 1) / sp;
@@ -450,14 +572,24 @@ template <int EPI>
 static void gemm_epi(hipStream_t s, float* ws, size_t ws_bytes, int M, int N, int K, const uint16_t* A, int64_t sam,
                      int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc,
                      const float* bias, uint16_t* Cbf) {
+  const bool big = gemm_big(M, N, sak, sbk);
+  const int tile = big ? 256 : 128;
   int Kc;
-  int sp = gemm_splits(M, N, K, &Kc);
+  int sp = gemm_splits(M, N, K, &Kc, tile);
   if (sp > 1 && (!ws || (size_t)sp * M * N * sizeof(float) > ws_bytes)) {
     sp = 1;  // no room for the partial tiles: one pass over K
     Kc = K;
   }
-  dim3 grid((N + 127) / 128, (M + 127) / 128, sp);
-  if (sp == 1) {
+  dim3 grid((N + tile - 1) / tile, (M + tile - 1) / tile, sp);
+  if (big) {
+    k_gemm_bf16_big<EPI><<<grid, kBigThreads, 0, s>>>(M, N, K, Kc, A, sam, B, sbn, C, ldc, bias, Cbf,
+                                                      sp == 1 ? nullptr : ws);
+    if (sp > 1) {
+      const int64_t mn = (int64_t)M * N;
+      k_splitk_reduce<EPI><<<(unsigned)std::min<int64_t>(4096, (mn + 255) / 256), 256, 0, s>>>(ws, sp, M, N, C, ldc,
+                                                                                               bias, Cbf);
+    }
+  } else if (sp == 1) {
     k_gemm_bf16<EPI><<<grid, 256, 0, s>>>(M, N, K, Kc, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf, nullptr);
   } else {
     k_gemm_bf16<EPI_NONE><<<grid, 256, 0, s>>>(M, N, K, Kc, A, sam, sak, B, sbk, sbn, C, ldc, bias, Cbf, ws);

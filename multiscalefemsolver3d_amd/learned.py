@@ -10,7 +10,7 @@ Pipeline (all data stays in device memory):
      batched CG; the zero guess stays the fallback (a guess worse than zero is
      rejected per problem) and every solve still runs to the same tolerance.
 
-`train` does steps 1-3 once; `solve_learned` is the per-batch hot path
+`train` does steps 1-3 once; `LearnedSolver.solve` is the per-batch hot path
 (features, network inference, guess assembly and the mixed batched solve);
 `run` chains both and compares against the zero and hat guesses.
 """
@@ -48,9 +48,10 @@ class Trained:
 
 
 def train(ctx, n=8, n_train=20000, hidden=256, layers=4, epochs=30, batch=256, lr=1e-3, tol=1e-8, seed_train=4,
-          log=print, tune=False):
+          log=print, tune=False, shape=None):
     """Solve `n_train` cells with the batched mixed solver, train the MLP on
-    (features -> basis corrections) and return it with the launch shapes."""
+    (features -> basis corrections) and return it with the launch shapes
+    (autotuned when `tune`, else `shape` or the library's defaults)."""
     nt = 6 * n ** 3
     nout = 3 * (n - 1) ** 3 * 24
     info = {"n": n, "n_train": n_train, "input": nt, "output": nout, "hidden": hidden, "layers": layers,
@@ -58,7 +59,7 @@ def train(ctx, n=8, n_train=20000, hidden=256, layers=4, epochs=30, batch=256, l
     cn, org, h, E = cells(n_train, n, seed_train)
     Btr = Batch(ctx, cn, org, h, E)
     Btr.assemble()
-    shape = {PREC_MIXED: {}, PREC_F64: {}}
+    shape = dict(shape) if shape else {PREC_MIXED: {}, PREC_F64: {}}
     if tune:
         # launch shapes tuned on the training cells (same subdivision as the
         # test cells) and reused for the test batch

@@ -8,7 +8,8 @@ import oracle as orc
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(70, 33, 17), (64, 128, 64), (3, 200, 129), (300, 264, 260), (70, 3000, 130)])
+@pytest.mark.parametrize("shape", [(70, 33, 17), (64, 128, 64), (3, 200, 129), (300, 264, 260), (70, 3000, 130),
+                                   (600, 1500, 520), (512, 1024, 768)])
 def test_gemm_forward_matches_fp64_on_bf16_operands(ctx, shape):
     """A single linear layer: output = X W^T + b with BF16 operands and FP32
     accumulation equals the FP64 product of the BF16-rounded operands to FP32

@@ -338,11 +338,18 @@ typedef struct {
   int maxit;       /* inner iteration cap per problem                         */
   double inner_tol;/* MIXED: FP32 inner relative tolerance                    */
   int use_graph;   /* capture the persistent launch in a hipGraph             */
-  int wg_size;     /* workgroup size (0 = autotuned)                          */
+  int wg_size;     /* workgroup size (0 = autotuned; 64..256, 512 for
+                      variant 3)                                             */
   int use_guess;   /* start from mfs_batch_set_guess values                   */
   int variant;     /* kernel variant: 0 = autotuned/default; rows carried
                       per thread: FP32 inner 1 -> 2 rows, 2 -> 3 rows;
-                      FP64 inner 1 -> 1 row, 2 -> 2 rows                    */
+                      FP64 inner 1 -> 1 row, 2 -> 2 rows; with r, d, q
+                      kept in registers: 3 -> 1 row (wg_size up to 512),
+                      4 -> 2 rows (these need nint <= rows * wg_size for
+                      every cell, else MFS_ERR_ARG)                         */
+  int fallback_wg_size;  /* MIXED: shape of the FP64 pass over the problems  */
+  int fallback_variant;  /* the FP32 launch could not solve (0 = the FP64
+                            autotuned shape, else as wg_size / variant)      */
 } mfs_batch_solve_desc;
 
 typedef struct {
@@ -420,7 +427,8 @@ int mfs_msfem_downscale(mfs_batch B, const double* u_coarse, int cell, double* u
 int mfs_msfem_error(mfs_batch B, const double* u_coarse, const double* u_fem, double* err3);
 /* Autotuner for mfs_batch_solve in desc->precision: times the whole batched
  * solve for every kernel variant (rows carried per thread), workgroup size
- * (64..256) and workgroups per CU (as many as the kernel's registers and LDS
+ * (64..256, up to 512 for variant 3) and workgroups per CU (as many as the
+ * kernel's registers and LDS
  * allow, or one fewer), keeps the fastest for later solves of this
  * precision and returns it as *best_wg = wg*100 + per_cu*10 + variant
  * (0-based; desc->variant takes it plus one) with

@@ -91,7 +91,8 @@ class BatchDesc(C.Structure):
 
 class BatchSolveDesc(C.Structure):
     _fields_ = [("precision", C.c_int), ("tol", C.c_double), ("maxit", C.c_int), ("inner_tol", C.c_double),
-                ("use_graph", C.c_int), ("wg_size", C.c_int), ("use_guess", C.c_int), ("variant", C.c_int)]
+                ("use_graph", C.c_int), ("wg_size", C.c_int), ("use_guess", C.c_int), ("variant", C.c_int),
+                ("fallback_wg_size", C.c_int), ("fallback_variant", C.c_int)]
 
 
 class BatchReport(C.Structure):
@@ -674,10 +675,11 @@ class BatchResult:
 
 
 def batch_solve_desc(precision=PREC_MIXED, tol=1e-8, maxit=20000, inner_tol=1e-4, use_graph=True, wg_size=0,
-                     use_guess=False, variant=0):
+                     use_guess=False, variant=0, fallback_wg_size=0, fallback_variant=0):
     d = BatchSolveDesc()
     d.precision, d.tol, d.maxit, d.inner_tol = precision, tol, maxit, inner_tol
     d.use_graph, d.wg_size, d.use_guess, d.variant = int(use_graph), wg_size, int(use_guess), variant
+    d.fallback_wg_size, d.fallback_variant = fallback_wg_size, fallback_variant
     return d
 
 

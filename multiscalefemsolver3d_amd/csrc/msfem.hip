@@ -58,11 +58,24 @@ constexpr int MAXN = 12; // largest supported cell subdivision (FP32 inner; FP64
 // Kernel variants per precision, picked by the autotuner: the number of
 // rows a thread carries through the inner-iteration phases at once (more
 // rows keep more loads in flight per wave, fewer rows leave registers for
-// two waves per SIMD).  (A one-row variant with the row's K values loaded
-// in chunks of 5 stencil points was measured and never won: it needs one
-// wave per SIMD.)
-constexpr int kRptF32[2] = {2, 3}, kRptF64[2] = {1, 2};
-constexpr int kVariants[2] = {2, 2};  // per precision: FP32 inner, FP64 inner
+// two waves per SIMD), and whether the per-row vectors r, d, q live in
+// registers (variants 3 and 4; only for batches whose cells all have
+// nint <= rows * workgroup size) or in a global scratch slab.  (Measured
+// and dropped: K rings of 5 stencil points, part of each row's K kept in
+// registers for the whole task, and forcing three or four waves per SIMD;
+// each costs more in spills than it hides in latency.)
+constexpr int kRptF32[4] = {2, 3, 1, 2}, kRptF64[4] = {1, 2, 1, 2};
+constexpr bool kRegVar[4] = {false, false, true, true};
+constexpr int kVariants[2] = {4, 4};  // per precision: FP32 inner, FP64 inner
+// Register-resident variants keep d (the inner correction) in LDS behind p,
+// except the one-row FP32 variant, which has the registers for it.
+constexpr bool d_in_lds(bool fp64, int rpt) { return fp64 || rpt != 1; }
+// Largest workgroup of a variant: one register-resident row per thread
+// needs a thread per interior node, so that form takes up to 512 threads.
+constexpr int max_threads(bool reg, int rpt) { return reg && rpt == 1 ? 512 : 256; }
+// Waves per SIMD a variant is compiled for: two where its registers allow
+// it (the compiler then spills cold code instead of dropping to one).
+constexpr int min_waves(bool fp64, int rpt, bool reg) { return (reg && rpt == 1) || (!reg && rpt <= (fp64 ? 1 : 2)) ? 2 : 1; }
 
 // Stencil offsets (di, dj, dk), sorted by linear offset (dk, dj, di lexicographic)
 __constant__ int c_stencil[NS][3];
@@ -161,7 +174,8 @@ struct SolveArgs {
   int* iters;            // [cell][24]
   double* relres;        // [cell][24]
   int* flags;            // [cell][24]: bit0 not converged, bit1 guess rejected,
-                         // bit2 FP32 inner solve failed, bit3 handed to the FP64 pass
+                         // bit2 FP32 inner solve failed, bit3 handed to the FP64 pass,
+                         // bit4 FP64 solve from the guess failed and was redone from zero
   // FP32 launch: tasks with a failed right-hand side are appended to the
   // FP64 pass's list (count, then cells, then groups); FP64 pass: its task
   // list and count (written by the FP32 launch), queue 0 only
@@ -542,6 +556,165 @@ __device__ __forceinline__ void phase_direction(int nint, const T* __restrict__ 
   }
 }
 
+// stencil_rows with the K values of PF stencil points in flight: point
+// s + PF is requested while point s is applied, from a ring of PF x RPT x 9
+// registers.  The point loop runs in chunks of PF points, each chunk
+// unrolled, so the ring is indexed statically while the code stays one
+// chunk long.  A wave then waits for a K load once per PF points instead of
+// once per point; PF x RPT x 9 stays below the 63 vector loads a wave can
+// have outstanding.  Same arithmetic, in the same order, as stencil_rows.
+template <typename KT, typename VT, int RPT, int PF, typename Load>
+__device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
+                                                const bool (&ok)[RPT], int m, Load load, VT (&acc)[RPT][3][RG]) {
+  static_assert(PF >= 1 && NS % PF == 0 && PF * RPT * 9 <= 63, "prefetch ring: PF must divide 15, <= 63 loads");
+  int vi[RPT], ci[RPT], cj[RPT], ck[RPT];
+  KT ring[PF][RPT][9];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    vi[q] = ok[q] ? v[q] : 0;
+    ci[q] = vi[q] % m + 1;
+    cj[q] = (vi[q] / m) % m + 1;
+    ck[q] = vi[q] / (m * m) + 1;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < RG; r++) acc[q][a][r] = 0;
+  }
+  // The row bases are redefined for the compiler on every call, so the
+  // 135 per-row K addresses are formed here, point by point, instead of
+  // being hoisted out of the iteration loop into registers they would spill.
+  const KT* kp[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    int64_t off = vi[q];
+    asm volatile("" : "+v"(off));
+    kp[q] = Kc + off;
+  }
+#pragma unroll
+  for (int p = 0; p < PF; p++)
+#pragma unroll
+    for (int q = 0; q < RPT; q++)
+#pragma unroll
+      for (int e = 0; e < 9; e++) ring[p][q][e] = kp[q][(int64_t)(p * 9 + e) * nint];
+#pragma unroll 1
+  for (int s0 = 0; s0 < NS; s0 += PF) {
+#pragma unroll
+    for (int p = 0; p < PF; p++) {
+      const int s = s0 + p;
+      VT kv[RPT][9];
+#pragma unroll
+      for (int q = 0; q < RPT; q++)
+#pragma unroll
+        for (int e = 0; e < 9; e++) kv[q][e] = (VT)ring[p][q][e];
+      if (s0 + PF < NS) {
+#pragma unroll
+        for (int q = 0; q < RPT; q++)
+#pragma unroll
+          for (int e = 0; e < 9; e++) ring[p][q][e] = kp[q][(int64_t)((s + PF) * 9 + e) * nint];
+      }
+      const int di = (int)((kOffI >> (2 * s)) & 3u) - 1;
+      const int dj = (int)((kOffJ >> (2 * s)) & 3u) - 1;
+      const int dk = (int)((kOffK >> (2 * s)) & 3u) - 1;
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int ni = ci[q] + di, nj = cj[q] + dj, nk = ck[q] + dk;
+        const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
+        const int u = in ? interior_index(ni, nj, nk, m) : vi[q];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          VT pv[RG];
+          load(u, b, pv);
+#pragma unroll
+          for (int r = 0; r < RG; r++) {
+            acc[q][0][r] += kv[q][0 * 3 + b] * pv[r];
+            acc[q][1][r] += kv[q][1 * 3 + b] * pv[r];
+            acc[q][2][r] += kv[q][2 * 3 + b] * pv[r];
+          }
+        }
+      }
+    }
+  }
+}
+
+// Register-resident forms of the three phases: thread t owns rows
+// t + q * blockDim.x (q < RPT) for the whole task, and keeps their r, q and
+// diagonal inverses in registers (d in registers or in LDS behind p), so an
+// inner iteration reads only K from memory (p stays in LDS: the stencil
+// reads the neighbours' entries).  Rows at or beyond nint (ok false) carry
+// values that are never used.
+template <typename T, int RPT, int PF>
+__device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, const double* __restrict__ K64, int nint,
+                                               int m, const T* P, T (&Qr)[RPT][3][RG], double (&w)[RG]) {
+  constexpr bool fp64 = std::is_same<T, double>::value;
+  auto ld = [&](int u, int b, T (&pv)[RG]) {
+#pragma unroll
+    for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
+  };
+  int vr[RPT];
+  bool ok[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    vr[q] = threadIdx.x + q * blockDim.x;
+    ok[q] = vr[q] < nint;
+  }
+  if constexpr (fp64)
+    stencil_rows_pf<double, double, RPT, PF>(K64, nint, vr, ok, m, ld, Qr);
+  else
+    stencil_rows_pf<float, float, RPT, PF>(K32, nint, vr, ok, m, ld, Qr);
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    if (!ok[q]) continue;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < RG; r++) w[r] += (double)(Qr[q][a][r] * P[((int64_t)vr[q] * 3 + a) * RG + r]);
+  }
+}
+
+template <typename T, int RPT, bool DLDS>
+__device__ __forceinline__ void phase_update_reg(int nint, const T (&dv)[RPT][3], const T* P, const T (&Qr)[RPT][3][RG],
+                                                 T (&Rr)[RPT][3][RG], T* DL, T (&Dr)[RPT][3][RG],
+                                                 const T (&alpha)[RG], double (&w)[2 * RG]) {
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int v = threadIdx.x + q * blockDim.x;
+    if (v >= nint) continue;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < RG; r++) {
+        const T pe = P[((int64_t)v * 3 + a) * RG + r];
+        if constexpr (DLDS)
+          DL[((int64_t)v * 3 + a) * RG + r] += alpha[r] * pe;
+        else
+          Dr[q][a][r] += alpha[r] * pe;
+        const T rv = Rr[q][a][r] - alpha[r] * Qr[q][a][r];
+        Rr[q][a][r] = rv;
+        const T z = dv[q][a] * rv;
+        w[r] += (double)(rv * z);
+        w[RG + r] += (double)(rv * rv);
+      }
+  }
+}
+
+template <typename T, int RPT>
+__device__ __forceinline__ void phase_direction_reg(int nint, const T (&dv)[RPT][3], T* P, const T (&Rr)[RPT][3][RG],
+                                                    const T (&beta)[RG], int on) {
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int v = threadIdx.x + q * blockDim.x;
+    if (v >= nint) continue;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < RG; r++) {
+        if (!((on >> r) & 1)) continue;
+        const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+        P[e] = dv[q][a] * Rr[q][a][r] + beta[r] * P[e];
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Workgroup all-reduce of N per-thread values (N a power of two, 8..32):
 // a butterfly reduce-scatter inside the wave (log2 N + remaining steps of
@@ -557,8 +730,10 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// wg_reduce_to_lanes: the same sums, left in lane k (< N) of every wave
+// (other lanes hold 0), for per-value work done one value per lane.
 template <int N>
-__device__ __forceinline__ void wg_allreduce(double (&v)[N], double* part, double (&tot)[N]) {
+__device__ __forceinline__ double wg_reduce_to_lanes(double (&v)[N], double* part) {
   static_assert(N >= 2 && N <= 32 && (N & (N - 1)) == 0, "N must be a power of two in 2..32");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   constexpr int LOG = N == 2 ? 1 : N == 4 ? 2 : N == 8 ? 3 : N == 16 ? 4 : 5;
@@ -586,9 +761,20 @@ __device__ __forceinline__ void wg_allreduce(double (&v)[N], double* part, doubl
   double s = 0.0;
   if (lane < N)
     for (int w = 0; w < nw; w++) s += part[w * N + lane];
+  return s;
+}
+
+template <int N>
+__device__ __forceinline__ void wg_allreduce(double (&v)[N], double* part, double (&tot)[N]) {
+  const double s = wg_reduce_to_lanes<N>(v, part);
 #pragma unroll
   for (int k = 0; k < N; k++) tot[k] = readlane_f64(s, k);
 }
+
+__device__ __forceinline__ float readlane_t(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane_t(double v, int l) { return readlane_f64(v, l); }
 
 // ---------------------------------------------------------------------------
 // Persistent batched solve.  T = inner precision (float: FP32 inner with FP64
@@ -613,21 +799,40 @@ struct RhsState {
   int itmin[2][RG];     // previous mark, and the inner iteration it was seen
   double ratio[RG];   // previous scale / new scale
   int done[RG], failed[RG], warm[RG], itbase[RG], it[RG];
+  int retried[RG], restart[RG];  // FP64 from a guess: one retry from zero
 };
 
-template <typename T, int RPT>
-__global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
+//
+// REG: r, d, q and the diagonal inverses of a thread's rows live in its
+// registers (rows t + k * blockDim.x, k < RPT; the host launches this form
+// only when every cell has nint <= RPT * blockDim.x).  Otherwise they live
+// in a per-workgroup scratch slab in global memory.
+template <typename T, int RPT, bool REG>
+__global__ __launch_bounds__(max_threads(REG, RPT))
+__attribute__((amdgpu_waves_per_eu(min_waves(std::is_same<T, double>::value, RPT, REG))))
+void k_batch_solve(SolveArgs A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* P = reinterpret_cast<T*>(smem_raw);
-  __shared__ double partA[4 * 2 * RG];
-  __shared__ double partB[4 * 2 * RG];
+  constexpr int kWaves = max_threads(REG, RPT) / 64;
+  __shared__ double partA[kWaves * 2 * RG];
+  __shared__ double partB[kWaves * 2 * RG];
   __shared__ RhsState S;
   __shared__ int task_s;
-  const int tid = threadIdx.x, nth = blockDim.x;
+  const int tid = threadIdx.x, nth = blockDim.x, lane = threadIdx.x & 63;
   constexpr bool fp64 = std::is_same<T, double>::value;
-  T* R = (fp64 ? (T*)A.scr64 : (T*)A.scr32) + (int64_t)blockIdx.x * 3 * A.scr_stride;
-  T* D = R + A.scr_stride;  // inner correction
-  T* Q = D + A.scr_stride;  // A p
+  T* R = REG ? nullptr : (fp64 ? (T*)A.scr64 : (T*)A.scr32) + (int64_t)blockIdx.x * 3 * A.scr_stride;
+  T* D = REG ? nullptr : R + A.scr_stride;  // inner correction
+  T* Q = REG ? nullptr : D + A.scr_stride;  // A p
+  constexpr int RQ = REG ? RPT : 1;
+  T Rr[RQ][3][RG], Qr[RQ][3][RG], dvr[RQ][3];
+  // REG: the inner correction d of the owned rows, in LDS behind p (each
+  // thread touches only its own rows' entries) or in registers
+  constexpr bool kDInLds = REG && d_in_lds(fp64, RPT);
+  T Dr[RQ][3][RG];  // unused (and removed) when d is in LDS
+  T* DL = kDInLds ? P + A.scr_stride : nullptr;
+  // stencil points whose K values are in flight in the register forms
+  constexpr int kPrefetch = fp64 ? 1 : 3;
+
   int tried = 0;  // queues found empty (thread 0)
   const int home = blockIdx.x % NQ;
   for (;;) {
@@ -665,6 +870,14 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
     const double* b64 = A.b64 + C.int0 * (3 * NB);
     double* x64 = A.x64 + C.int0 * (3 * NB);
     const T* dinv = fp64 ? (const T*)(A.dinv64 + C.int0 * 3) : (const T*)(A.dinv32 + C.int0 * 3);
+    if constexpr (REG) {
+#pragma unroll
+      for (int q = 0; q < RQ; q++) {
+        const int v = tid + q * nth;
+#pragma unroll
+        for (int a = 0; a < 3; a++) dvr[q][a] = v < nint ? dinv[v * 3 + a] : (T)0;
+      }
+    }
     const int q0 = grp * RG;
     // the FP64 pass re-solves a task: its outcome replaces the FP32 flags
     // (bit 1, a rejected guess, and bit 3, the hand-over, stay)
@@ -672,9 +885,11 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
     // FP32 inner solves that need more than a few times the FP64 count are
     // handed to the FP64 pass (CG iterations grow like n for these cells)
     const int inner_cap = min(A.inner_cap, fp64 ? A.inner_cap : 20 * n + 60);
-    // an FP32 inner solve whose recursive r.r has not dropped 4x within this
-    // many iterations is stagnating (FP32 cannot resolve the problem)
-    constexpr int kStagWindow = 60;
+    // an inner solve whose recursive r.r has not dropped 4x within this many
+    // iterations is stagnating: FP32 cannot resolve the problem, or (FP64)
+    // CG is stalled on an indefinite operator (the reference's element
+    // convention gives local operators with negative eigenvalues)
+    constexpr int kStagWindow = fp64 ? 1000 : 60;
 
     // ||b||^2 per RHS
     {
@@ -697,6 +912,7 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
           S.warm[r] = 0;
           S.it[r] = 0;
           S.prev[r] = 1e300;
+          S.retried[r] = 0;
           S.scale[r] = 1.0;
         }
     }
@@ -722,6 +938,25 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
 #pragma unroll
           for (int r = 0; r < RG; r++) pv[r] = x64[(int64_t)u * 3 * NB + b * NB + q0 + r];
         };
+        if constexpr (REG) {
+          // one owned row at a time (the accumulators are FP64)
+#pragma unroll
+          for (int q = 0; q < RQ; q++) {
+            const int vr[1] = {tid + q * nth};
+            const bool ok[1] = {vr[0] < nint};
+            double acc[1][3][RG];
+            stencil_rows<double, double, 1>(K64, nint, vr, ok, m, ld, acc);
+            if (!ok[0]) continue;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+              for (int r = 0; r < RG; r++) {
+                const double rv = b64[(int64_t)vr[0] * 3 * NB + a * NB + q0 + r] - acc[0][a][r];
+                w[r] += rv * rv;
+                Rr[q][a][r] = (T)rv;
+              }
+          }
+        } else
         for (int v0 = tid; v0 < nint; v0 += RR * nth) {
           int vr[RR];
           bool ok[RR];
@@ -754,6 +989,21 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
 #pragma unroll
         for (int r = 0; r < RG; r++) bad |= (!(rr64[r] <= S.bb[r])) << r;
         if (bad) {
+          if constexpr (REG) {
+#pragma unroll
+            for (int q = 0; q < RQ; q++) {
+              const int v = tid + q * nth;
+              if (v >= nint) continue;
+#pragma unroll
+              for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int r = 0; r < RG; r++)
+                  if ((bad >> r) & 1) {
+                    x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
+                    Rr[q][a][r] = (T)b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
+                  }
+            }
+          } else
           for (int v = tid; v < nint; v += nth)
             for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -770,11 +1020,22 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
       }
       if (tid == 0)
         for (int r = 0; r < RG; r++) {
-          const double rr = rr64[r];
-          S.rr64[r] = rr;
+          double rr = rr64[r];
           // an outer step that did not reduce the true residual means the
           // inner precision cannot solve this problem: stop it and report it
           if (!S.done[r] && outer > 0 && !(rr < 0.9 * S.prev[r])) S.failed[r] = 1;
+          // FP64 from a guess: a failed problem is solved once more from
+          // zero (CG on an indefinite operator can stall from one start and
+          // converge from another); its residual is then b
+          S.restart[r] = 0;
+          if (fp64 && A.use_guess && S.failed[r] && !S.done[r] && !S.retried[r]) {
+            S.failed[r] = 0;
+            S.retried[r] = 1;
+            S.restart[r] = 1;
+            S.warm[r] = 0;
+            rr = S.bb[r];
+          }
+          S.rr64[r] = rr;
           S.prev[r] = rr;
           // continue the FP32 iteration only if its recursion stayed
           // consistent with the true residual (within 2x in norm)
@@ -790,6 +1051,38 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
           S.itbase[r] = S.it[r];
         }
       __syncthreads();
+      if constexpr (fp64) {
+        int rsm = 0;
+#pragma unroll
+        for (int r = 0; r < RG; r++) rsm |= (S.restart[r] != 0) << r;
+        if (rsm) {
+          // x = 0 and r = b for the restarted problems (own rows / all rows)
+          if constexpr (REG) {
+#pragma unroll
+            for (int q = 0; q < RQ; q++) {
+              const int v = tid + q * nth;
+              if (v >= nint) continue;
+#pragma unroll
+              for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int r = 0; r < RG; r++)
+                  if ((rsm >> r) & 1) {
+                    x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
+                    Rr[q][a][r] = (T)b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
+                  }
+            }
+          } else {
+            for (int v = tid; v < nint; v += nth)
+              for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int r = 0; r < RG; r++)
+                  if ((rsm >> r) & 1) {
+                    x64[(int64_t)v * 3 * NB + a * NB + q0 + r] = 0.0;
+                    R[((int64_t)v * 3 + a) * RG + r] = (T)b64[(int64_t)v * 3 * NB + a * NB + q0 + r];
+                  }
+          }
+        }
+      }
       int live = 0, left = 0;
 #pragma unroll
       for (int r = 0; r < RG; r++)
@@ -809,6 +1102,26 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
         T inv[RG];
 #pragma unroll
         for (int r = 0; r < RG; r++) inv[r] = ((live >> r) & 1) ? (T)(1.0 / S.scale[r]) : (T)0;
+        if constexpr (REG) {
+#pragma unroll
+          for (int q = 0; q < RQ; q++) {
+            if (tid + q * nth >= nint) continue;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+              for (int r = 0; r < RG; r++) {
+                const T rv = ((live >> r) & 1) ? (T)((double)Rr[q][a][r] * (double)inv[r]) : (T)0;
+                Rr[q][a][r] = rv;
+                if constexpr (kDInLds)
+                  DL[((int64_t)(tid + q * nth) * 3 + a) * RG + r] = 0;
+                else
+                  Dr[q][a][r] = 0;
+                const T z = dvr[q][a] * rv;
+                w[r] += (double)(rv * z);
+                w[RG + r] += (double)(rv * rv);
+              }
+          }
+        } else
         for (int v = tid; v < nint; v += nth)
           for (int a = 0; a < 3; a++) {
             const T dv = dinv[v * 3 + a];
@@ -838,6 +1151,21 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
             S.rrmin[0][r] = tot[RG + r];
             S.itmin[0][r] = 0;
           }
+        if constexpr (REG) {
+#pragma unroll
+          for (int q = 0; q < RQ; q++) {
+            const int v = tid + q * nth;
+            if (v >= nint) continue;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+              for (int r = 0; r < RG; r++) {
+                const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+                const T z = dvr[q][a] * Rr[q][a][r];
+                P[e] = beta[r] != (T)0 ? z + beta[r] * P[e] : z;
+              }
+          }
+        } else
         for (int v = tid; v < nint; v += nth)
           for (int a = 0; a < 3; a++) {
             const T dv = dinv[v * 3 + a];
@@ -860,68 +1188,97 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
         // q = K p, p.q
         T alpha[RG];
         {
-          double w[RG], pq[RG];
+          double w[RG];
 #pragma unroll
           for (int r = 0; r < RG; r++) w[r] = 0;
-          phase_spmv<T, RPT>(K32, K64, nint, m, P, Q, w);
-          wg_allreduce<RG>(w, partB, pq);
+          if constexpr (REG)
+            phase_spmv_reg<T, RQ, kPrefetch>(K32, K64, nint, m, P, Qr, w);
+          else
+            phase_spmv<T, RPT>(K32, K64, nint, m, P, Q, w);
+          // alpha of right-hand side r computed in lane r, then broadcast
+          const double pql = wg_reduce_to_lanes<RG>(w, partB);
+          T al = 0;
+          if (lane < RG && ((act >> lane) & 1) && pql != 0.0) al = (T)(S.rz[par][lane] / pql);
 #pragma unroll
-          for (int r = 0; r < RG; r++)
-            alpha[r] = (((act >> r) & 1) && pq[r] != 0.0) ? (T)(S.rz[par][r] / pq[r]) : (T)0;
+          for (int r = 0; r < RG; r++) alpha[r] = readlane_t(al, r);
         }
         // x += alpha p, r -= alpha q, z = dinv r; r.z, r.r
         int act2 = act;
         T beta[RG];
         {
-          double w[2 * RG], tot[2 * RG];
+          double w[2 * RG];
 #pragma unroll
           for (int r = 0; r < 2 * RG; r++) w[r] = 0;
-          phase_update<T, RPT>(nint, dinv, P, Q, R, D, alpha, w);
-          wg_allreduce<2 * RG>(w, partA, tot);
-#pragma unroll
-          for (int r = 0; r < RG; r++) {
-            beta[r] = 0;
-            if (!((act >> r) & 1)) continue;
-            const double rz2 = tot[r], rr2 = tot[RG + r];
-            const bool stop = !(rr2 == rr2) || rr2 <= S.itol2[r] || S.itbase[r] + inner_it + 1 >= A.maxit;
-            bool stag = false;
-            if (!fp64 && !stop) {
-              const int itm = rr2 < 0.25 * S.rrmin[par][r] ? inner_it + 1 : S.itmin[par][r];
+          if constexpr (REG)
+            phase_update_reg<T, RQ, kDInLds>(nint, dvr, P, Qr, Rr, DL, Dr, alpha, w);
+          else
+            phase_update<T, RPT>(nint, dinv, P, Q, R, D, alpha, w);
+          // lane r < RG decides right-hand side r from its r.z (lane r) and
+          // r.r (lane RG + r); the decisions are gathered with ballots
+          const double rzl = wg_reduce_to_lanes<2 * RG>(w, partA);
+          const double rrl = __shfl(rzl, (lane + RG) & 63, 64);
+          const bool mine = lane < RG && ((act >> lane) & 1);
+          bool cont = false, stag = false;
+          T be = 0;
+          if (mine) {
+            const bool stop =
+                !(rrl == rrl) || rrl <= S.itol2[lane] || S.itbase[lane] + inner_it + 1 >= A.maxit;
+            if (!stop) {
+              const int itm = rrl < 0.25 * S.rrmin[par][lane] ? inner_it + 1 : S.itmin[par][lane];
               stag = inner_it + 1 - itm > kStagWindow;
             }
-            if (stop || stag) act2 &= ~(1 << r);
-            else beta[r] = (T)(rz2 / S.rz[par][r]);
-            if (stag) capfail |= 1 << r;  // recorded by thread 0 after the x update
+            cont = !(stop || stag);
+            if (cont) be = (T)(rzl / S.rz[par][lane]);
           }
-          if (tid == 0)
-            for (int r = 0; r < RG; r++) {
-              const bool mark = ((act >> r) & 1) && tot[RG + r] < 0.25 * S.rrmin[par][r];
-              S.rrmin[par ^ 1][r] = mark ? tot[RG + r] : S.rrmin[par][r];
-              S.itmin[par ^ 1][r] = mark ? inner_it + 1 : S.itmin[par][r];
-              if (!((act >> r) & 1)) {
-                S.rz[par ^ 1][r] = S.rz[par][r];
-                continue;
-              }
-              if ((act2 >> r) & 1) {
-                S.rz[par ^ 1][r] = tot[r];
-              } else {
-                // stopped: keep (p, r.z) of this iteration for the next step
-                S.rz[par ^ 1][r] = S.rz[par][r];
-                S.rzkeep[r] = S.rz[par][r];
-                S.it[r] = S.itbase[r] + inner_it + 1;
-                S.warm[r] = tot[RG + r] <= S.itol2[r];
-                S.rrpred[r] = tot[RG + r];
-              }
+          act2 = (int)(__ballot(cont) & ((1u << RG) - 1));
+          capfail |= (int)(__ballot(stag) & ((1u << RG) - 1));  // recorded after the x update
+#pragma unroll
+          for (int r = 0; r < RG; r++) beta[r] = readlane_t(be, r);
+          if (tid < RG) {
+            const int r = tid;
+            const bool on = (act >> r) & 1;
+            const bool mark = on && rrl < 0.25 * S.rrmin[par][r];
+            S.rrmin[par ^ 1][r] = mark ? rrl : S.rrmin[par][r];
+            S.itmin[par ^ 1][r] = mark ? inner_it + 1 : S.itmin[par][r];
+            if (!on) {
+              S.rz[par ^ 1][r] = S.rz[par][r];
+            } else if (cont) {
+              S.rz[par ^ 1][r] = rzl;
+            } else {
+              // stopped: keep (p, r.z) of this iteration for the next step
+              S.rz[par ^ 1][r] = S.rz[par][r];
+              S.rzkeep[r] = S.rz[par][r];
+              S.it[r] = S.itbase[r] + inner_it + 1;
+              S.warm[r] = rrl <= S.itol2[r];
+              S.rrpred[r] = rrl;
             }
+          }
         }
         // p = z + beta p (continuing RHS only)
-        phase_direction<T, RPT>(nint, dinv, P, R, beta, act2);
+        if constexpr (REG)
+          phase_direction_reg<T, RQ>(nint, dvr, P, Rr, beta, act2);
+        else
+          phase_direction<T, RPT>(nint, dinv, P, R, beta, act2);
         __syncthreads();
         act = act2;
         par ^= 1;
         inner_it++;
       }
       // x64 += scale * d (not for problems whose inner solve failed)
+      if constexpr (REG) {
+#pragma unroll
+        for (int q = 0; q < RQ; q++) {
+          const int v = tid + q * nth;
+          if (v >= nint) continue;
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int r = 0; r < RG; r++)
+              if (((live >> r) & 1) && !((capfail >> r) & 1) && !S.failed[r])
+                x64[(int64_t)v * 3 * NB + a * NB + q0 + r] +=
+                    S.scale[r] * (double)(kDInLds ? DL[((int64_t)v * 3 + a) * RG + r] : Dr[q][a][r]);
+        }
+      } else
       for (int v = tid; v < nint; v += nth)
         for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -946,6 +1303,7 @@ __global__ __launch_bounds__(256) void k_batch_solve(SolveArgs A) {
       A.relres[cell * NB + q0 + r] = sqrt(S.rr64[r] / S.bb[r]);
       if (!(S.rr64[r] <= A.tol * A.tol * S.bb[r])) A.flags[cell * NB + q0 + r] |= 1;
       if (S.failed[r]) A.flags[cell * NB + q0 + r] |= A.fb_count ? (4 | 8) : 4;
+      if (S.retried[r]) A.flags[cell * NB + q0 + r] |= 16;
     }
     if (A.fb_count && tid == 0) {
       bool any = false;
@@ -1807,15 +2165,18 @@ static void boundary_functions(const mfs_batch_s* B, const std::vector<double>& 
       }
 }
 
-static size_t lds_bytes(mfs_batch B, int prec) {
-  return (size_t)B->max_nint * 3 * RG * (prec == MFS_PREC_F64 ? 8 : 4);
+// p, and for the register-resident variants also d, per workgroup
+static size_t lds_bytes(mfs_batch B, int prec, int var) {
+  const int rpt = prec == MFS_PREC_F64 ? kRptF64[var] : kRptF32[var];
+  const bool dl = kRegVar[var] && d_in_lds(prec == MFS_PREC_F64, rpt);
+  return (size_t)B->max_nint * 3 * RG * (prec == MFS_PREC_F64 ? 8 : 4) * (dl ? 2 : 1);
 }
 
 // Per-workgroup scratch for a launch shape; must run before any capture.
-static int batch_scratch(mfs_batch B, int precision, int per_cu) {
+static int batch_scratch(mfs_batch B, int precision, int per_cu, int var) {
   const int grid = B->ctx->cu_count * per_cu;
   const int64_t stride = (int64_t)B->max_nint * 3 * RG;
-  const size_t need = (size_t)grid * 3 * stride * (precision == MFS_PREC_F64 ? 8 : 4);
+  const size_t need = kRegVar[var] ? 0 : (size_t)grid * 3 * stride * (precision == MFS_PREC_F64 ? 8 : 4);
   if (need > B->scr_bytes) {
     MFS_HIP(hipStreamSynchronize(B->ctx->stream));
     dfree(B->d_scr);
@@ -1830,9 +2191,28 @@ static int batch_scratch(mfs_batch B, int precision, int per_cu) {
 // Launch reset + persistent solve on the context stream (capturable: no
 // allocation, no memset; the queue and flags are cleared by a kernel).
 static const void* batch_kernel(int precision, int var) {
-  if (precision == MFS_PREC_F64)
-    return var == 0 ? (const void*)k_batch_solve<double, kRptF64[0]> : (const void*)k_batch_solve<double, kRptF64[1]>;
-  return var == 0 ? (const void*)k_batch_solve<float, kRptF32[0]> : (const void*)k_batch_solve<float, kRptF32[1]>;
+  if (precision == MFS_PREC_F64) {
+    switch (var) {
+      case 0: return (const void*)k_batch_solve<double, kRptF64[0], kRegVar[0]>;
+      case 1: return (const void*)k_batch_solve<double, kRptF64[1], kRegVar[1]>;
+      case 2: return (const void*)k_batch_solve<double, kRptF64[2], kRegVar[2]>;
+      default: return (const void*)k_batch_solve<double, kRptF64[3], kRegVar[3]>;
+    }
+  }
+  switch (var) {
+    case 0: return (const void*)k_batch_solve<float, kRptF32[0], kRegVar[0]>;
+    case 1: return (const void*)k_batch_solve<float, kRptF32[1], kRegVar[1]>;
+    case 2: return (const void*)k_batch_solve<float, kRptF32[2], kRegVar[2]>;
+    default: return (const void*)k_batch_solve<float, kRptF32[3], kRegVar[3]>;
+  }
+}
+
+// A variant takes workgroups up to its thread bound; the register-resident
+// ones also need every cell's rows covered by the workgroup's threads.
+static bool variant_fits(mfs_batch B, int precision, int var, int wg) {
+  const int rpt = precision == MFS_PREC_F64 ? kRptF64[var] : kRptF32[var];
+  if (wg > max_threads(kRegVar[var], rpt)) return false;
+  return !kRegVar[var] || (B->max_nint <= rpt * wg && lds_bytes(B, precision, var) <= 160 * 1024 - 8192);
 }
 
 // pass: BL_MAIN = the whole batch, BL_MAIN_LIST = the whole batch in FP32,
@@ -1843,9 +2223,14 @@ enum { BL_MAIN = 0, BL_MAIN_LIST = 1, BL_FP64_FALLBACK = 2 };
 static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int per_cu, int var, int pass) {
   hipStream_t s = B->ctx->stream;
   const int grid = B->ctx->cu_count * per_cu;
-  const size_t lds = lds_bytes(B, d->precision);
   const int64_t stride = (int64_t)B->max_nint * 3 * RG;
-  const size_t need = (size_t)grid * 3 * stride * (d->precision == MFS_PREC_F64 ? 8 : 4);
+  if (var < 0 || var >= kVariants[d->precision == MFS_PREC_F64 ? 1 : 0] || !variant_fits(B, d->precision, var, wg)) {
+    set_error("mfs_batch_solve: variants 3 and 4 (register-resident rows) need nint <= rows per thread x workgroup "
+              "size for every cell and twice the LDS; workgroups above 256 threads are for variant 3 only");
+    return MFS_ERR_ARG;
+  }
+  const size_t lds = lds_bytes(B, d->precision, var);
+  const size_t need = kRegVar[var] ? 0 : (size_t)grid * 3 * stride * (d->precision == MFS_PREC_F64 ? 8 : 4);
   if (need > B->scr_bytes) {
     set_error("mfs_batch_solve: scratch not allocated for this launch shape");
     return MFS_ERR_ARG;
@@ -1890,18 +2275,8 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   // count) are cleared by a kernel: the launch stays capturable
   k_batch_reset<<<std::max(1, (B->ncells * NB + 255) / 256), 256, 0, s>>>(
       B->d_queue, B->d_flags, fb ? 0 : B->ncells * NB, list_failures ? B->d_fb : nullptr);
-  if (d->precision == MFS_PREC_F64) {
-    if (var == 0)
-      k_batch_solve<double, kRptF64[0]><<<grid, wg, lds, s>>>(A);
-    else
-      k_batch_solve<double, kRptF64[1]><<<grid, wg, lds, s>>>(A);
-  } else {
-    if (var == 0)
-      k_batch_solve<float, kRptF32[0]><<<grid, wg, lds, s>>>(A);
-    else
-      k_batch_solve<float, kRptF32[1]><<<grid, wg, lds, s>>>(A);
-  }
-  MFS_HIP(hipGetLastError());
+  void* args[] = {&A};
+  MFS_HIP(hipLaunchKernel(batch_kernel(d->precision, var), dim3(grid), dim3(wg), args, lds, s));
   return MFS_OK;
 }
 
@@ -1910,11 +2285,15 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
   MFS_CHECK_ARG(B->assembled, "mfs_batch_solve: call mfs_batch_assemble first");
   MFS_CHECK_ARG(d->precision == MFS_PREC_F64 || d->precision == MFS_PREC_MIXED, "mfs_batch_solve: bad precision");
   MFS_CHECK_ARG(d->tol > 0, "mfs_batch_solve: tol must be positive");
-  MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 256),
-                "mfs_batch_solve: wg_size must be a multiple of 64 in 64..256");
+  MFS_CHECK_ARG(d->wg_size == 0 || (d->wg_size % 64 == 0 && d->wg_size >= 64 && d->wg_size <= 512),
+                "mfs_batch_solve: wg_size must be a multiple of 64 in 64..512");
   MFS_CHECK_ARG(d->variant >= 0 && d->variant <= kVariants[d->precision == MFS_PREC_F64 ? 1 : 0],
-                "mfs_batch_solve: variant must be 0 (autotuned), 1 or 2");
-  if (lds_bytes(B, d->precision) > 160 * 1024 - 8192) {
+                "mfs_batch_solve: variant must be 0 (autotuned) or 1..4");
+  MFS_CHECK_ARG(d->fallback_variant >= 0 && d->fallback_variant <= kVariants[1] &&
+                    (d->fallback_wg_size == 0 ||
+                     (d->fallback_wg_size % 64 == 0 && d->fallback_wg_size >= 64 && d->fallback_wg_size <= 512)),
+                "mfs_batch_solve: fallback_variant must be 0..4 and fallback_wg_size a multiple of 64 in 64..512");
+  if (lds_bytes(B, d->precision, 0) > 160 * 1024 - 8192) {
     set_error("mfs_batch_solve: cells too large for the LDS-resident search directions (FP64 inner needs n <= 10)");
     return MFS_ERR_UNSUPPORTED;
   }
@@ -1924,7 +2303,7 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
 // Workgroups of size wg that fit one CU at once (registers and LDS), at
 // least 1.
 static int resident_per_cu(mfs_batch B, int precision, int wg, int var) {
-  const size_t lds = lds_bytes(B, precision);
+  const size_t lds = lds_bytes(B, precision, var);
   const void* fn = batch_kernel(precision, var);
   (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int nb = 0;
@@ -1982,18 +2361,18 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
   // launch lists those tasks on the device and the FP64 pass, queued right
   // behind it, reads the list: no host round trip between the two.
   // the FP64 pass needs its search directions in LDS (cells with n <= 10)
-  const bool mixed = d->precision == MFS_PREC_MIXED && lds_bytes(B, MFS_PREC_F64) <= 160 * 1024 - 8192;
+  const bool mixed = d->precision == MFS_PREC_MIXED && lds_bytes(B, MFS_PREC_F64, 0) <= 160 * 1024 - 8192;
   mfs_batch_solve_desc d64 = *d;
   d64.precision = MFS_PREC_F64;
   d64.use_guess = 1;
-  d64.variant = 0;
-  d64.wg_size = 0;
+  d64.variant = d->fallback_variant;
+  d64.wg_size = d->fallback_wg_size;
   int wg64 = 0, pc64 = 0, var64 = 0;
   if (mixed) {
     MFS_TRY(pick_shape(B, &d64, &wg64, &pc64, &var64));
-    MFS_TRY(batch_scratch(B, MFS_PREC_F64, pc64));
+    MFS_TRY(batch_scratch(B, MFS_PREC_F64, pc64, var64));
   }
-  MFS_TRY(batch_scratch(B, d->precision, pc));
+  MFS_TRY(batch_scratch(B, d->precision, pc, var));
   hipStream_t s = B->ctx->stream;
   hipEvent_t ev[3];
   for (auto& e : ev) MFS_HIP(hipEventCreate(&e));
@@ -2058,7 +2437,8 @@ extern "C" int mfs_batch_solve(mfs_batch B, const mfs_batch_solve_desc* d, mfs_b
 }
 
 // Autotuner: times the whole batched solve for each kernel variant (rows
-// carried per thread), each candidate workgroup size (64..256 threads) and
+// carried per thread, slab or registers), each workgroup size the variant
+// takes (64..256 threads; up to 512 for one register row per thread) and
 // as many workgroups per CU as fit at once (from the kernel's register and
 // LDS use) or one fewer, and keeps the fastest shape for this precision.
 extern "C" int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* d, int* best_wg, double* best_ms) {
@@ -2067,7 +2447,8 @@ extern "C" int mfs_batch_autotune(mfs_batch B, const mfs_batch_solve_desc* d, in
   double best = 1e30;
   int bw = 256, bp = 1, bv = 1;
   for (int var = 0; var < kVariants[t]; var++)
-  for (int wg : {64, 128, 192, 256}) {
+  for (int wg : {64, 128, 192, 256, 320, 384, 448, 512}) {
+    if (!variant_fits(B, d->precision, var, wg)) continue;
     const int res = resident_per_cu(B, d->precision, wg, var);
     for (int pc : {res, res - 1}) {
       if (pc < 1 || (pc == res - 1 && res == 1)) continue;

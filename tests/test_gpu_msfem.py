@@ -39,11 +39,13 @@ def test_bases_match_oracle(ctx, small, precision):
 
 @pytest.mark.parametrize("precision,variant,wg", [
     (p, v, w) for p in (mfs.PREC_MIXED, mfs.PREC_F64)
-    for v, w in ((1, 64), (1, 256), (2, 64), (2, 192))])
+    for v, w in ((1, 64), (1, 256), (2, 64), (2, 192), (3, 64), (3, 512), (4, 64), (4, 192))])
 def test_kernel_variants_and_shapes_match_oracle(ctx, small, precision, variant, wg):
-    """Every kernel variant (rows carried per thread) at small and large
-    workgroups, where rows per thread exceed or fall short of the variant's
-    interleave, gives the oracle's bases."""
+    """Every kernel variant (rows carried per thread; variants 3 and 4 keep
+    the rows' vectors in registers) at small and large workgroups, where
+    rows per thread exceed or fall short of the variant's interleave (with
+    192 or 512 threads most threads of variants 3 and 4 own no row of these
+    cells), gives the oracle's bases."""
     p, B = small
     rep = B.solve(precision=precision, tol=1e-10, use_graph=False, variant=variant, wg_size=wg)
     assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-10
@@ -54,10 +56,58 @@ def test_kernel_variants_and_shapes_match_oracle(ctx, small, precision, variant,
         assert err < 1e-8, (c, err)
 
 
+def test_register_variant_rejects_cells_larger_than_the_workgroup(ctx):
+    """Variants 3 and 4 hold each row of a cell in one thread's registers: a
+    cell of 8^3 hexes (343 interior nodes) does not fit 256 threads x 1 row
+    or 64 threads x 2 rows, and such solves are refused instead of run; the
+    shapes that fit give the slab variant's solution in both precisions."""
+    p = problems.msfem_cells(1, n_fixed=8, seed_E=3)
+    B = make_batch(ctx, p)
+    for var, wg in ((3, 256), (4, 64), (4, 384)):
+        with pytest.raises(mfs.MfsError, match="register-resident"):
+            B.solve(precision=mfs.PREC_MIXED, tol=1e-8, variant=var, wg_size=wg)
+    B.solve(precision=mfs.PREC_F64, tol=1e-10, variant=1, wg_size=256)
+    x1 = B.basis(0)
+    for prec in (mfs.PREC_MIXED, mfs.PREC_F64):
+        for var, wg in ((3, 384), (3, 512), (4, 192), (4, 256)):
+            rep = B.solve(precision=prec, tol=1e-10, variant=var, wg_size=wg)
+            assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-10
+            x = B.basis(0)
+            assert np.linalg.norm(x - x1) / np.linalg.norm(x1) < 1e-8, (prec, var, wg)
+    B.close()
+
+
+def test_fp64_pass_retries_a_stalled_problem_from_zero(ctx):
+    """A cell (8^3 hexes, reference element convention: indefinite local
+    operators) whose FP32 solves fail for some bases; the FP64 pass starts
+    from where FP32 stopped, and with the one-row register variant one of
+    those starts made FP64 CG stall at 2.7e-8 until maxit.  The pass now
+    detects the stall, restarts that problem from zero (flag bit 16) and
+    every basis reaches the tolerance, matching a plain FP64 solve."""
+    from multiscalefemsolver3d_amd import learned
+    c = 18263  # of learned.cells(20000, 8, seed 4), the batch that showed it
+    E = mfs.field_lognormal(6 * 8 ** 3, 4, 1000 + c, 0, learned.MU_LOGE, learned.SIGMA_LOGE)
+    B = mfs.Batch(ctx, [8], [[c / 32.0, 0.0, 0.0]], 1.0 / 32, E)
+    B.assemble()
+    with pytest.raises(mfs.MfsError, match="fallback_variant"):
+        B.solve(precision=mfs.PREC_MIXED, tol=1e-8, fallback_variant=5)
+    rep = B.solve(precision=mfs.PREC_MIXED, tol=1e-8, variant=3, wg_size=384, fallback_variant=3,
+                  fallback_wg_size=384)
+    assert rep.n_not_converged == 0 and rep.max_true_relres <= 1e-8
+    assert rep.n_precision_fallback > 0
+    it, rr, fl = B.status()
+    assert (fl & 1).sum() == 0 and rr.max() <= 1e-8
+    x = B.basis(0)
+    B.solve(precision=mfs.PREC_F64, tol=1e-10, variant=1, wg_size=256)
+    x64 = B.basis(0)
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-5
+    B.close()
+
+
 def test_autotuner_shape_is_used_and_consistent(ctx, small):
     p, B = small
     wg, per_cu, var, ms = B.autotune(precision=mfs.PREC_MIXED, tol=1e-10)
-    assert wg in (64, 128, 192, 256) and per_cu >= 1 and var in (0, 1) and ms > 0
+    assert wg in (64, 128, 192, 256) and per_cu >= 1 and var in (0, 1, 2) and ms > 0
     B.solve(precision=mfs.PREC_MIXED, tol=1e-10)
     tuned = np.concatenate([B.basis(c).ravel() for c in range(B.ncells)])
     B.solve(precision=mfs.PREC_F64, tol=1e-10, variant=1, wg_size=256)

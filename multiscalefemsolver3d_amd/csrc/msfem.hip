@@ -351,49 +351,20 @@ constexpr uint32_t pack_offsets(int dim) {
 }
 constexpr uint32_t kOffI = pack_offsets(0), kOffJ = pack_offsets(1), kOffK = pack_offsets(2);
 
-template <typename KT, typename VT, typename Load>
-__device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint, int v, int i, int j, int k, int m,
-                                            Load load, VT (&acc)[3][RG]) {
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int r = 0; r < RG; r++) acc[a][r] = 0;
-  // The 9 values of stencil point s+1 are loaded while point s is applied
-  // (one point of prefetch keeps the register count at two waves per SIMD).
-  const KT* kp = Kc + v;
-  KT kn[9];
-#pragma unroll
-  for (int e = 0; e < 9; e++) kn[e] = kp[(int64_t)e * nint];
-#pragma unroll 1
-  for (int s = 0; s < NS; s++) {
-    VT kv[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) kv[e] = (VT)kn[e];
-    if (s + 1 < NS) {
-      kp += (int64_t)9 * nint;
-#pragma unroll
-      for (int e = 0; e < 9; e++) kn[e] = kp[(int64_t)e * nint];
-    }
-    // Blocks coupling to boundary nodes are stored as zeros; their column
-    // index is replaced by the row itself, so no lane skips a point.
-    const int ni = i + (int)((kOffI >> (2 * s)) & 3u) - 1;
-    const int nj = j + (int)((kOffJ >> (2 * s)) & 3u) - 1;
-    const int nk = k + (int)((kOffK >> (2 * s)) & 3u) - 1;
-    const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
-    const int u = in ? interior_index(ni, nj, nk, m) : v;
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-      VT pv[RG];
-      load(u, b, pv);
-#pragma unroll
-      for (int r = 0; r < RG; r++) {
-        acc[0][r] += kv[0 * 3 + b] * pv[r];
-        acc[1][r] += kv[1 * 3 + b] * pv[r];
-        acc[2][r] += kv[2 * 3 + b] * pv[r];
-      }
-    }
-  }
+// Row offset of stencil point s in the cell's interior numbering
+// (x fastest, m interior nodes per edge): di + m (dj + m dk).
+__device__ __forceinline__ int stencil_offset(int s, int m) {
+  const int di = (int)((kOffI >> (2 * s)) & 3u) - 1;
+  const int dj = (int)((kOffJ >> (2 * s)) & 3u) - 1;
+  const int dk = (int)((kOffK >> (2 * s)) & 3u) - 1;
+  return di + m * (dj + m * dk);
 }
+// The row whose p (or x) a stencil point reads.  An interior neighbour is
+// row v + off.  A neighbour on the cell boundary has a zero K block (the
+// cell assembly stores 0 there), so any row of the cell may be read for it:
+// the sum clamped to the cell's rows is one instruction instead of the
+// neighbour's coordinates, its bounds test and a select.
+__device__ __forceinline__ int neighbour_row(int v, int off, int nint) { return min(max(v + off, 0), nint - 1); }
 
 // The same product for RPT rows of one thread at once (rows v[k]; rows with
 // ok[k] false are computed on row 0 and discarded by the caller).  The K
@@ -403,15 +374,12 @@ __device__ __forceinline__ void stencil_row(const KT* __restrict__ Kc, int nint,
 template <typename KT, typename VT, int RPT, typename Load>
 __device__ __forceinline__ void stencil_rows(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
                                              const bool (&ok)[RPT], int m, Load load, VT (&acc)[RPT][3][RG]) {
-  int vi[RPT], ci[RPT], cj[RPT], ck[RPT];
+  int vi[RPT];
   const KT* kp[RPT];
   KT kn[RPT][9];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     vi[q] = ok[q] ? v[q] : 0;
-    ci[q] = vi[q] % m + 1;
-    cj[q] = (vi[q] / m) % m + 1;
-    ck[q] = vi[q] / (m * m) + 1;
     kp[q] = Kc + vi[q];
 #pragma unroll
     for (int e = 0; e < 9; e++) kn[q][e] = kp[q][(int64_t)e * nint];
@@ -435,14 +403,10 @@ __device__ __forceinline__ void stencil_rows(const KT* __restrict__ Kc, int nint
         for (int e = 0; e < 9; e++) kn[q][e] = kp[q][(int64_t)e * nint];
       }
     }
-    const int di = (int)((kOffI >> (2 * s)) & 3u) - 1;
-    const int dj = (int)((kOffJ >> (2 * s)) & 3u) - 1;
-    const int dk = (int)((kOffK >> (2 * s)) & 3u) - 1;
+    const int off = stencil_offset(s, m);
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
-      const int ni = ci[q] + di, nj = cj[q] + dj, nk = ck[q] + dk;
-      const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
-      const int u = in ? interior_index(ni, nj, nk, m) : vi[q];
+      const int u = neighbour_row(vi[q], off, nint);
 #pragma unroll
       for (int b = 0; b < 3; b++) {
         VT pv[RG];
@@ -557,8 +521,8 @@ __device__ __forceinline__ void phase_direction(int nint, const T* __restrict__ 
 }
 
 // stencil_rows with the K values of PF stencil points in flight: point
-// s + PF is requested while point s is applied, from a ring of PF x RPT x 9
-// registers.  The point loop runs in chunks of PF points, each chunk
+// s + PF is requested right after point s is applied, into the ring slot
+// point s used (PF x RPT x 9 registers).  The point loop runs in chunks of PF points, each chunk
 // unrolled, so the ring is indexed statically while the code stays one
 // chunk long.  A wave then waits for a K load once per PF points instead of
 // once per point; PF x RPT x 9 stays below the 63 vector loads a wave can
@@ -567,14 +531,11 @@ template <typename KT, typename VT, int RPT, int PF, typename Load>
 __device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
                                                 const bool (&ok)[RPT], int m, Load load, VT (&acc)[RPT][3][RG]) {
   static_assert(PF >= 1 && NS % PF == 0 && PF * RPT * 9 <= 63, "prefetch ring: PF must divide 15, <= 63 loads");
-  int vi[RPT], ci[RPT], cj[RPT], ck[RPT];
+  int vi[RPT];
   KT ring[PF][RPT][9];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     vi[q] = ok[q] ? v[q] : 0;
-    ci[q] = vi[q] % m + 1;
-    cj[q] = (vi[q] / m) % m + 1;
-    ck[q] = vi[q] / (m * m) + 1;
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -601,36 +562,29 @@ __device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int n
 #pragma unroll
     for (int p = 0; p < PF; p++) {
       const int s = s0 + p;
-      VT kv[RPT][9];
-#pragma unroll
-      for (int q = 0; q < RPT; q++)
-#pragma unroll
-        for (int e = 0; e < 9; e++) kv[q][e] = (VT)ring[p][q][e];
-      if (s0 + PF < NS) {
-#pragma unroll
-        for (int q = 0; q < RPT; q++)
-#pragma unroll
-          for (int e = 0; e < 9; e++) ring[p][q][e] = kp[q][(int64_t)((s + PF) * 9 + e) * nint];
-      }
-      const int di = (int)((kOffI >> (2 * s)) & 3u) - 1;
-      const int dj = (int)((kOffJ >> (2 * s)) & 3u) - 1;
-      const int dk = (int)((kOffK >> (2 * s)) & 3u) - 1;
+      const int off = stencil_offset(s, m);
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
-        const int ni = ci[q] + di, nj = cj[q] + dj, nk = ck[q] + dk;
-        const bool in = ni >= 1 && ni <= m && nj >= 1 && nj <= m && nk >= 1 && nk <= m;
-        const int u = in ? interior_index(ni, nj, nk, m) : vi[q];
+        const int u = neighbour_row(vi[q], off, nint);
 #pragma unroll
         for (int b = 0; b < 3; b++) {
           VT pv[RG];
           load(u, b, pv);
 #pragma unroll
           for (int r = 0; r < RG; r++) {
-            acc[q][0][r] += kv[q][0 * 3 + b] * pv[r];
-            acc[q][1][r] += kv[q][1 * 3 + b] * pv[r];
-            acc[q][2][r] += kv[q][2 * 3 + b] * pv[r];
+            acc[q][0][r] += (VT)ring[p][q][0 * 3 + b] * pv[r];
+            acc[q][1][r] += (VT)ring[p][q][1 * 3 + b] * pv[r];
+            acc[q][2][r] += (VT)ring[p][q][2 * 3 + b] * pv[r];
           }
         }
+      }
+      // the slot just used is refilled with the point PF later (requesting
+      // it after the point's products needs no copy of the slot)
+      if (s0 + PF < NS) {
+#pragma unroll
+        for (int q = 0; q < RPT; q++)
+#pragma unroll
+          for (int e = 0; e < 9; e++) ring[p][q][e] = kp[q][(int64_t)((s + PF) * 9 + e) * nint];
       }
     }
   }
@@ -747,8 +701,15 @@ __device__ __forceinline__ double wg_reduce_to_lanes(double (&v)[N], double* par
     const bool up = (lane & o) != 0;
 #pragma unroll
     for (int i = 0; i < h; i++) {
-      const double send = up ? a[i] : a[h + i];
-      const double keep = up ? a[h + i] : a[i];
+      // the pair is made opaque first: otherwise the compiler turns
+      // "up ? a[i] : a[h + i]" into a[up ? i : h + i], an array read with a
+      // lane-dependent index, and lowers it to a compare-and-select chain
+      // over all N values (about 1,000 extra VALU instructions per
+      // reduction of 16 values)
+      double lo = a[i], hi = a[h + i];
+      asm volatile("" : "+v"(lo), "+v"(hi));
+      const double send = up ? lo : hi;
+      const double keep = up ? hi : lo;
       a[i] = keep + __shfl_xor(send, o, 64);
     }
   }

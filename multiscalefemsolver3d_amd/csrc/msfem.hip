@@ -615,13 +615,18 @@ __device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, co
     stencil_rows_pf<double, double, RPT, PF>(K64, nint, vr, ok, m, ld, Qr);
   else
     stencil_rows_pf<float, float, RPT, PF>(K32, nint, vr, ok, m, ld, Qr);
+  // the row's three components are summed in T before the FP64 partial
+  // (one conversion and FP64 add per right-hand side instead of three)
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     if (!ok[q]) continue;
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int r = 0; r < RG; r++) {
+      T t = 0;
 #pragma unroll
-      for (int r = 0; r < RG; r++) w[r] += (double)(Qr[q][a][r] * P[((int64_t)vr[q] * 3 + a) * RG + r]);
+      for (int a = 0; a < 3; a++) t += Qr[q][a][r] * P[((int64_t)vr[q] * 3 + a) * RG + r];
+      w[r] += (double)t;
+    }
   }
 }
 
@@ -633,6 +638,9 @@ __device__ __forceinline__ void phase_update_reg(int nint, const T (&dv)[RPT][3]
   for (int q = 0; q < RPT; q++) {
     const int v = threadIdx.x + q * blockDim.x;
     if (v >= nint) continue;
+    T tz[RG], tr[RG];  // the row's three components summed in T first
+#pragma unroll
+    for (int r = 0; r < RG; r++) tz[r] = tr[r] = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -645,9 +653,14 @@ __device__ __forceinline__ void phase_update_reg(int nint, const T (&dv)[RPT][3]
         const T rv = Rr[q][a][r] - alpha[r] * Qr[q][a][r];
         Rr[q][a][r] = rv;
         const T z = dv[q][a] * rv;
-        w[r] += (double)(rv * z);
-        w[RG + r] += (double)(rv * rv);
+        tz[r] += rv * z;
+        tr[r] += rv * rv;
       }
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+      w[r] += (double)tz[r];
+      w[RG + r] += (double)tr[r];
+    }
   }
 }
 

@@ -351,6 +351,20 @@ constexpr uint32_t pack_offsets(int dim) {
 }
 constexpr uint32_t kOffI = pack_offsets(0), kOffJ = pack_offsets(1), kOffK = pack_offsets(2);
 
+// Loads through a raw buffer resource (gfx9 descriptor word 3; the range
+// check returns 0 beyond num_records).
+constexpr int kBufferRsrcWord3 = 0x00020000;
+template <typename KT>
+__device__ __forceinline__ KT buffer_load(__amdgpu_buffer_rsrc_t rs, int voff, int soff);
+template <>
+__device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t rs, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+}
+
 // Row offset of stencil point s in the cell's interior numbering
 // (x fastest, m interior nodes per edge): di + m (dj + m dk).
 __device__ __forceinline__ int stencil_offset(int s, int m) {
@@ -541,22 +555,25 @@ __device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int n
 #pragma unroll
       for (int r = 0; r < RG; r++) acc[q][a][r] = 0;
   }
-  // The row bases are redefined for the compiler on every call, so the
-  // 135 per-row K addresses are formed here, point by point, instead of
-  // being hoisted out of the iteration loop into registers they would spill.
-  const KT* kp[RPT];
+  // K is read through a buffer resource over the cell's matrix: the row's
+  // byte offset is the (per-thread) VGPR offset and the entry's byte offset
+  // (s*9+e)*nint*sizeof(KT) the (uniform) SGPR offset, so a load costs no
+  // vector address arithmetic.  The stride is redefined for the compiler
+  // on every call, so the 135 entry offsets are formed here, point by
+  // point, instead of being hoisted out of the iteration loop.
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<KT*>(Kc), (short)0, NS * 9 * nint * (int)sizeof(KT), kBufferRsrcWord3);
+  int vo[RPT];
 #pragma unroll
-  for (int q = 0; q < RPT; q++) {
-    int64_t off = vi[q];
-    asm volatile("" : "+v"(off));
-    kp[q] = Kc + off;
-  }
+  for (int q = 0; q < RPT; q++) vo[q] = vi[q] * (int)sizeof(KT);
+  int stride = nint * (int)sizeof(KT);
+  asm volatile("" : "+s"(stride));
 #pragma unroll
   for (int p = 0; p < PF; p++)
 #pragma unroll
     for (int q = 0; q < RPT; q++)
 #pragma unroll
-      for (int e = 0; e < 9; e++) ring[p][q][e] = kp[q][(int64_t)(p * 9 + e) * nint];
+      for (int e = 0; e < 9; e++) ring[p][q][e] = buffer_load<KT>(rs, vo[q], (p * 9 + e) * stride);
 #pragma unroll 1
   for (int s0 = 0; s0 < NS; s0 += PF) {
 #pragma unroll
@@ -584,7 +601,7 @@ __device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int n
 #pragma unroll
         for (int q = 0; q < RPT; q++)
 #pragma unroll
-          for (int e = 0; e < 9; e++) ring[p][q][e] = kp[q][(int64_t)((s + PF) * 9 + e) * nint];
+          for (int e = 0; e < 9; e++) ring[p][q][e] = buffer_load<KT>(rs, vo[q], ((s + PF) * 9 + e) * stride);
       }
     }
   }

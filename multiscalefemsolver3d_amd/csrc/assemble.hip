@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -44,6 +45,13 @@ struct mfs_asm_s {
   uint8_t* d_fixed = nullptr;   // per scalar DOF
   double* d_g = nullptr;        // per scalar DOF (fixed values)
   double* d_fbase = nullptr;    // Neumann loads per scalar DOF
+  // fused per-slice path (when every slice touches at most kMaxSliceTets tets)
+  bool fused = false;
+  size_t fused_lds = 0;
+  int64_t* d_sptr = nullptr;    // n_slices+1
+  int32_t* d_stet = nullptr;    // per slice, its tets (ascending)
+  uint16_t* d_gil16 = nullptr;  // interleaved lists of local codes (local tet << 4 | i << 2 | j)
+  uint8_t* d_slice_fix = nullptr;  // per slice: a row or an entry column is fixed
 };
 
 // ---------------------------------------------------------------------------
@@ -51,59 +59,73 @@ struct mfs_asm_s {
 // ---------------------------------------------------------------------------
 // Gradients of the 4 barycentric functions: grad phi_1..3 = rows of J^{-T}
 // with J = [p1-p0, p2-p0, p3-p0]; grad phi_0 = -(sum).  V = |det J| / 6.
+// o[0..11] = g[4][3], o[12] = V sigma (scalar) or V lambda, o[13] = 0 or
+// V mu, o[14] = V.
+__device__ __forceinline__ void tet_record(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
+                                           const double* __restrict__ coef, int64_t e, int problem, double nu,
+                                           double (&o)[15]) {
+  // no fused multiply-adds: every assembly path (and the oracle, built with
+  // -ffp-contract=off) rounds the same operations the same way
+#pragma clang fp contract(off)
+  const int4 t = *reinterpret_cast<const int4*>(tets + 4 * e);
+  double p[4][3];
+  const int32_t id[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+  for (int v = 0; v < 4; v++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) p[v][a] = xyz[3 * (int64_t)id[v] + a];
+  double J[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) J[r][a] = p[r + 1][a] - p[0][a];
+  // cofactors: inverse(J)^T rows give grad phi_{1..3}
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+  const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+  const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double inv = 1.0 / det;
+  double g[4][3];
+  // grad phi_k (k=1..3) = column k-1 of J^{-1} = (cofactor row k-1) / det
+  g[1][0] = c00 * inv; g[1][1] = c01 * inv; g[1][2] = c02 * inv;
+  g[2][0] = c10 * inv; g[2][1] = c11 * inv; g[2][2] = c12 * inv;
+  g[3][0] = c20 * inv; g[3][1] = c21 * inv; g[3][2] = c22 * inv;
+#pragma unroll
+  for (int a = 0; a < 3; a++) g[0][a] = -(g[1][a] + g[2][a] + g[3][a]);
+  const double V = fabs(det) / 6.0;
+#pragma unroll
+  for (int v = 0; v < 4; v++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) o[3 * v + a] = g[v][a];
+  const double c = coef[e];
+  if (problem == MFS_PROBLEM_SCALAR) {
+    o[12] = V * c;
+    o[13] = 0.0;
+  } else {
+    const double lam = c * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double mu = c / (2.0 * (1.0 + nu));
+    o[12] = V * lam;
+    o[13] = V * mu;
+  }
+  o[14] = V;
+}
+
 __global__ __launch_bounds__(256) void k_tet_geometry(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
                                                       const double* __restrict__ coef, int64_t nt, int problem,
                                                       double nu, double* __restrict__ out) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nt; e += (int64_t)gridDim.x * blockDim.x) {
-    const int4 t = *reinterpret_cast<const int4*>(tets + 4 * e);
-    double p[4][3];
-    const int32_t id[4] = {t.x, t.y, t.z, t.w};
+    double o[15];
+    tet_record(xyz, tets, coef, e, problem, nu, o);
+    double* d = out + 16 * e;
 #pragma unroll
-    for (int v = 0; v < 4; v++)
-#pragma unroll
-      for (int a = 0; a < 3; a++) p[v][a] = xyz[3 * (int64_t)id[v] + a];
-    double J[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-      for (int a = 0; a < 3; a++) J[r][a] = p[r + 1][a] - p[0][a];
-    // cofactors: inverse(J)^T rows give grad phi_{1..3}
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-    const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-    const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-    const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-    const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-    const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    const double inv = 1.0 / det;
-    double g[4][3];
-    // grad phi_k (k=1..3) = column k-1 of J^{-1} = (cofactor row k-1) / det
-    g[1][0] = c00 * inv; g[1][1] = c01 * inv; g[1][2] = c02 * inv;
-    g[2][0] = c10 * inv; g[2][1] = c11 * inv; g[2][2] = c12 * inv;
-    g[3][0] = c20 * inv; g[3][1] = c21 * inv; g[3][2] = c22 * inv;
-#pragma unroll
-    for (int a = 0; a < 3; a++) g[0][a] = -(g[1][a] + g[2][a] + g[3][a]);
-    const double V = fabs(det) / 6.0;
-    double* o = out + 16 * e;
-#pragma unroll
-    for (int v = 0; v < 4; v++)
-#pragma unroll
-      for (int a = 0; a < 3; a++) o[3 * v + a] = g[v][a];
-    const double c = coef[e];
-    if (problem == MFS_PROBLEM_SCALAR) {
-      o[12] = V * c;
-      o[13] = 0.0;
-    } else {
-      const double lam = c * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-      const double mu = c / (2.0 * (1.0 + nu));
-      o[12] = V * lam;
-      o[13] = V * mu;
-    }
-    o[14] = V;
-    o[15] = 0.0;
+    for (int k = 0; k < 15; k++) d[k] = o[k];
+    d[15] = 0.0;
   }
 }
 
@@ -115,6 +137,7 @@ constexpr uint32_t kNoContribution = 0xffffffffu;
 //   the left of D)), added to acc.
 template <int BS, int CONV>
 __device__ __forceinline__ void add_block(const double (&g)[8], double (&acc)[BS * BS]) {
+#pragma clang fp contract(off)
   const double* dI = g;
   const double* dJ = g + 3;
   const double dd = dI[0] * dJ[0] + dI[1] * dJ[1] + dI[2] * dJ[2];
@@ -199,6 +222,7 @@ __global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ b
                                                    const uint8_t* __restrict__ fixed, const double* __restrict__ g,
                                                    const double* __restrict__ fbase, double* __restrict__ rhs,
                                                    double* __restrict__ x0, int64_t nb) {
+#pragma clang fp contract(off)
   constexpr int C = 64;
   constexpr int U = 4;  // entries whose index / flag loads are in flight together
   for (int64_t R = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; R < nb; R += (int64_t)gridDim.x * blockDim.x) {
@@ -256,6 +280,184 @@ __global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ b
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused assembly, one workgroup of 1,024 threads per SELL slice (64 block
+// rows; the slice's LDS records allow one workgroup per CU, so its 16 waves
+// supply the latency hiding):
+//  1. the records of the tets touching the slice's rows (host list, in
+//     ascending tet order) are computed into LDS, 11 doubles each
+//     (grad phi_1..3, the two material factors; grad phi_0 is re-formed
+//     as -(sum) exactly as tet_record forms it);
+//  2. one thread per SELL entry sums its contributions (16-bit local codes,
+//     interleaved per group of 64 entries, ascending tet order) from LDS,
+//     turns fixed rows into identity rows, and writes the FP64 value, its
+//     FP32 copy and, at the diagonal entry, the inverse diagonal;
+//  3. the slice's rows then get their right-hand side; where the host
+//     flagged a fixed column in the slice, the row moves those columns to
+//     the right-hand side in slot order (as k_dirichlet does) and
+//     refreshes its inverse diagonal.
+// The arithmetic and its order are those of k_tet_geometry + k_gather +
+// k_dirichlet + the FP32 / diagonal derivation, so both paths give the
+// same bits; this one reads each tet record from LDS instead of HBM and
+// writes each value once.
+constexpr int kRec = 11;               // doubles per tet record in LDS
+constexpr int kMaxSliceTets = 1700;    // 1700 * 88 B = 149.6 KB of LDS
+constexpr uint16_t kNoContribution16 = 0xffff;
+
+constexpr int kSliceThreads = 1024;  // 16 waves share one slice's LDS records
+
+template <int BS, int CONV>
+__global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
+    const double* __restrict__ xyz, const int32_t* __restrict__ tets, const double* __restrict__ coef, int problem,
+    double nu, const int64_t* __restrict__ sptr, const int32_t* __restrict__ stet, const int64_t* __restrict__ goff,
+    const uint16_t* __restrict__ gil, const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
+    const int32_t* __restrict__ browptr, const int32_t* __restrict__ col, const uint8_t* __restrict__ slice_fix,
+    const uint8_t* __restrict__ fixed, const double* __restrict__ g, const double* __restrict__ fbase,
+    double* __restrict__ val64, float* __restrict__ val32, double* __restrict__ dinv64, float* __restrict__ dinv32,
+    double* __restrict__ rhs, double* __restrict__ x0, int64_t nb) {
+#pragma clang fp contract(off)
+  constexpr int C = 64;
+  extern __shared__ double rec[];
+  const int64_t q = blockIdx.x;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int64_t t0 = sptr[q];
+  const int T = (int)(sptr[q + 1] - t0);
+  for (int t = tid; t < T; t += nth) {
+    double o[15];
+    tet_record(xyz, tets, coef, stet[t0 + t], problem, nu, o);
+    double* r = rec + (int64_t)t * kRec;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r[k] = o[3 + k];
+    r[9] = o[12];
+    r[10] = o[13];
+  }
+  __syncthreads();
+  const int w = slice_width[q];
+  const int64_t sb = slice_base[q];
+  constexpr int KB = 32;  // list positions whose codes are requested together
+  for (int p = tid; p < w * C; p += nth) {
+    const int j = p / C, lane = p % C;
+    const int64_t grp = sb / C + j;
+    const int64_t k0 = goff[grp];
+    const int K = (int)((goff[grp + 1] - k0) / C);
+    double acc[BS * BS];
+#pragma unroll
+    for (int e = 0; e < BS * BS; e++) acc[e] = 0.0;
+    for (int kb = 0; kb < K; kb += KB) {
+      // all codes of the batch in flight at once, then the sums from LDS
+      uint16_t c[KB];
+#pragma unroll
+      for (int u = 0; u < KB; u++) c[u] = kb + u < K ? gil[k0 + (int64_t)(kb + u) * C + lane] : kNoContribution16;
+#pragma unroll
+      for (int u = 0; u < KB; u++) {
+        if (c[u] == kNoContribution16) continue;
+        const double* r = rec + (int64_t)(c[u] >> 4) * kRec;
+        const int i = (c[u] >> 2) & 3, jj = c[u] & 3;
+        double gr[8];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          // grad phi_0 = -(grad phi_1 + grad phi_2 + grad phi_3), as tet_record forms it
+          const double g0 = -(r[a] + r[3 + a] + r[6 + a]);
+          gr[a] = i == 0 ? g0 : r[3 * (i - 1) + a];
+          gr[3 + a] = jj == 0 ? g0 : r[3 * (jj - 1) + a];
+        }
+        gr[6] = r[9];
+        gr[7] = r[10];
+        add_block<BS, CONV>(gr, acc);
+      }
+    }
+    const int64_t R = q * C + lane;
+    const int64_t ent = sb + (int64_t)j * C + lane;
+    const int64_t vbase = (sb + (int64_t)j * C) * (BS * BS) + lane;
+    const bool real = R < nb && j < browptr[R + 1] - browptr[R];  // padding entries stay 0
+    int32_t cc = 0;
+    if (real) {
+      cc = col[ent];
+#pragma unroll
+      for (int a = 0; a < BS; a++)
+        if (fixed[BS * R + a])
+#pragma unroll
+          for (int b = 0; b < BS; b++) acc[BS * a + b] = ((int64_t)BS * cc + b == BS * R + a) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < BS * BS; e++) {
+      val64[vbase + (int64_t)e * C] = acc[e];
+      val32[vbase + (int64_t)e * C] = (float)acc[e];
+    }
+    if (real && cc == R) {
+#pragma unroll
+      for (int a = 0; a < BS; a++) {
+        const double d = acc[BS * a + a];
+        const double inv = d != 0.0 ? 1.0 / d : 1.0;
+        dinv64[BS * R + a] = inv;
+        dinv32[BS * R + a] = (float)inv;
+      }
+    }
+  }
+  __syncthreads();  // the slice's values are visible to the row pass below
+  if (tid < C) {
+    const int64_t R = q * C + tid;
+    if (R < nb) {
+      double f[BS];
+      bool rf[BS];
+#pragma unroll
+      for (int a = 0; a < BS; a++) {
+        f[a] = fbase[BS * R + a];
+        rf[a] = fixed[BS * R + a] != 0;
+      }
+      if (slice_fix[q]) {
+        const int len = browptr[R + 1] - browptr[R];
+        int jd = -1;
+        constexpr int U3 = 8;  // entries whose column and flag loads are in flight together
+        for (int j0 = 0; j0 < len; j0 += U3) {
+          int32_t cj0[U3];
+          uint8_t fx[U3][BS];
+#pragma unroll
+          for (int u = 0; u < U3; u++) cj0[u] = col[sb + (int64_t)min(j0 + u, len - 1) * C + tid];
+#pragma unroll
+          for (int u = 0; u < U3; u++)
+#pragma unroll
+            for (int b = 0; b < BS; b++) fx[u][b] = fixed[(int64_t)BS * cj0[u] + b];
+#pragma unroll
+          for (int u = 0; u < U3; u++) {
+            if (j0 + u >= len) continue;
+            if (cj0[u] == R) jd = j0 + u;
+            const int64_t vb = (sb + (int64_t)(j0 + u) * C) * (BS * BS) + tid;
+#pragma unroll
+            for (int a = 0; a < BS; a++)
+#pragma unroll
+              for (int b = 0; b < BS; b++) {
+                const int64_t cj = (int64_t)BS * cj0[u] + b;
+                if (!rf[a] && fx[u][b]) {
+                  const int64_t vi = vb + (int64_t)(BS * a + b) * C;
+                  f[a] -= val64[vi] * g[cj];
+                  val64[vi] = 0.0;
+                  val32[vi] = 0.0f;
+                }
+              }
+          }
+        }
+        if (jd >= 0) {
+          const int64_t vb = (sb + (int64_t)jd * C) * (BS * BS) + tid;
+#pragma unroll
+          for (int a = 0; a < BS; a++) {
+            const double d = val64[vb + (int64_t)(BS * a + a) * C];
+            const double inv = d != 0.0 ? 1.0 / d : 1.0;
+            dinv64[BS * R + a] = inv;
+            dinv32[BS * R + a] = (float)inv;
+          }
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < BS; a++) {
+        const int64_t i = BS * R + a;
+        rhs[i] = rf[a] ? g[i] : f[a];
+        x0[i] = rf[a] ? g[i] : 0.0;
+      }
+    }
+  }
+}
+
 static inline unsigned grid_cap(int64_t n, int64_t cap = 65536) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap));
 }
@@ -279,6 +481,10 @@ extern "C" int mfs_assemble_destroy(mfs_asm S) {
   dfree(S->d_fixed);
   dfree(S->d_g);
   dfree(S->d_fbase);
+  dfree(S->d_sptr);
+  dfree(S->d_stet);
+  dfree(S->d_gil16);
+  dfree(S->d_slice_fix);
   mfs_ctx c = S->ctx;
   delete S;
   ctx_release(c);
@@ -357,14 +563,25 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   int st = mat_create_from_pattern(ctx, bs, std::move(P), &S->A);
   // contribution lists interleaved per group of 64 SELL entries (k-th
   // contribution of the group's 64 entries side by side, padded to the
-  // group's longest list)
-  std::vector<int64_t> goff;
+  // group's longest list).  Fused path: codes local to the slice's tet
+  // list (16 bits); otherwise global codes (32 bits).
+  std::vector<int64_t> goff, sptr;
   std::vector<uint32_t> gil;
+  std::vector<uint16_t> gil16;
+  std::vector<int32_t> stet;
+  std::vector<uint8_t> slice_fix;
   if (st == MFS_OK) {
     const SellLayout& L = S->A->sell;
     const Pattern& Pt = S->A->pat;
     const int C = L.C;
     const int64_t ngroups = L.slice_base[L.n_slices] / C;
+    build_slice_tets(nn, nt, rt.data(), C, sptr, stet);
+    int64_t tmax = 0;
+    for (int64_t q = 0; q < L.n_slices; q++) tmax = std::max(tmax, sptr[q + 1] - sptr[q]);
+    // MFS_ASSEMBLY_SLICES=0 keeps the general path (A/B runs and tests)
+    const char* ev = std::getenv("MFS_ASSEMBLY_SLICES");
+    S->fused = tmax <= kMaxSliceTets && !(ev && ev[0] == '0');
+    S->fused_lds = (size_t)std::max<int64_t>(1, tmax) * kRec * sizeof(double);
     goff.assign(ngroups + 1, 0);
     auto slot_of = [&](int64_t q, int j, int lane) -> int64_t {
       const int64_t r = q * C + lane;
@@ -382,17 +599,49 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
         goff[L.slice_base[q] / C + j + 1] = K * C;
       }
     for (int64_t g = 0; g < ngroups; g++) goff[g + 1] += goff[g];
-    gil.assign(goff[ngroups], kNoContribution);
-#pragma omp parallel for schedule(static)
-    for (int64_t q = 0; q < L.n_slices; q++)
-      for (int j = 0; j < L.slice_width[q]; j++) {
-        const int64_t base = goff[L.slice_base[q] / C + j];
-        for (int lane = 0; lane < C; lane++) {
-          const int64_t sl = slot_of(q, j, lane);
-          if (sl < 0) continue;
-          for (int64_t k = G.ptr[sl]; k < G.ptr[sl + 1]; k++) gil[base + (k - G.ptr[sl]) * C + lane] = G.entry[k];
+    if (S->fused) {
+      gil16.assign(goff[ngroups], kNoContribution16);
+      slice_fix.assign(L.n_slices, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+      for (int64_t q = 0; q < L.n_slices; q++) {
+        const int32_t* t0 = stet.data() + sptr[q];
+        const int32_t* t1 = stet.data() + sptr[q + 1];
+        bool fx = false;
+        for (int lane = 0; lane < C && !fx; lane++) {
+          const int64_t r = q * C + lane;
+          if (r >= Pt.n) break;
+          for (int a = 0; a < bs; a++) fx = fx || fixed[bs * r + a];
+          for (int64_t k = Pt.rowptr[r]; k < Pt.rowptr[r + 1] && !fx; k++)
+            for (int a = 0; a < bs; a++) fx = fx || fixed[(int64_t)bs * Pt.col[k] + a];
+        }
+        slice_fix[q] = fx;
+        for (int j = 0; j < L.slice_width[q]; j++) {
+          const int64_t base = goff[L.slice_base[q] / C + j];
+          for (int lane = 0; lane < C; lane++) {
+            const int64_t sl = slot_of(q, j, lane);
+            if (sl < 0) continue;
+            for (int64_t k = G.ptr[sl]; k < G.ptr[sl + 1]; k++) {
+              const uint32_t code = G.entry[k];
+              const int32_t e = (int32_t)(code >> 4);
+              const int64_t loc = std::lower_bound(t0, t1, e) - t0;
+              gil16[base + (k - G.ptr[sl]) * C + lane] = (uint16_t)((loc << 4) | (code & 15));
+            }
+          }
         }
       }
+    } else {
+      gil.assign(goff[ngroups], kNoContribution);
+#pragma omp parallel for schedule(static)
+      for (int64_t q = 0; q < L.n_slices; q++)
+        for (int j = 0; j < L.slice_width[q]; j++) {
+          const int64_t base = goff[L.slice_base[q] / C + j];
+          for (int lane = 0; lane < C; lane++) {
+            const int64_t sl = slot_of(q, j, lane);
+            if (sl < 0) continue;
+            for (int64_t k = G.ptr[sl]; k < G.ptr[sl + 1]; k++) gil[base + (k - G.ptr[sl]) * C + lane] = G.entry[k];
+          }
+        }
+    }
   }
   if (st == MFS_OK) st = mfs_vec_create(ctx, ndof, MFS_F64, &S->b);
   if (st == MFS_OK) st = mfs_vec_create(ctx, ndof, MFS_F64, &S->x0);
@@ -402,9 +651,16 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   alloc((void**)&S->d_xyz, 8 * 3 * nn);
   alloc((void**)&S->d_tets, 4 * 4 * nt);
   alloc((void**)&S->d_coef, 8 * nt);
-  alloc((void**)&S->d_tet, 8 * 16 * nt);
+  if (!S->fused) alloc((void**)&S->d_tet, 8 * 16 * nt);
   alloc((void**)&S->d_goff, 8 * goff.size());
-  alloc((void**)&S->d_gil, 4 * std::max<size_t>(1, gil.size()));
+  if (S->fused) {
+    alloc((void**)&S->d_gil16, 2 * std::max<size_t>(1, gil16.size()));
+    alloc((void**)&S->d_sptr, 8 * sptr.size());
+    alloc((void**)&S->d_stet, 4 * std::max<size_t>(1, stet.size()));
+    alloc((void**)&S->d_slice_fix, std::max<size_t>(1, slice_fix.size()));
+  } else {
+    alloc((void**)&S->d_gil, 4 * std::max<size_t>(1, gil.size()));
+  }
   alloc((void**)&S->d_browptr, 4 * browptr.size());
   alloc((void**)&S->d_fixed, ndof);
   alloc((void**)&S->d_g, 8 * ndof);
@@ -418,6 +674,13 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   MFS_HIP(hipMemcpyAsync(S->d_tets, rt.data(), 4 * 4 * nt, hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(S->d_goff, goff.data(), 8 * goff.size(), hipMemcpyHostToDevice, s));
   if (!gil.empty()) MFS_HIP(hipMemcpyAsync(S->d_gil, gil.data(), 4 * gil.size(), hipMemcpyHostToDevice, s));
+  if (S->fused) {
+    if (!gil16.empty())
+      MFS_HIP(hipMemcpyAsync(S->d_gil16, gil16.data(), 2 * gil16.size(), hipMemcpyHostToDevice, s));
+    MFS_HIP(hipMemcpyAsync(S->d_sptr, sptr.data(), 8 * sptr.size(), hipMemcpyHostToDevice, s));
+    if (!stet.empty()) MFS_HIP(hipMemcpyAsync(S->d_stet, stet.data(), 4 * stet.size(), hipMemcpyHostToDevice, s));
+    MFS_HIP(hipMemcpyAsync(S->d_slice_fix, slice_fix.data(), slice_fix.size(), hipMemcpyHostToDevice, s));
+  }
   MFS_HIP(hipMemcpyAsync(S->d_browptr, browptr.data(), 4 * browptr.size(), hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(S->d_fixed, fixed.data(), ndof, hipMemcpyHostToDevice, s));
   MFS_HIP(hipMemcpyAsync(S->d_g, g.data(), 8 * ndof, hipMemcpyHostToDevice, s));
@@ -427,7 +690,34 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   return MFS_OK;
 }
 
+static int assemble_fused(mfs_asm S, double nu) {
+  hipStream_t s = S->ctx->stream;
+  mfs_mat A = S->A;
+  const unsigned grid = (unsigned)A->sell.n_slices;
+  const size_t lds = S->fused_lds;
+#define MFS_ASM_SLICE(BS, CONV)                                                                                     \
+  do {                                                                                                              \
+    (void)hipFuncSetAttribute((const void*)k_assemble_slice<BS, CONV>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                              (int)lds);                                                                            \
+    k_assemble_slice<BS, CONV><<<grid, kSliceThreads, lds, s>>>(                                                    \
+        S->d_xyz, S->d_tets, S->d_coef, S->problem, nu, S->d_sptr, S->d_stet, S->d_goff, S->d_gil16,                \
+        A->d_slice_base, A->d_slice_width, S->d_browptr, A->d_col, S->d_slice_fix, S->d_fixed, S->d_g, S->d_fbase,  \
+        A->d_val64, A->d_val32, A->d_dinv64, A->d_dinv32, (double*)S->b->d, (double*)S->x0->d, A->nb);             \
+  } while (0)
+  if (S->bs == 1)
+    MFS_ASM_SLICE(1, MFS_ELEM_REFERENCE);
+  else if (S->convention == MFS_ELEM_TEXTBOOK)
+    MFS_ASM_SLICE(3, MFS_ELEM_TEXTBOOK);
+  else
+    MFS_ASM_SLICE(3, MFS_ELEM_REFERENCE);
+#undef MFS_ASM_SLICE
+  MFS_HIP(hipGetLastError());
+  A->val32_valid = true;
+  return MFS_OK;
+}
+
 static int assemble_kernels(mfs_asm S, double nu) {
+  if (S->fused) return assemble_fused(S, nu);
   hipStream_t s = S->ctx->stream;
   mfs_mat A = S->A;
   k_tet_geometry<<<grid_cap(S->nt), 256, 0, s>>>(S->d_xyz, S->d_tets, S->d_coef, S->nt, S->problem, nu, S->d_tet);

@@ -45,8 +45,7 @@ static inline double rand_normal(uint64_t seed, uint64_t stream, uint64_t idx) {
 // Portrait: row r lists every node sharing a tet with r (r included), sorted.
 // Built from the node->tet adjacency, rows in parallel.
 // ---------------------------------------------------------------------------
-static void node_tets(int64_t nn, int64_t nt, const int32_t* tets, std::vector<int64_t>& ptr,
-                      std::vector<int32_t>& list) {
+void node_tets(int64_t nn, int64_t nt, const int32_t* tets, std::vector<int64_t>& ptr, std::vector<int32_t>& list) {
   ptr.assign(nn + 1, 0);
   for (int64_t k = 0; k < 4 * nt; k++) ptr[tets[k] + 1]++;
   for (int64_t i = 0; i < nn; i++) ptr[i + 1] += ptr[i];
@@ -54,6 +53,29 @@ static void node_tets(int64_t nn, int64_t nt, const int32_t* tets, std::vector<i
   std::vector<int64_t> cur(ptr.begin(), ptr.end() - 1);
   for (int64_t e = 0; e < nt; e++)  // ascending tet order within each node
     for (int v = 0; v < 4; v++) list[cur[tets[4 * e + v]]++] = (int32_t)e;
+}
+
+// Distinct tets touching the nodes [q*C, (q+1)*C) of each slice q, in
+// ascending order.
+void build_slice_tets(int64_t nn, int64_t nt, const int32_t* tets, int C, std::vector<int64_t>& sptr,
+                      std::vector<int32_t>& stet) {
+  std::vector<int64_t> nptr;
+  std::vector<int32_t> nlist;
+  node_tets(nn, nt, tets, nptr, nlist);
+  const int64_t ns = (nn + C - 1) / C;
+  std::vector<std::vector<int32_t>> per(ns);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t q = 0; q < ns; q++) {
+    std::vector<int32_t>& v = per[q];
+    v.assign(nlist.begin() + nptr[q * C], nlist.begin() + nptr[std::min<int64_t>(nn, (q + 1) * C)]);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  sptr.assign(ns + 1, 0);
+  for (int64_t q = 0; q < ns; q++) sptr[q + 1] = sptr[q] + (int64_t)per[q].size();
+  stet.resize(sptr[ns]);
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < ns; q++) std::copy(per[q].begin(), per[q].end(), stet.begin() + sptr[q]);
 }
 
 void build_pattern(int64_t n, int64_t nt, const int32_t* tets, Pattern& P) {

@@ -26,6 +26,11 @@ struct GatherLists {
   std::vector<uint32_t> entry;    // 16 per tet in total
 };
 void build_gather_lists(const Pattern& P, int64_t n_tets, const int32_t* tets, GatherLists& G);
+// node -> tets containing it (ascending tet order per node)
+void node_tets(int64_t nn, int64_t nt, const int32_t* tets, std::vector<int64_t>& ptr, std::vector<int32_t>& list);
+// per slice of C consecutive nodes, the distinct tets touching them (ascending)
+void build_slice_tets(int64_t nn, int64_t nt, const int32_t* tets, int C, std::vector<int64_t>& sptr,
+                      std::vector<int32_t>& stet);
 
 // Mechanical renumbering (Dirichlet first, then first appearance by material).
 int64_t renumber_mech(int64_t nn, int64_t nt, const int32_t* tets, const int32_t* mat, int64_t nfix,

@@ -36,6 +36,37 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
+@pytest.mark.parametrize("kind,conv", [("scalar", 0), ("elastic", mfs.ELEM_REFERENCE), ("elastic", mfs.ELEM_TEXTBOOK)])
+def test_fused_slice_assembly_matches_general_path(ctx, kind, conv, monkeypatch):
+    """The per-slice assembly (tet records in LDS, one pass writing the FP64
+    and FP32 values and the inverse diagonal, fixed DOFs in the same launch)
+    gives the general path's bits (tet records in HBM, gather, elimination,
+    derivation): matrix values, right-hand side, start vector, and a mixed
+    PCG run (which reads the FP32 values and the FP32 inverse diagonal)
+    with the same iterations and solution.  MFS_ASSEMBLY_SLICES=0 selects
+    the general path."""
+    if kind == "scalar":
+        p = problems.laplace_inclusions(14, count=40, rmin=0.05, rmax=0.15)
+    else:
+        p = problems.elastic_cube(9, field="grf")
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MFS_ASSEMBLY_SLICES", flag)
+        S = assemble(ctx, p, conv)
+        rp, col, val = S.A.export_csr()
+        x = mfs.Vec(ctx, S.A.n)
+        x.copy_from(S.x0)
+        # 200 iterations at most: identical iterates are the point, not convergence
+        method = mfs.PCG_JACOBI if kind == "scalar" or conv == mfs.ELEM_TEXTBOOK else mfs.MINRES
+        rep = mfs.Solver(S.A, precision=mfs.PREC_MIXED, tol=1e-9, maxit=200, method=method).run(S.b, x, check=False)
+        out.append((rp, col, val, S.b.download(), S.x0.download(), x.download(), rep.iterations))
+    (rp1, c1, v1, b1, x01, xs1, it1), (rp0, c0, v0, b0, x00, xs0, it0) = out
+    assert np.array_equal(rp1, rp0) and np.array_equal(c1, c0)
+    assert np.array_equal(v1, v0), np.abs(v1 - v0).max()
+    assert np.array_equal(b1, b0) and np.array_equal(x01, x00)
+    assert it1 == it0 and np.array_equal(xs1, xs0)
+
+
 # --- SpMV / BLAS -----------------------------------------------------------------
 
 @pytest.mark.parametrize("kind", ["csr", "bsr3"])

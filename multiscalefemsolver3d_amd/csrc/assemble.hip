@@ -45,13 +45,15 @@ struct mfs_asm_s {
   uint8_t* d_fixed = nullptr;   // per scalar DOF
   double* d_g = nullptr;        // per scalar DOF (fixed values)
   double* d_fbase = nullptr;    // Neumann loads per scalar DOF
-  // fused per-slice path (when every slice touches at most kMaxSliceTets tets)
+  // fused per-chunk path (when every chunk touches at most kMaxSliceTets tets);
+  // a chunk is chunk_rows consecutive block rows of one SELL slice
   bool fused = false;
+  int chunk_rows = 64;
   size_t fused_lds = 0;
-  int64_t* d_sptr = nullptr;    // n_slices+1
-  int32_t* d_stet = nullptr;    // per slice, its tets (ascending)
-  uint16_t* d_gil16 = nullptr;  // interleaved lists of local codes (local tet << 4 | i << 2 | j)
-  uint8_t* d_slice_fix = nullptr;  // per slice: a row or an entry column is fixed
+  int64_t* d_sptr = nullptr;    // n_chunks+1
+  int32_t* d_stet = nullptr;    // per chunk, its tets (ascending)
+  uint16_t* d_gil16 = nullptr;  // interleaved lists of local codes (chunk-local tet << 4 | i << 2 | j)
+  uint8_t* d_slice_fix = nullptr;  // per chunk: a row or an entry column is fixed
 };
 
 // ---------------------------------------------------------------------------
@@ -281,19 +283,22 @@ __global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ b
 }
 
 // ---------------------------------------------------------------------------
-// Fused assembly, one workgroup of 1,024 threads per SELL slice (64 block
-// rows; the slice's LDS records allow one workgroup per CU, so its 16 waves
-// supply the latency hiding):
-//  1. the records of the tets touching the slice's rows (host list, in
+// Fused assembly, one workgroup per chunk of CL consecutive block rows of a
+// SELL slice (CL = 64, 32 or 16; a slice's 64 rows split into 64 / CL
+// chunks, each with its own tet list, so a smaller chunk needs less LDS and
+// lets more workgroups share a CU; along a row of nodes the chunks' tet
+// lists hardly overlap, so the split recomputes few records):
+//  1. the records of the tets touching the chunk's rows (host list, in
 //     ascending tet order) are computed into LDS, 11 doubles each
 //     (grad phi_1..3, the two material factors; grad phi_0 is re-formed
 //     as -(sum) exactly as tet_record forms it);
-//  2. one thread per SELL entry sums its contributions (16-bit local codes,
+//  2. one thread per SELL entry of the chunk sums its contributions (16-bit
+//     chunk-local codes,
 //     interleaved per group of 64 entries, ascending tet order) from LDS,
 //     turns fixed rows into identity rows, and writes the FP64 value, its
 //     FP32 copy and, at the diagonal entry, the inverse diagonal;
-//  3. the slice's rows then get their right-hand side; where the host
-//     flagged a fixed column in the slice, the row moves those columns to
+//  3. the chunk's rows then get their right-hand side; where the host
+//     flagged a fixed column in the chunk, the row moves those columns to
 //     the right-hand side in slot order (as k_dirichlet does) and
 //     refreshes its inverse diagonal.
 // The arithmetic and its order are those of k_tet_geometry + k_gather +
@@ -303,11 +308,13 @@ __global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ b
 constexpr int kRec = 11;               // doubles per tet record in LDS
 constexpr int kMaxSliceTets = 1700;    // 1700 * 88 B = 149.6 KB of LDS
 constexpr uint16_t kNoContribution16 = 0xffff;
+constexpr int kDefaultChunkRows = 16;
+// threads per chunk workgroup: 16 per chunk row, so that one pass over the
+// chunk's entries (about 15 per row for the elastic stencil) needs one round
+constexpr int kChunkThreads(int cl) { return 16 * cl; }
 
-constexpr int kSliceThreads = 1024;  // 16 waves share one slice's LDS records
-
-template <int BS, int CONV>
-__global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
+template <int BS, int CONV, int CL, int NTH>
+__global__ __launch_bounds__(NTH) void k_assemble_slice(
     const double* __restrict__ xyz, const int32_t* __restrict__ tets, const double* __restrict__ coef, int problem,
     double nu, const int64_t* __restrict__ sptr, const int32_t* __restrict__ stet, const int64_t* __restrict__ goff,
     const uint16_t* __restrict__ gil, const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
@@ -316,12 +323,15 @@ __global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
     double* __restrict__ val64, float* __restrict__ val32, double* __restrict__ dinv64, float* __restrict__ dinv32,
     double* __restrict__ rhs, double* __restrict__ x0, int64_t nb) {
 #pragma clang fp contract(off)
-  constexpr int C = 64;
+  constexpr int C = 64, SUB = C / CL;
+  static_assert(C % CL == 0, "a chunk is a whole fraction of a slice");
   extern __shared__ double rec[];
-  const int64_t q = blockIdx.x;
-  const int tid = threadIdx.x, nth = blockDim.x;
-  const int64_t t0 = sptr[q];
-  const int T = (int)(sptr[q + 1] - t0);
+  const int64_t chunk = blockIdx.x;
+  const int64_t q = chunk / SUB;           // SELL slice
+  const int h0 = (int)(chunk % SUB) * CL;  // first lane of the chunk
+  const int tid = threadIdx.x, nth = NTH;
+  const int64_t t0 = sptr[chunk];
+  const int T = (int)(sptr[chunk + 1] - t0);
   for (int t = tid; t < T; t += nth) {
     double o[15];
     tet_record(xyz, tets, coef, stet[t0 + t], problem, nu, o);
@@ -334,9 +344,9 @@ __global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
   __syncthreads();
   const int w = slice_width[q];
   const int64_t sb = slice_base[q];
-  constexpr int KB = 32;  // list positions whose codes are requested together
-  for (int p = tid; p < w * C; p += nth) {
-    const int j = p / C, lane = p % C;
+  constexpr int KB = 16;  // list positions whose codes are requested together
+  for (int p = tid; p < w * CL; p += nth) {
+    const int j = p / CL, lane = h0 + p % CL;
     const int64_t grp = sb / C + j;
     const int64_t k0 = goff[grp];
     const int K = (int)((goff[grp + 1] - k0) / C);
@@ -394,9 +404,10 @@ __global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
       }
     }
   }
-  __syncthreads();  // the slice's values are visible to the row pass below
-  if (tid < C) {
-    const int64_t R = q * C + tid;
+  __syncthreads();  // the chunk's values are visible to the row pass below
+  if (tid < CL) {
+    const int lane = h0 + tid;
+    const int64_t R = q * C + lane;
     if (R < nb) {
       double f[BS];
       bool rf[BS];
@@ -405,7 +416,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
         f[a] = fbase[BS * R + a];
         rf[a] = fixed[BS * R + a] != 0;
       }
-      if (slice_fix[q]) {
+      if (slice_fix[chunk]) {
         const int len = browptr[R + 1] - browptr[R];
         int jd = -1;
         constexpr int U3 = 8;  // entries whose column and flag loads are in flight together
@@ -413,7 +424,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
           int32_t cj0[U3];
           uint8_t fx[U3][BS];
 #pragma unroll
-          for (int u = 0; u < U3; u++) cj0[u] = col[sb + (int64_t)min(j0 + u, len - 1) * C + tid];
+          for (int u = 0; u < U3; u++) cj0[u] = col[sb + (int64_t)min(j0 + u, len - 1) * C + lane];
 #pragma unroll
           for (int u = 0; u < U3; u++)
 #pragma unroll
@@ -422,7 +433,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
           for (int u = 0; u < U3; u++) {
             if (j0 + u >= len) continue;
             if (cj0[u] == R) jd = j0 + u;
-            const int64_t vb = (sb + (int64_t)(j0 + u) * C) * (BS * BS) + tid;
+            const int64_t vb = (sb + (int64_t)(j0 + u) * C) * (BS * BS) + lane;
 #pragma unroll
             for (int a = 0; a < BS; a++)
 #pragma unroll
@@ -438,7 +449,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_assemble_slice(
           }
         }
         if (jd >= 0) {
-          const int64_t vb = (sb + (int64_t)jd * C) * (BS * BS) + tid;
+          const int64_t vb = (sb + (int64_t)jd * C) * (BS * BS) + lane;
 #pragma unroll
           for (int a = 0; a < BS; a++) {
             const double d = val64[vb + (int64_t)(BS * a + a) * C];
@@ -575,9 +586,16 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
     const Pattern& Pt = S->A->pat;
     const int C = L.C;
     const int64_t ngroups = L.slice_base[L.n_slices] / C;
-    build_slice_tets(nn, nt, rt.data(), C, sptr, stet);
+    // MFS_ASSEMBLY_CHUNK=64|32|16 picks the chunk height (A/B runs and tests)
+    const char* evc = std::getenv("MFS_ASSEMBLY_CHUNK");
+    const int want = evc ? std::atoi(evc) : kDefaultChunkRows;
+    S->chunk_rows = want == 16 || want == 32 || want == 64 ? want : kDefaultChunkRows;
+    const int CL = S->chunk_rows, SUB = C / CL;
+    build_slice_tets(nn, nt, rt.data(), CL, sptr, stet);
+    const int64_t n_chunks = L.n_slices * SUB;
+    if ((int64_t)sptr.size() < n_chunks + 1) sptr.resize(n_chunks + 1, sptr.back());  // padded rows of the last slice
     int64_t tmax = 0;
-    for (int64_t q = 0; q < L.n_slices; q++) tmax = std::max(tmax, sptr[q + 1] - sptr[q]);
+    for (int64_t c = 0; c < n_chunks; c++) tmax = std::max(tmax, sptr[c + 1] - sptr[c]);
     // MFS_ASSEMBLY_SLICES=0 keeps the general path (A/B runs and tests)
     const char* ev = std::getenv("MFS_ASSEMBLY_SLICES");
     S->fused = tmax <= kMaxSliceTets && !(ev && ev[0] == '0');
@@ -601,23 +619,25 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
     for (int64_t g = 0; g < ngroups; g++) goff[g + 1] += goff[g];
     if (S->fused) {
       gil16.assign(goff[ngroups], kNoContribution16);
-      slice_fix.assign(L.n_slices, 0);
+      slice_fix.assign(n_chunks, 0);
 #pragma omp parallel for schedule(dynamic, 16)
-      for (int64_t q = 0; q < L.n_slices; q++) {
-        const int32_t* t0 = stet.data() + sptr[q];
-        const int32_t* t1 = stet.data() + sptr[q + 1];
+      for (int64_t ck = 0; ck < n_chunks; ck++) {
+        const int64_t q = ck / SUB;
+        const int h0 = (int)(ck % SUB) * CL;
+        const int32_t* t0 = stet.data() + sptr[ck];
+        const int32_t* t1 = stet.data() + sptr[ck + 1];
         bool fx = false;
-        for (int lane = 0; lane < C && !fx; lane++) {
+        for (int lane = h0; lane < h0 + CL && !fx; lane++) {
           const int64_t r = q * C + lane;
           if (r >= Pt.n) break;
           for (int a = 0; a < bs; a++) fx = fx || fixed[bs * r + a];
           for (int64_t k = Pt.rowptr[r]; k < Pt.rowptr[r + 1] && !fx; k++)
             for (int a = 0; a < bs; a++) fx = fx || fixed[(int64_t)bs * Pt.col[k] + a];
         }
-        slice_fix[q] = fx;
+        slice_fix[ck] = fx;
         for (int j = 0; j < L.slice_width[q]; j++) {
           const int64_t base = goff[L.slice_base[q] / C + j];
-          for (int lane = 0; lane < C; lane++) {
+          for (int lane = h0; lane < h0 + CL; lane++) {
             const int64_t sl = slot_of(q, j, lane);
             if (sl < 0) continue;
             for (int64_t k = G.ptr[sl]; k < G.ptr[sl + 1]; k++) {
@@ -693,16 +713,27 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
 static int assemble_fused(mfs_asm S, double nu) {
   hipStream_t s = S->ctx->stream;
   mfs_mat A = S->A;
-  const unsigned grid = (unsigned)A->sell.n_slices;
+  const int CL = S->chunk_rows;
+  const unsigned grid = (unsigned)(A->sell.n_slices * (64 / CL));
   const size_t lds = S->fused_lds;
-#define MFS_ASM_SLICE(BS, CONV)                                                                                     \
+#define MFS_ASM_CHUNK(BS, CONV, CLV)                                                                                \
   do {                                                                                                              \
-    (void)hipFuncSetAttribute((const void*)k_assemble_slice<BS, CONV>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
-                              (int)lds);                                                                            \
-    k_assemble_slice<BS, CONV><<<grid, kSliceThreads, lds, s>>>(                                                    \
+    constexpr int NTH = kChunkThreads(CLV);                                                                         \
+    auto kern = k_assemble_slice<BS, CONV, CLV, NTH>;                                                               \
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);             \
+    kern<<<grid, NTH, lds, s>>>(                                                                                    \
         S->d_xyz, S->d_tets, S->d_coef, S->problem, nu, S->d_sptr, S->d_stet, S->d_goff, S->d_gil16,                \
         A->d_slice_base, A->d_slice_width, S->d_browptr, A->d_col, S->d_slice_fix, S->d_fixed, S->d_g, S->d_fbase,  \
         A->d_val64, A->d_val32, A->d_dinv64, A->d_dinv32, (double*)S->b->d, (double*)S->x0->d, A->nb);             \
+  } while (0)
+#define MFS_ASM_SLICE(BS, CONV)      \
+  do {                               \
+    if (CL == 16)                    \
+      MFS_ASM_CHUNK(BS, CONV, 16);   \
+    else if (CL == 32)               \
+      MFS_ASM_CHUNK(BS, CONV, 32);   \
+    else                             \
+      MFS_ASM_CHUNK(BS, CONV, 64);   \
   } while (0)
   if (S->bs == 1)
     MFS_ASM_SLICE(1, MFS_ELEM_REFERENCE);
@@ -711,6 +742,7 @@ static int assemble_fused(mfs_asm S, double nu) {
   else
     MFS_ASM_SLICE(3, MFS_ELEM_REFERENCE);
 #undef MFS_ASM_SLICE
+#undef MFS_ASM_CHUNK
   MFS_HIP(hipGetLastError());
   A->val32_valid = true;
   return MFS_OK;

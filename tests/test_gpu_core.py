@@ -43,15 +43,17 @@ def test_fused_slice_assembly_matches_general_path(ctx, kind, conv, monkeypatch)
     gives the general path's bits (tet records in HBM, gather, elimination,
     derivation): matrix values, right-hand side, start vector, and a mixed
     PCG run (which reads the FP32 values and the FP32 inverse diagonal)
-    with the same iterations and solution.  MFS_ASSEMBLY_SLICES=0 selects
-    the general path."""
+    with the same iterations and solution, for every chunk height (64, 32
+    and 16 block rows per workgroup).  MFS_ASSEMBLY_SLICES=0 selects the
+    general path, MFS_ASSEMBLY_CHUNK the chunk height."""
     if kind == "scalar":
         p = problems.laplace_inclusions(14, count=40, rmin=0.05, rmax=0.15)
     else:
         p = problems.elastic_cube(9, field="grf")
     out = []
-    for flag in ("1", "0"):
+    for flag, chunk in (("0", "64"), ("1", "64"), ("1", "32"), ("1", "16")):
         monkeypatch.setenv("MFS_ASSEMBLY_SLICES", flag)
+        monkeypatch.setenv("MFS_ASSEMBLY_CHUNK", chunk)
         S = assemble(ctx, p, conv)
         rp, col, val = S.A.export_csr()
         x = mfs.Vec(ctx, S.A.n)
@@ -60,11 +62,12 @@ def test_fused_slice_assembly_matches_general_path(ctx, kind, conv, monkeypatch)
         method = mfs.PCG_JACOBI if kind == "scalar" or conv == mfs.ELEM_TEXTBOOK else mfs.MINRES
         rep = mfs.Solver(S.A, precision=mfs.PREC_MIXED, tol=1e-9, maxit=200, method=method).run(S.b, x, check=False)
         out.append((rp, col, val, S.b.download(), S.x0.download(), x.download(), rep.iterations))
-    (rp1, c1, v1, b1, x01, xs1, it1), (rp0, c0, v0, b0, x00, xs0, it0) = out
-    assert np.array_equal(rp1, rp0) and np.array_equal(c1, c0)
-    assert np.array_equal(v1, v0), np.abs(v1 - v0).max()
-    assert np.array_equal(b1, b0) and np.array_equal(x01, x00)
-    assert it1 == it0 and np.array_equal(xs1, xs0)
+    rp0, c0, v0, b0, x00, xs0, it0 = out[0]
+    for rp1, c1, v1, b1, x01, xs1, it1 in out[1:]:
+        assert np.array_equal(rp1, rp0) and np.array_equal(c1, c0)
+        assert np.array_equal(v1, v0), np.abs(v1 - v0).max()
+        assert np.array_equal(b1, b0) and np.array_equal(x01, x00)
+        assert it1 == it0 and np.array_equal(xs1, xs0)
 
 
 # --- SpMV / BLAS -----------------------------------------------------------------

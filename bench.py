@@ -253,6 +253,12 @@ def bench_headline(mfs, ctx, args):
     gms, gtf = mfs.gemm_bf16_time(ctx, 8192, 8192, 8192, reps=5)
     detail["gemm_bf16_kernel_8192cubed"] = {"ms": round(gms, 3), "tflops": round(gtf, 1),
                                             "frac_dense_bf16_peak": round(gtf / 2500.0, 4)}
+    # the VALU multiply-add rate the solver kernels are compiled for (plain
+    # FP32, no packed instructions) beside FP64: the arithmetic ceiling of
+    # the FP32-inner / FP64 speedup
+    detail["valu_fma_tflops"] = {"f32_unpacked": round(mfs.valu_fma_rate(ctx, mfs.F32), 1),
+                                 "f64": round(mfs.valu_fma_rate(ctx, mfs.F64), 1),
+                                 "published_peaks": {"f32_vector": 157.3, "f64_vector": 78.6}}
     cpu = None
     if not args.no_cpu:
         cpu = msfem_cpu_baseline_uniform(learned, n, ncells, args.cpu_budget)

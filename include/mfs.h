@@ -473,6 +473,11 @@ int mfs_mlp_forward_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_vec Y, int 
  * reps launches back to back: average ms and TFLOP/s (2MNK per launch).
  * No reference counterpart (the learned guess is specific to this build). */
 int mfs_gemm_bf16_time(mfs_ctx ctx, int M, int N, int K, int reps, double* avg_ms, double* tflops);
+/* Measured VALU multiply-add rate (TFLOP/s, 2 flops per FMA) of the
+ * device in MFS_F32 (plain, unpacked instructions, as every kernel of this
+ * library is compiled) or MFS_F64.  No reference counterpart: a
+ * measurement that explains the mixed-precision speedups of the solvers. */
+int mfs_valu_fma_rate(mfs_ctx ctx, int dtype, double* tflops);
 /* Average device time (ms) and achieved TFLOP/s of the forward+backward GEMMs
  * for one minibatch of `batch`. */
 int mfs_mlp_time_step(mfs_mlp M, int batch, int reps, double* avg_ms, double* tflops);

@@ -170,6 +170,14 @@ def gemm_bf16_time(ctx, M, N, K, reps=10):
     ms, tf = C.c_double(), C.c_double()
     _chk(lib.mfs_gemm_bf16_time(ctx.h, M, N, K, reps, C.byref(ms), C.byref(tf)), "mfs_gemm_bf16_time")
     return ms.value, tf.value
+_sig("mfs_valu_fma_rate", C.c_int, [vp, C.c_int, c_dp])
+
+
+def valu_fma_rate(ctx, dtype):
+    """Measured VALU multiply-add rate in TFLOP/s for F32 (unpacked) or F64."""
+    tf = C.c_double()
+    _chk(lib.mfs_valu_fma_rate(ctx.h, dtype, C.byref(tf)), "mfs_valu_fma_rate")
+    return tf.value
 _sig("mfs_solve", C.c_int, [vp, C.POINTER(SolveDesc), vp, vp, C.POINTER(SolveReport)])
 _sig("mfs_assemble_create", C.c_int, [vp, C.POINTER(FemDesc), C.POINTER(vp)])
 _sig("mfs_assemble_run", C.c_int, [vp, c_dp, C.c_double])

@@ -859,6 +859,56 @@ extern "C" int mfs_gemm_bf16_time(mfs_ctx ctx, int M, int N, int K, int reps, do
   return MFS_OK;
 }
 
+// Eight independent multiply-add chains per thread, `iters` steps each: the
+// VALU issue rate of plain (unpacked) FP32 and of FP64 arithmetic, as the
+// solver kernels are compiled.  The sum is stored so nothing is removed.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fma_rate(T* __restrict__ out, int iters, T a, T b) {
+  T v[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = (T)(threadIdx.x + c);
+  for (int i = 0; i < iters; i++)
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = fma(v[c], a, b);
+  T sum = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) sum += v[c];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
+extern "C" int mfs_valu_fma_rate(mfs_ctx ctx, int dtype, double* tflops) {
+  MFS_CHECK_ARG(ctx && tflops && (dtype == MFS_F32 || dtype == MFS_F64), "mfs_valu_fma_rate: bad argument");
+  MFS_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  constexpr int kBlocks = 256 * 32, kThreads = 256, kIters = 8192;
+  void* out = nullptr;
+  int st = dmalloc(&out, (size_t)kBlocks * kThreads * 8);
+  if (st != MFS_OK) return st;
+  auto launch = [&]() {
+    if (dtype == MFS_F32)
+      k_fma_rate<float><<<kBlocks, kThreads, 0, s>>>((float*)out, kIters, 0.9999999f, 1e-7f);
+    else
+      k_fma_rate<double><<<kBlocks, kThreads, 0, s>>>((double*)out, kIters, 0.9999999, 1e-7);
+  };
+  launch();  // warm-up
+  hipEvent_t e0, e1;
+  MFS_HIP(hipEventCreate(&e0));
+  MFS_HIP(hipEventCreate(&e1));
+  constexpr int kReps = 5;
+  MFS_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < kReps; r++) launch();
+  MFS_HIP(hipEventRecord(e1, s));
+  MFS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  MFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dfree(out);
+  const double flops = 2.0 * 8 * (double)kIters * kBlocks * kThreads;
+  *tflops = flops / (ms / kReps * 1e-3) / 1e12;
+  return MFS_OK;
+}
+
 extern "C" int mfs_mlp_time_step(mfs_mlp M, int batch, int reps, double* avg_ms, double* tflops) {
   MFS_CHECK_ARG(M && batch > 0 && reps > 0 && avg_ms, "mfs_mlp_time_step: bad argument");
   MFS_TRY(mlp_capacity(M, batch));

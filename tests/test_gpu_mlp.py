@@ -61,3 +61,13 @@ def test_training_reduces_loss(ctx):
     net.train(X, T, batch=128, epochs=30)
     l1, _, _ = net.loss_grad(X[:256], T[:256])
     assert l1 < 0.1 * l0, (l0, l1)
+
+
+def test_valu_fma_rate_probe(ctx):
+    """The VALU multiply-add probe runs in both precisions and returns rates
+    below the device's published vector peaks (157.3 / 78.6 TFLOP/s) with
+    some measurement slack."""
+    f32 = mfs.valu_fma_rate(ctx, mfs.F32)
+    f64 = mfs.valu_fma_rate(ctx, mfs.F64)
+    assert 1.0 < f32 < 1.05 * 157.3
+    assert 1.0 < f64 < 1.05 * 78.6

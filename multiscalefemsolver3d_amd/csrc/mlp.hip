@@ -191,19 +191,28 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int Kc, 
 // Large GEMMs with both operands contiguous in k (the forward pass X W^T and
 // the benchmark): 256x256 output tile per 512-thread workgroup, each of the
 // 8 waves a 128x64 block as 4x2 v_mfma_f32_32x32x16_bf16 tiles, K staged
-// through LDS in steps of 32 with two buffers.  A wave reads (128 + 64) x 32
-// operand values from LDS per 128 x 64 x 32 of MFMA work, a third more
-// arithmetic per LDS byte than the 64x64 quadrants of k_gemm_bf16, and two
-// waves per SIMD overlap one's LDS reads with the other's MFMAs.  Operand
-// rows beyond M / N and k beyond the split's end are read as zeros.
-// Split-K and epilogue as k_gemm_bf16.
+// through LDS in steps of 64 with two buffers (four MFMA sub-steps per
+// barrier, so the fragment reads of one sub-step overlap the MFMAs of the
+// previous one).  A wave reads (128 + 64) x 16 operand values from LDS per
+// 128 x 64 x 16 of MFMA work, a third more arithmetic per LDS byte than the
+// 64x64 quadrants of k_gemm_bf16, and two waves per SIMD overlap one's LDS
+// reads with the other's MFMAs.  Interior tiles load without bounds checks;
+// elsewhere operand rows beyond M / N and k beyond the split's end are read
+// as zeros.  Split-K and epilogue as k_gemm_bf16.  Measured alternatives at
+// 8192^3: steps of 32 (884 TFLOP/s), three LDS stages (879), 4 waves of
+// 128x128 with AGPR accumulators (836), s_setprio around the MFMAs (-3 %).
 constexpr int kBigThreads = 512;
-__device__ __forceinline__ void tile256_load(uint4 (&v)[2], const uint16_t* __restrict__ src, int64_t s_row, int row0,
-                                             int k0, int R, int K) {
+constexpr int kBigK = 64;  // k values per LDS stage (2 stages of 2 x 256 x 72 bf16 = 144 KB)
+// KS k values per LDS stage; rows padded to KS + 8 so that the 16-byte
+// fragment reads of 8 consecutive rows fall in distinct banks
+template <int KS, int NT = kBigThreads>
+__device__ __forceinline__ void tile256_load(uint4 (&v)[KS * 32 / NT], const uint16_t* __restrict__ src,
+                                             int64_t s_row, int row0, int k0, int R, int K) {
+  constexpr int CPR = KS / 8;  // 16-byte chunks per row
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int c = threadIdx.x + q * kBigThreads;
-    const int row = c >> 2, kk = (c & 3) * 8;
+  for (int q = 0; q < KS * 32 / NT; q++) {
+    const int c = threadIdx.x + q * NT;
+    const int row = c / CPR, kk = (c % CPR) * 8;
     const int gr = row0 + row, gk = k0 + kk;
     const uint16_t* ptr = src + (int64_t)gr * s_row + gk;
     if (gr < R && gk + 8 <= K && ((uintptr_t)ptr & 15) == 0) {
@@ -216,23 +225,37 @@ __device__ __forceinline__ void tile256_load(uint4 (&v)[2], const uint16_t* __re
     }
   }
 }
-__device__ __forceinline__ void tile256_store(uint16_t (*dst)[40], const uint4 (&v)[2]) {
+// Interior tiles (all 256 rows in range, the whole K step inside the split,
+// 16-byte aligned rows): no bounds checks
+template <int KS, int NT = kBigThreads>
+__device__ __forceinline__ void tile256_load_interior(uint4 (&v)[KS * 32 / NT], const uint16_t* __restrict__ src,
+                                                      int64_t s_row, int row0, int k0) {
+  constexpr int CPR = KS / 8;
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int c = threadIdx.x + q * kBigThreads;
-    *reinterpret_cast<uint4*>(&dst[c >> 2][(c & 3) * 8]) = v[q];
+  for (int q = 0; q < KS * 32 / NT; q++) {
+    const int c = threadIdx.x + q * NT;
+    v[q] = *reinterpret_cast<const uint4*>(src + (int64_t)(row0 + c / CPR) * s_row + k0 + (c % CPR) * 8);
+  }
+}
+template <int KS, int NT = kBigThreads>
+__device__ __forceinline__ void tile256_store(uint16_t (*dst)[KS + 8], const uint4 (&v)[KS * 32 / NT]) {
+  constexpr int CPR = KS / 8;
+#pragma unroll
+  for (int q = 0; q < KS * 32 / NT; q++) {
+    const int c = threadIdx.x + q * NT;
+    *reinterpret_cast<uint4*>(&dst[c / CPR][(c % CPR) * 8]) = v[q];
   }
 }
 
-template <int EPI>
+template <int EPI, int KS>
 __global__ __launch_bounds__(kBigThreads) void k_gemm_bf16_big(int M, int N, int K, int Kc,
                                                                const uint16_t* __restrict__ A, int64_t sam,
                                                                const uint16_t* __restrict__ B, int64_t sbn,
                                                                float* __restrict__ C, int64_t ldc,
                                                                const float* __restrict__ bias,
                                                                uint16_t* __restrict__ Cbf, float* __restrict__ ws) {
-  __shared__ __align__(16) uint16_t As[2][256][40];
-  __shared__ __align__(16) uint16_t Bs[2][256][40];
+  __shared__ __align__(16) uint16_t As[2][256][KS + 8];
+  __shared__ __align__(16) uint16_t Bs[2][256][KS + 8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 2, wn = w & 3;  // 2 x 4 waves of 128 x 64
   const int m0 = blockIdx.y * 256, n0 = blockIdx.x * 256;
@@ -244,24 +267,37 @@ __global__ __launch_bounds__(kBigThreads) void k_gemm_bf16_big(int M, int N, int
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[i][j][e] = 0.f;
   const int kb = blockIdx.z * Kc, ke = min(K, kb + Kc);
-  uint4 ra[2], rb[2];
+  uint4 ra[KS / 16], rb[KS / 16];
+  // workgroup-uniform: every row of both operand tiles in range and aligned
+  const bool interior = m0 + 256 <= M && n0 + 256 <= N && (sam & 7) == 0 && (sbn & 7) == 0 &&
+                        (kb & 7) == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
   if (kb < ke) {
-    tile256_load(ra, A, sam, m0, kb, M, ke);
-    tile256_load(rb, B, sbn, n0, kb, N, ke);
-    tile256_store(As[0], ra);
-    tile256_store(Bs[0], rb);
+    if (interior && kb + KS <= ke) {
+      tile256_load_interior<KS>(ra, A, sam, m0, kb);
+      tile256_load_interior<KS>(rb, B, sbn, n0, kb);
+    } else {
+      tile256_load<KS>(ra, A, sam, m0, kb, M, ke);
+      tile256_load<KS>(rb, B, sbn, n0, kb, N, ke);
+    }
+    tile256_store<KS>(As[0], ra);
+    tile256_store<KS>(Bs[0], rb);
   }
   __syncthreads();
   int buf = 0;
   const int fr = lane & 31, fk = 8 * (lane >> 5);
-  for (int k0 = kb; k0 < ke; k0 += 32) {
-    const bool more = k0 + 32 < ke;
+  for (int k0 = kb; k0 < ke; k0 += KS) {
+    const bool more = k0 + KS < ke;
     if (more) {
-      tile256_load(ra, A, sam, m0, k0 + 32, M, ke);
-      tile256_load(rb, B, sbn, n0, k0 + 32, N, ke);
+      if (interior && k0 + 2 * KS <= ke) {
+        tile256_load_interior<KS>(ra, A, sam, m0, k0 + KS);
+        tile256_load_interior<KS>(rb, B, sbn, n0, k0 + KS);
+      } else {
+        tile256_load<KS>(ra, A, sam, m0, k0 + KS, M, ke);
+        tile256_load<KS>(rb, B, sbn, n0, k0 + KS, N, ke);
+      }
     }
 #pragma unroll
-    for (int ks = 0; ks < 32; ks += 16) {
+    for (int ks = 0; ks < KS; ks += 16) {
       bf16x8 af[4], bfr[2];
 #pragma unroll
       for (int i = 0; i < 4; i++)
@@ -276,8 +312,8 @@ __global__ __launch_bounds__(kBigThreads) void k_gemm_bf16_big(int M, int N, int
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
     if (more) {
-      tile256_store(As[buf ^ 1], ra);
-      tile256_store(Bs[buf ^ 1], rb);
+      tile256_store<KS>(As[buf ^ 1], ra);
+      tile256_store<KS>(Bs[buf ^ 1], rb);
     }
     __syncthreads();
     buf ^= 1;
@@ -582,8 +618,8 @@ static void gemm_epi(hipStream_t s, float* ws, size_t ws_bytes, int M, int N, in
   }
   dim3 grid((N + tile - 1) / tile, (M + tile - 1) / tile, sp);
   if (big) {
-    k_gemm_bf16_big<EPI><<<grid, kBigThreads, 0, s>>>(M, N, K, Kc, A, sam, B, sbn, C, ldc, bias, Cbf,
-                                                      sp == 1 ? nullptr : ws);
+    k_gemm_bf16_big<EPI, kBigK><<<grid, kBigThreads, 0, s>>>(M, N, K, Kc, A, sam, B, sbn, C, ldc, bias, Cbf,
+                                                               sp == 1 ? nullptr : ws);
     if (sp > 1) {
       const int64_t mn = (int64_t)M * N;
       k_splitk_reduce<EPI><<<(unsigned)std::min<int64_t>(4096, (mn + 255) / 256), 256, 0, s>>>(ws, sp, M, N, C, ldc,

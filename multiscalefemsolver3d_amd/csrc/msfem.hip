@@ -53,6 +53,17 @@ namespace {
 
 constexpr int NS = 15;   // stencil points
 constexpr int RG = 8;    // right-hand sides per task
+// Row stride (values) of p, and of d when it lives in LDS, in the register
+// variants: the 3 x RG values of a row plus padding.  One row per lane, so
+// the 16-byte reads of a wave address consecutive rows; 112 B (FP32) and
+// 208 B (FP64) per row put any 8 (or 16) consecutive rows in distinct banks
+// (the packed 96 B / 192 B rows had 2- / 4-way conflicts).  The slab
+// variants keep the packed rows, which leaves room for two workgroups per
+// CU on the larger cells.
+template <typename T>
+constexpr int lds_row_padded() { return 3 * RG + (sizeof(T) == 8 ? 2 : 4); }
+template <typename T>
+__device__ __forceinline__ int64_t lds_at(int v, int a, int r) { return (int64_t)v * lds_row_padded<T>() + a * RG + r; }
 constexpr int NB = 24;   // bases per cell
 constexpr int MAXN = 12; // largest supported cell subdivision (FP32 inner; FP64 inner: 10)
 // Kernel variants per precision, picked by the autotuner: the number of
@@ -188,6 +199,7 @@ struct SolveArgs {
   int inner_cap;  // iterations one inner solve may take before the RHS is
                   // declared failed in this precision (FP32: handed to FP64)
   int use_guess;
+  int max_nint;  // largest cell of the batch (rows of the LDS vectors)
 };
 
 __device__ __forceinline__ int interior_index(int i, int j, int k, int m) {
@@ -619,7 +631,7 @@ __device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, co
   constexpr bool fp64 = std::is_same<T, double>::value;
   auto ld = [&](int u, int b, T (&pv)[RG]) {
 #pragma unroll
-    for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
+    for (int r = 0; r < RG; r++) pv[r] = P[lds_at<T>(u, b, r)];
   };
   int vr[RPT];
   bool ok[RPT];
@@ -641,7 +653,7 @@ __device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, co
     for (int r = 0; r < RG; r++) {
       T t = 0;
 #pragma unroll
-      for (int a = 0; a < 3; a++) t += Qr[q][a][r] * P[((int64_t)vr[q] * 3 + a) * RG + r];
+      for (int a = 0; a < 3; a++) t += Qr[q][a][r] * P[lds_at<T>(vr[q], a, r)];
       w[r] += (double)t;
     }
   }
@@ -662,9 +674,9 @@ __device__ __forceinline__ void phase_update_reg(int nint, const T (&dv)[RPT][3]
     for (int a = 0; a < 3; a++)
 #pragma unroll
       for (int r = 0; r < RG; r++) {
-        const T pe = P[((int64_t)v * 3 + a) * RG + r];
+        const T pe = P[lds_at<T>(v, a, r)];
         if constexpr (DLDS)
-          DL[((int64_t)v * 3 + a) * RG + r] += alpha[r] * pe;
+          DL[lds_at<T>(v, a, r)] += alpha[r] * pe;
         else
           Dr[q][a][r] += alpha[r] * pe;
         const T rv = Rr[q][a][r] - alpha[r] * Qr[q][a][r];
@@ -693,7 +705,7 @@ __device__ __forceinline__ void phase_direction_reg(int nint, const T (&dv)[RPT]
 #pragma unroll
       for (int r = 0; r < RG; r++) {
         if (!((on >> r) & 1)) continue;
-        const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+        const int64_t e = lds_at<T>(v, a, r);
         P[e] = dv[q][a] * Rr[q][a][r] + beta[r] * P[e];
       }
   }
@@ -820,7 +832,7 @@ void k_batch_solve(SolveArgs A) {
   // thread touches only its own rows' entries) or in registers
   constexpr bool kDInLds = REG && d_in_lds(fp64, RPT);
   T Dr[RQ][3][RG];  // unused (and removed) when d is in LDS
-  T* DL = kDInLds ? P + A.scr_stride : nullptr;
+  T* DL = kDInLds ? P + (int64_t)A.max_nint * lds_row_padded<T>() : nullptr;
   // stencil points whose K values are in flight in the register forms
   constexpr int kPrefetch = fp64 ? 1 : 3;
 
@@ -1104,7 +1116,7 @@ void k_batch_solve(SolveArgs A) {
                 const T rv = ((live >> r) & 1) ? (T)((double)Rr[q][a][r] * (double)inv[r]) : (T)0;
                 Rr[q][a][r] = rv;
                 if constexpr (kDInLds)
-                  DL[((int64_t)(tid + q * nth) * 3 + a) * RG + r] = 0;
+                  DL[lds_at<T>(tid + q * nth, a, r)] = 0;
                 else
                   Dr[q][a][r] = 0;
                 const T z = dvr[q][a] * rv;
@@ -1151,7 +1163,7 @@ void k_batch_solve(SolveArgs A) {
             for (int a = 0; a < 3; a++)
 #pragma unroll
               for (int r = 0; r < RG; r++) {
-                const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+                const int64_t e = lds_at<T>(v, a, r);
                 const T z = dvr[q][a] * Rr[q][a][r];
                 P[e] = beta[r] != (T)0 ? z + beta[r] * P[e] : z;
               }
@@ -1267,7 +1279,7 @@ void k_batch_solve(SolveArgs A) {
             for (int r = 0; r < RG; r++)
               if (((live >> r) & 1) && !((capfail >> r) & 1) && !S.failed[r])
                 x64[(int64_t)v * 3 * NB + a * NB + q0 + r] +=
-                    S.scale[r] * (double)(kDInLds ? DL[((int64_t)v * 3 + a) * RG + r] : Dr[q][a][r]);
+                    S.scale[r] * (double)(kDInLds ? DL[lds_at<T>(v, a, r)] : Dr[q][a][r]);
         }
       } else
       for (int v = tid; v < nint; v += nth)
@@ -2160,7 +2172,9 @@ static void boundary_functions(const mfs_batch_s* B, const std::vector<double>& 
 static size_t lds_bytes(mfs_batch B, int prec, int var) {
   const int rpt = prec == MFS_PREC_F64 ? kRptF64[var] : kRptF32[var];
   const bool dl = kRegVar[var] && d_in_lds(prec == MFS_PREC_F64, rpt);
-  return (size_t)B->max_nint * 3 * RG * (prec == MFS_PREC_F64 ? 8 : 4) * (dl ? 2 : 1);
+  const int row = !kRegVar[var] ? 3 * RG
+                  : prec == MFS_PREC_F64 ? lds_row_padded<double>() : lds_row_padded<float>();
+  return (size_t)B->max_nint * row * (prec == MFS_PREC_F64 ? 8 : 4) * (dl ? 2 : 1);
 }
 
 // Per-workgroup scratch for a launch shape; must run before any capture.
@@ -2249,6 +2263,7 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   A.scr32 = (float*)B->d_scr;
   A.scr64 = (double*)B->d_scr;
   A.scr_stride = stride;
+  A.max_nint = B->max_nint;
   A.iters = B->d_iters;
   A.relres = B->d_relres;
   A.flags = B->d_flags;

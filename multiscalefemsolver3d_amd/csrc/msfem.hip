@@ -53,17 +53,18 @@ namespace {
 
 constexpr int NS = 15;   // stencil points
 constexpr int RG = 8;    // right-hand sides per task
-// Row stride (values) of p, and of d when it lives in LDS, in the register
-// variants: the 3 x RG values of a row plus padding.  One row per lane, so
-// the 16-byte reads of a wave address consecutive rows; 112 B (FP32) and
-// 208 B (FP64) per row put any 8 (or 16) consecutive rows in distinct banks
-// (the packed 96 B / 192 B rows had 2- / 4-way conflicts).  The slab
-// variants keep the packed rows, which leaves room for two workgroups per
-// CU on the larger cells.
-template <typename T>
-constexpr int lds_row_padded() { return 3 * RG + (sizeof(T) == 8 ? 2 : 4); }
-template <typename T>
-__device__ __forceinline__ int64_t lds_at(int v, int a, int r) { return (int64_t)v * lds_row_padded<T>() + a * RG + r; }
+// Row stride (values) of p in LDS, and of d when it lives there: the 3 x RG
+// values of a row plus padding.  A wave's lanes own consecutive rows, so its
+// reads of one component address consecutive rows; with packed rows (96 B
+// FP32, 192 B FP64) they had 2- / 4-way bank conflicts.  FP64 and the FP32
+// register variants use 16-byte reads: 208 B and 112 B rows put any 8 (or
+// 16) consecutive rows in distinct banks.  The FP32 slab variants use 104 B
+// rows (8-byte reads, distinct banks for 16 consecutive rows), which keeps
+// two workgroups per CU on cells of up to 729 rows.
+template <typename T, bool REG>
+constexpr int lds_row() { return sizeof(T) == 8 ? 3 * RG + 2 : REG ? 3 * RG + 4 : 3 * RG + 2; }
+template <typename T, bool REG>
+__device__ __forceinline__ int64_t lds_at(int v, int a, int r) { return (int64_t)v * lds_row<T, REG>() + a * RG + r; }
 constexpr int NB = 24;   // bases per cell
 constexpr int MAXN = 12; // largest supported cell subdivision (FP32 inner; FP64 inner: 10)
 // Kernel variants per precision, picked by the autotuner: the number of
@@ -459,7 +460,7 @@ __device__ __forceinline__ void phase_spmv(const float* __restrict__ K32, const 
   const int tid = threadIdx.x, nth = blockDim.x;
   auto ld = [&](int u, int b, T (&pv)[RG]) {
 #pragma unroll
-    for (int r = 0; r < RG; r++) pv[r] = P[((int64_t)u * 3 + b) * RG + r];
+    for (int r = 0; r < RG; r++) pv[r] = P[lds_at<T, false>(u, b, r)];
   };
   constexpr int RP = RPT > 0 ? RPT : 1;
   for (int v0 = tid; v0 < nint; v0 += RP * nth) {
@@ -484,7 +485,7 @@ __device__ __forceinline__ void phase_spmv(const float* __restrict__ K32, const 
         for (int r = 0; r < RG; r++) {
           const int64_t e = ((int64_t)vr[q] * 3 + a) * RG + r;
           Q[e] = acc[q][a][r];
-          w[r] += (double)(acc[q][a][r] * P[e]);
+          w[r] += (double)(acc[q][a][r] * P[lds_at<T, false>(vr[q], a, r)]);
         }
     }
   }
@@ -508,7 +509,7 @@ __device__ __forceinline__ void phase_update(int nint, const T* __restrict__ din
 #pragma unroll
         for (int r = 0; r < RG; r++) {
           const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-          const T pe = P[e];
+          const T pe = P[lds_at<T, false>(v, a, r)];
           D[e] += alpha[r] * pe;
           const T rv = R[e] - alpha[r] * Q[e];
           R[e] = rv;
@@ -538,8 +539,8 @@ __device__ __forceinline__ void phase_direction(int nint, const T* __restrict__ 
 #pragma unroll
         for (int r = 0; r < RG; r++) {
           if (!((on >> r) & 1)) continue;
-          const int64_t e = ((int64_t)v * 3 + a) * RG + r;
-          P[e] = dv * R[e] + beta[r] * P[e];
+          const int64_t e = ((int64_t)v * 3 + a) * RG + r, ep = lds_at<T, false>(v, a, r);
+          P[ep] = dv * R[e] + beta[r] * P[ep];
         }
       }
     }
@@ -631,7 +632,7 @@ __device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, co
   constexpr bool fp64 = std::is_same<T, double>::value;
   auto ld = [&](int u, int b, T (&pv)[RG]) {
 #pragma unroll
-    for (int r = 0; r < RG; r++) pv[r] = P[lds_at<T>(u, b, r)];
+    for (int r = 0; r < RG; r++) pv[r] = P[lds_at<T, true>(u, b, r)];
   };
   int vr[RPT];
   bool ok[RPT];
@@ -653,7 +654,7 @@ __device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, co
     for (int r = 0; r < RG; r++) {
       T t = 0;
 #pragma unroll
-      for (int a = 0; a < 3; a++) t += Qr[q][a][r] * P[lds_at<T>(vr[q], a, r)];
+      for (int a = 0; a < 3; a++) t += Qr[q][a][r] * P[lds_at<T, true>(vr[q], a, r)];
       w[r] += (double)t;
     }
   }
@@ -674,9 +675,9 @@ __device__ __forceinline__ void phase_update_reg(int nint, const T (&dv)[RPT][3]
     for (int a = 0; a < 3; a++)
 #pragma unroll
       for (int r = 0; r < RG; r++) {
-        const T pe = P[lds_at<T>(v, a, r)];
+        const T pe = P[lds_at<T, true>(v, a, r)];
         if constexpr (DLDS)
-          DL[lds_at<T>(v, a, r)] += alpha[r] * pe;
+          DL[lds_at<T, true>(v, a, r)] += alpha[r] * pe;
         else
           Dr[q][a][r] += alpha[r] * pe;
         const T rv = Rr[q][a][r] - alpha[r] * Qr[q][a][r];
@@ -705,7 +706,7 @@ __device__ __forceinline__ void phase_direction_reg(int nint, const T (&dv)[RPT]
 #pragma unroll
       for (int r = 0; r < RG; r++) {
         if (!((on >> r) & 1)) continue;
-        const int64_t e = lds_at<T>(v, a, r);
+        const int64_t e = lds_at<T, true>(v, a, r);
         P[e] = dv[q][a] * Rr[q][a][r] + beta[r] * P[e];
       }
   }
@@ -832,7 +833,7 @@ void k_batch_solve(SolveArgs A) {
   // thread touches only its own rows' entries) or in registers
   constexpr bool kDInLds = REG && d_in_lds(fp64, RPT);
   T Dr[RQ][3][RG];  // unused (and removed) when d is in LDS
-  T* DL = kDInLds ? P + (int64_t)A.max_nint * lds_row_padded<T>() : nullptr;
+  T* DL = kDInLds ? P + (int64_t)A.max_nint * lds_row<T, REG>() : nullptr;
   // stencil points whose K values are in flight in the register forms
   constexpr int kPrefetch = fp64 ? 1 : 3;
 
@@ -1116,7 +1117,7 @@ void k_batch_solve(SolveArgs A) {
                 const T rv = ((live >> r) & 1) ? (T)((double)Rr[q][a][r] * (double)inv[r]) : (T)0;
                 Rr[q][a][r] = rv;
                 if constexpr (kDInLds)
-                  DL[lds_at<T>(tid + q * nth, a, r)] = 0;
+                  DL[lds_at<T, true>(tid + q * nth, a, r)] = 0;
                 else
                   Dr[q][a][r] = 0;
                 const T z = dvr[q][a] * rv;
@@ -1163,7 +1164,7 @@ void k_batch_solve(SolveArgs A) {
             for (int a = 0; a < 3; a++)
 #pragma unroll
               for (int r = 0; r < RG; r++) {
-                const int64_t e = lds_at<T>(v, a, r);
+                const int64_t e = lds_at<T, true>(v, a, r);
                 const T z = dvr[q][a] * Rr[q][a][r];
                 P[e] = beta[r] != (T)0 ? z + beta[r] * P[e] : z;
               }
@@ -1174,9 +1175,9 @@ void k_batch_solve(SolveArgs A) {
             const T dv = dinv[v * 3 + a];
 #pragma unroll
             for (int r = 0; r < RG; r++) {
-              const int64_t e = ((int64_t)v * 3 + a) * RG + r;
+              const int64_t e = ((int64_t)v * 3 + a) * RG + r, ep = lds_at<T, false>(v, a, r);
               const T z = dv * R[e];
-              P[e] = beta[r] != (T)0 ? z + beta[r] * P[e] : z;
+              P[ep] = beta[r] != (T)0 ? z + beta[r] * P[ep] : z;
             }
           }
         __syncthreads();
@@ -1279,7 +1280,7 @@ void k_batch_solve(SolveArgs A) {
             for (int r = 0; r < RG; r++)
               if (((live >> r) & 1) && !((capfail >> r) & 1) && !S.failed[r])
                 x64[(int64_t)v * 3 * NB + a * NB + q0 + r] +=
-                    S.scale[r] * (double)(kDInLds ? DL[lds_at<T>(v, a, r)] : Dr[q][a][r]);
+                    S.scale[r] * (double)(kDInLds ? DL[lds_at<T, true>(v, a, r)] : Dr[q][a][r]);
         }
       } else
       for (int v = tid; v < nint; v += nth)
@@ -2172,8 +2173,8 @@ static void boundary_functions(const mfs_batch_s* B, const std::vector<double>& 
 static size_t lds_bytes(mfs_batch B, int prec, int var) {
   const int rpt = prec == MFS_PREC_F64 ? kRptF64[var] : kRptF32[var];
   const bool dl = kRegVar[var] && d_in_lds(prec == MFS_PREC_F64, rpt);
-  const int row = !kRegVar[var] ? 3 * RG
-                  : prec == MFS_PREC_F64 ? lds_row_padded<double>() : lds_row_padded<float>();
+  const int row = prec == MFS_PREC_F64 ? (kRegVar[var] ? lds_row<double, true>() : lds_row<double, false>())
+                                        : (kRegVar[var] ? lds_row<float, true>() : lds_row<float, false>());
   return (size_t)B->max_nint * row * (prec == MFS_PREC_F64 ? 8 : 4) * (dl ? 2 : 1);
 }
 

@@ -40,10 +40,10 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_K1 = 29470310
 # HBM bytes of one k_batch_solve<float,1,true> launch on the headline
 # workload (learned guess, launch shape 384 / variant 3): rocprofv3 --pmc
-# FETCH_SIZE 64,123,287.7 KiB x2.000 + WRITE_SIZE 15,431,690.9 KiB x1.000
+# FETCH_SIZE 62,661,689.5 KiB x2.000 + WRITE_SIZE 15,494,735.2 KiB x1.000
 # (the copy calibration above).  The register-resident kernel keeps r, q, d
-# and p on chip, so this is 0.064 of the algorithmic bytes.
-TRAFFIC_BATCH = 147126544691
+# and p on chip, so this is 0.062 of the algorithmic bytes.
+TRAFFIC_BATCH = 144197748941
 
 
 def log(*a):
@@ -141,8 +141,13 @@ def main():
         "dtype": "f32 inner / f64 outer (bf16 mfma inference)",
         "data": "synthetic (seeded lognormal moduli; the network is trained on cells the solver generates)",
         "config": head["config"],
+        # FP64 GPU CG on the same problems: the batched register-resident
+        # kernel (same design as the mixed one, the ratio quoted), and plain
+        # FP64 PCG through mfs_solve one problem at a time (sampled)
         "fp64_gpu_cg_ms": head["fp64_batched_zero_ms"],
         "speedup_vs_fp64_gpu_cg": round(head["fp64_batched_zero_ms"] / head["ms_per_step"], 3),
+        "fp64_plain_gpu_cg_one_at_a_time_ms": head["detail"]["fp64_plain_cg_one_problem_at_a_time"]["ms_extrapolated"],
+        "speedup_vs_fp64_plain_gpu_cg": head["detail"]["fp64_plain_cg_one_problem_at_a_time"]["speedup_learned_vs_plain"],
         "headline": head["detail"],
         "roofline": head["roofline"],
         "cpu_baseline": head["cpu_baseline"],
@@ -221,6 +226,14 @@ def bench_headline(mfs, ctx, args):
     ms64 = 1000.0 * w64
     log(f"[bench] FP64 batched zero guess {ms64:.1f} ms (mean it {it64.mean():.1f}); mixed zero guess "
         f"{1000 * w0:.1f} ms (mean it {it0.mean():.1f}); learned {ms:.1f} ms -> {ms64 / ms:.2f}x")
+    # plain FP64 GPU CG (mfs_solve, Jacobi-PCG) on one local problem at a time,
+    # sampled on two cells of the batch and extrapolated to all problems
+    nu = 0.3  # the batch's Poisson ratio (mfs.Batch default)
+    sec = float(np.mean([fp64_plain_cg_seconds(mfs, ctx, n, tuple(org[c]), h, E[6 * n ** 3 * c:6 * n ** 3 * (c + 1)], nu)
+                         for c in (0, 1)]))
+    plain = {"ms_per_problem": round(sec * 1e3, 3), "sample": "2 cells x 24 problems",
+             "ms_extrapolated": round(sec * 1e3 * 24 * ncells, 1),
+             "speedup_learned_vs_plain": round(sec * 1e3 * 24 * ncells / ms, 1)}
     detail = {
         "problems": 24 * ncells, "cell_n": n, "interior_dofs_per_cell": 3 * (n - 1) ** 3,
         "all_converged_to_1e-8": ok, "max_true_relres": max(r.max_true_relres for r in reps),
@@ -234,6 +247,7 @@ def bench_headline(mfs, ctx, args):
         "fp32_to_fp64_fallbacks": reps[-1].n_precision_fallback, "rejected_guesses": reps[-1].n_fallback,
         "mixed_zero_guess_ms": round(1000 * w0, 2),
         "speedup_vs_mixed_zero_guess": round(1000 * w0 / ms, 3),
+        "fp64_plain_cg_one_problem_at_a_time": plain,
         "training": dict(model.info, wall_s=round(train_wall, 1)),
         "cell_assembly_s": round(asm_s, 3),
         "timing": "wall clock of K steps with the context synchronised on both sides",
@@ -537,29 +551,9 @@ def bench_msfem(mfs, problems, ctx, cpu=True):
                      "not_converged": r.n_not_converged, "fp32_to_fp64_fallbacks": r.n_precision_fallback}
     # plain FP64 GPU CG, one local problem at a time (sample of cells)
     sample = [int(np.nonzero(p["cell_n"] == n)[0][0]) for n in (6, 8, 10)]
-    per_problem = []
-    for c in sample:
-        n = int(p["cell_n"][c])
-        xyz, tets = mfs.mesh_box(n, origin=tuple(p["origin"][c]), h=(p["h"] / n,) * 3)
-        s = n + 1
-        ijk = np.array([[i, j, k] for k in range(s) for j in range(s) for i in range(s)])
-        bnd = np.nonzero(((ijk == 0) | (ijk == n)).any(1))[0]
-        X = ijk / n
-        times = []
-        for q in range(24):
-            cc, d = q // 3, q % 3
-            hat = ((X[:, 0] if cc & 1 else 1 - X[:, 0]) * (X[:, 1] if cc & 2 else 1 - X[:, 1]) *
-                   (X[:, 2] if cc & 4 else 1 - X[:, 2]))
-            g = np.zeros((len(bnd), 3))
-            g[:, d] = hat[bnd]
-            S = mfs.Assembly(ctx, mfs.PROBLEM_ELASTIC, xyz, tets, fixed_nodes=bnd, fixed_values=g)
-            S.run(problems.cell_E(p, c), p["nu"])
-            x = mfs.Vec(ctx, S.A.n)
-            x.copy_from(S.x0)
-            rep = mfs.Solver(S.A, precision=mfs.PREC_F64, tol=1e-8).run(S.b, x)
-            times.append(rep.seconds)
-            S.close()
-        per_problem.append((n, float(np.mean(times))))
+    per_problem = [(int(p["cell_n"][c]), fp64_plain_cg_seconds(mfs, ctx, int(p["cell_n"][c]), tuple(p["origin"][c]),
+                                                                p["h"], problems.cell_E(p, c), p["nu"]))
+                   for c in sample]
     # extrapolate with the batch's cell-size histogram
     counts = {int(n): int((p["cell_n"] == n).sum()) for n in np.unique(p["cell_n"])}
     t_of = dict(per_problem)
@@ -582,6 +576,33 @@ def bench_msfem(mfs, problems, ctx, cpu=True):
         res["cpu_baseline"] = msfem_cpu_baseline(p, counts)
     log(f"[bench] msfem: {res}")
     return res
+
+
+def fp64_plain_cg_seconds(mfs, ctx, n, origin, h, E, nu):
+    """Mean device time of one local problem of a cell (n^3 hexes, per-tet
+    modulus E) solved on its own through mfs_solve: FP64 Jacobi-PCG to 1e-8,
+    the 24 hat-function boundary conditions one after the other."""
+    import numpy as np
+    xyz, tets = mfs.mesh_box(n, origin=origin, h=(h / n,) * 3)
+    s = n + 1
+    ijk = np.array([[i, j, k] for k in range(s) for j in range(s) for i in range(s)])
+    bnd = np.nonzero(((ijk == 0) | (ijk == n)).any(1))[0]
+    X = ijk / n
+    times = []
+    for q in range(24):
+        cc, d = q // 3, q % 3
+        hat = ((X[:, 0] if cc & 1 else 1 - X[:, 0]) * (X[:, 1] if cc & 2 else 1 - X[:, 1]) *
+               (X[:, 2] if cc & 4 else 1 - X[:, 2]))
+        g = np.zeros((len(bnd), 3))
+        g[:, d] = hat[bnd]
+        S = mfs.Assembly(ctx, mfs.PROBLEM_ELASTIC, xyz, tets, fixed_nodes=bnd, fixed_values=g)
+        S.run(E, nu)
+        x = mfs.Vec(ctx, S.A.n)
+        x.copy_from(S.x0)
+        rep = mfs.Solver(S.A, precision=mfs.PREC_F64, tol=1e-8).run(S.b, x)
+        times.append(rep.seconds)
+        S.close()
+    return float(np.mean(times))
 
 
 def bench_msfem_reference_trace(mfs, ctx, p):

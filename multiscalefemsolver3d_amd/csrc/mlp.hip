@@ -199,8 +199,13 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int Kc, 
 // reads with the other's MFMAs.  Interior tiles load without bounds checks;
 // elsewhere operand rows beyond M / N and k beyond the split's end are read
 // as zeros.  Split-K and epilogue as k_gemm_bf16.  Measured alternatives at
-// 8192^3: steps of 32 (884 TFLOP/s), three LDS stages (879), 4 waves of
-// 128x128 with AGPR accumulators (836), s_setprio around the MFMAs (-3 %).
+// 8192^3: steps of 32 (884 TFLOP/s), three LDS stages (879), fragments read
+// one sub-step ahead (+0.5 %), 4 waves of 128x128 with AGPR accumulators
+// (836; 940 with the next sub-step's reads interleaved between the MFMAs),
+// s_setprio around the MFMAs (-3 %).  The main loop is LDS-bound: per step
+// the 8 waves read 192 KiB and write 64 KiB of LDS, as many cycles at 128 B
+// per clock as the step's MFMAs at peak, and the barrier keeps the waves of
+// all SIMDs in the same phase.
 constexpr int kBigThreads = 512;
 constexpr int kBigK = 64;  // k values per LDS stage (2 stages of 2 x 256 x 72 bf16 = 144 KB)
 // KS k values per LDS stage; rows padded to KS + 8 so that the 16-byte

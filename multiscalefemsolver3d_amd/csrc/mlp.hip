@@ -202,7 +202,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int Kc, 
 // 8192^3: steps of 32 (884 TFLOP/s), three LDS stages (879), fragments read
 // one sub-step ahead (+0.5 %), 4 waves of 128x128 with AGPR accumulators
 // (836; 940 with the next sub-step's reads interleaved between the MFMAs),
-// s_setprio around the MFMAs (-3 %).  The main loop is LDS-bound: per step
+// s_setprio around the MFMAs (-3 %), waves 4-7 one sub-step behind waves
+// 0-3 so that the two waves of a SIMD alternate reads and MFMAs (+1 %).  The main loop is LDS-bound: per step
 // the 8 waves read 192 KiB and write 64 KiB of LDS, as many cycles at 128 B
 // per clock as the step's MFMAs at peak, and the barrier keeps the waves of
 // all SIMDs in the same phase.

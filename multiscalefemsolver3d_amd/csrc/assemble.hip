@@ -289,12 +289,12 @@ __global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ b
 // lets more workgroups share a CU; along a row of nodes the chunks' tet
 // lists hardly overlap, so the split recomputes few records):
 //  1. the records of the tets touching the chunk's rows (host list, in
-//     ascending tet order) are computed into LDS, 11 doubles each
-//     (grad phi_1..3, the two material factors; grad phi_0 is re-formed
-//     as -(sum) exactly as tet_record forms it);
+//     ascending tet order) are computed into LDS, 14 doubles each
+//     (grad phi_0..3 and the two material factors, as tet_record forms
+//     them);
 //  2. one thread per SELL entry of the chunk sums its contributions (16-bit
-//     chunk-local codes,
-//     interleaved per group of 64 entries, ascending tet order) from LDS,
+//     chunk-local codes, interleaved per group of 64 entries, ascending tet
+//     order) from LDS,
 //     turns fixed rows into identity rows, and writes the FP64 value, its
 //     FP32 copy and, at the diagonal entry, the inverse diagonal;
 //  3. the chunk's rows then get their right-hand side; where the host
@@ -305,8 +305,8 @@ __global__ __launch_bounds__(256) void k_dirichlet(const int32_t* __restrict__ b
 // k_dirichlet + the FP32 / diagonal derivation, so both paths give the
 // same bits; this one reads each tet record from LDS instead of HBM and
 // writes each value once.
-constexpr int kRec = 11;               // doubles per tet record in LDS
-constexpr int kMaxSliceTets = 1700;    // 1700 * 88 B = 149.6 KB of LDS
+constexpr int kRec = 14;               // doubles per tet record in LDS
+constexpr int kMaxSliceTets = 1400;    // 1400 * 112 B = 153.1 KB of LDS
 constexpr uint16_t kNoContribution16 = 0xffff;
 constexpr int kDefaultChunkRows = 16;
 // threads per chunk workgroup: 16 per chunk row, so that one pass over the
@@ -337,9 +337,9 @@ __global__ __launch_bounds__(NTH) void k_assemble_slice(
     tet_record(xyz, tets, coef, stet[t0 + t], problem, nu, o);
     double* r = rec + (int64_t)t * kRec;
 #pragma unroll
-    for (int k = 0; k < 9; k++) r[k] = o[3 + k];
-    r[9] = o[12];
-    r[10] = o[13];
+    for (int k = 0; k < 12; k++) r[k] = o[k];
+    r[12] = o[12];
+    r[13] = o[13];
   }
   __syncthreads();
   const int w = slice_width[q];
@@ -366,13 +366,11 @@ __global__ __launch_bounds__(NTH) void k_assemble_slice(
         double gr[8];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-          // grad phi_0 = -(grad phi_1 + grad phi_2 + grad phi_3), as tet_record forms it
-          const double g0 = -(r[a] + r[3 + a] + r[6 + a]);
-          gr[a] = i == 0 ? g0 : r[3 * (i - 1) + a];
-          gr[3 + a] = jj == 0 ? g0 : r[3 * (jj - 1) + a];
+          gr[a] = r[3 * i + a];
+          gr[3 + a] = r[3 * jj + a];
         }
-        gr[6] = r[9];
-        gr[7] = r[10];
+        gr[6] = r[12];
+        gr[7] = r[13];
         add_block<BS, CONV>(gr, acc);
       }
     }

@@ -52,6 +52,7 @@ struct mfs_asm_s {
   size_t fused_lds = 0;
   int64_t* d_sptr = nullptr;    // n_chunks+1
   int32_t* d_stet = nullptr;    // per chunk, its tets (ascending)
+  int4* d_snode = nullptr;      // per chunk, its tets' node ids (same order)
   uint16_t* d_gil16 = nullptr;  // interleaved lists of local codes (chunk-local tet << 4 | i << 2 | j)
   uint8_t* d_slice_fix = nullptr;  // per chunk: a row or an entry column is fixed
 };
@@ -63,13 +64,11 @@ struct mfs_asm_s {
 // with J = [p1-p0, p2-p0, p3-p0]; grad phi_0 = -(sum).  V = |det J| / 6.
 // o[0..11] = g[4][3], o[12] = V sigma (scalar) or V lambda, o[13] = 0 or
 // V mu, o[14] = V.
-__device__ __forceinline__ void tet_record(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
-                                           const double* __restrict__ coef, int64_t e, int problem, double nu,
-                                           double (&o)[15]) {
+__device__ __forceinline__ void tet_record_nodes(const double* __restrict__ xyz, int4 t, double c, int problem,
+                                                 double nu, double (&o)[15]) {
   // no fused multiply-adds: every assembly path (and the oracle, built with
   // -ffp-contract=off) rounds the same operations the same way
 #pragma clang fp contract(off)
-  const int4 t = *reinterpret_cast<const int4*>(tets + 4 * e);
   double p[4][3];
   const int32_t id[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
@@ -105,7 +104,6 @@ __device__ __forceinline__ void tet_record(const double* __restrict__ xyz, const
   for (int v = 0; v < 4; v++)
 #pragma unroll
     for (int a = 0; a < 3; a++) o[3 * v + a] = g[v][a];
-  const double c = coef[e];
   if (problem == MFS_PROBLEM_SCALAR) {
     o[12] = V * c;
     o[13] = 0.0;
@@ -116,6 +114,12 @@ __device__ __forceinline__ void tet_record(const double* __restrict__ xyz, const
     o[13] = V * mu;
   }
   o[14] = V;
+}
+
+__device__ __forceinline__ void tet_record(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
+                                           const double* __restrict__ coef, int64_t e, int problem, double nu,
+                                           double (&o)[15]) {
+  tet_record_nodes(xyz, *reinterpret_cast<const int4*>(tets + 4 * e), coef[e], problem, nu, o);
 }
 
 __global__ __launch_bounds__(256) void k_tet_geometry(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
@@ -316,7 +320,8 @@ constexpr int kChunkThreads(int cl) { return 16 * cl; }
 template <int BS, int CONV, int CL, int NTH>
 __global__ __launch_bounds__(NTH) void k_assemble_slice(
     const double* __restrict__ xyz, const int32_t* __restrict__ tets, const double* __restrict__ coef, int problem,
-    double nu, const int64_t* __restrict__ sptr, const int32_t* __restrict__ stet, const int64_t* __restrict__ goff,
+    double nu, const int64_t* __restrict__ sptr, const int32_t* __restrict__ stet, const int4* __restrict__ snode,
+    const int64_t* __restrict__ goff,
     const uint16_t* __restrict__ gil, const int64_t* __restrict__ slice_base, const int32_t* __restrict__ slice_width,
     const int32_t* __restrict__ browptr, const int32_t* __restrict__ col, const uint8_t* __restrict__ slice_fix,
     const uint8_t* __restrict__ fixed, const double* __restrict__ g, const double* __restrict__ fbase,
@@ -332,32 +337,83 @@ __global__ __launch_bounds__(NTH) void k_assemble_slice(
   const int tid = threadIdx.x, nth = NTH;
   const int64_t t0 = sptr[chunk];
   const int T = (int)(sptr[chunk + 1] - t0);
+  const int w = slice_width[q];
+  const int64_t sb = slice_base[q];
+  constexpr int KB = 16;  // list positions whose codes are requested together
+  // the list offsets and first KB codes of this thread's first entry are
+  // requested before the records, so they arrive while phase 1 runs
+  int64_t pk0 = 0;
+  int pK = 0;
+  uint16_t pc[KB];
+  // and the entry's row facts for the write-out: row length, column, fixed DOFs
+  int plen = 0;
+  int32_t pcol = 0;
+  uint8_t pfx[BS];
+#pragma unroll
+  for (int a = 0; a < BS; a++) pfx[a] = 0;
+  if (tid < w * CL) {
+    const int j = tid / CL, lane = h0 + tid % CL;
+    const int64_t grp = sb / C + j;
+    pk0 = goff[grp];
+    pK = (int)((goff[grp + 1] - pk0) / C);
+    const int64_t R = q * C + lane;
+    if (R < nb) {
+      plen = browptr[R + 1] - browptr[R];
+      pcol = col[sb + (int64_t)j * C + lane];
+#pragma unroll
+      for (int a = 0; a < BS; a++) pfx[a] = fixed[BS * R + a];
+    }
+  }
+  // phase 1: the chunk list carries each tet's node ids, so the coordinate
+  // loads wait for one list load (and the modulus for the tet id)
   for (int t = tid; t < T; t += nth) {
+    const int4 nd = snode[t0 + t];
+    const int32_t e = stet[t0 + t];
+    if (t == tid && tid < w * CL) {
+      const int lane = h0 + tid % CL;
+#pragma unroll
+      for (int u = 0; u < KB; u++) pc[u] = u < pK ? gil[pk0 + (int64_t)u * C + lane] : kNoContribution16;
+    }
     double o[15];
-    tet_record(xyz, tets, coef, stet[t0 + t], problem, nu, o);
+    tet_record_nodes(xyz, nd, coef[e], problem, nu, o);
     double* r = rec + (int64_t)t * kRec;
 #pragma unroll
     for (int k = 0; k < 12; k++) r[k] = o[k];
     r[12] = o[12];
     r[13] = o[13];
   }
+  if (tid >= T && tid < w * CL) {  // a chunk with fewer tets than threads
+    const int lane = h0 + tid % CL;
+#pragma unroll
+    for (int u = 0; u < KB; u++) pc[u] = u < pK ? gil[pk0 + (int64_t)u * C + lane] : kNoContribution16;
+  }
   __syncthreads();
-  const int w = slice_width[q];
-  const int64_t sb = slice_base[q];
-  constexpr int KB = 16;  // list positions whose codes are requested together
   for (int p = tid; p < w * CL; p += nth) {
     const int j = p / CL, lane = h0 + p % CL;
-    const int64_t grp = sb / C + j;
-    const int64_t k0 = goff[grp];
-    const int K = (int)((goff[grp + 1] - k0) / C);
+    const bool first = p == tid;  // the prefetched entry
+    int64_t k0;
+    int K;
+    if (first) {
+      k0 = pk0;
+      K = pK;
+    } else {
+      const int64_t grp = sb / C + j;
+      k0 = goff[grp];
+      K = (int)((goff[grp + 1] - k0) / C);
+    }
     double acc[BS * BS];
 #pragma unroll
     for (int e = 0; e < BS * BS; e++) acc[e] = 0.0;
     for (int kb = 0; kb < K; kb += KB) {
       // all codes of the batch in flight at once, then the sums from LDS
       uint16_t c[KB];
+      if (first && kb == 0) {
 #pragma unroll
-      for (int u = 0; u < KB; u++) c[u] = kb + u < K ? gil[k0 + (int64_t)(kb + u) * C + lane] : kNoContribution16;
+        for (int u = 0; u < KB; u++) c[u] = pc[u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < KB; u++) c[u] = kb + u < K ? gil[k0 + (int64_t)(kb + u) * C + lane] : kNoContribution16;
+      }
 #pragma unroll
       for (int u = 0; u < KB; u++) {
         if (c[u] == kNoContribution16) continue;
@@ -377,13 +433,14 @@ __global__ __launch_bounds__(NTH) void k_assemble_slice(
     const int64_t R = q * C + lane;
     const int64_t ent = sb + (int64_t)j * C + lane;
     const int64_t vbase = (sb + (int64_t)j * C) * (BS * BS) + lane;
-    const bool real = R < nb && j < browptr[R + 1] - browptr[R];  // padding entries stay 0
+    // padding entries stay 0
+    const bool real = R < nb && j < (first ? plen : browptr[R + 1] - browptr[R]);
     int32_t cc = 0;
     if (real) {
-      cc = col[ent];
+      cc = first ? pcol : col[ent];
 #pragma unroll
       for (int a = 0; a < BS; a++)
-        if (fixed[BS * R + a])
+        if (first ? pfx[a] : fixed[BS * R + a])
 #pragma unroll
           for (int b = 0; b < BS; b++) acc[BS * a + b] = ((int64_t)BS * cc + b == BS * R + a) ? 1.0 : 0.0;
     }
@@ -492,6 +549,7 @@ extern "C" int mfs_assemble_destroy(mfs_asm S) {
   dfree(S->d_fbase);
   dfree(S->d_sptr);
   dfree(S->d_stet);
+  dfree(S->d_snode);
   dfree(S->d_gil16);
   dfree(S->d_slice_fix);
   mfs_ctx c = S->ctx;
@@ -578,6 +636,7 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
   std::vector<uint32_t> gil;
   std::vector<uint16_t> gil16;
   std::vector<int32_t> stet;
+  std::vector<int4> snode;
   std::vector<uint8_t> slice_fix;
   if (st == MFS_OK) {
     const SellLayout& L = S->A->sell;
@@ -598,6 +657,14 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
     const char* ev = std::getenv("MFS_ASSEMBLY_SLICES");
     S->fused = tmax <= kMaxSliceTets && !(ev && ev[0] == '0');
     S->fused_lds = (size_t)std::max<int64_t>(1, tmax) * kRec * sizeof(double);
+    if (S->fused) {
+      snode.resize(stet.size());
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < (int64_t)stet.size(); i++) {
+        const int32_t* t = rt.data() + 4 * (int64_t)stet[i];
+        snode[i] = make_int4(t[0], t[1], t[2], t[3]);
+      }
+    }
     goff.assign(ngroups + 1, 0);
     auto slot_of = [&](int64_t q, int j, int lane) -> int64_t {
       const int64_t r = q * C + lane;
@@ -675,6 +742,7 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
     alloc((void**)&S->d_gil16, 2 * std::max<size_t>(1, gil16.size()));
     alloc((void**)&S->d_sptr, 8 * sptr.size());
     alloc((void**)&S->d_stet, 4 * std::max<size_t>(1, stet.size()));
+    alloc((void**)&S->d_snode, 16 * std::max<size_t>(1, stet.size()));
     alloc((void**)&S->d_slice_fix, std::max<size_t>(1, slice_fix.size()));
   } else {
     alloc((void**)&S->d_gil, 4 * std::max<size_t>(1, gil.size()));
@@ -697,6 +765,8 @@ extern "C" int mfs_assemble_create(mfs_ctx ctx, const mfs_fem_desc* d, mfs_asm* 
       MFS_HIP(hipMemcpyAsync(S->d_gil16, gil16.data(), 2 * gil16.size(), hipMemcpyHostToDevice, s));
     MFS_HIP(hipMemcpyAsync(S->d_sptr, sptr.data(), 8 * sptr.size(), hipMemcpyHostToDevice, s));
     if (!stet.empty()) MFS_HIP(hipMemcpyAsync(S->d_stet, stet.data(), 4 * stet.size(), hipMemcpyHostToDevice, s));
+    if (!snode.empty())
+      MFS_HIP(hipMemcpyAsync(S->d_snode, snode.data(), 16 * snode.size(), hipMemcpyHostToDevice, s));
     MFS_HIP(hipMemcpyAsync(S->d_slice_fix, slice_fix.data(), slice_fix.size(), hipMemcpyHostToDevice, s));
   }
   MFS_HIP(hipMemcpyAsync(S->d_browptr, browptr.data(), 4 * browptr.size(), hipMemcpyHostToDevice, s));
@@ -720,7 +790,7 @@ static int assemble_fused(mfs_asm S, double nu) {
     auto kern = k_assemble_slice<BS, CONV, CLV, NTH>;                                                               \
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);             \
     kern<<<grid, NTH, lds, s>>>(                                                                                    \
-        S->d_xyz, S->d_tets, S->d_coef, S->problem, nu, S->d_sptr, S->d_stet, S->d_goff, S->d_gil16,                \
+        S->d_xyz, S->d_tets, S->d_coef, S->problem, nu, S->d_sptr, S->d_stet, S->d_snode, S->d_goff, S->d_gil16,   \
         A->d_slice_base, A->d_slice_width, S->d_browptr, A->d_col, S->d_slice_fix, S->d_fixed, S->d_g, S->d_fbase,  \
         A->d_val64, A->d_val32, A->d_dinv64, A->d_dinv32, (double*)S->b->d, (double*)S->x0->d, A->nb);             \
   } while (0)

@@ -64,17 +64,19 @@ struct mfs_asm_s {
 // with J = [p1-p0, p2-p0, p3-p0]; grad phi_0 = -(sum).  V = |det J| / 6.
 // o[0..11] = g[4][3], o[12] = V sigma (scalar) or V lambda, o[13] = 0 or
 // V mu, o[14] = V.
-__device__ __forceinline__ void tet_record_nodes(const double* __restrict__ xyz, int4 t, double c, int problem,
-                                                 double nu, double (&o)[15]) {
-  // no fused multiply-adds: every assembly path (and the oracle, built with
-  // -ffp-contract=off) rounds the same operations the same way
-#pragma clang fp contract(off)
-  double p[4][3];
+__device__ __forceinline__ void tet_coords(const double* __restrict__ xyz, int4 t, double (&p)[4][3]) {
   const int32_t id[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
   for (int v = 0; v < 4; v++)
 #pragma unroll
     for (int a = 0; a < 3; a++) p[v][a] = xyz[3 * (int64_t)id[v] + a];
+}
+
+__device__ __forceinline__ void tet_record_from(const double (&p)[4][3], double c, int problem, double nu,
+                                                double (&o)[15]) {
+  // no fused multiply-adds: every assembly path (and the oracle, built with
+  // -ffp-contract=off) rounds the same operations the same way
+#pragma clang fp contract(off)
   double J[3][3];
 #pragma unroll
   for (int r = 0; r < 3; r++)
@@ -119,7 +121,9 @@ __device__ __forceinline__ void tet_record_nodes(const double* __restrict__ xyz,
 __device__ __forceinline__ void tet_record(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
                                            const double* __restrict__ coef, int64_t e, int problem, double nu,
                                            double (&o)[15]) {
-  tet_record_nodes(xyz, *reinterpret_cast<const int4*>(tets + 4 * e), coef[e], problem, nu, o);
+  double p[4][3];
+  tet_coords(xyz, *reinterpret_cast<const int4*>(tets + 4 * e), p);
+  tet_record_from(p, coef[e], problem, nu, o);
 }
 
 __global__ __launch_bounds__(256) void k_tet_geometry(const double* __restrict__ xyz, const int32_t* __restrict__ tets,
@@ -349,8 +353,13 @@ __global__ __launch_bounds__(NTH) void k_assemble_slice(
   int plen = 0;
   int32_t pcol = 0;
   uint8_t pfx[BS];
+  double pfb[BS];  // Neumann loads of the row pass's row (threads < CL)
 #pragma unroll
-  for (int a = 0; a < BS; a++) pfx[a] = 0;
+  for (int a = 0; a < BS; a++) {
+    pfx[a] = 0;
+    pfb[a] = 0.0;
+  }
+  const bool chunk_fix = slice_fix[chunk] != 0;
   if (tid < w * CL) {
     const int j = tid / CL, lane = h0 + tid % CL;
     const int64_t grp = sb / C + j;
@@ -362,25 +371,43 @@ __global__ __launch_bounds__(NTH) void k_assemble_slice(
       pcol = col[sb + (int64_t)j * C + lane];
 #pragma unroll
       for (int a = 0; a < BS; a++) pfx[a] = fixed[BS * R + a];
+      if (tid < CL)  // the row pass below handles this thread's row
+#pragma unroll
+        for (int a = 0; a < BS; a++) pfb[a] = fbase[BS * R + a];
     }
   }
   // phase 1: the chunk list carries each tet's node ids, so the coordinate
-  // loads wait for one list load (and the modulus for the tet id)
-  for (int t = tid; t < T; t += nth) {
-    const int4 nd = snode[t0 + t];
-    const int32_t e = stet[t0 + t];
-    if (t == tid && tid < w * CL) {
-      const int lane = h0 + tid % CL;
-#pragma unroll
-      for (int u = 0; u < KB; u++) pc[u] = u < pK ? gil[pk0 + (int64_t)u * C + lane] : kNoContribution16;
-    }
-    double o[15];
-    tet_record_nodes(xyz, nd, coef[e], problem, nu, o);
+  // loads wait for one list load (and the modulus for the tet id); a thread
+  // with two records requests both before computing either
+  auto put = [&](int t, const double (&o)[15]) {
     double* r = rec + (int64_t)t * kRec;
 #pragma unroll
     for (int k = 0; k < 12; k++) r[k] = o[k];
     r[12] = o[12];
     r[13] = o[13];
+  };
+  for (int t = tid; t < T; t += 2 * nth) {
+    const bool two = t + nth < T;
+    const int4 nd = snode[t0 + t];
+    const int32_t e = stet[t0 + t];
+    const int4 nd2 = two ? snode[t0 + t + nth] : nd;
+    const int32_t e2 = two ? stet[t0 + t + nth] : e;
+    if (t == tid && tid < w * CL) {
+      const int lane = h0 + tid % CL;
+#pragma unroll
+      for (int u = 0; u < KB; u++) pc[u] = u < pK ? gil[pk0 + (int64_t)u * C + lane] : kNoContribution16;
+    }
+    double p[4][3], p2[4][3];
+    tet_coords(xyz, nd, p);
+    if (two) tet_coords(xyz, nd2, p2);
+    const double ce = coef[e], ce2 = two ? coef[e2] : 0.0;
+    double o[15];
+    tet_record_from(p, ce, problem, nu, o);
+    put(t, o);
+    if (two) {
+      tet_record_from(p2, ce2, problem, nu, o);
+      put(t + nth, o);
+    }
   }
   if (tid >= T && tid < w * CL) {  // a chunk with fewer tets than threads
     const int lane = h0 + tid % CL;
@@ -468,10 +495,10 @@ __global__ __launch_bounds__(NTH) void k_assemble_slice(
       bool rf[BS];
 #pragma unroll
       for (int a = 0; a < BS; a++) {
-        f[a] = fbase[BS * R + a];
-        rf[a] = fixed[BS * R + a] != 0;
+        f[a] = pfb[a];
+        rf[a] = pfx[a] != 0;
       }
-      if (slice_fix[chunk]) {
+      if (chunk_fix) {
         const int len = browptr[R + 1] - browptr[R];
         int jd = -1;
         constexpr int U3 = 8;  // entries whose column and flag loads are in flight together

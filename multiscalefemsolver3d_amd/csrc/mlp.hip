@@ -670,13 +670,15 @@ static size_t gemm_ws_bytes(mfs_mlp M, int B) {
 }
 
 // forward on actbf[0] (already filled, B rows): act[L] holds the output
-static void mlp_forward_dev(mfs_mlp M, int B) {
+// out: where the last layer goes ([B][dims[L]], row stride dims[L]); the
+// activation buffer when null
+static void mlp_forward_dev(mfs_mlp M, int B, float* out = nullptr) {
   for (int l = 0; l < M->L; l++) {
     const int nin = M->dims[l], nout = M->dims[l + 1];
     const bool last = l == M->L - 1;
     // Y[B][out] = X[B][in] W[out][in]^T + b
-    gemm(M, last ? EPI_BIAS : EPI_BIAS_RELU, B, nout, nin, M->actbf[l], nin, 1, M->Wbf[l], 1, nin, M->act[l + 1], nout,
-         M->b[l], last ? nullptr : M->actbf[l + 1]);
+    gemm(M, last ? EPI_BIAS : EPI_BIAS_RELU, B, nout, nin, M->actbf[l], nin, 1, M->Wbf[l], 1, nin,
+         last && out ? out : M->act[l + 1], nout, M->b[l], last ? nullptr : M->actbf[l + 1]);
   }
 }
 
@@ -850,8 +852,7 @@ extern "C" int mfs_mlp_forward_vec(mfs_mlp M, int64_t n_samples, mfs_vec X, mfs_
   for (int64_t b0 = 0; b0 < n_samples; b0 += batch) {
     const int B = (int)std::min<int64_t>(batch, n_samples - b0);
     k_to_bf16<<<g1d((int64_t)B * D0), 256, 0, s>>>((const float*)X->d + b0 * D0, M->actbf[0], (int64_t)B * D0);
-    mlp_forward_dev(M, B);
-    MFS_HIP(hipMemcpyAsync((float*)Y->d + b0 * DL, M->act[M->L], 4 * (int64_t)B * DL, hipMemcpyDeviceToDevice, s));
+    mlp_forward_dev(M, B, (float*)Y->d + b0 * DL);  // the last layer straight into Y
   }
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipStreamSynchronize(s));

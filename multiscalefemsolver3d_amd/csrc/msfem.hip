@@ -201,6 +201,7 @@ struct SolveArgs {
                   // declared failed in this precision (FP32: handed to FP64)
   int use_guess;
   int max_nint;  // largest cell of the batch (rows of the LDS vectors)
+  int stage_x;   // register variants: the residual pass stages the FP64 iterate in LDS
 };
 
 __device__ __forceinline__ int interior_index(int i, int j, int k, int m) {
@@ -834,6 +835,12 @@ void k_batch_solve(SolveArgs A) {
   constexpr bool kDInLds = REG && d_in_lds(fp64, RPT);
   T Dr[RQ][3][RG];  // unused (and removed) when d is in LDS
   T* DL = kDInLds ? P + (int64_t)A.max_nint * lds_row<T, REG>() : nullptr;
+  // REG with stage_x: the FP64 iterate of the task's rows for the residual
+  // pass, in the FP64 d region (free during that pass) or behind p
+  double* XS = !REG ? nullptr
+                    : (kDInLds && fp64) ? reinterpret_cast<double*>(DL)
+                                        : reinterpret_cast<double*>(P + (int64_t)A.max_nint * lds_row<T, REG>() *
+                                                                            (kDInLds ? 2 : 1));
   // stencil points whose K values are in flight in the register forms
   constexpr int kPrefetch = fp64 ? 1 : 3;
 
@@ -943,13 +950,36 @@ void k_batch_solve(SolveArgs A) {
           for (int r = 0; r < RG; r++) pv[r] = x64[(int64_t)u * 3 * NB + b * NB + q0 + r];
         };
         if constexpr (REG) {
+          // when the launch has room for it, the task's FP64 iterate is
+          // staged in LDS (own rows, one barrier), so the stencil reads its
+          // neighbours' values from there instead of from global memory
+          if (A.stage_x) {
+#pragma unroll
+            for (int q = 0; q < RQ; q++) {
+              const int v = tid + q * nth;
+              if (v >= nint) continue;
+#pragma unroll
+              for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int r = 0; r < RG; r++)
+                  XS[lds_at<double, true>(v, a, r)] = x64[(int64_t)v * 3 * NB + a * NB + q0 + r];
+            }
+            __syncthreads();
+          }
+          auto lds_ld = [&](int u, int b, double (&pv)[RG]) {
+#pragma unroll
+            for (int r = 0; r < RG; r++) pv[r] = XS[lds_at<double, true>(u, b, r)];
+          };
           // one owned row at a time (the accumulators are FP64)
 #pragma unroll
           for (int q = 0; q < RQ; q++) {
             const int vr[1] = {tid + q * nth};
             const bool ok[1] = {vr[0] < nint};
             double acc[1][3][RG];
-            stencil_rows<double, double, 1>(K64, nint, vr, ok, m, ld, acc);
+            if (A.stage_x)
+              stencil_rows<double, double, 1>(K64, nint, vr, ok, m, lds_ld, acc);
+            else
+              stencil_rows<double, double, 1>(K64, nint, vr, ok, m, ld, acc);
             if (!ok[0]) continue;
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -2178,6 +2208,22 @@ static size_t lds_bytes(mfs_batch B, int prec, int var) {
   return (size_t)B->max_nint * row * (prec == MFS_PREC_F64 ? 8 : 4) * (dl ? 2 : 1);
 }
 
+// The residual pass of the register variants stages the task's FP64 iterate
+// in LDS: FP64 in the d region (free during that pass), FP32 one row per
+// thread (already one workgroup per CU) in a region behind p, when it fits.
+static size_t stage_x_extra(mfs_batch B, int prec, int var) {
+  if (!kRegVar[var] || prec == MFS_PREC_F64) return 0;
+  return (size_t)B->max_nint * lds_row<double, true>() * 8;
+}
+static bool stage_x(mfs_batch B, int prec, int var) {
+  if (!kRegVar[var]) return false;
+  if (prec != MFS_PREC_F64 && kRptF32[var] != 1) return false;
+  return lds_bytes(B, prec, var) + stage_x_extra(B, prec, var) <= 160 * 1024 - 8192;
+}
+static size_t launch_lds(mfs_batch B, int prec, int var) {
+  return lds_bytes(B, prec, var) + (stage_x(B, prec, var) ? stage_x_extra(B, prec, var) : 0);
+}
+
 // Per-workgroup scratch for a launch shape; must run before any capture.
 static int batch_scratch(mfs_batch B, int precision, int per_cu, int var) {
   const int grid = B->ctx->cu_count * per_cu;
@@ -2235,7 +2281,7 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
               "size for every cell and twice the LDS; workgroups above 256 threads are for variant 3 only");
     return MFS_ERR_ARG;
   }
-  const size_t lds = lds_bytes(B, d->precision, var);
+  const size_t lds = launch_lds(B, d->precision, var);
   const size_t need = kRegVar[var] ? 0 : (size_t)grid * 3 * stride * (d->precision == MFS_PREC_F64 ? 8 : 4);
   if (need > B->scr_bytes) {
     set_error("mfs_batch_solve: scratch not allocated for this launch shape");
@@ -2265,6 +2311,7 @@ static int batch_launch(mfs_batch B, const mfs_batch_solve_desc* d, int wg, int 
   A.scr64 = (double*)B->d_scr;
   A.scr_stride = stride;
   A.max_nint = B->max_nint;
+  A.stage_x = stage_x(B, d->precision, var) ? 1 : 0;
   A.iters = B->d_iters;
   A.relres = B->d_relres;
   A.flags = B->d_flags;
@@ -2310,7 +2357,7 @@ static int batch_check_desc(mfs_batch B, const mfs_batch_solve_desc* d) {
 // Workgroups of size wg that fit one CU at once (registers and LDS), at
 // least 1.
 static int resident_per_cu(mfs_batch B, int precision, int wg, int var) {
-  const size_t lds = lds_bytes(B, precision, var);
+  const size_t lds = launch_lds(B, precision, var);
   const void* fn = batch_kernel(precision, var);
   (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int nb = 0;

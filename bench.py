@@ -221,7 +221,7 @@ def bench_headline(mfs, ctx, args):
         return best
     w64, r64 = timed(precision=mfs.PREC_F64, **model.shape[mfs.PREC_F64])
     it64, _, _ = B.status()
-    w0, r0 = timed(precision=mfs.PREC_MIXED, **model.shape[mfs.PREC_MIXED])
+    w0, r0 = timed(precision=mfs.PREC_MIXED, **learned.mixed_shape(model.shape))
     it0, _, _ = B.status()
     ms64 = 1000.0 * w64
     log(f"[bench] FP64 batched zero guess {ms64:.1f} ms (mean it {it64.mean():.1f}); mixed zero guess "

@@ -67,7 +67,7 @@ def train(ctx, n=8, n_train=20000, hidden=256, layers=4, epochs=30, batch=256, l
             wg, _, var, _ = Btr.autotune(precision=prec, tol=tol)
             shape[prec] = {"wg_size": wg, "variant": var + 1}
         info["autotuned_shape"] = {"mixed": shape[PREC_MIXED], "fp64": shape[PREC_F64]}
-    rep = Btr.solve(precision=PREC_MIXED, tol=tol, **shape[PREC_MIXED])
+    rep = Btr.solve(precision=PREC_MIXED, tol=tol, **mixed_shape(shape))
     info["train_solve_ms"] = round(rep.seconds * 1e3, 1)
     X = Vec(ctx, n_train * nt, F32)
     T = Vec(ctx, n_train * nout, F32)
@@ -89,6 +89,18 @@ def train(ctx, n=8, n_train=20000, hidden=256, layers=4, epochs=30, batch=256, l
         f"fwd+bwd {ms:.2f} ms/batch = {tf:.1f} TFLOP/s")
     del X, T
     return Trained(net, n, shape, info)
+
+
+def mixed_shape(shape):
+    """Launch shape of a mixed batched solve: the mixed shape for the FP32
+    launch and the FP64 shape for the FP64 pass over its failures."""
+    out = dict(shape[PREC_MIXED])
+    f64 = shape.get(PREC_F64) or {}
+    if f64.get("wg_size"):
+        out["fallback_wg_size"] = f64["wg_size"]
+    if f64.get("variant"):
+        out["fallback_variant"] = f64["variant"]
+    return out
 
 
 class LearnedSolver:
@@ -113,7 +125,7 @@ class LearnedSolver:
         """One step: inference + guess + batched solve (returns the batch
         report; the caller times the whole step)."""
         self.infer()
-        return self.B.solve(precision=PREC_MIXED, tol=self.tol, use_guess=True, **self.m.shape[PREC_MIXED])
+        return self.B.solve(precision=PREC_MIXED, tol=self.tol, use_guess=True, **mixed_shape(self.m.shape))
 
     def close(self):
         del self.X, self.Y
@@ -137,7 +149,7 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     res = {}
     # the plain FP64 batched solve from the zero guess (the comparison point)
     res["fp64_zero"] = B.solve(precision=PREC_F64, tol=tol, **shape[PREC_F64])
-    r0 = B.solve(precision=PREC_MIXED, tol=tol, **shape[PREC_MIXED])
+    r0 = B.solve(precision=PREC_MIXED, tol=tol, **mixed_shape(shape))
     it0, rr0, _ = B.status()
     res["zero"] = r0
     # reference solution for guess-quality figures
@@ -146,7 +158,7 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     corr = Tref.download()
     del Tref
     B.guess_from(None)
-    r_hat = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True, **shape[PREC_MIXED])
+    r_hat = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True, **mixed_shape(shape))
     it_hat, _, _ = B.status()
     res["hat"] = r_hat
     L = LearnedSolver(ctx, model, B, n_test, tol)
@@ -156,7 +168,7 @@ def run(ctx, n=8, n_train=20000, n_test=4096, hidden=256, layers=4, epochs=30, b
     ctx.sync()
     infer_s = time.perf_counter() - t1
     pred = L.Y.download()
-    r_l = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True, **shape[PREC_MIXED])
+    r_l = B.solve(precision=PREC_MIXED, tol=tol, use_guess=True, **mixed_shape(shape))
     it_l, rr_l, fl_l = B.status()
     res["learned"] = r_l
     L.close()

@@ -728,6 +728,26 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// Exchange with a partner lane that differs in bit log2(o) and agrees in the
+// bits above it: DPP moves inside a row of 16 lanes for o <= 8 (quad
+// permutes for 1 and 2, half-row and row mirrors for 4 and 8; a mirror
+// also flips the lower bits, which the butterfly below does not rely on),
+// ds_bpermute for 16 and 32.  Every pairing is an involution.
+__device__ __forceinline__ int dpp_partner_i32(int v, int o) {
+  switch (o) {
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 4: return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    default: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false); // row_mirror (o = 8)
+  }
+}
+__device__ __forceinline__ double partner_f64(double v, int o) {
+  if (o >= 16) return __shfl_xor(v, o, 64);
+  const int lo = dpp_partner_i32(__double2loint(v), o);
+  const int hi = dpp_partner_i32(__double2hiint(v), o);
+  return __hiloint2double(hi, lo);
+}
+
 // wg_reduce_to_lanes: the same sums, left in lane k (< N) of every wave
 // (other lanes hold 0), for per-value work done one value per lane.
 template <int N>
@@ -754,11 +774,11 @@ __device__ __forceinline__ double wg_reduce_to_lanes(double (&v)[N], double* par
       asm volatile("" : "+v"(lo), "+v"(hi));
       const double send = up ? lo : hi;
       const double keep = up ? hi : lo;
-      a[i] = keep + __shfl_xor(send, o, 64);
+      a[i] = keep + partner_f64(send, o);
     }
   }
 #pragma unroll
-  for (int o = 32 >> LOG; o > 0; o >>= 1) a[0] += __shfl_xor(a[0], o, 64);
+  for (int o = 32 >> LOG; o > 0; o >>= 1) a[0] += partner_f64(a[0], o);
   // value index held by this lane: its top LOG bits (bit 5 most significant)
   const int idx = lane >> (6 - LOG);
   if ((lane & ((64 >> LOG) - 1)) == 0) part[wave * N + idx] = a[0];

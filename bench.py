@@ -40,10 +40,10 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_K1 = 29470310
 # HBM bytes of one k_batch_solve<float,1,true> launch on the headline
 # workload (learned guess, launch shape 384 / variant 3): rocprofv3 --pmc
-# FETCH_SIZE 62,516,845.1 KiB x2.000 + WRITE_SIZE 15,364,045.2 KiB x1.000
+# FETCH_SIZE 61,643,930.2 KiB x2.000 + WRITE_SIZE 14,990,975.1 KiB x1.000
 # (the copy calibration above).  The register-resident kernel keeps r, q, d
-# and p on chip, so this is 0.062 of the algorithmic bytes.
-TRAFFIC_BATCH = 143767281050
+# and p on chip, so this is 0.061 of the algorithmic bytes.
+TRAFFIC_BATCH = 141597527552
 
 
 def log(*a):

@@ -305,9 +305,47 @@ def bench_gemm(mfs, ctx):
     ctx.sync()
     inf_ms = (time.perf_counter() - t0) * 1e3
     flops = 2.0 * ns * (3072 * 256 + 256 * 256 * 2 + 256 * 24696)
+    del X, Y, net
     return {"gemm_8192cubed_ms": round(ms, 3), "gemm_tflops": round(tf, 1),
             "frac_dense_bf16_peak": round(tf / 2500.0, 4), "inference_ms": round(inf_ms, 3),
-            "inference_tflops": round(flops / (inf_ms * 1e-3) / 1e12, 1)}
+            "inference_tflops": round(flops / (inf_ms * 1e-3) / 1e12, 1),
+            "vendor_library_8192cubed": vendor_gemm_bf16(8192)}
+
+
+def vendor_gemm_bf16(n, reps=5):
+    """The vendor library's BF16 GEMM (torch.mm, which calls hipBLASLt) on
+    the same n^3 product, row-major contiguous operands, for comparison
+    with the kernel above.  Timed like it: HIP events on torch's stream
+    around reps back-to-back launches after three warm-up launches.  It runs
+    in a child process that loads only torch, so that torch's HIP runtime
+    and this library's do not share a process."""
+    import subprocess
+    code = (
+        "import json, torch\n"
+        f"n, reps = {int(n)}, {int(reps)}\n"
+        "g = torch.Generator(device='cuda').manual_seed(0)\n"
+        "a = torch.randn(n, n, device='cuda', dtype=torch.bfloat16, generator=g)\n"
+        "b = torch.randn(n, n, device='cuda', dtype=torch.bfloat16, generator=g)\n"
+        "c = torch.empty(n, n, device='cuda', dtype=torch.bfloat16)\n"
+        "for _ in range(3): torch.mm(a, b, out=c)\n"
+        "torch.cuda.synchronize()\n"
+        "e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)\n"
+        "e0.record()\n"
+        "for _ in range(reps): torch.mm(a, b, out=c)\n"
+        "e1.record(); e1.synchronize()\n"
+        "print(json.dumps({'ms': e0.elapsed_time(e1) / reps, 'torch': torch.__version__}))\n")
+    try:
+        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        log(f"[bench] vendor GEMM not run: {e}")
+        return None
+    if out.returncode != 0:
+        log(f"[bench] vendor GEMM failed ({out.returncode}): {out.stderr.strip()[-400:]}")
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    tf = 2.0 * n ** 3 / (r["ms"] * 1e-3) / 1e12
+    return {"library": "torch.mm (hipBLASLt), BF16 output", "torch": r["torch"],
+            "ms": round(r["ms"], 3), "tflops": round(tf, 1), "frac_dense_bf16_peak": round(tf / 2500.0, 4)}
 
 
 def msfem_cpu_baseline_uniform(learned, n, ncells, budget):

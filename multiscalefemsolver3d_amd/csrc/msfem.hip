@@ -588,7 +588,7 @@ __device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int n
     for (int q = 0; q < RPT; q++)
 #pragma unroll
       for (int e = 0; e < 9; e++) ring[p][q][e] = buffer_load<KT>(rs, vo[q], (p * 9 + e) * stride);
-#pragma unroll 1
+#pragma unroll
   for (int s0 = 0; s0 < NS; s0 += PF) {
 #pragma unroll
     for (int p = 0; p < PF; p++) {

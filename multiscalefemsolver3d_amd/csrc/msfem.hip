@@ -1233,12 +1233,10 @@ void k_batch_solve(SolveArgs A) {
         __syncthreads();
       }
       int inner_it = 0, capfail = 0;
-      while (act) {
-        if (inner_it >= inner_cap) {
-          capfail |= act;  // recorded by thread 0 after the x update
-          act = 0;
-          break;
-        }
+      // tested at the bottom: r and d then stay in one set of registers
+      // across iterations (a test at the top made the compiler carry them
+      // in two sets and copy 48 values per iteration)
+      if (act && inner_cap > 0) do {
         // q = K p, p.q
         T alpha[RG];
         {
@@ -1317,7 +1315,10 @@ void k_batch_solve(SolveArgs A) {
         act = act2;
         par ^= 1;
         inner_it++;
-      }
+      } while (act && inner_it < inner_cap);
+      // right-hand sides still active here reached the iteration cap
+      capfail |= act;  // recorded by thread 0 after the x update
+      act = 0;
       // x64 += scale * d (not for problems whose inner solve failed)
       if constexpr (REG) {
 #pragma unroll

@@ -763,6 +763,31 @@ __device__ __forceinline__ double wg_reduce_to_lanes(double (&v)[N], double* par
   for (int t = 0; t < LOG; t++) {
     const int o = 32 >> t, h = N >> (t + 1);
     const bool up = (lane & o) != 0;
+    if (o >= 16) {
+      // v_permlane32_swap / v_permlane16_swap of (a[i], a[h+i]) leaves, in
+      // every lane, the value it keeps in one output and its partner's copy
+      // of that value in the other: one swap per 32-bit half and an add,
+      // with no selects and no LDS round trip
+#pragma unroll
+      for (int i = 0; i < h; i++) {
+        const unsigned al = (unsigned)__double2loint(a[i]), ah = (unsigned)__double2hiint(a[i]);
+        const unsigned bl = (unsigned)__double2loint(a[h + i]), bh = (unsigned)__double2hiint(a[h + i]);
+        double x0, x1;
+        if (o == 32) {
+          const auto l = __builtin_amdgcn_permlane32_swap(al, bl, false, false);
+          const auto hh = __builtin_amdgcn_permlane32_swap(ah, bh, false, false);
+          x0 = __hiloint2double((int)hh[0], (int)l[0]);
+          x1 = __hiloint2double((int)hh[1], (int)l[1]);
+        } else {
+          const auto l = __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+          const auto hh = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+          x0 = __hiloint2double((int)hh[0], (int)l[0]);
+          x1 = __hiloint2double((int)hh[1], (int)l[1]);
+        }
+        a[i] = x0 + x1;
+      }
+      continue;
+    }
 #pragma unroll
     for (int i = 0; i < h; i++) {
       // the pair is made opaque first: otherwise the compiler turns

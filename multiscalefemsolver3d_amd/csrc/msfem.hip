@@ -548,46 +548,74 @@ __device__ __forceinline__ void phase_direction(int nint, const T* __restrict__ 
   }
 }
 
-// stencil_rows with the K values of PF stencil points in flight: point
-// s + PF is requested right after point s is applied, into the ring slot
-// point s used (PF x RPT x 9 registers).  The point loop runs in chunks of PF points, each chunk
-// unrolled, so the ring is indexed statically while the code stays one
-// chunk long.  A wave then waits for a K load once per PF points instead of
-// once per point; PF x RPT x 9 stays below the 63 vector loads a wave can
-// have outstanding.  Same arithmetic, in the same order, as stencil_rows.
-template <typename KT, typename VT, int RPT, int PF, typename Load>
-__device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
-                                                const bool (&ok)[RPT], int m, Load load, VT (&acc)[RPT][3][RG]) {
-  static_assert(PF >= 1 && NS % PF == 0 && PF * RPT * 9 <= 63, "prefetch ring: PF must divide 15, <= 63 loads");
-  int vi[RPT];
-  KT ring[PF][RPT][9];
-#pragma unroll
-  for (int q = 0; q < RPT; q++) {
-    vi[q] = ok[q] ? v[q] : 0;
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-      for (int r = 0; r < RG; r++) acc[q][a][r] = 0;
-  }
-  // K is read through a buffer resource over the cell's matrix: the row's
-  // byte offset is the (per-thread) VGPR offset and the entry's byte offset
-  // (s*9+e)*nint*sizeof(KT) the (uniform) SGPR offset, so a load costs no
-  // vector address arithmetic.  The stride is redefined for the compiler
-  // on every call, so the 135 entry offsets are formed here, point by
-  // point, instead of being hoisted out of the iteration loop.
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<KT*>(Kc), (short)0, NS * 9 * nint * (int)sizeof(KT), kBufferRsrcWord3);
+// K of one cell through a buffer resource: the row's byte offset is the
+// (per-thread) VGPR offset and the entry's byte offset (s*9+e)*nint*sizeof(KT)
+// the (uniform) SGPR offset, so a load costs no vector address arithmetic.
+// The stride is redefined for the compiler on every call, so the 135 entry
+// offsets are formed where they are used instead of being hoisted out of
+// the iteration loop.
+template <typename KT, int RPT>
+struct KRows {
+  __amdgpu_buffer_rsrc_t rs;
   int vo[RPT];
+  int stride;
+  __device__ __forceinline__ KRows(const KT* Kc, int nint, const int (&vi)[RPT]) {
+    rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<KT*>(Kc), (short)0, NS * 9 * nint * (int)sizeof(KT),
+                                           kBufferRsrcWord3);
 #pragma unroll
-  for (int q = 0; q < RPT; q++) vo[q] = vi[q] * (int)sizeof(KT);
-  int stride = nint * (int)sizeof(KT);
-  asm volatile("" : "+s"(stride));
+    for (int q = 0; q < RPT; q++) vo[q] = vi[q] * (int)sizeof(KT);
+    stride = nint * (int)sizeof(KT);
+    asm volatile("" : "+s"(stride));
+  }
+  __device__ __forceinline__ KT at(int q, int s, int e) const { return buffer_load<KT>(rs, vo[q], (s * 9 + e) * stride); }
+};
+
+template <int RPT>
+__device__ __forceinline__ void clamp_rows(const int (&v)[RPT], const bool (&ok)[RPT], int (&vi)[RPT]) {
+#pragma unroll
+  for (int q = 0; q < RPT; q++) vi[q] = ok[q] ? v[q] : 0;
+}
+
+// Requests the K values of stencil points 0 .. PF-1 into the ring that
+// stencil_rows_pf consumes.
+template <typename KT, int RPT, int PF>
+__device__ __forceinline__ void k_ring_request(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
+                                               const bool (&ok)[RPT], KT (&ring)[PF][RPT][9]) {
+  int vi[RPT];
+  clamp_rows<RPT>(v, ok, vi);
+  const KRows<KT, RPT> kr(Kc, nint, vi);
 #pragma unroll
   for (int p = 0; p < PF; p++)
 #pragma unroll
     for (int q = 0; q < RPT; q++)
 #pragma unroll
-      for (int e = 0; e < 9; e++) ring[p][q][e] = buffer_load<KT>(rs, vo[q], (p * 9 + e) * stride);
+      for (int e = 0; e < 9; e++) ring[p][q][e] = kr.at(q, p, e);
+}
+
+// stencil_rows with the K values of PF stencil points in flight: the ring
+// holds points 0 .. PF-1 on entry (k_ring_request, or the previous call),
+// and point (s + PF) mod 15 is requested right after point s is applied,
+// into the slot point s used (PF x RPT x 9 registers).  The ring thus wraps:
+// the last points request the first ones for the next product with the same
+// K, which arrive while the caller runs its update and reductions.  The
+// point loop is unrolled, so the ring is indexed statically and needs no
+// copies.  A wave then waits for a K load once per PF points instead of
+// once per point; PF x RPT x 9 stays below the 63 vector loads a wave can
+// have outstanding.  Same arithmetic, in the same order, as stencil_rows.
+template <typename KT, typename VT, int RPT, int PF, typename Load>
+__device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
+                                                const bool (&ok)[RPT], int m, Load load, KT (&ring)[PF][RPT][9],
+                                                VT (&acc)[RPT][3][RG]) {
+  static_assert(PF >= 1 && NS % PF == 0 && PF * RPT * 9 <= 63, "prefetch ring: PF must divide 15, <= 63 loads");
+  int vi[RPT];
+  clamp_rows<RPT>(v, ok, vi);
+#pragma unroll
+  for (int q = 0; q < RPT; q++)
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < RG; r++) acc[q][a][r] = 0;
+  const KRows<KT, RPT> kr(Kc, nint, vi);
 #pragma unroll
   for (int s0 = 0; s0 < NS; s0 += PF) {
 #pragma unroll
@@ -609,14 +637,13 @@ __device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int n
           }
         }
       }
-      // the slot just used is refilled with the point PF later (requesting
-      // it after the point's products needs no copy of the slot)
-      if (s0 + PF < NS) {
+      // the slot just used is refilled with the point PF later, modulo 15
+      // (requesting it after the point's products needs no copy of the slot)
+      const int sn = (s + PF) % NS;
 #pragma unroll
-        for (int q = 0; q < RPT; q++)
+      for (int q = 0; q < RPT; q++)
 #pragma unroll
-          for (int e = 0; e < 9; e++) ring[p][q][e] = buffer_load<KT>(rs, vo[q], ((s + PF) * 9 + e) * stride);
-      }
+        for (int e = 0; e < 9; e++) ring[p][q][e] = kr.at(q, sn, e);
     }
   }
 }
@@ -629,7 +656,8 @@ __device__ __forceinline__ void stencil_rows_pf(const KT* __restrict__ Kc, int n
 // values that are never used.
 template <typename T, int RPT, int PF>
 __device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, const double* __restrict__ K64, int nint,
-                                               int m, const T* P, T (&Qr)[RPT][3][RG], double (&w)[RG]) {
+                                               int m, const T* P, T (&ring)[PF][RPT][9], T (&Qr)[RPT][3][RG],
+                                               double (&w)[RG]) {
   constexpr bool fp64 = std::is_same<T, double>::value;
   auto ld = [&](int u, int b, T (&pv)[RG]) {
 #pragma unroll
@@ -643,9 +671,9 @@ __device__ __forceinline__ void phase_spmv_reg(const float* __restrict__ K32, co
     ok[q] = vr[q] < nint;
   }
   if constexpr (fp64)
-    stencil_rows_pf<double, double, RPT, PF>(K64, nint, vr, ok, m, ld, Qr);
+    stencil_rows_pf<double, double, RPT, PF>(K64, nint, vr, ok, m, ld, ring, Qr);
   else
-    stencil_rows_pf<float, float, RPT, PF>(K32, nint, vr, ok, m, ld, Qr);
+    stencil_rows_pf<float, float, RPT, PF>(K32, nint, vr, ok, m, ld, ring, Qr);
   // the row's three components are summed in T before the FP64 partial
   // (one conversion and FP64 add per right-hand side instead of three)
 #pragma unroll
@@ -1261,6 +1289,23 @@ void k_batch_solve(SolveArgs A) {
       // tested at the bottom: r and d then stay in one set of registers
       // across iterations (a test at the top made the compiler carry them
       // in two sets and copy 48 values per iteration)
+      // K of the first stencil points, carried from one product to the next
+      T kring[REG ? kPrefetch : 1][RQ][9];
+      if constexpr (REG) {
+        if (act && inner_cap > 0) {
+          int vr[RQ];
+          bool ok[RQ];
+#pragma unroll
+          for (int q = 0; q < RQ; q++) {
+            vr[q] = tid + q * nth;
+            ok[q] = vr[q] < nint;
+          }
+          if constexpr (fp64)
+            k_ring_request<double, RQ, kPrefetch>(K64, nint, vr, ok, kring);
+          else
+            k_ring_request<float, RQ, kPrefetch>(K32, nint, vr, ok, kring);
+        }
+      }
       if (act && inner_cap > 0) do {
         // q = K p, p.q
         T alpha[RG];
@@ -1269,7 +1314,7 @@ void k_batch_solve(SolveArgs A) {
 #pragma unroll
           for (int r = 0; r < RG; r++) w[r] = 0;
           if constexpr (REG)
-            phase_spmv_reg<T, RQ, kPrefetch>(K32, K64, nint, m, P, Qr, w);
+            phase_spmv_reg<T, RQ, kPrefetch>(K32, K64, nint, m, P, kring, Qr, w);
           else
             phase_spmv<T, RPT>(K32, K64, nint, m, P, Q, w);
           // alpha of right-hand side r computed in lane r, then broadcast

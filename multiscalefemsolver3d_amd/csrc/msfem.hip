@@ -1049,9 +1049,12 @@ void k_batch_solve(SolveArgs A) {
             const int vr[1] = {tid + q * nth};
             const bool ok[1] = {vr[0] < nint};
             double acc[1][3][RG];
-            if (A.stage_x)
-              stencil_rows<double, double, 1>(K64, nint, vr, ok, m, lds_ld, acc);
-            else
+            if (A.stage_x) {
+              // unrolled, with the next point's K in flight
+              double kr1[1][1][9];
+              k_ring_request<double, 1, 1>(K64, nint, vr, ok, kr1);
+              stencil_rows_pf<double, double, 1, 1>(K64, nint, vr, ok, m, lds_ld, kr1, acc);
+            } else
               stencil_rows<double, double, 1>(K64, nint, vr, ok, m, ld, acc);
             if (!ok[0]) continue;
 #pragma unroll

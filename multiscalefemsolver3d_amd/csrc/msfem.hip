@@ -2710,19 +2710,23 @@ __device__ __forceinline__ double hat_at(int v, int n, int a, int q) {
   return ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
 }
 
-// T[cell][(v*3+a)*24+q] = x - hat (all cells the same n); mode 1: x = hat + G
-__global__ void k_targets_guess(const Cell* __restrict__ cells, int ncells, int n, int64_t per_cell,
+// T[cell][(v*3+a)*24+q] = x - hat (all cells the same n); mode 1: x = hat + G.
+// blockIdx.y walks the cells and blockIdx.x the cell's per_cell entries, so
+// the entry index is 32-bit and its split into (v, a, q) divides by
+// constants.
+__global__ void k_targets_guess(const Cell* __restrict__ cells, int ncells, int n, int per_cell,
                                 double* __restrict__ x64, float* __restrict__ TG, int mode) {
-  const int64_t total = per_cell * ncells;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t cell = e / per_cell, r = e % per_cell;
-    const int q = (int)(r % NB), a = (int)((r / NB) % 3), v = (int)(r / (3 * NB));
-    const int64_t at = cells[cell].int0 * 3 * NB + r;
-    const double hat = hat_at(v, n, a, q);
-    if (mode == 0)
-      TG[e] = (float)(x64[at] - hat);
-    else
-      x64[at] = hat + (TG ? (double)TG[e] : 0.0);
+  for (int cell = blockIdx.y; cell < ncells; cell += gridDim.y) {
+    double* xc = x64 + cells[cell].int0 * 3 * NB;
+    float* tc = TG ? TG + (int64_t)cell * per_cell : nullptr;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < per_cell; r += gridDim.x * blockDim.x) {
+      const int q = r % NB, a = (r / NB) % 3, v = r / (3 * NB);
+      const double hat = hat_at(v, n, a, q);
+      if (mode == 0)
+        tc[r] = (float)(xc[r] - hat);
+      else
+        xc[r] = hat + (tc ? (double)tc[r] : 0.0);
+    }
   }
 }
 
@@ -2750,9 +2754,11 @@ static int targets_guess(mfs_batch B, mfs_vec V, int mode) {
   MFS_CHECK_ARG(n > 0, "mfs_batch_targets/guess: all cells must have the same subdivision");
   const int64_t per_cell = (int64_t)(n - 1) * (n - 1) * (n - 1) * 3 * NB;
   if (V) MFS_CHECK_ARG(V->prec == MFS_F32 && V->n == per_cell * B->ncells, "mfs_batch_targets/guess: bad vector");
-  const int64_t total = per_cell * B->ncells;
-  k_targets_guess<<<(unsigned)std::min<int64_t>(65536, (total + 255) / 256), 256, 0, B->ctx->stream>>>(
-      B->d_cells, B->ncells, n, per_cell, B->d_x64, V ? (float*)V->d : nullptr, mode);
+  MFS_CHECK_ARG(mode == 1 || V, "mfs_batch_targets: bad vector");
+  // 1,024 entries per workgroup along x (4 per thread), cells along y
+  const dim3 grid((unsigned)((per_cell + 1023) / 1024), (unsigned)std::min(B->ncells, 65535));
+  k_targets_guess<<<grid, 256, 0, B->ctx->stream>>>(B->d_cells, B->ncells, n, (int)per_cell, B->d_x64,
+                                                    V ? (float*)V->d : nullptr, mode);
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipStreamSynchronize(B->ctx->stream));
   return MFS_OK;

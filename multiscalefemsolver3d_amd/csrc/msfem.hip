@@ -2014,6 +2014,8 @@ struct mfs_batch_s {
   double* d_ft = nullptr;  // reference traces: face traces
   int* d_face_order = nullptr;
   unsigned long long* d_fstats = nullptr;  // [0] worst relative residual bits, [1] failures
+  double* d_hat = nullptr;  // learned guess: the hat interpolant of one cell's entries
+  int64_t hat_len = 0;
 };
 
 static int batch_free(mfs_batch B) {
@@ -2022,7 +2024,7 @@ static int batch_free(mfs_batch B) {
   void* ptrs[] = {B->d_cells, B->d_int0, B->d_tmpls, B->d_gptr, B->d_gent, B->d_E, B->d_K32, B->d_K64,
                   B->d_dinv32, B->d_dinv64, B->d_b64, B->d_x64, B->d_iters, B->d_relres, B->d_flags,
                   B->d_task_cell, B->d_task_group, B->d_fb, B->d_queue, B->d_scr, B->d_et, B->d_ft, B->d_face_order,
-                  B->d_fstats};
+                  B->d_fstats, B->d_hat};
   for (void* p : ptrs) dfree(p);
   mfs_ctx c = B->ctx;
   delete B;
@@ -2710,18 +2712,25 @@ __device__ __forceinline__ double hat_at(int v, int n, int a, int q) {
   return ((c & 1) ? X : 1 - X) * ((c & 2) ? Y : 1 - Y) * ((c & 4) ? Z : 1 - Z);
 }
 
+// The hat interpolant at every entry (v*3+a)*24+q of one n-cell: the same
+// for every cell of a batch of equal subdivisions.
+__global__ void k_hat_table(int n, int per_cell, double* __restrict__ hat) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < per_cell) hat[r] = hat_at(r / (3 * NB), n, (r / NB) % 3, r % NB);
+}
+
 // T[cell][(v*3+a)*24+q] = x - hat (all cells the same n); mode 1: x = hat + G.
-// blockIdx.y walks the cells and blockIdx.x the cell's per_cell entries, so
-// the entry index is 32-bit and its split into (v, a, q) divides by
-// constants.
-__global__ void k_targets_guess(const Cell* __restrict__ cells, int ncells, int n, int per_cell,
-                                double* __restrict__ x64, float* __restrict__ TG, int mode) {
+// blockIdx.y walks the cells and blockIdx.x the cell's per_cell entries;
+// hat comes from k_hat_table (L2-resident, 8 bytes per entry), so an entry
+// costs a load and a store and no divisions.
+__global__ void k_targets_guess(const Cell* __restrict__ cells, int ncells, int per_cell,
+                                const double* __restrict__ hat_tab, double* __restrict__ x64,
+                                float* __restrict__ TG, int mode) {
   for (int cell = blockIdx.y; cell < ncells; cell += gridDim.y) {
     double* xc = x64 + cells[cell].int0 * 3 * NB;
     float* tc = TG ? TG + (int64_t)cell * per_cell : nullptr;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < per_cell; r += gridDim.x * blockDim.x) {
-      const int q = r % NB, a = (r / NB) % 3, v = r / (3 * NB);
-      const double hat = hat_at(v, n, a, q);
+      const double hat = hat_tab[r];
       if (mode == 0)
         tc[r] = (float)(xc[r] - hat);
       else
@@ -2755,9 +2764,19 @@ static int targets_guess(mfs_batch B, mfs_vec V, int mode) {
   const int64_t per_cell = (int64_t)(n - 1) * (n - 1) * (n - 1) * 3 * NB;
   if (V) MFS_CHECK_ARG(V->prec == MFS_F32 && V->n == per_cell * B->ncells, "mfs_batch_targets/guess: bad vector");
   MFS_CHECK_ARG(mode == 1 || V, "mfs_batch_targets: bad vector");
+  if (B->hat_len != per_cell) {
+    dfree(B->d_hat);
+    B->d_hat = nullptr;
+    B->hat_len = 0;
+    const int st = dmalloc((void**)&B->d_hat, sizeof(double) * per_cell);
+    if (st != MFS_OK) return st;
+    B->hat_len = per_cell;
+  }
+  k_hat_table<<<(unsigned)((per_cell + 255) / 256), 256, 0, B->ctx->stream>>>(n, (int)per_cell, B->d_hat);
+  MFS_HIP(hipGetLastError());
   // 1,024 entries per workgroup along x (4 per thread), cells along y
   const dim3 grid((unsigned)((per_cell + 1023) / 1024), (unsigned)std::min(B->ncells, 65535));
-  k_targets_guess<<<grid, 256, 0, B->ctx->stream>>>(B->d_cells, B->ncells, n, (int)per_cell, B->d_x64,
+  k_targets_guess<<<grid, 256, 0, B->ctx->stream>>>(B->d_cells, B->ncells, (int)per_cell, B->d_hat, B->d_x64,
                                                     V ? (float*)V->d : nullptr, mode);
   MFS_HIP(hipGetLastError());
   MFS_HIP(hipStreamSynchronize(B->ctx->stream));

@@ -18,3 +18,51 @@ def test_learned_guess_small(ctx):
     assert out["correction_rel_error"]["learned_guess"] < 0.8, out
     assert out["iterations_saved_vs_zero"]["learned"] > 0.05, out
     assert out["iterations_saved_learned_vs_hat"] > 0.0, out
+
+
+def q1_hat_values(n):
+    """The Q1 coarse basis of an n-cell at its interior nodes, laid out as
+    the batch's guesses: entry (v*3 + a)*24 + q of interior node v = i + m*(j
+    + m*k) (m = n - 1, fine node (i+1, j+1, k+1)/n), basis q = 3*c + d of
+    corner c (bit 0: x = 1, bit 1: y = 1, bit 2: z = 1) and direction d.
+    The corner's hat is prod(1 - |x - corner|), written independently of the
+    library's form."""
+    import numpy as np
+    m = n - 1
+    g = (np.arange(m) + 1.0) / n
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")  # node v = i + m*(j + m*k)
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    out = np.zeros((m ** 3, 3, 24))
+    for c in range(8):
+        corner = np.array([c & 1, (c >> 1) & 1, (c >> 2) & 1], dtype=float)
+        hat = np.prod(1.0 - np.abs(pts - corner), axis=1)
+        for d in range(3):
+            out[:, d, 3 * c + d] = hat
+    return out.ravel()
+
+
+def test_guess_and_targets_use_the_q1_hat(ctx):
+    """targets(T) = x - hat and guess_from(G) = hat + G against an
+    independent restatement of the hat functions, and their round trip."""
+    import numpy as np
+    n, count = 5, 3
+    cn, org, h, E = learned.cells(count, n, 7)
+    B = mfs.Batch(ctx, cn, org, h, E)
+    hat = np.tile(q1_hat_values(n), count)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=hat.size)
+    B.set_guess(x)
+    T = mfs.Vec(ctx, hat.size, mfs.F32)
+    B.targets(T)
+    np.testing.assert_allclose(T.download(), (x - hat).astype(np.float32), rtol=1e-6, atol=1e-7)
+    # guess_from(None): x = hat exactly, so the targets are exactly zero
+    B.guess_from(None)
+    B.targets(T)
+    assert not np.any(T.download())
+    # guess_from(G) then targets: (hat + G) - hat gives G back bit for bit
+    G = rng.normal(size=hat.size).astype(np.float32)
+    Gv = mfs.Vec(ctx, hat.size, mfs.F32, G)
+    B.guess_from(Gv)
+    B.targets(T)
+    np.testing.assert_array_equal(T.download(), G)
+    B.close()

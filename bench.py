@@ -72,8 +72,8 @@ def main():
     ap.add_argument("--cells", type=int, default=32768, help="headline: coarse cells (32^3)")
     ap.add_argument("--train-cells", type=int, default=20000)
     ap.add_argument("--epochs", type=int, default=20)
-    ap.add_argument("--shape", default="", help="headline batch-solve launch shape WG,VARIANT for both precisions "
-                                               "instead of the autotuner (counter runs)")
+    ap.add_argument("--shape", default="", help="headline batch-solve launch shape WG,VARIANT for both precisions, "
+                                               "or WG,VARIANT,WG64,VARIANT64, instead of the autotuner (counter runs)")
     ap.add_argument("--n", type=int, default=64, help="Laplace leg: mesh hexes per edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work per leg")
@@ -170,8 +170,10 @@ def bench_headline(mfs, ctx, args):
     t0 = time.time()
     shape = None
     if args.shape:
-        wg, var = (int(v) for v in args.shape.split(","))
-        shape = {mfs.PREC_MIXED: {"wg_size": wg, "variant": var}, mfs.PREC_F64: {"wg_size": wg, "variant": var}}
+        v = [int(x) for x in args.shape.split(",")]
+        wg, var = v[0], v[1]
+        wg64, var64 = (v[2], v[3]) if len(v) == 4 else (wg, var)
+        shape = {mfs.PREC_MIXED: {"wg_size": wg, "variant": var}, mfs.PREC_F64: {"wg_size": wg64, "variant": var64}}
     model = learned.train(ctx, n=n, n_train=args.train_cells, hidden=256, layers=4, epochs=args.epochs, batch=256,
                           lr=1e-3, tol=1e-8, log=log, tune=shape is None, shape=shape)
     train_wall = time.time() - t0

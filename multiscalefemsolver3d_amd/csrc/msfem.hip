@@ -769,6 +769,12 @@ __device__ __forceinline__ int dpp_partner_i32(int v, int o) {
     default: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false); // row_mirror (o = 8)
   }
 }
+// lane l of every row of 16 gets lane l ^ 8's value (DPP row_ror:8)
+__device__ __forceinline__ double row_ror8_f64(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x128, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x128, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double partner_f64(double v, int o) {
   if (o >= 16) return __shfl_xor(v, o, 64);
   const int lo = dpp_partner_i32(__double2loint(v), o);
@@ -1321,9 +1327,12 @@ void k_batch_solve(SolveArgs A) {
           else
             phase_spmv<T, RPT>(K32, K64, nint, m, P, Q, w);
           // alpha of right-hand side r computed in lane r, then broadcast
+          // (the state is read before the reduction's barrier: it was
+          // written before the previous iteration's last barrier)
+          const double rzp = lane < RG ? S.rz[par][lane] : 0.0;
           const double pql = wg_reduce_to_lanes<RG>(w, partB);
           T al = 0;
-          if (lane < RG && ((act >> lane) & 1) && pql != 0.0) al = (T)(S.rz[par][lane] / pql);
+          if (lane < RG && ((act >> lane) & 1) && pql != 0.0) al = (T)(rzp / pql);
 #pragma unroll
           for (int r = 0; r < RG; r++) alpha[r] = readlane_t(al, r);
         }
@@ -1339,21 +1348,26 @@ void k_batch_solve(SolveArgs A) {
           else
             phase_update<T, RPT>(nint, dinv, P, Q, R, D, alpha, w);
           // lane r < RG decides right-hand side r from its r.z (lane r) and
-          // r.r (lane RG + r); the decisions are gathered with ballots
+          // r.r (lane RG + r, moved to lane r within the row of 16); the
+          // decisions are gathered with ballots.  The state it needs is read
+          // before the reduction's barrier (written before an earlier one).
+          static_assert(RG == 8, "r.r is moved from lane r + 8 by a row rotation of 8");
+          const int sl = lane < RG ? lane : 0;
+          const double rz_old = S.rz[par][sl], itol2 = S.itol2[sl], rrmin = S.rrmin[par][sl];
+          const int itbase = S.itbase[sl], itmin = S.itmin[par][sl];
           const double rzl = wg_reduce_to_lanes<2 * RG>(w, partA);
-          const double rrl = __shfl(rzl, (lane + RG) & 63, 64);
+          const double rrl = row_ror8_f64(rzl);
           const bool mine = lane < RG && ((act >> lane) & 1);
           bool cont = false, stag = false;
           T be = 0;
           if (mine) {
-            const bool stop =
-                !(rrl == rrl) || rrl <= S.itol2[lane] || S.itbase[lane] + inner_it + 1 >= A.maxit;
+            const bool stop = !(rrl == rrl) || rrl <= itol2 || itbase + inner_it + 1 >= A.maxit;
             if (!stop) {
-              const int itm = rrl < 0.25 * S.rrmin[par][lane] ? inner_it + 1 : S.itmin[par][lane];
+              const int itm = rrl < 0.25 * rrmin ? inner_it + 1 : itmin;
               stag = inner_it + 1 - itm > kStagWindow;
             }
             cont = !(stop || stag);
-            if (cont) be = (T)(rzl / S.rz[par][lane]);
+            if (cont) be = (T)(rzl / rz_old);
           }
           act2 = (int)(__ballot(cont) & ((1u << RG) - 1));
           capfail |= (int)(__ballot(stag) & ((1u << RG) - 1));  // recorded after the x update

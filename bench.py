@@ -40,10 +40,10 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_K1 = 29470310
 # HBM bytes of one k_batch_solve<float,1,true> launch on the headline
 # workload (learned guess, launch shape 384 / variant 3): rocprofv3 --pmc
-# FETCH_SIZE 58,467,410.1 KiB x2.000 + WRITE_SIZE 14,459,422.6 KiB x1.000
+# FETCH_SIZE 57,946,409.9 KiB x2.000 + WRITE_SIZE 14,526,056.7 KiB x1.000
 # (the copy calibration above).  The register-resident kernel keeps r, q, d
 # and p on chip, so this is 0.058 of the algorithmic bytes.
-TRAFFIC_BATCH = 134547704627
+TRAFFIC_BATCH = 133548929536
 
 
 def log(*a):

@@ -2030,6 +2030,7 @@ struct mfs_batch_s {
   unsigned long long* d_fstats = nullptr;  // [0] worst relative residual bits, [1] failures
   double* d_hat = nullptr;  // learned guess: the hat interpolant of one cell's entries
   int64_t hat_len = 0;
+  unsigned long long* d_report = nullptr;  // k_batch_report's six sums
 };
 
 static int batch_free(mfs_batch B) {
@@ -2038,7 +2039,7 @@ static int batch_free(mfs_batch B) {
   void* ptrs[] = {B->d_cells, B->d_int0, B->d_tmpls, B->d_gptr, B->d_gent, B->d_E, B->d_K32, B->d_K64,
                   B->d_dinv32, B->d_dinv64, B->d_b64, B->d_x64, B->d_iters, B->d_relres, B->d_flags,
                   B->d_task_cell, B->d_task_group, B->d_fb, B->d_queue, B->d_scr, B->d_et, B->d_ft, B->d_face_order,
-                  B->d_fstats, B->d_hat};
+                  B->d_fstats, B->d_hat, B->d_report};
   for (void* p : ptrs) dfree(p);
   mfs_ctx c = B->ctx;
   delete B;
@@ -2187,6 +2188,7 @@ extern "C" int mfs_batch_create(mfs_ctx ctx, const mfs_batch_desc* d, mfs_batch*
   alloc((void**)&B->d_task_group, sizeof(int) * B->ntasks);
   alloc((void**)&B->d_fb, sizeof(int) * (1 + 2 * (size_t)B->ntasks));
   alloc((void**)&B->d_queue, sizeof(unsigned) * NQ * QSTRIDE);
+  alloc((void**)&B->d_report, sizeof(unsigned long long) * 6);
   if (B->trace == MFS_TRACE_REFERENCE) {
     alloc((void**)&B->d_et, sizeof(double) * B->et_total);
     alloc((void**)&B->d_ft, sizeof(double) * B->ft_total);
@@ -2485,30 +2487,62 @@ static int pick_shape(mfs_batch B, const mfs_batch_solve_desc* d, int* wg, int* 
   return MFS_OK;
 }
 
+// The solve report reduced on the device, so that a solve hands 48 bytes to
+// the host instead of three per-problem arrays: [0] largest iteration count,
+// [1] sum of the counts, [2] bits of the largest true relative residual
+// (non-negative doubles order like their bit patterns; NaN is skipped),
+// [3] problems not converged, [4] FP64 retries from zero, [5] problems
+// handed from FP32 to FP64.  out is cleared by the caller.
+__global__ void k_batch_report(const int* __restrict__ it, const int* __restrict__ fl,
+                               const double* __restrict__ rr, int nq, unsigned long long* __restrict__ out) {
+  unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    const unsigned long long i = (unsigned long long)max(it[q], 0);
+    const int f = fl[q];
+    const double r = rr[q];
+    v[0] = max(v[0], i);
+    v[1] += i;
+    if (r == r) v[2] = max(v[2], (unsigned long long)__double_as_longlong(r > 0.0 ? r : 0.0));
+    v[3] += f & 1;
+    v[4] += (f >> 1) & 1;
+    v[5] += (f >> 3) & 1;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const unsigned long long w = __shfl_xor(v[k], o, 64);
+      v[k] = (k == 0 || k == 2) ? max(v[k], w) : v[k] + w;
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out + 0, v[0]);
+    atomicAdd(out + 1, v[1]);
+    atomicMax(out + 2, v[2]);
+    atomicAdd(out + 3, v[3]);
+    atomicAdd(out + 4, v[4]);
+    atomicAdd(out + 5, v[5]);
+  }
+}
+
 static int batch_collect(mfs_batch B, mfs_batch_report* rep) {
   hipStream_t s = B->ctx->stream;
   const int nq = B->ncells * NB;
-  std::vector<int> it(nq), fl(nq);
-  std::vector<double> rr(nq);
-  MFS_HIP(hipMemcpyAsync(it.data(), B->d_iters, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
-  MFS_HIP(hipMemcpyAsync(fl.data(), B->d_flags, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
-  MFS_HIP(hipMemcpyAsync(rr.data(), B->d_relres, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+  unsigned long long h[6];
+  MFS_HIP(hipMemsetAsync(B->d_report, 0, sizeof(h), s));
+  k_batch_report<<<(unsigned)std::min(1024, (nq + 255) / 256), 256, 0, s>>>(B->d_iters, B->d_flags, B->d_relres, nq,
+                                                                            B->d_report);
+  MFS_HIP(hipGetLastError());
+  MFS_HIP(hipMemcpyAsync(h, B->d_report, sizeof(h), hipMemcpyDeviceToHost, s));
   MFS_HIP(hipStreamSynchronize(s));
-  rep->max_iterations = 0;
-  rep->mean_iterations = 0;
-  rep->max_true_relres = 0;
-  rep->n_not_converged = 0;
-  rep->n_fallback = 0;
-  rep->n_precision_fallback = 0;
-  for (int q = 0; q < nq; q++) {
-    rep->max_iterations = std::max(rep->max_iterations, it[q]);
-    rep->mean_iterations += it[q];
-    rep->max_true_relres = std::max(rep->max_true_relres, rr[q]);
-    rep->n_not_converged += fl[q] & 1;
-    rep->n_fallback += (fl[q] >> 1) & 1;
-    rep->n_precision_fallback += (fl[q] >> 3) & 1;
-  }
-  rep->mean_iterations /= nq;
+  rep->max_iterations = (int)h[0];
+  rep->mean_iterations = (double)h[1] / nq;
+  double rmax;
+  std::memcpy(&rmax, &h[2], sizeof(rmax));
+  rep->max_true_relres = rmax;
+  rep->n_not_converged = (int)h[3];
+  rep->n_fallback = (int)h[4];
+  rep->n_precision_fallback = (int)h[5];
   return MFS_OK;
 }
 

@@ -2768,21 +2768,28 @@ __global__ void k_hat_table(int n, int per_cell, double* __restrict__ hat) {
 }
 
 // T[cell][(v*3+a)*24+q] = x - hat (all cells the same n); mode 1: x = hat + G.
-// blockIdx.y walks the cells and blockIdx.x the cell's per_cell entries;
-// hat comes from k_hat_table (L2-resident, 8 bytes per entry), so an entry
-// costs a load and a store and no divisions.
+// blockIdx.y walks the cells and blockIdx.x the cell's per_cell entries, two
+// adjacent entries per thread (per_cell = 72 (n-1)^3 is even and a cell's
+// entries start 16-byte aligned, so they move as float2 / double2); hat
+// comes from k_hat_table (L2-resident), so an entry costs a load and a store
+// and no divisions.
 __global__ void k_targets_guess(const Cell* __restrict__ cells, int ncells, int per_cell,
                                 const double* __restrict__ hat_tab, double* __restrict__ x64,
                                 float* __restrict__ TG, int mode) {
+  const int half = per_cell / 2;
+  const double2* ht = reinterpret_cast<const double2*>(hat_tab);
   for (int cell = blockIdx.y; cell < ncells; cell += gridDim.y) {
-    double* xc = x64 + cells[cell].int0 * 3 * NB;
-    float* tc = TG ? TG + (int64_t)cell * per_cell : nullptr;
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < per_cell; r += gridDim.x * blockDim.x) {
-      const double hat = hat_tab[r];
-      if (mode == 0)
-        tc[r] = (float)(xc[r] - hat);
-      else
-        xc[r] = hat + (tc ? (double)tc[r] : 0.0);
+    double2* xc = reinterpret_cast<double2*>(x64 + cells[cell].int0 * 3 * NB);
+    float2* tc = TG ? reinterpret_cast<float2*>(TG + (int64_t)cell * per_cell) : nullptr;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < half; r += gridDim.x * blockDim.x) {
+      const double2 hat = ht[r];
+      if (mode == 0) {
+        const double2 x = xc[r];
+        tc[r] = make_float2((float)(x.x - hat.x), (float)(x.y - hat.y));
+      } else {
+        const float2 g = tc ? tc[r] : make_float2(0.f, 0.f);
+        xc[r] = make_double2(hat.x + (double)g.x, hat.y + (double)g.y);
+      }
     }
   }
 }
@@ -2822,8 +2829,8 @@ static int targets_guess(mfs_batch B, mfs_vec V, int mode) {
   }
   k_hat_table<<<(unsigned)((per_cell + 255) / 256), 256, 0, B->ctx->stream>>>(n, (int)per_cell, B->d_hat);
   MFS_HIP(hipGetLastError());
-  // 1,024 entries per workgroup along x (4 per thread), cells along y
-  const dim3 grid((unsigned)((per_cell + 1023) / 1024), (unsigned)std::min(B->ncells, 65535));
+  // 2,048 entries per workgroup along x (2 x 4 per thread), cells along y
+  const dim3 grid((unsigned)((per_cell + 2047) / 2048), (unsigned)std::min(B->ncells, 65535));
   k_targets_guess<<<grid, 256, 0, B->ctx->stream>>>(B->d_cells, B->ncells, (int)per_cell, B->d_hat, B->d_x64,
                                                     V ? (float*)V->d : nullptr, mode);
   MFS_HIP(hipGetLastError());

@@ -77,6 +77,28 @@ def test_register_variant_rejects_cells_larger_than_the_workgroup(ctx):
     B.close()
 
 
+def test_solve_report_matches_the_per_problem_status(ctx):
+    """The solve report is reduced on the device; it must equal the same
+    reductions of the per-problem arrays (iterations, true residuals, flags)
+    for converged solves in both precisions and for one capped at too few
+    iterations, where problems are reported as not converged."""
+    p = problems.msfem_cells(3, n_lo=3, n_hi=5, seed_E=9)
+    B = make_batch(ctx, p)
+    runs = (dict(precision=mfs.PREC_MIXED, tol=1e-10), dict(precision=mfs.PREC_F64, tol=1e-10),
+            dict(precision=mfs.PREC_F64, tol=1e-10, maxit=5))
+    for kw in runs:
+        rep = B.solve(check=False, use_graph=False, **kw)
+        it, rr, fl = B.status()
+        assert rep.max_iterations == max(0, int(it.max())), kw
+        assert rep.mean_iterations == np.maximum(it, 0).sum() / it.size, kw
+        assert rep.max_true_relres == max(0.0, float(np.max(rr[~np.isnan(rr)], initial=0.0))), kw
+        assert rep.n_not_converged == int((fl & 1).sum()), kw
+        assert rep.n_fallback == int(((fl >> 1) & 1).sum()), kw
+        assert rep.n_precision_fallback == int(((fl >> 3) & 1).sum()), kw
+    assert rep.n_not_converged > 0 and rep.status != 0
+    B.close()
+
+
 def test_fp64_pass_retries_a_stalled_problem_from_zero(ctx):
     """A cell (8^3 hexes, reference element convention: indefinite local
     operators) whose FP32 solves fail for some bases; the FP64 pass starts

@@ -450,6 +450,64 @@ __device__ __forceinline__ void stencil_rows(const KT* __restrict__ Kc, int nint
   }
 }
 
+// stencil_rows for the slab variants: the same products in the same order,
+// with two K register sets used in turn (the next point's K is requested
+// into one while the other is applied, two points per loop trip), so no set
+// is copied from one point to the next.
+template <typename KT, typename VT, int RPT, typename Load>
+__device__ __forceinline__ void stencil_rows_pp(const KT* __restrict__ Kc, int nint, const int (&v)[RPT],
+                                                const bool (&ok)[RPT], int m, Load load, VT (&acc)[RPT][3][RG]) {
+  int vi[RPT];
+  const KT* kp[RPT];
+  KT ka[RPT][9], kb[RPT][9];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    vi[q] = ok[q] ? v[q] : 0;
+    kp[q] = Kc + vi[q];
+#pragma unroll
+    for (int e = 0; e < 9; e++) ka[q][e] = kp[q][(int64_t)e * nint];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int r = 0; r < RG; r++) acc[q][a][r] = 0;
+  }
+  auto request = [&](KT (&kn)[RPT][9]) {
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      kp[q] += (int64_t)9 * nint;
+#pragma unroll
+      for (int e = 0; e < 9; e++) kn[q][e] = kp[q][(int64_t)e * nint];
+    }
+  };
+  auto apply = [&](int s, const KT (&kv)[RPT][9]) {
+    const int off = stencil_offset(s, m);
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int u = neighbour_row(vi[q], off, nint);
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        VT pv[RG];
+        load(u, b, pv);
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+          acc[q][0][r] += (VT)kv[q][0 * 3 + b] * pv[r];
+          acc[q][1][r] += (VT)kv[q][1 * 3 + b] * pv[r];
+          acc[q][2][r] += (VT)kv[q][2 * 3 + b] * pv[r];
+        }
+      }
+    }
+  };
+  static_assert(NS % 2 == 1, "the last stencil point is applied after the paired loop");
+#pragma unroll 1
+  for (int s = 0; s + 1 < NS; s += 2) {
+    request(kb);  // point s + 1
+    apply(s, ka);
+    request(ka);  // point s + 2 (NS is odd, so it exists)
+    apply(s + 1, kb);
+  }
+  apply(NS - 1, ka);
+}
+
 // Inner-iteration vector phases over RPT rows of one thread at a time
 // (separate restrict-qualified arrays, so the loads of the RPT rows are
 // issued together).
@@ -474,9 +532,9 @@ __device__ __forceinline__ void phase_spmv(const float* __restrict__ K32, const 
     }
     T acc[RP][3][RG];
     if constexpr (fp64)
-      stencil_rows<double, double, RP>(K64, nint, vr, ok, m, ld, acc);
+      stencil_rows_pp<double, double, RP>(K64, nint, vr, ok, m, ld, acc);
     else
-      stencil_rows<float, float, RP>(K32, nint, vr, ok, m, ld, acc);
+      stencil_rows_pp<float, float, RP>(K32, nint, vr, ok, m, ld, acc);
 #pragma unroll
     for (int q = 0; q < RP; q++) {
       if (!ok[q]) continue;
@@ -1082,7 +1140,7 @@ void k_batch_solve(SolveArgs A) {
             ok[q] = vr[q] < nint;
           }
           double acc[RR][3][RG];
-          stencil_rows<double, double, RR>(K64, nint, vr, ok, m, ld, acc);
+          stencil_rows_pp<double, double, RR>(K64, nint, vr, ok, m, ld, acc);
 #pragma unroll
           for (int q = 0; q < RR; q++) {
             if (!ok[q]) continue;
